@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""k-Shape benchmark on one MI355X: n=100000 series of length m=512, k=16
+clusters (FFT length 1024), the large single-job workload with the precision
+path (FP32 SBD, split-BF16 MFMA for S).
+
+A step is one full k-Shape iteration (shape extraction for all k clusters,
+then the fused SBD assignment of all n series) on data resident in HBM.
+`value` = SBD pairs per second over full iterations (n*k per iteration).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "times-c_amd"))
+
+METRIC = "SBD pairs/s + k-Shape iter/s at n=100k,m=512,k=16; p50/p99 job latency"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def fft_size(m):
+    return 1 << (2 * m - 1).bit_length()
+
+
+def sbd_assign_flops(n, m, k):
+    """Algorithmic FP32 flops of one assignment (FFTW convention 2.5 F log2 F
+    per real transform): forward transform + norm per series, and per pair a
+    conj-multiply of F/2+1 bins, an inverse transform and the 2m-1 lag max."""
+    F = fft_size(m)
+    t = 2.5 * F * math.log2(F)
+    return n * (t + 2 * m) + n * k * (t + 6 * (F // 2 + 1) + 2 * m - 1)
+
+
+def iteration_bytes(n, m, k):
+    """Compulsory HBM bytes of one iteration: X read by refine and by assign,
+    label/dist/shift, centroids and their spectra."""
+    F = fft_size(m)
+    return 2 * n * m * 4 + 3 * n * 4 + k * m * 4 + k * (F // 2 + 1) * 8
+
+
+def cpu_baseline(X, k, sample, seed):
+    """FP64 oracle (the C restatement of the reference's CPU path) on a bounded
+    sample of the same workload, all host cores: one refine + one assign."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+
+    import timesc
+
+    Xs = oracle_py.zscore(X[:sample].astype(np.float64))
+    lab = timesc.init_labels(seed, sample, k)
+    C0 = np.zeros((k, X.shape[1]))
+    threads = os.cpu_count() or 1
+    # untimed first iteration (zero centroids, no alignment), then one timed
+    # iteration with alignment, like the GPU's timed steps
+    C, _, _ = oracle_py.refine(Xs, lab, C0, seed=seed, it=0, nthreads=threads)
+    lab, _, _ = oracle_py.assign(Xs, C, nthreads=threads)
+    t0 = time.perf_counter()
+    C, _, _ = oracle_py.refine(Xs, lab, C, seed=seed, it=1, nthreads=threads)
+    t1 = time.perf_counter()
+    oracle_py.assign(Xs, C, nthreads=threads)
+    t2 = time.perf_counter()
+    pairs = sample * k
+    return {
+        "value": pairs / (t2 - t0),
+        "unit": "SBD pairs/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"iteration 2 (refine {t1 - t0:.1f} s + assign {t2 - t1:.1f} s) on the first "
+                  f"{sample} series (m={X.shape[1]}, k={k}), FP64 oracle, OpenMP",
+        "assign_pairs_per_s": pairs / (t2 - t1),
+        "iter_per_s": 1.0 / (t2 - t0),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=100000)
+    ap.add_argument("--m", type=int, default=512)
+    ap.add_argument("--k", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-sample", type=int, default=100000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only; run with --gpus 1", file=sys.stderr)
+        return 2
+    import timesc
+
+    n, m, k = args.n, args.m, args.k
+    print(f"# generating n={n} m={m} k={k}", file=sys.stderr, flush=True)
+    X, y = timesc.generate(args.seed, n, m, k, 0.5)
+    ctx = timesc.Context(0)
+    dx = ctx.to_device(X)
+    ctx.znorm(dx, ddof=0)
+    ctx.fit_begin(dx, k, max_iter=1 << 30, seed=args.seed)
+    for _ in range(args.warmup):
+        ctx.fit_step()
+    ctx.sync()
+    s0 = ctx.fit_query()
+    print(f"# warmup done ({args.warmup} iterations)", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.fit_step()
+    ctx.sync()
+    t1 = time.perf_counter()
+    s1 = ctx.fit_query()
+    res = ctx.fit_end(n, k, m)
+    dt = t1 - t0
+    ms_step = dt / args.steps * 1e3
+    ms_assign = (s1["ms_assign"] - s0["ms_assign"]) / args.steps
+    ms_refine = (s1["ms_refine"] - s0["ms_refine"]) / args.steps
+    flops = sbd_assign_flops(n, m, k)
+    achieved = flops / (ms_assign * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": n * k * args.steps / dt,
+        "unit": "SBD pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"k-Shape iteration, n={n} m={m} k={k} F={fft_size(m)}, sigma=0.5, "
+                        "FP32 SBD, split-BF16 MFMA S",
+            "n": n, "m": m, "k": k, "fft": fft_size(m), "s_precision": "split-bf16",
+        },
+        "iter_per_s": args.steps / dt,
+        "assign_ms": ms_assign,
+        "refine_ms": ms_refine,
+        "assign_pairs_per_s": n * k / (ms_assign * 1e-3),
+        "hbm_fraction_of_iteration": iteration_bytes(n, m, k) / (PEAK_HBM_GBS * 1e9) / (ms_step * 1e-3),
+        "power_steps_max": s1["power_steps_max"],
+        "p50_job_ms": None,
+        "p99_job_ms": None,
+        "roofline": {
+            "bound": "fp32",
+            "kernel": "k_sbd (fused FFT SBD assignment)",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None,
+        },
+        "cpu_baseline": None,
+    }
+    dx.free()
+    ctx.close()
+    if not args.no_cpu_baseline:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(X, k, min(args.cpu_sample, n), args.seed)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,222 @@
+/*
+ * timesc.h -- C ABI of the MI355X k-Shape engine (libtimesc_hip.so).
+ *
+ * The engine replaces the GPU path of rbowS/Times-C.  Each entry point below
+ * names the reference interface it stands in for (paths relative to the
+ * reference repository).  All entry points:
+ *   - take plain pointers and sizes ("d_" = device memory, "h_" = host);
+ *   - return an int status, 0 = TCX_OK (the reference prints and calls
+ *     exit(-1) instead, check.cuh:7-19; timesc_ref.hpp restores that
+ *     behaviour for drop-in callers);
+ *   - are synchronous with respect to the host unless stated otherwise and
+ *     run on the context's stream;
+ *   - follow the reference's CPU path semantics (Code/test/test_timesC.py)
+ *     for every result: FFT length F = 1 << (2m-1).bit_length(), first-index
+ *     argmax over lags, first-index argmin over clusters, alignment sign of
+ *     _preSBD_extract_shape, zscore with nan_to_num.
+ * Series are rows of a row-major [n][m] FP32 matrix.
+ */
+#ifndef TIMESC_H
+#define TIMESC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define TCX_OK 0
+#define TCX_ERR_ARG 1         /* invalid argument (sizes, k > n, null pointer) */
+#define TCX_ERR_HIP 2         /* a HIP runtime call failed                     */
+#define TCX_ERR_NOMEM 3       /* device allocation failed                      */
+#define TCX_ERR_BUDGET 4      /* request exceeds the context's HBM budget      */
+#define TCX_ERR_IO 5          /* file missing, unreadable or malformed         */
+#define TCX_ERR_UNSUPPORTED 6 /* option not supported by this build            */
+#define TCX_ERR_NUMERIC 7     /* kernel reported a numerical failure           */
+
+/* alignment sign conventions (see tcx_sbd_align) */
+#define TCX_CONV_CPU 0    /* cc(member, centroid): test_timesC.py:129-132 (default) */
+#define TCX_CONV_KSHAPE 1 /* cc(centroid, member): SBD.cuh:61,78                   */
+
+/* precision of S = Y^T Y in shape extraction */
+#define TCX_S_SPLIT_BF16 0 /* hi+lo BF16 MFMA, FP32 accumulation (default)      */
+#define TCX_S_BF16 1       /* single BF16 MFMA product, FP32 accumulation        */
+
+typedef struct tcx_ctx tcx_ctx;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* tcx_last_error(void);
+
+/* Context: device, stream, cached device workspace, FFT twiddle tables.
+ * hbm_budget = 0 means no limit beyond the device's memory.  No reference
+ * counterpart: the reference allocates and frees per call
+ * (kshape.cuh:209-216, 317-323; ncc.cuh:484-488). */
+int tcx_create(int device, size_t hbm_budget, tcx_ctx** out);
+int tcx_destroy(tcx_ctx* ctx);
+/* Bytes of workspace the context currently holds. */
+size_t tcx_workspace_bytes(const tcx_ctx* ctx);
+
+/* Device memory helpers for hosts without their own allocator (tests,
+ * the CLI).  Stand in for the cudaMalloc/cudaMemcpy/cudaFree calls of
+ * test_kshape.cu:35-36, 74. */
+int tcx_malloc(tcx_ctx* ctx, void** d_ptr, size_t bytes);
+int tcx_free(tcx_ctx* ctx, void* d_ptr);
+int tcx_memcpy_h2d(tcx_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
+int tcx_memcpy_d2h(tcx_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
+int tcx_memset(tcx_ctx* ctx, void* d_ptr, int value, size_t bytes);
+int tcx_synchronize(tcx_ctx* ctx);
+
+/* ------------------------------------------------------------ primitives */
+
+/* In-place z-normalization of each row: (x - mean) / std_ddof, FP64
+ * accumulation, constant rows -> 0.
+ * Replaces z_norm_gpu_x<float> (Code/include/normalization.cuh:149-177);
+ * semantics of zscore (Code/test/test_timesC.py:42-52). */
+int tcx_znorm(tcx_ctx* ctx, float* d_x, int rows, int cols, int ddof);
+
+/* Materialized normalized cross-correlation d_out[x][y][2m-1]
+ * (lags -(m-1)..m-1), linear (non-aliasing) FFT length.
+ * Replaces NCC_3D<float> (Code/include/ncc.cuh:467-517);
+ * values of _ncc_c_3dim (Code/test/test_timesC.py:105-114). */
+int tcx_ncc(tcx_ctx* ctx, const float* d_x, const float* d_y, float* d_out,
+            int x_rows, int y_rows, int cols);
+
+/* In-place alignment of `rows` members to one centroid: shift each member
+ * by the lag of its maximum NCC with zero fill.  A centroid that is all
+ * zeros leaves the members unchanged (test_timesC.py:128).
+ * Replaces sbd3D<float> (Code/include/SBD.cuh:68-85); semantics of
+ * _preSBD_extract_shape (Code/test/test_timesC.py:127-133). */
+int tcx_sbd_align(tcx_ctx* ctx, float* d_members, const float* d_center,
+                  int rows, int cols, int convention);
+
+/* Fused SBD assignment: dist[i] = min_j (1 - max_lag NCC(x_i, c_j)),
+ * label[i] = first argmin (zero-based), shift[i] = lag of the maximum for
+ * the chosen centroid (the shift the next refinement applies).  d_dmat
+ * (n*k, may be NULL) receives all distances; d_dist and d_shift may be NULL.
+ * Replaces NCC_3D + get_pos_idx + the inertia reduce
+ * (Code/include/kshape.cuh:257-271, 35-96); semantics of
+ * test_timesC.py:258-260. */
+int tcx_assign(tcx_ctx* ctx, const float* d_x, const float* d_centroids,
+               int n, int m, int k, int* d_label, float* d_dist, int* d_shift,
+               float* d_dmat);
+
+/* Shape extraction: group rows by d_label (zero-based), align each member
+ * by d_shift[i] (ignored for clusters whose old centroid is all zero; a
+ * NULL d_shift means: compute each member's shift against its own old
+ * centroid, as _preSBD_extract_shape does),
+ * z-normalize members (ddof=1), S_j = Y_j^T Y_j on MFMA, centre with
+ * rank-1 updates, power iteration with an FP64 Rayleigh-quotient test,
+ * sign fix by the lowest-index member, z-normalize (ddof=1).
+ * d_centroids: in = old centroids (k*m), out = new centroids; an empty
+ * cluster gets a zero centroid.  `iter` selects the power-iteration start
+ * vectors.  *sum_class (may be NULL) = number of non-empty clusters.
+ * Replaces extract_shape<float> (Code/include/shapeExtract.cuh:1057-1181);
+ * semantics of _preSBD_extract_shape + _extract_shape
+ * (Code/test/test_timesC.py:127-204). */
+int tcx_extract_shape(tcx_ctx* ctx, const int* d_label, const float* d_x,
+                      const int* d_shift, float* d_centroids, int n, int m,
+                      int k, int iter, int* sum_class);
+
+/* ------------------------------------------------------------------- fit */
+
+typedef struct {
+  int max_iter;          /* iterations cap (test_timesC.py:221 uses 10)      */
+  int init_mode;         /* 0: random labels, 1: random centres (init_chose)  */
+  uint64_t seed;         /* initial labels / centres and power start vectors */
+  const int* init_labels;/* optional host labels (n, zero-based), mode 0     */
+  const int* init_rows;  /* optional host row indices (k), mode 1            */
+  int convention;        /* TCX_CONV_CPU (default) or TCX_CONV_KSHAPE         */
+  int s_precision;       /* TCX_S_SPLIT_BF16 (default) or TCX_S_BF16          */
+  int cal_select;        /* 0: power iteration (1 is not supported yet)       */
+  double power_tol;      /* relative Rayleigh-quotient tolerance, 0 -> 1e-12 */
+  int* labels_per_iter;  /* optional host buffer (max_iter*n): labels after
+                            each iteration's assignment                      */
+} tcx_fit_opts;
+
+typedef struct {
+  int iterations;        /* iterations run                                  */
+  int converged;         /* 1 if labels stopped changing                    */
+  int sum_class;         /* non-empty clusters after the last refinement    */
+  int power_steps_max;   /* largest power-iteration step count of the run   */
+  int fallbacks;         /* eigen-fallback branches taken                    */
+  double ms_total;       /* wall time of the fit (host clock)               */
+  double ms_refine;      /* summed refinement time (HIP events)             */
+  double ms_assign;      /* summed assignment time (HIP events)             */
+} tcx_fit_stats;
+
+/* Fill opts with defaults (max_iter 10, seed 0, CPU convention,
+ * split-BF16, power_tol 1e-12). */
+void tcx_fit_opts_default(tcx_fit_opts* opts);
+
+/* Full k-Shape fit on z-normalized device data.  h_labels (n) receives
+ * zero-based labels, h_centroids (k*m, may be NULL) the final centroids,
+ * stats may be NULL.  The loop is _kshape (test_timesC.py:213-265) for
+ * init_mode 0 and the k_shape_tsl order (kshape.cuh:376-461) for 1. */
+int tcx_kshape_fit(tcx_ctx* ctx, const float* d_x, int n, int m, int k,
+                   const tcx_fit_opts* opts, int* h_labels, float* h_centroids,
+                   tcx_fit_stats* stats);
+
+/* Stepwise form of the same fit (one context runs one fit at a time):
+ * tcx_fit_begin sets up the initial state, each tcx_fit_step runs one
+ * iteration (refine + assign, or assign + stop test + refine for
+ * init_mode 1) and reports in *changed (may be NULL) whether any label
+ * changed; the caller decides when to stop.  tcx_fit_end copies out the
+ * result (h_labels, h_centroids and stats may each be NULL). */
+int tcx_fit_begin(tcx_ctx* ctx, const float* d_x, int n, int m, int k,
+                  const tcx_fit_opts* opts);
+int tcx_fit_step(tcx_ctx* ctx, int* changed);
+int tcx_fit_end(tcx_ctx* ctx, int* h_labels, float* h_centroids,
+                tcx_fit_stats* stats);
+/* Statistics of the fit in progress (no reference counterpart). */
+int tcx_fit_query(tcx_ctx* ctx, tcx_fit_stats* stats);
+
+/* Reference-shaped fit: one-based labels into h_idx, *iter_num in = cap,
+ * out = iterations run, *sum_class out = non-empty clusters.
+ * Replaces k_shape<float> (Code/include/kshape.cuh:197-324). */
+int tcx_kshape(tcx_ctx* ctx, const float* d_x, int* h_idx, int n, int m,
+               int k, int* iter_num, int* sum_class, int cal_select);
+
+/* Centre-initialized fit (init_chose=1).  h_x is accepted for signature
+ * parity and unused: the centres are rows of the z-normalized d_x.
+ * Replaces k_shape_tsl<float> (Code/include/kshape.cuh:332-474). */
+int tcx_kshape_tsl(tcx_ctx* ctx, const float* d_x, const float* h_x,
+                   int* h_idx, int n, int m, int k, int* iter_num,
+                   int* sum_class, int cal_select);
+
+/* ------------------------------------------------------------- host I/O */
+
+/* Labeled CSV "label,v1,...,vm" per line.  *data and *labels are
+ * allocated with malloc (free with tcx_free_host).
+ * Replaces readFile<float> (Code/include/readfile.cuh:12-67). */
+int tcx_read_labeled_csv(const char* path, float** data, int** labels,
+                         int* rows, int* cols);
+/* Unlabeled CSV "v1,...,vm" per line.
+ * Replaces read_unlabeled_File<float> (Code/include/readfile.cuh:72-116). */
+int tcx_read_unlabeled_csv(const char* path, float** data, int* rows,
+                           int* cols);
+/* "l,l,...,l," on one line.
+ * Replaces writeFile<int> (Code/include/readfile.cuh:120-129). */
+int tcx_write_labels(const char* path, const int* labels, int n);
+void tcx_free_host(void* p);
+
+/* --------------------------------------------- synthetic data (host) */
+
+/* Seeded generator of the benchmark workloads: per class a prototype
+ * sum_{q<3} A_q sin(2 pi f_q t/m + phi_q) + exp(-(t-mu)^2/(2 (m/20)^2));
+ * series a*roll(proto, s) + b + N(0, sigma^2).  class_frac (K, may be NULL
+ * for balanced classes) gives class proportions.  X (n*m) and y (n) out.
+ * No reference counterpart (the reference's dataset is not available). */
+int tcx_generate(uint64_t seed, long n, int m, int K, double sigma,
+                 const double* class_frac, float* X, int* y);
+/* Initial labels in [0,k) and k distinct initial centre rows. */
+int tcx_init_labels(uint64_t seed, long n, int k, int* labels);
+int tcx_init_rows(uint64_t seed, long n, int k, int* rows);
+/* Power-iteration start vector of (iter, cluster), FP64 (length m). */
+void tcx_power_v0(uint64_t seed, int iter, int cluster, int m, double* v0);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

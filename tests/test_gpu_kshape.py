@@ -1,0 +1,134 @@
+"""GPU parity of shape extraction and of the full k-Shape loop against the
+FP64 oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+import timesc as T
+from parity import CENT_RTOL, assert_labels_match, centroid_errors, label_mismatch_report
+
+pytestmark = pytest.mark.gpu
+
+
+def _znormed(ctx, seed, n, m, K, sigma=0.3, frac=None):
+    X, y = T.generate(seed, n, m, K, sigma, frac)
+    d = ctx.to_device(X)
+    ctx.znorm(d, ddof=0)
+    return d, d.numpy(), y
+
+
+@pytest.mark.parametrize("m,n,k", [(128, 512, 4), (48, 300, 6), (512, 2000, 16)])
+@pytest.mark.parametrize("aligned", [False, True])
+def test_extract_shape_matches_oracle(ctx, m, n, k, aligned):
+    d, Z, y = _znormed(ctx, m + k, n, m, k)
+    labels = T.init_labels(5, n, k) if not aligned else y.astype(np.int32)
+    labels[labels == k - 1] = 0  # cluster k-1 empty -> zero centroid
+    if aligned:
+        C0 = O.zscore(Z[T.init_rows(9, n, k)].astype(np.float64), ddof=1).astype(np.float32)
+    else:
+        C0 = np.zeros((k, m), np.float32)
+    dl, dc = ctx.to_device(labels), ctx.to_device(C0)
+    try:
+        sc = ctx.extract_shape(dl, d, None, dc, it=2)
+        got = dc.numpy()
+    finally:
+        for a in (dl, dc, d):
+            a.free()
+    ref, counts, infos = O.refine(Z.astype(np.float64), labels, C0.astype(np.float64), seed=0, it=2)
+    assert sc == int(np.sum(counts > 0))
+    assert np.all(got[k - 1] == 0)
+    errs = centroid_errors(got, ref)
+    assert max(errs) <= CENT_RTOL, errs
+
+
+def test_extract_shape_edge_clusters(ctx):
+    """single-member cluster -> zscore_ddof1(member); constant members -> 0."""
+    m, n = 64, 10
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((n, m)).astype(np.float32)
+    X[7:] = 1.5  # cluster 2: constant rows, z-normalized to zeros
+    Z = O.zscore(X.astype(np.float64)).astype(np.float32)
+    labels = np.array([0, 0, 0, 0, 0, 0, 1, 2, 2, 2], np.int32)
+    d, dl = ctx.to_device(Z), ctx.to_device(labels)
+    dc = ctx.to_device(np.zeros((3, m), np.float32))
+    try:
+        ctx.extract_shape(dl, d, None, dc)
+        got = dc.numpy()
+    finally:
+        for a in (d, dl, dc):
+            a.free()
+    one = O.zscore(Z[6].astype(np.float64), ddof=1)
+    assert np.abs(got[1] - one).max() <= 1e-5
+    assert np.all(got[2] == 0)
+    assert abs(got[0].mean()) <= 1e-5 and abs(got[0].std(ddof=1) - 1) <= 1e-5
+
+
+def test_stepwise_parity_cfg0(ctx):
+    """Each refine and assign step from identical inputs (the GPU's previous
+    state) matches the oracle: centroids to 1e-4 rel-L2, labels up to ties."""
+    n, m, k = 512, 128, 4
+    d, Z, y = _znormed(ctx, 0, n, m, k)
+    Z64 = Z.astype(np.float64)
+    labels = T.init_labels(0, n, k)
+    C = np.zeros((k, m), np.float32)
+    dl, dc = ctx.to_device(labels), ctx.to_device(C)
+    try:
+        for it in range(12):
+            C_old = dc.numpy().astype(np.float64)
+            lab_in = dl.numpy()
+            ctx.extract_shape(dl, d, None, dc, it=it)
+            Cg = dc.numpy()
+            Co, _, _ = O.refine(Z64, lab_in, C_old, seed=0, it=it)
+            errs = centroid_errors(Cg, Co)
+            assert max(errs) <= CENT_RTOL, (it, errs)
+            lab, dist, sh = ctx.assign(d, dc)
+            olab, odist, osh, odm = O.assign(Z64, Cg.astype(np.float64), dmat=True)
+            assert_labels_match(lab, olab, odm, what=f"iteration {it}")
+            ctx.h2d(dl, lab)
+    finally:
+        for a in (d, dl, dc):
+            a.free()
+
+
+def test_fit_cfg0_trajectory(ctx):
+    """Plain k-Shape, n=512, m=128, k=4, random-index init, 20 iterations:
+    labels after every iteration equal the oracle's; final centroids within
+    1e-4 rel-L2."""
+    n, m, k, iters = 512, 128, 4, 20
+    d, Z, y = _znormed(ctx, 0, n, m, k)
+    init = T.init_labels(0, n, k)
+    try:
+        res = ctx.fit(d, k, max_iter=iters, init_labels=init, seed=0, labels_per_iter=True)
+    finally:
+        d.free()
+    hist, cent = O.kshape(Z.astype(np.float64), k, init_labels=init, max_iter=iters, seed=0)
+    got = res["labels_per_iter"]
+    for t in range(min(len(got), len(hist))):
+        if not np.array_equal(got[t], hist[t]):
+            # only a near-tie may separate the trajectories
+            C_prev = None
+            raise AssertionError(f"trajectory diverges at iteration {t}: "
+                                 f"{np.sum(got[t] != hist[t])} labels differ")
+    assert len(got) == len(hist)
+    errs = centroid_errors(res["centroids"], cent)
+    assert max(errs) <= CENT_RTOL, errs
+
+
+def test_fit_tsl_cfg1_small(ctx):
+    """Centre initialisation (init_chose=1), long series with F=8192:
+    n=137, m=2709, k=2, 60/40 classes."""
+    n, m, k, iters = 137, 2709, 2, 8
+    d, Z, y = _znormed(ctx, 1, n, m, k, frac=[0.6, 0.4])
+    rows = T.init_rows(1, n, k)
+    try:
+        res = ctx.fit(d, k, max_iter=iters, init_mode=1, init_rows=rows, seed=1,
+                      labels_per_iter=True)
+    finally:
+        d.free()
+    hist, cent = O.kshape(Z.astype(np.float64), k, init_mode=1, init_rows=rows,
+                          max_iter=iters, seed=1)
+    got = res["labels_per_iter"]
+    assert len(got) == len(hist)
+    assert np.array_equal(got, hist)
+    errs = centroid_errors(res["centroids"], cent)
+    assert max(errs) <= CENT_RTOL, errs

@@ -1,0 +1,131 @@
+"""GPU parity of the SBD path (z-norm, NCC, assignment, alignment) against
+the FP64 oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+import timesc as T
+from parity import SBD_TIE, assert_labels_match
+
+pytestmark = pytest.mark.gpu
+
+# one length per supported FFT size F = 1 << (2m-1).bit_length()
+LENGTHS = [2, 3, 5, 9, 17, 33, 65, 128, 200, 512, 1000, 2048, 2709, 4096]
+
+
+def _data(seed, n, m, k, sigma=0.3):
+    X, y = T.generate(seed, n, m, k, sigma)
+    return X, y
+
+
+@pytest.mark.parametrize("ddof", [0, 1])
+def test_znorm_matches_zscore(ctx, ddof):
+    X, _ = _data(1, 300, 77, 3)
+    X[5, :] = 2.0  # constant row -> zeros (nan_to_num)
+    X[6, :] = 0.0
+    d = ctx.to_device(X)
+    try:
+        ctx.znorm(d, ddof=ddof)
+        got = d.numpy()
+    finally:
+        d.free()
+    ref = O.zscore(X.astype(np.float64), ddof=ddof)
+    assert np.abs(got - ref).max() <= 2e-6
+    assert np.all(got[5] == 0) and np.all(got[6] == 0)
+
+
+@pytest.mark.parametrize("m", LENGTHS)
+def test_ncc_matches_oracle(ctx, m):
+    rng = np.random.default_rng(m)
+    xr, yr = 5, 3
+    X = rng.standard_normal((xr, m)).astype(np.float32)
+    Y = rng.standard_normal((yr, m)).astype(np.float32)
+    Y[2] = 0.0  # zero norm -> NCC 0 everywhere
+    dx, dy = ctx.to_device(X), ctx.to_device(Y)
+    try:
+        got = ctx.ncc(dx, dy)
+    finally:
+        dx.free()
+        dy.free()
+    assert got.shape == (xr, yr, 2 * m - 1)
+    for i in range(xr):
+        for j in range(yr):
+            ref = O.ncc(X[i].astype(np.float64), Y[j].astype(np.float64))
+            assert np.abs(got[i, j] - ref).max() <= 2e-6, (i, j)
+    assert np.all(got[:, 2, :] == 0)
+
+
+@pytest.mark.parametrize("m,n,k", [(2, 31, 3), (17, 101, 4), (128, 512, 4), (512, 1001, 16),
+                                   (1000, 77, 5), (2709, 40, 2), (4096, 9, 3)])
+def test_assign_matches_oracle(ctx, m, n, k):
+    X, _ = _data(m + n, n, m, k)
+    d = ctx.to_device(X)
+    try:
+        ctx.znorm(d, ddof=0)
+        Z = d.numpy()
+        rows = T.init_rows(3, n, k)
+        C = Z[rows].copy()
+        if k > 2:
+            C[k - 1] = 0.0  # zero centroid: distance exactly 1
+        dc = ctx.to_device(C)
+        lab, dist, sh, dm = ctx.assign(d, dc, dmat=True)
+        dc.free()
+    finally:
+        d.free()
+    olab, odist, osh, odm = O.assign(Z.astype(np.float64), C.astype(np.float64), dmat=True)
+    err = np.abs(dm - odm).max()
+    assert err <= 2e-6, f"max SBD error {err:.3e}"
+    assert_labels_match(lab, olab, odm)
+    same = lab == olab
+    assert np.abs(dist[same] - odist[same]).max() <= 2e-6
+    if k > 2:
+        assert np.all(dm[:, k - 1] == 1.0)
+    # shifts: equal unless the oracle's NCC has a near-tie between the lags
+    for i in np.nonzero(same & (sh != osh))[0]:
+        cc = O.ncc(Z[i].astype(np.float64), C[lab[i]].astype(np.float64))
+        assert abs(cc[sh[i] + m - 1] - cc[osh[i] + m - 1]) <= SBD_TIE, (i, sh[i], osh[i])
+
+
+@pytest.mark.parametrize("convention", [T.CONV_CPU, T.CONV_KSHAPE])
+@pytest.mark.parametrize("m", [16, 128, 700])
+def test_sbd_align_matches_oracle(ctx, convention, m):
+    X, _ = _data(m, 40, m, 2)
+    Z = O.zscore(X.astype(np.float64)).astype(np.float32)
+    c = Z[0].copy()
+    members = Z[1:].copy()
+    dm_, dc = ctx.to_device(members), ctx.to_device(c[None, :])
+    try:
+        ctx.sbd_align(dm_, dc, convention)
+        got = dm_.numpy()
+        # all-zero centre: members unchanged (test_timesC.py:128)
+        dz = ctx.to_device(np.zeros((1, m), np.float32))
+        dm2 = ctx.to_device(members)
+        ctx.sbd_align(dm2, dz, convention)
+        unchanged = dm2.numpy()
+        dz.free()
+        dm2.free()
+    finally:
+        dm_.free()
+        dc.free()
+    assert np.array_equal(unchanged, members)
+    for i in range(members.shape[0]):
+        s = O.align_shift(members[i].astype(np.float64), c.astype(np.float64), convention)
+        ref = O.roll_zeropad(members[i].astype(np.float64), s)
+        assert np.array_equal(got[i], ref.astype(np.float32)), i
+
+
+def test_assign_known_shift(ctx):
+    """member = roll_zeropad(c, d): the CPU-path shift is +d."""
+    m = 256
+    X, _ = _data(11, 1, m, 1)
+    c = O.zscore(X[0].astype(np.float64)).astype(np.float32)
+    d = 13
+    mem = O.roll_zeropad(c.astype(np.float64), d).astype(np.float32)
+    dx, dc = ctx.to_device(np.stack([mem, c])), ctx.to_device(c[None, :])
+    try:
+        lab, dist, sh = ctx.assign(dx, dc)
+    finally:
+        dx.free()
+        dc.free()
+    assert sh[0] == d and sh[1] == 0
+    assert abs(dist[1]) <= 1e-6

@@ -1,0 +1,615 @@
+// Shape-extraction kernels: stable grouping of rows by label, shift-align +
+// z-normalize + BF16 (hi, lo) pack into a transposed panel, S_j = Y_j^T Y_j
+// on MFMA with FP32 accumulation, and the per-cluster power iteration with
+// rank-1 centering, FP64 Rayleigh-quotient test, sign fix and final z-norm.
+// Semantics: _preSBD_extract_shape + _extract_shape,
+// Code/test/test_timesC.py:127-204 (grouping order :228-243).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "tcx_device.hpp"
+#include "tcx_kernels.hpp"
+#include "tcx_rng.hpp"
+
+namespace tcx {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ uint16_t f2bf_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// --------------------------------------------------------------- helpers
+__global__ void k_fill_int(int* p, long count, int value) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (long)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+// -------------------------------------------------------------- grouping
+// Per block of kGroupChunk rows: label histogram.
+__global__ __launch_bounds__(256) void k_group_count(const int* __restrict__ label, long n,
+                                                     int k, int* __restrict__ blk_count,
+                                                     int* __restrict__ bad) {
+  extern __shared__ int cnt[];
+  for (int j = threadIdx.x; j < k; j += blockDim.x) cnt[j] = 0;
+  __syncthreads();
+  const long b0 = (long)blockIdx.x * kGroupChunk;
+  const long b1 = b0 + kGroupChunk < n ? b0 + kGroupChunk : n;
+  for (long i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+    const int l = label[i];
+    if (l >= 0 && l < k)
+      atomicAdd(&cnt[l], 1);
+    else
+      *bad = 1;  // out-of-range label: the row is left out, the host reports it
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < k; j += blockDim.x) blk_count[(long)blockIdx.x * k + j] = cnt[j];
+}
+
+// Exclusive scan over blocks per cluster; cluster sizes; padded segment
+// offsets (each cluster's columns start at a multiple of kPackPad).
+__global__ __launch_bounds__(256) void k_group_scan(int nblk, int k,
+                                                    const int* __restrict__ blk_count,
+                                                    int* __restrict__ blk_base,
+                                                    int* __restrict__ counts,
+                                                    long* __restrict__ offsets) {
+  for (int j = threadIdx.x; j < k; j += blockDim.x) {
+    int run = 0;
+    for (int b = 0; b < nblk; ++b) {
+      blk_base[(long)b * k + j] = run;
+      run += blk_count[(long)b * k + j];
+    }
+    counts[j] = run;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long o = 0;
+    offsets[0] = 0;
+    for (int j = 0; j < k; ++j) {
+      o += ((long)counts[j] + kPackPad - 1) / kPackPad * kPackPad;
+      offsets[j + 1] = o;
+    }
+  }
+}
+
+// Stable placement: perm[offsets[l] + rank] = row, rank = number of earlier
+// rows with the same label.  One wave per block of rows; ranks inside a
+// 64-row slice come from ballots over the distinct labels of the slice.
+__global__ __launch_bounds__(64) void k_group_place(const int* __restrict__ label, long n,
+                                                    int k, const int* __restrict__ blk_base,
+                                                    const long* __restrict__ offsets,
+                                                    int* __restrict__ perm) {
+  extern __shared__ int run[];
+  const int lane = threadIdx.x;
+  for (int j = lane; j < k; j += kWave) run[j] = blk_base[(long)blockIdx.x * k + j];
+  wave_lds_sync();
+  const long b0 = (long)blockIdx.x * kGroupChunk;
+  const long b1 = b0 + kGroupChunk < n ? b0 + kGroupChunk : n;
+  const uint64_t below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  for (long base = b0; base < b1; base += kWave) {
+    const long row = base + lane;
+    int lab = row < b1 ? label[row] : -1;
+    if (lab >= k) lab = -1;
+    bool todo = lab >= 0;
+    for (;;) {
+      uint64_t pending = __ballot(todo);
+      if (pending == 0) break;
+      const int leader = __ffsll((unsigned long long)pending) - 1;
+      const int j = __shfl(lab, leader);
+      const uint64_t mask = __ballot(lab == j);
+      if (lab == j) {
+        const int rank = run[j] + __popcll(mask & below);
+        perm[offsets[j] + rank] = (int)row;
+        todo = false;
+      }
+      wave_lds_sync();
+      if (lane == leader) run[j] += __popcll(mask);
+      wave_lds_sync();
+    }
+  }
+}
+
+// ------------------------------------------------------------ align + pack
+__device__ __forceinline__ int cluster_of(long p, const long* offsets, int k) {
+  int lo = 0, hi = k;  // offsets[lo] <= p < offsets[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (offsets[mid] <= p) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Block of 64 packed columns.  For each column (a member of cluster cl):
+// a = roll_zeropad(x_row, s) with s = shift[row] unless the old centroid of
+// cl is all zero (test_timesC.py:128-132); y = zscore(a, ddof=1) with FP64
+// statistics (:142); y -> (hi, lo) BF16 with y ~= hi + lo, written
+// transposed: Y[t][p].  The first column of a cluster (its lowest-index
+// member) also writes its aligned row to a0 for the sign fix (:163).
+__global__ __launch_bounds__(256) void k_align_pack(
+    const float* __restrict__ x, int m, const int* __restrict__ perm, long n_pad,
+    const long* __restrict__ offsets, int k, const int* __restrict__ shift,
+    const int* __restrict__ czero, uint16_t* __restrict__ yhi, uint16_t* __restrict__ ylo,
+    float* __restrict__ a0) {
+  __shared__ double s_mean[64], s_sd[64];
+  __shared__ int s_row[64], s_sh[64];
+  __shared__ uint16_t t_hi[64][66], t_lo[64][66];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const long p0 = (long)blockIdx.x * 64;
+  for (int q = wid; q < 64; q += 4) {
+    const long p = p0 + q;
+    const int row = p < n_pad ? perm[p] : -1;
+    int s = 0, cl = 0;
+    if (row >= 0) {
+      cl = cluster_of(p, offsets, k);
+      s = czero[cl] ? 0 : shift[row];
+    }
+    const float* xr = x + (long)(row >= 0 ? row : 0) * m;
+    double sum = 0.0;
+    if (row >= 0)
+      for (int t = lane; t < m; t += kWave) {
+        int src = t - s;
+        sum += (src >= 0 && src < m) ? (double)xr[src] : 0.0;
+      }
+    sum = wave_sum_f64(sum);
+    const double mean = sum / (double)m;
+    double ss = 0.0;
+    if (row >= 0)
+      for (int t = lane; t < m; t += kWave) {
+        int src = t - s;
+        double d = ((src >= 0 && src < m) ? (double)xr[src] : 0.0) - mean;
+        ss += d * d;
+      }
+    ss = wave_sum_f64(ss);
+    if (lane == 0) {
+      s_row[q] = row;
+      s_sh[q] = s;
+      s_mean[q] = mean;
+      s_sd[q] = sqrt(ss / (double)(m - 1));
+    }
+    if (row >= 0 && p == offsets[cl]) {
+      float* o = a0 + (long)cl * m;
+      for (int t = lane; t < m; t += kWave) {
+        int src = t - s;
+        o[t] = (src >= 0 && src < m) ? xr[src] : 0.f;
+      }
+    }
+  }
+  __syncthreads();
+  for (int t0 = 0; t0 < m; t0 += 64) {
+#pragma unroll 4
+    for (int it = 0; it < 16; ++it) {
+      const int idx = it * 256 + tid;
+      const int q = idx >> 6, tt = idx & 63;
+      const int t = t0 + tt;
+      const int row = s_row[q];
+      float f = 0.f;
+      if (row >= 0 && t < m) {
+        const int src = t - s_sh[q];
+        const double a = (src >= 0 && src < m) ? (double)x[(long)row * m + src] : 0.0;
+        const double y = (a - s_mean[q]) / s_sd[q];
+        f = isnan(y) ? 0.f : (isinf(y) ? (y > 0 ? 3.402823466e38f : -3.402823466e38f) : (float)y);
+      }
+      const uint16_t h = f2bf_rne(f);
+      t_hi[tt][q] = h;
+      t_lo[tt][q] = f2bf_rne(f - bf2f(h));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int pass = 0; pass < 8; ++pass) {
+      const int tt = pass * 8 + (tid >> 5);
+      const int qq = (tid & 31) * 2;
+      const int t = t0 + tt;
+      if (t < m) {
+        uint32_t h = (uint32_t)t_hi[tt][qq] | ((uint32_t)t_hi[tt][qq + 1] << 16);
+        uint32_t l = (uint32_t)t_lo[tt][qq] | ((uint32_t)t_lo[tt][qq + 1] << 16);
+        *reinterpret_cast<uint32_t*>(yhi + (long)t * n_pad + p0 + qq) = h;
+        *reinterpret_cast<uint32_t*>(ylo + (long)t * n_pad + p0 + qq) = l;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------------- S = Y^T Y
+// One 128x128 tile of S_j over packed columns [p0, p1) per block; four waves
+// in a 2x2 arrangement, each 2x2 MFMA 32x32x16 BF16 blocks with FP32
+// accumulation.  split != 0 adds hi*lo + lo*hi (split-BF16).  Operands come
+// straight from the transposed panel: lane l supplies row (l % 32) and
+// columns p + 8*(l / 32) .. +7, one 16-byte load per fragment.
+__global__ __launch_bounds__(256) void k_syrk(const SyrkItem* __restrict__ items,
+                                              const uint16_t* __restrict__ yhi,
+                                              const uint16_t* __restrict__ ylo, long n_pad,
+                                              int m, int split, float* __restrict__ part) {
+  const SyrkItem it = items[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wi = wave >> 1, wj = wave & 1;
+  const int kofs = 8 * (lane >> 5);
+  int ra[2], rb[2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    ra[a] = it.ti * kSyrkTile + wi * 64 + a * 32 + (lane & 31);
+    rb[a] = it.tj * kSyrkTile + wj * 64 + a * 32 + (lane & 31);
+  }
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
+  const bf16x8 zero8 = {};
+  for (long p = it.p0; p < it.p1; p += 16) {
+    bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const long oa = (long)ra[a] * n_pad + p + kofs;
+      const long ob = (long)rb[a] * n_pad + p + kofs;
+      ah[a] = ra[a] < m ? *reinterpret_cast<const bf16x8*>(yhi + oa) : zero8;
+      bh[a] = rb[a] < m ? *reinterpret_cast<const bf16x8*>(yhi + ob) : zero8;
+      if (split) {
+        al[a] = ra[a] < m ? *reinterpret_cast<const bf16x8*>(ylo + oa) : zero8;
+        bl[a] = rb[a] < m ? *reinterpret_cast<const bf16x8*>(ylo + ob) : zero8;
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+        if (split) {
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+        }
+      }
+  }
+  float* o = part + (long)it.part * kSyrkTile * kSyrkTile;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int i = wi * 64 + a * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+        const int j = wj * 64 + b * 32 + (lane & 31);
+        o[i * kSyrkTile + j] = acc[a][b][r];
+      }
+}
+
+// Sum the K-chunk partials of each tile in a fixed order and write the tile
+// and its mirror, so S_j is exactly symmetric (diagonal tiles keep i <= j).
+__global__ __launch_bounds__(256) void k_syrk_reduce(const SyrkTile* __restrict__ tiles,
+                                                     const float* __restrict__ part, int m,
+                                                     float* __restrict__ S) {
+  const SyrkTile tl = tiles[blockIdx.x];
+  float* Sj = S + (long)tl.cluster * m * m;
+  for (int e = threadIdx.x; e < kSyrkTile * kSyrkTile; e += blockDim.x) {
+    const int i = e / kSyrkTile, jj = e % kSyrkTile;
+    const int gi = tl.ti * kSyrkTile + i, gj = tl.tj * kSyrkTile + jj;
+    if (gi >= m || gj >= m) continue;
+    if (tl.ti == tl.tj && jj < i) continue;
+    float s = 0.f;
+    for (int c = 0; c < tl.nparts; ++c) s += part[(long)(tl.part0 + c) * kSyrkTile * kSyrkTile + e];
+    Sj[(long)gi * m + gj] = s;
+    Sj[(long)gj * m + gi] = s;
+  }
+}
+
+// ---------------------------------------------------------- power iteration
+constexpr int kPowerThreads = 1024;
+constexpr int kPowerWaves = kPowerThreads / kWave;
+
+__device__ __forceinline__ double block_sum_f64(double v, double* scratch) {
+  v = wave_sum_f64(v);
+  const int wid = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) scratch[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < kPowerWaves; ++w) s += scratch[w];
+  return s;
+}
+
+__device__ __forceinline__ int block_min_int(int v, int* scratch) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    int ov = __shfl_xor(v, o, 64);
+    v = ov < v ? ov : v;
+  }
+  const int wid = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) scratch[wid] = v;
+  __syncthreads();
+  int s = scratch[0];
+#pragma unroll
+  for (int w = 1; w < kPowerWaves; ++w) s = scratch[w] < s ? scratch[w] : s;
+  return s;
+}
+
+// w = M v with M = P S P = S - (r 1^T + 1 r^T)/m + (sigma/m^2) 1 1^T
+// (test_timesC.py:145-149 in rank-1 form; r = S 1, sigma = 1^T r), FP32 S,
+// FP64 v and accumulation.  Returns (|w|^2, v.w).
+__device__ void center_matvec(const float* __restrict__ Sj, int m, const double* v,
+                              const double* r, double sigma, double* w, double* scratch,
+                              double& nw2, double& vw) {
+  double sv = 0.0, rv = 0.0;
+  for (int i = threadIdx.x; i < m; i += kPowerThreads) {
+    sv += v[i];
+    rv += r[i] * v[i];
+  }
+  sv = block_sum_f64(sv, scratch);
+  rv = block_sum_f64(rv, scratch + kPowerWaves);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const double inv_m = 1.0 / (double)m;
+  for (int i = wid; i < m; i += kPowerWaves) {
+    const float* row = Sj + (long)i * m;
+    double acc = 0.0;
+    for (int c = lane; c < m; c += kWave) acc += (double)row[c] * v[c];
+    acc = wave_sum_f64(acc);
+    if (lane == 0) w[i] = acc - (r[i] * sv + rv) * inv_m + sigma * sv * inv_m * inv_m;
+  }
+  __syncthreads();
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < m; i += kPowerThreads) {
+    a += w[i] * w[i];
+    b += v[i] * w[i];
+  }
+  nw2 = block_sum_f64(a, scratch + 2 * kPowerWaves);
+  vw = block_sum_f64(b, scratch + 3 * kPowerWaves);
+}
+
+// Runs power steps v <- M v / |M v| until the Rayleigh quotient changes by
+// <= tol relative (FP64) or `steps_cap` steps.  Returns 0, or 1 if M v = 0.
+__device__ int power_steps(const float* Sj, int m, double* v, const double* r, double sigma,
+                           double* w, double* scratch, int steps_cap, double tol, int& steps,
+                           int& converged) {
+  double lam_prev = 0.0;
+  for (int t = 0; t < steps_cap; ++t) {
+    double nw2, vw;
+    center_matvec(Sj, m, v, r, sigma, w, scratch, nw2, vw);
+    if (!(nw2 > 0.0)) return 1;
+    const double nw = sqrt(nw2);
+    for (int i = threadIdx.x; i < m; i += kPowerThreads) v[i] = w[i] / nw;
+    __syncthreads();
+    ++steps;
+    if (t > 0 && fabs(vw - lam_prev) <= tol * fabs(vw)) {
+      converged = 1;
+      break;
+    }
+    lam_prev = vw;
+  }
+  return 0;
+}
+
+__device__ void sign_fix(const float* a0, double* c, double* w, int m, double* scratch,
+                         int& flipped) {
+  double s1 = 0.0, s2 = 0.0;
+  for (int i = threadIdx.x; i < m; i += kPowerThreads) {
+    double d1 = (double)a0[i] - c[i], d2 = (double)a0[i] + c[i];
+    s1 += d1 * d1;
+    s2 += d2 * d2;
+  }
+  s1 = block_sum_f64(s1, scratch);
+  s2 = block_sum_f64(s2, scratch + kPowerWaves);
+  if (sqrt(s1) >= sqrt(s2)) {  // test_timesC.py:165
+    for (int i = threadIdx.x; i < m; i += kPowerThreads) {
+      c[i] = -c[i];
+      w[i] = -w[i];
+    }
+    flipped = 1;
+  }
+  __syncthreads();
+}
+
+// One block per cluster.  Steps of test_timesC.py:152-204: power iteration
+// (start vector = power_v0(seed, iter, cluster, .)), lambda = v.Mv, sign fix
+// by a0, the vecleft/vecright sign test and the signed-sum residual test,
+// then zscore(c, ddof=1).  The eigh fallback keeps iterating the power
+// method (the top eigenvector it converges to is eigh's last column up to
+// sign; the sign fix then picks the same sign).
+__global__ __launch_bounds__(kPowerThreads) void k_power(
+    const float* __restrict__ S, int m, const int* __restrict__ counts,
+    const float* __restrict__ a0all, uint64_t seed, int iter, int max_steps, double tol,
+    float* __restrict__ c_out, PowerInfo* __restrict__ info) {
+  extern __shared__ double sh[];
+  double* v = sh;
+  double* w = sh + m;
+  double* r = sh + 2 * m;
+  double* scratch = sh + 3 * m;  // 8 * kPowerWaves doubles
+  int* iscratch = reinterpret_cast<int*>(scratch + 6 * kPowerWaves);
+  const int j = blockIdx.x;
+  const float* Sj = S + (long)j * m * m;
+  const float* a0 = a0all + (long)j * m;
+  float* out = c_out + (long)j * m;
+  PowerInfo inf = {};
+  if (counts[j] == 0) {  // test_timesC.py:139-140
+    for (int i = threadIdx.x; i < m; i += kPowerThreads) out[i] = 0.f;
+    if (threadIdx.x == 0) {
+      inf.empty = 1;
+      info[j] = inf;
+    }
+    return;
+  }
+  // r = S 1, sigma = 1^T S 1
+  {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int i = wid; i < m; i += kPowerWaves) {
+      const float* row = Sj + (long)i * m;
+      double acc = 0.0;
+      for (int c = lane; c < m; c += kWave) acc += (double)row[c];
+      acc = wave_sum_f64(acc);
+      if (lane == 0) r[i] = acc;
+    }
+  }
+  for (int i = threadIdx.x; i < m; i += kPowerThreads) v[i] = power_v0(seed, iter, j, i);
+  __syncthreads();
+  double sigma = 0.0, n0 = 0.0;
+  for (int i = threadIdx.x; i < m; i += kPowerThreads) {
+    sigma += r[i];
+    n0 += v[i] * v[i];
+  }
+  sigma = block_sum_f64(sigma, scratch);
+  n0 = sqrt(block_sum_f64(n0, scratch + kPowerWaves));
+  for (int i = threadIdx.x; i < m; i += kPowerThreads) v[i] /= n0;
+  __syncthreads();
+
+  int steps = 0, converged = 0;
+  if (power_steps(Sj, m, v, r, sigma, w, scratch, max_steps, tol, steps, converged)) {
+    // M v = 0: NumPy's v / 0 gives NaN, zscore(NaN) -> 0 (nan_to_num)
+    for (int i = threadIdx.x; i < m; i += kPowerThreads) out[i] = 0.f;
+    if (threadIdx.x == 0) {
+      inf.zero = 1;
+      inf.steps = steps;
+      info[j] = inf;
+    }
+    return;
+  }
+  double nw2, vw;
+  center_matvec(Sj, m, v, r, sigma, w, scratch, nw2, vw);  // w = M v
+  double vv = 0.0;
+  for (int i = threadIdx.x; i < m; i += kPowerThreads) vv += v[i] * v[i];
+  vv = block_sum_f64(vv, scratch + 4 * kPowerWaves);
+  double lambda = vw / vv;  // :37
+  int flipped = 0, fallback = 0;
+  if (lambda == 0.0) {
+    fallback = 1;
+  } else {
+    sign_fix(a0, v, w, m, scratch, flipped);  // :163-166
+    // :170-174 first index where vecleft (= M c = w) and vecright (= lambda c)
+    // are both non-zero
+    int pos = 0x7fffffff;
+    for (int i = threadIdx.x; i < m; i += kPowerThreads)
+      if (w[i] != 0.0 && lambda * v[i] != 0.0) {
+        pos = i;
+        break;
+      }
+    pos = block_min_int(pos, iscratch);
+    if (pos == 0x7fffffff) pos = 0;
+    bool neg = (w[pos] < 0.0 && lambda * v[pos] > 0.0);  // :175
+    if (neg) lambda = -lambda;
+    double sd = 0.0;  // :178-179
+    for (int i = threadIdx.x; i < m; i += kPowerThreads) {
+      double vr = neg ? -w[i] : lambda * v[i];
+      sd += vr - w[i];
+    }
+    sd = block_sum_f64(sd, scratch + 5 * kPowerWaves);
+    if (lambda <= 0.0 || fabs(sd) > (double)m * 0.01) fallback = 1;  // :183
+  }
+  if (fallback) {  // :184-192
+    if (!converged) {
+      int more = 0;
+      power_steps(Sj, m, v, r, sigma, w, scratch, 8 * max_steps, tol, more, converged);
+      steps += more;
+    }
+    center_matvec(Sj, m, v, r, sigma, w, scratch, nw2, vw);
+    flipped = 0;
+    sign_fix(a0, v, w, m, scratch, flipped);
+  }
+  // :204 zscore(c, ddof=1)
+  double s = 0.0;
+  for (int i = threadIdx.x; i < m; i += kPowerThreads) s += v[i];
+  s = block_sum_f64(s, scratch);
+  const double mean = s / (double)m;
+  double ss = 0.0;
+  for (int i = threadIdx.x; i < m; i += kPowerThreads) {
+    double d = v[i] - mean;
+    ss += d * d;
+  }
+  ss = block_sum_f64(ss, scratch + kPowerWaves);
+  const double sdv = sqrt(ss / (double)(m - 1));
+  for (int i = threadIdx.x; i < m; i += kPowerThreads) {
+    double y = (v[i] - mean) / sdv;
+    out[i] = isnan(y) ? 0.f : (float)y;
+  }
+  if (threadIdx.x == 0) {
+    inf.lambda = lambda;
+    inf.steps = steps;
+    inf.converged = converged;
+    inf.fallback = fallback;
+    inf.flipped = flipped;
+    info[j] = inf;
+  }
+}
+
+// ---------------------------------------------------------------- launchers
+hipError_t launch_fill_int(int* p, long count, int value, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  long blocks = (count + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)blocks), dim3(256), 0, st, p, count, value);
+  return hipGetLastError();
+}
+
+hipError_t launch_group_count(const int* label, long n, int k, int* blk_count, int* bad,
+                              hipStream_t st) {
+  long nblk = (n + kGroupChunk - 1) / kGroupChunk;
+  hipLaunchKernelGGL(k_group_count, dim3((unsigned)nblk), dim3(256), sizeof(int) * k, st,
+                     label, n, k, blk_count, bad);
+  return hipGetLastError();
+}
+
+hipError_t launch_group_scan(int nblk, int k, const int* blk_count, int* blk_base, int* counts,
+                             long* offsets, hipStream_t st) {
+  hipLaunchKernelGGL(k_group_scan, dim3(1), dim3(256), 0, st, nblk, k, blk_count, blk_base,
+                     counts, offsets);
+  return hipGetLastError();
+}
+
+hipError_t launch_group_place(const int* label, long n, int k, const int* blk_base,
+                              const long* offsets, int* perm, hipStream_t st) {
+  long nblk = (n + kGroupChunk - 1) / kGroupChunk;
+  hipLaunchKernelGGL(k_group_place, dim3((unsigned)nblk), dim3(64), sizeof(int) * k, st, label,
+                     n, k, blk_base, offsets, perm);
+  return hipGetLastError();
+}
+
+hipError_t launch_align_pack(const float* x, int m, const int* perm, long n_pad,
+                             const long* offsets, int k, const int* shift, const int* czero,
+                             uint16_t* yhi, uint16_t* ylo, float* a0, hipStream_t st) {
+  if (n_pad <= 0) return hipSuccess;
+  long blocks = n_pad / 64;
+  hipLaunchKernelGGL(k_align_pack, dim3((unsigned)blocks), dim3(256), 0, st, x, m, perm, n_pad,
+                     offsets, k, shift, czero, yhi, ylo, a0);
+  return hipGetLastError();
+}
+
+hipError_t launch_syrk(const SyrkItem* items, int nitems, const uint16_t* yhi,
+                       const uint16_t* ylo, long n_pad, int m, int split, float* part,
+                       hipStream_t st) {
+  if (nitems <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_syrk, dim3(nitems), dim3(256), 0, st, items, yhi, ylo, n_pad, m, split,
+                     part);
+  return hipGetLastError();
+}
+
+hipError_t launch_syrk_reduce(const SyrkTile* tiles, int ntiles, const float* part, int m,
+                              float* S, hipStream_t st) {
+  if (ntiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_syrk_reduce, dim3(ntiles), dim3(256), 0, st, tiles, part, m, S);
+  return hipGetLastError();
+}
+
+static size_t power_smem(int m) {
+  return sizeof(double) * (3 * (size_t)m + 8 * kPowerWaves) + sizeof(int) * kPowerWaves;
+}
+
+hipError_t launch_power(const float* S, int m, int k, const int* counts, const float* a0,
+                        uint64_t seed, int iter, int max_steps, double tol, float* c_out,
+                        PowerInfo* info, hipStream_t st) {
+  hipLaunchKernelGGL(k_power, dim3(k), dim3(kPowerThreads), power_smem(m), st, S, m, counts,
+                     a0, seed, iter, max_steps, tol, c_out, info);
+  return hipGetLastError();
+}
+
+hipError_t configure_extract_kernels() {
+  return hipFuncSetAttribute((const void*)k_power, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)power_smem(4096));
+}
+
+}  // namespace tcx

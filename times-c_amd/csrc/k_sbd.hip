@@ -1,0 +1,334 @@
+// SBD kernels: z-normalization, centroid spectra, the fused
+// FFT -> conj-multiply -> inverse FFT -> max-over-lags -> argmin assignment,
+// the materialized NCC and the zero-fill shift.
+//
+// Layout: series are rows of a row-major FP32 [n][m] matrix.  A spectrum is
+// F complex values stored lane-strided ([s][lane], s < P, lane < L) so that
+// the lane group that produced it reads it back with coalesced loads.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "tcx_device.hpp"
+#include "tcx_kernels.hpp"
+
+namespace tcx {
+
+// ---------------------------------------------------------------- z-norm
+// One wave per row, FP64 accumulation of mean and of the centred sum of
+// squares; (x - mean) / sqrt(ss / (m - ddof)), NaN -> 0 (nan_to_num).
+// Semantics of zscore, Code/test/test_timesC.py:42-52.
+__global__ __launch_bounds__(256) void k_znorm(float* __restrict__ x, long rows,
+                                               int m, int ddof) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= rows) return;
+  float* r = x + row * (long)m;
+  double s = 0.0;
+  for (int t = lane; t < m; t += kWave) s += (double)r[t];
+  s = wave_sum_f64(s);
+  const double mean = s / (double)m;
+  double ss = 0.0;
+  for (int t = lane; t < m; t += kWave) {
+    double d = (double)r[t] - mean;
+    ss += d * d;
+  }
+  ss = wave_sum_f64(ss);
+  const double sd = sqrt(ss / (double)(m - ddof));
+  for (int t = lane; t < m; t += kWave) {
+    double v = ((double)r[t] - mean) / sd;
+    float f;
+    if (isnan(v))
+      f = 0.f;
+    else if (isinf(v))
+      f = v > 0 ? 3.402823466e38f : -3.402823466e38f;
+    else
+      f = (float)v;
+    r[t] = f;
+  }
+}
+
+// ------------------------------------------------------------- spectra
+// Forward FFT of each row (zero padded to F, imaginary part 0), stored
+// lane-strided, plus the row's L2 norm and an all-zero flag.
+template <int F>
+__global__ __launch_bounds__(FftGeom<F>::kBlock) void k_spectra(
+    const float* __restrict__ y, int rows, int m, const float2* __restrict__ twg,
+    float2* __restrict__ spec, float* __restrict__ norms, int* __restrict__ zero) {
+  using G = FftGeom<F>;
+  constexpr int L = G::L, P = G::P;
+  extern __shared__ float2 smem[];
+  float2* tw = smem;
+  float* red = reinterpret_cast<float*>(smem + F + G::kGroups * fft_buf_len<F>());
+  for (int i = threadIdx.x; i < F; i += blockDim.x) tw[i] = twg[i];
+  __syncthreads();
+  const int g = threadIdx.x / L, lane = threadIdx.x % L;
+  float2* buf = smem + F + g * fft_buf_len<F>();
+  for (int row = blockIdx.x * G::kGroups + g; row < rows; row += gridDim.x * G::kGroups) {
+    const float* yr = y + (long)row * m;
+    float2 v[P];
+    float ss = 0.f;
+    int nz = 0;
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+      int t = lane + L * s;
+      float a = t < m ? yr[t] : 0.f;
+      v[s] = make_float2(a, 0.f);
+      ss += a * a;
+      nz |= (a != 0.f);
+    }
+    ss = group_sum<L>(ss, red);
+    float fnz = group_sum<L>((float)nz, red);
+    fft<F, L, false>(v, buf, tw, lane);
+    float2* o = spec + (long)row * F;
+#pragma unroll
+    for (int s = 0; s < P; ++s) o[s * L + lane] = v[s];
+    if (lane == 0) {
+      norms[row] = sqrtf(ss);
+      if (zero) zero[row] = fnz == 0.f;
+    }
+  }
+}
+
+// ------------------------------------------------ fused SBD / assignment
+// Two series share one complex transform: z = x_a + i x_b.  For centroid j
+// with spectrum C_j, ifft(FFT(z) conj(C_j)) = cc_a + i cc_b, the two
+// circular cross-correlations of length F (F >= 2m-1, so no aliasing).
+// ASSIGN: per series keep the centroid with the smallest
+//   dist = 1 - (max_lag cc / F) / (|x| |c_j|)       (first argmin)
+// and the lag key of its maximum (first argmax), test_timesC.py:258-260.
+// NCC: write cc / F / (|x| |y|) for all 2m-1 lags (ncc.cuh:467-517 layout
+// out[x][y][lag]).
+template <int F, int MODE>
+__global__ __launch_bounds__(FftGeom<F>::kBlock) void k_sbd(
+    const float* __restrict__ x, long n, int m, int k,
+    const float2* __restrict__ cspec, const float* __restrict__ cnorm,
+    const float2* __restrict__ twg, int rev, int* __restrict__ label,
+    float* __restrict__ dist, int* __restrict__ shift, float* __restrict__ dmat,
+    int* __restrict__ changed, float* __restrict__ ncc_out,
+    const int* __restrict__ fixed_label) {
+  using G = FftGeom<F>;
+  constexpr int L = G::L, P = G::P;
+  extern __shared__ float2 smem[];
+  float2* tw = smem;
+  float* red = reinterpret_cast<float*>(smem + F + G::kGroups * fft_buf_len<F>());
+  for (int i = threadIdx.x; i < F; i += blockDim.x) tw[i] = twg[i];
+  __syncthreads();
+  const int g = threadIdx.x / L, lane = threadIdx.x % L;
+  float2* buf = smem + F + g * fft_buf_len<F>();
+  const long npairs = (n + 1) / 2;
+  const int nlag = 2 * m - 1;
+  for (long pr = (long)blockIdx.x * G::kGroups + g; pr < npairs;
+       pr += (long)gridDim.x * G::kGroups) {
+    const long ia = 2 * pr, ib = 2 * pr + 1;
+    const bool hasb = ib < n;
+    const float* xa = x + ia * (long)m;
+    const float* xb = x + (hasb ? ib : ia) * (long)m;
+    float2 zx[P];
+    float sa = 0.f, sb = 0.f;
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+      int t = lane + L * s;
+      float a = t < m ? xa[t] : 0.f;
+      float b = (hasb && t < m) ? xb[t] : 0.f;
+      zx[s] = make_float2(a, b);
+      sa += a * a;
+      sb += b * b;
+    }
+    sa = group_sum<L>(sa, red);
+    sb = group_sum<L>(sb, red);
+    const float na = sqrtf(sa), nb = sqrtf(sb);
+    fft<F, L, false>(zx, buf, tw, lane);
+
+    float bestA = INFINITY, bestB = INFINITY;
+    int labA = 0, labB = 0, keyA = 0, keyB = 0;
+    // fixed_label: report the lag for the given centroid instead of the argmin
+    const int fixA = fixed_label ? fixed_label[ia] : -1;
+    const int fixB = (fixed_label && hasb) ? fixed_label[ib] : -1;
+    for (int j = 0; j < k; ++j) {
+      const float2* cj = cspec + (long)j * F;
+      float2 z[P];
+#pragma unroll
+      for (int s = 0; s < P; ++s) z[s] = cmulc(zx[s], cj[s * L + lane]);
+      fft<F, L, true>(z, buf, tw, lane);
+      const float cn = cnorm[j];
+      const float dena = na * cn, denb = nb * cn;
+      if constexpr (MODE == kModeAssign) {
+        float va = -INFINITY, vb = -INFINITY;
+        int ka = 0x7fffffff, kb = 0x7fffffff;
+#pragma unroll
+        for (int s = 0; s < P; ++s) {
+          int key = lag_key(lane + L * s, m, F, rev);
+          if (key >= 0) {
+            if (z[s].x > va || (z[s].x == va && key < ka)) {
+              va = z[s].x;
+              ka = key;
+            }
+            if (z[s].y > vb || (z[s].y == vb && key < kb)) {
+              vb = z[s].y;
+              kb = key;
+            }
+          }
+        }
+        group_argmax<L>(va, ka, red);
+        group_argmax<L>(vb, kb, red);
+        // den == 0: NumPy divides by inf, every lag is 0, argmax is index 0
+        float da = 1.f, db = 1.f;
+        if (dena != 0.f) da = 1.f - (va / (float)F) / dena; else ka = 0;
+        if (denb != 0.f) db = 1.f - (vb / (float)F) / denb; else kb = 0;
+        if (fixed_label ? (j == fixA) : (da < bestA)) { bestA = da; labA = j; keyA = ka; }
+        if (fixed_label ? (j == fixB) : (db < bestB)) { bestB = db; labB = j; keyB = kb; }
+        if (dmat && lane == 0) {
+          dmat[ia * k + j] = da;
+          if (hasb) dmat[ib * k + j] = db;
+        }
+      } else {
+        const float sca = dena != 0.f ? 1.f : 0.f, scb = denb != 0.f ? 1.f : 0.f;
+        const float dA = dena != 0.f ? dena : 1.f, dB = denb != 0.f ? denb : 1.f;
+        float* oa = ncc_out + (ia * (long)k + j) * nlag;
+        float* ob = ncc_out + (ib * (long)k + j) * nlag;
+#pragma unroll
+        for (int s = 0; s < P; ++s) {
+          int key = lag_key(lane + L * s, m, F, 0);
+          if (key >= 0) {
+            oa[key] = sca * ((z[s].x / (float)F) / dA);
+            if (hasb) ob[key] = scb * ((z[s].y / (float)F) / dB);
+          }
+        }
+      }
+    }
+    if constexpr (MODE == kModeAssign) {
+      if (lane == 0) {
+        if (changed && label[ia] != labA) *changed = 1;
+        label[ia] = labA;
+        if (dist) dist[ia] = bestA;
+        if (shift) shift[ia] = keyA + 1 - m;
+        if (hasb) {
+          if (changed && label[ib] != labB) *changed = 1;
+          label[ib] = labB;
+          if (dist) dist[ib] = bestB;
+          if (shift) shift[ib] = keyB + 1 - m;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------- zero-fill shift
+// out[t] = in[t - s] (0 outside), s = shift[row]; skipped when *skip != 0.
+// roll_zeropad, test_timesC.py:55-77.  In place: one block per row, the row
+// is staged in LDS.
+__global__ __launch_bounds__(256) void k_apply_shift(float* __restrict__ x, long rows, int m,
+                                                    const int* __restrict__ shift,
+                                                    const int* __restrict__ skip) {
+  extern __shared__ float row_s[];
+  if (skip && *skip) return;
+  for (long row = blockIdx.x; row < rows; row += gridDim.x) {
+    float* r = x + row * (long)m;
+    for (int t = threadIdx.x; t < m; t += blockDim.x) row_s[t] = r[t];
+    __syncthreads();
+    const int s = shift[row];
+    for (int t = threadIdx.x; t < m; t += blockDim.x) {
+      long src = (long)t - s;
+      r[t] = (src >= 0 && src < m) ? row_s[src] : 0.f;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- launchers
+template <int F>
+static size_t sbd_smem() {
+  using G = FftGeom<F>;
+  return sizeof(float2) * (F + G::kGroups * fft_buf_len<F>()) + sizeof(float) * 64;
+}
+
+template <int F>
+static hipError_t launch_spectra_F(const float* y, int rows, int m, const float2* tw,
+                                   float2* spec, float* norms, int* zero, hipStream_t st) {
+  using G = FftGeom<F>;
+  int blocks = (rows + G::kGroups - 1) / G::kGroups;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL((k_spectra<F>), dim3(blocks), dim3(G::kBlock), sbd_smem<F>(), st, y,
+                     rows, m, tw, spec, norms, zero);
+  return hipGetLastError();
+}
+
+template <int F, int MODE>
+static hipError_t launch_sbd_F(const SbdArgs& a, hipStream_t st) {
+  using G = FftGeom<F>;
+  long npairs = (a.n + 1) / 2;
+  long blocks = (npairs + G::kGroups - 1) / G::kGroups;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL((k_sbd<F, MODE>), dim3((unsigned)blocks), dim3(G::kBlock),
+                     sbd_smem<F>(), st, a.x, a.n, a.m, a.k, a.cspec, a.cnorm, a.tw, a.rev,
+                     a.label, a.dist, a.shift, a.dmat, a.changed, a.ncc_out, a.fixed_label);
+  return hipGetLastError();
+}
+
+#define TCX_FOR_EACH_F(X) \
+  X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
+
+hipError_t launch_spectra(int F, const float* y, int rows, int m, const float2* tw,
+                          float2* spec, float* norms, int* zero, hipStream_t st) {
+  switch (F) {
+#define X(FF) \
+  case FF:    \
+    return launch_spectra_F<FF>(y, rows, m, tw, spec, norms, zero, st);
+    TCX_FOR_EACH_F(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_sbd(int F, int mode, const SbdArgs& a, hipStream_t st) {
+  switch (F) {
+#define X(FF)                                                     \
+  case FF:                                                        \
+    return mode == kModeAssign ? launch_sbd_F<FF, kModeAssign>(a, st) \
+                               : launch_sbd_F<FF, kModeNcc>(a, st);
+    TCX_FOR_EACH_F(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+// Kernels whose dynamic LDS exceeds the default per-launch limit.
+hipError_t configure_sbd_kernels() {
+  hipError_t e = hipSuccess;
+#define X(FF)                                                                                 \
+  if (e == hipSuccess)                                                                        \
+    e = hipFuncSetAttribute((const void*)k_sbd<FF, kModeAssign>,                              \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sbd_smem<FF>()); \
+  if (e == hipSuccess)                                                                        \
+    e = hipFuncSetAttribute((const void*)k_sbd<FF, kModeNcc>,                                 \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sbd_smem<FF>()); \
+  if (e == hipSuccess)                                                                        \
+    e = hipFuncSetAttribute((const void*)k_spectra<FF>,                                       \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sbd_smem<FF>());
+  TCX_FOR_EACH_F(X)
+#undef X
+  return e;
+}
+
+hipError_t launch_znorm(float* x, long rows, int m, int ddof, hipStream_t st) {
+  if (rows <= 0) return hipSuccess;
+  long blocks = (rows + 3) / 4;
+  hipLaunchKernelGGL(k_znorm, dim3((unsigned)blocks), dim3(256), 0, st, x, rows, m, ddof);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_shift(float* x, long rows, int m, const int* shift, const int* skip,
+                              hipStream_t st) {
+  if (rows <= 0) return hipSuccess;
+  long blocks = rows < 4096 ? rows : 4096;
+  hipLaunchKernelGGL(k_apply_shift, dim3((unsigned)blocks), dim3(256), sizeof(float) * m, st,
+                     x, rows, m, shift, skip);
+  return hipGetLastError();
+}
+
+}  // namespace tcx

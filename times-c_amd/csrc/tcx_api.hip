@@ -1,0 +1,740 @@
+// C ABI of the engine: context, workspace, primitives and the k-Shape fit
+// loop.  Host code; every kernel runs on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "tcx_host.hpp"
+#include "tcx_kernels.hpp"
+#include "timesc.h"
+
+namespace tcx {
+
+namespace {
+thread_local std::string g_err;
+}
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+void clear_error() { g_err.clear(); }
+
+}  // namespace tcx
+
+using tcx::fail;
+
+#define TCX_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(TCX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+#define TCX_TRY(call)            \
+  do {                           \
+    int rc_ = (call);            \
+    if (rc_ != TCX_OK) return rc_; \
+  } while (0)
+
+// Workspace slots (grow-only device buffers held by the context).
+enum Slot {
+  kSlotCspec, kSlotCnorm, kSlotCzero, kSlotLabel, kSlotShift, kSlotDist, kSlotCent,
+  kSlotCentNew, kSlotChanged, kSlotBlkCount, kSlotBlkBase, kSlotCounts, kSlotOffsets,
+  kSlotPerm, kSlotBad, kSlotYhi, kSlotYlo, kSlotA0, kSlotItems, kSlotTiles, kSlotPart,
+  kSlotS, kSlotInfo, kSlotDummy, kNumSlots
+};
+
+struct FitState {
+  bool active = false;
+  const float* d_x = nullptr;
+  int n = 0, m = 0, k = 0;
+  tcx_fit_opts o{};
+  double tol = 1e-12;
+  int split = 1;
+  int it = 0, converged = 0, sum_class = 0, steps_max = 0, fallbacks = 0;
+  float ms_ref = 0.f, ms_asg = 0.f, ms_asg_last = 0.f;
+  int *label = nullptr, *shift = nullptr, *changed = nullptr, *zero = nullptr;
+  float *dist = nullptr, *cent = nullptr, *norm = nullptr;
+  float2* spec = nullptr;
+  std::chrono::steady_clock::time_point t_start;
+};
+
+struct tcx_ctx {
+  FitState fit;
+  int device = 0;
+  size_t budget = 0;
+  hipStream_t stream = nullptr;
+  std::map<int, float2*> twiddles;
+  void* slot_ptr[kNumSlots] = {};
+  size_t slot_bytes[kNumSlots] = {};
+  size_t ws_bytes = 0;
+  size_t user_bytes = 0;
+  std::map<void*, size_t> user_allocs;
+  hipEvent_t ev[4] = {};
+  // host copies kept alive until the stream has consumed them
+  std::vector<tcx::SyrkItem> items;
+  std::vector<tcx::SyrkTile> tiles;
+};
+
+namespace {
+
+int fft_size(int m) {
+  unsigned v = (unsigned)(2 * m - 1);
+  int bl = 0;
+  while (v) {
+    ++bl;
+    v >>= 1;
+  }
+  return 1 << bl;
+}
+
+int set_dev(tcx_ctx* c) {
+  if (!c) return fail(TCX_ERR_ARG, "null context");
+  TCX_HIP(hipSetDevice(c->device));
+  return TCX_OK;
+}
+
+int check_budget(tcx_ctx* c, size_t extra) {
+  if (c->budget && c->ws_bytes + c->user_bytes + extra > c->budget)
+    return fail(TCX_ERR_BUDGET, "HBM budget exceeded: need " +
+                                    std::to_string(c->ws_bytes + c->user_bytes + extra) +
+                                    " bytes, budget " + std::to_string(c->budget));
+  return TCX_OK;
+}
+
+template <typename T>
+int ws(tcx_ctx* c, Slot s, size_t count, T** out) {
+  size_t bytes = sizeof(T) * (count ? count : 1);
+  bytes = (bytes + 255) & ~(size_t)255;
+  if (c->slot_bytes[s] < bytes) {
+    if (c->slot_ptr[s]) {
+      TCX_HIP(hipStreamSynchronize(c->stream));
+      TCX_HIP(hipFree(c->slot_ptr[s]));
+      c->ws_bytes -= c->slot_bytes[s];
+      c->slot_ptr[s] = nullptr;
+      c->slot_bytes[s] = 0;
+    }
+    TCX_TRY(check_budget(c, bytes));
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess)
+      return fail(TCX_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    c->slot_ptr[s] = p;
+    c->slot_bytes[s] = bytes;
+    c->ws_bytes += bytes;
+  }
+  *out = reinterpret_cast<T*>(c->slot_ptr[s]);
+  return TCX_OK;
+}
+
+int twiddles(tcx_ctx* c, int F, const float2** out) {
+  auto it = c->twiddles.find(F);
+  if (it != c->twiddles.end()) {
+    *out = it->second;
+    return TCX_OK;
+  }
+  std::vector<float2> h(F);
+  const double two_pi = 6.283185307179586476925286766559;
+  for (int q = 0; q < F; ++q) {
+    double a = two_pi * (double)q / (double)F;
+    h[q] = make_float2((float)std::cos(a), (float)-std::sin(a));
+  }
+  TCX_TRY(check_budget(c, sizeof(float2) * F));
+  float2* d = nullptr;
+  if (hipMalloc(&d, sizeof(float2) * F) != hipSuccess)
+    return fail(TCX_ERR_NOMEM, "twiddle table allocation failed");
+  TCX_HIP(hipMemcpy(d, h.data(), sizeof(float2) * F, hipMemcpyHostToDevice));
+  c->twiddles[F] = d;
+  c->ws_bytes += sizeof(float2) * F;
+  *out = d;
+  return TCX_OK;
+}
+
+int check_series(int n, int m) {
+  if (n < 1) return fail(TCX_ERR_ARG, "need at least one series");
+  if (m < 2) return fail(TCX_ERR_ARG, "series length must be >= 2");
+  if (m > tcx::kMaxFft / 2) return fail(TCX_ERR_UNSUPPORTED, "series length > 4096 is not supported");
+  return TCX_OK;
+}
+
+// Spectra, norms and all-zero flags of k rows.
+int spectra(tcx_ctx* c, const float* d_rows, int rows, int m, float2** spec, float** norm,
+            int** zero) {
+  int F = fft_size(m);
+  const float2* tw;
+  TCX_TRY(twiddles(c, F, &tw));
+  TCX_TRY(ws(c, kSlotCspec, (size_t)rows * F, spec));
+  TCX_TRY(ws(c, kSlotCnorm, (size_t)rows, norm));
+  TCX_TRY(ws(c, kSlotCzero, (size_t)rows, zero));
+  TCX_HIP(tcx::launch_spectra(F, d_rows, rows, m, tw, *spec, *norm, *zero, c->stream));
+  return TCX_OK;
+}
+
+int assign_core(tcx_ctx* c, const float* d_x, long n, int m, int k, const float2* cspec,
+                const float* cnorm, int rev, int* label, float* dist, int* shift, float* dmat,
+                int* changed) {
+  int F = fft_size(m);
+  const float2* tw;
+  TCX_TRY(twiddles(c, F, &tw));
+  tcx::SbdArgs a;
+  a.x = d_x; a.n = n; a.m = m; a.k = k; a.cspec = cspec; a.cnorm = cnorm; a.tw = tw;
+  a.rev = rev; a.label = label; a.dist = dist; a.shift = shift; a.dmat = dmat;
+  a.changed = changed;
+  TCX_HIP(tcx::launch_sbd(F, tcx::kModeAssign, a, c->stream));
+  return TCX_OK;
+}
+
+struct RefineResult {
+  int sum_class = 0;
+  int steps_max = 0;
+  int fallbacks = 0;
+};
+
+// One refinement: group, align + pack, S, power iteration.  Reads the
+// labels, shifts and old-centroid zero flags; writes k*m new centroids.
+int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_label,
+           const int* d_shift, const int* d_czero, float* d_cout, int iter, uint64_t seed,
+           int split, double tol, RefineResult* res) {
+  hipStream_t st = c->stream;
+  long nblk = (n + tcx::kGroupChunk - 1) / tcx::kGroupChunk;
+  int *blk_count, *blk_base, *counts, *perm, *bad;
+  long* offsets;
+  TCX_TRY(ws(c, kSlotBlkCount, (size_t)nblk * k, &blk_count));
+  TCX_TRY(ws(c, kSlotBlkBase, (size_t)nblk * k, &blk_base));
+  TCX_TRY(ws(c, kSlotCounts, (size_t)k, &counts));
+  TCX_TRY(ws(c, kSlotOffsets, (size_t)k + 1, &offsets));
+  TCX_TRY(ws(c, kSlotBad, 1, &bad));
+  TCX_HIP(tcx::launch_fill_int(bad, 1, 0, st));
+  TCX_HIP(tcx::launch_group_count(d_label, n, k, blk_count, bad, st));
+  TCX_HIP(tcx::launch_group_scan((int)nblk, k, blk_count, blk_base, counts, offsets, st));
+  std::vector<int> h_counts(k);
+  std::vector<long> h_off(k + 1);
+  int h_bad = 0;
+  TCX_HIP(hipMemcpyAsync(h_counts.data(), counts, sizeof(int) * k, hipMemcpyDeviceToHost, st));
+  TCX_HIP(hipMemcpyAsync(h_off.data(), offsets, sizeof(long) * (k + 1), hipMemcpyDeviceToHost, st));
+  TCX_HIP(hipMemcpyAsync(&h_bad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+  TCX_HIP(hipStreamSynchronize(st));
+  if (h_bad) return fail(TCX_ERR_ARG, "label outside [0, k)");
+  const long n_pad = h_off[k];
+  TCX_TRY(ws(c, kSlotPerm, (size_t)(n_pad > 0 ? n_pad : 1), &perm));
+  TCX_HIP(tcx::launch_fill_int(perm, n_pad, -1, st));
+  TCX_HIP(tcx::launch_group_place(d_label, n, k, blk_base, offsets, perm, st));
+  uint16_t *yhi, *ylo;
+  float* a0;
+  TCX_TRY(ws(c, kSlotYhi, (size_t)m * (n_pad > 0 ? n_pad : 1), &yhi));
+  TCX_TRY(ws(c, kSlotYlo, (size_t)m * (n_pad > 0 ? n_pad : 1), &ylo));
+  TCX_TRY(ws(c, kSlotA0, (size_t)k * m, &a0));
+  TCX_HIP(tcx::launch_align_pack(d_x, m, perm, n_pad, offsets, k, d_shift, d_czero, yhi, ylo,
+                                 a0, st));
+  // S tiles: upper triangle of ceil(m/128)^2 tiles per non-empty cluster,
+  // K split so that the grid has at least ~2 blocks per CU.
+  const int T = (m + tcx::kSyrkTile - 1) / tcx::kSyrkTile;
+  const int tiles_per = T * (T + 1) / 2;
+  int nonempty = 0;
+  for (int j = 0; j < k; ++j) nonempty += h_counts[j] > 0;
+  const long total_tiles = (long)tiles_per * nonempty;
+  const long want = 512;
+  const long split_f = total_tiles >= want ? 1 : (want + total_tiles - 1) / std::max(1L, total_tiles);
+  c->items.clear();
+  c->tiles.clear();
+  for (int j = 0; j < k; ++j) {
+    if (h_counts[j] == 0) continue;
+    const long K = h_off[j + 1] - h_off[j];
+    long chunk = (K + split_f - 1) / split_f;
+    chunk = (chunk + tcx::kPackPad - 1) / tcx::kPackPad * tcx::kPackPad;
+    for (int ti = 0; ti < T; ++ti)
+      for (int tj = ti; tj < T; ++tj) {
+        tcx::SyrkTile tl{j, ti, tj, (int)c->items.size(), 0};
+        for (long p = 0; p < K; p += chunk) {
+          tcx::SyrkItem it{j, ti, tj, (int)c->items.size(), h_off[j] + p,
+                           h_off[j] + std::min(K, p + chunk)};
+          c->items.push_back(it);
+          ++tl.nparts;
+        }
+        c->tiles.push_back(tl);
+      }
+  }
+  tcx::SyrkItem* d_items;
+  tcx::SyrkTile* d_tiles;
+  float *part, *S;
+  tcx::PowerInfo* info;
+  TCX_TRY(ws(c, kSlotItems, c->items.size(), &d_items));
+  TCX_TRY(ws(c, kSlotTiles, c->tiles.size(), &d_tiles));
+  TCX_TRY(ws(c, kSlotPart, c->items.size() * tcx::kSyrkTile * tcx::kSyrkTile, &part));
+  TCX_TRY(ws(c, kSlotS, (size_t)k * m * m, &S));
+  TCX_TRY(ws(c, kSlotInfo, (size_t)k, &info));
+  if (!c->items.empty()) {
+    TCX_HIP(hipMemcpyAsync(d_items, c->items.data(), sizeof(tcx::SyrkItem) * c->items.size(),
+                           hipMemcpyHostToDevice, st));
+    TCX_HIP(hipMemcpyAsync(d_tiles, c->tiles.data(), sizeof(tcx::SyrkTile) * c->tiles.size(),
+                           hipMemcpyHostToDevice, st));
+  }
+  TCX_HIP(tcx::launch_syrk(d_items, (int)c->items.size(), yhi, ylo, n_pad, m, split, part, st));
+  TCX_HIP(tcx::launch_syrk_reduce(d_tiles, (int)c->tiles.size(), part, m, S, st));
+  const int max_steps = m > 1000 ? 1000 : m;  // test_timesC.py:152-156
+  TCX_HIP(tcx::launch_power(S, m, k, counts, a0, seed, iter, max_steps, tol, d_cout, info, st));
+  std::vector<tcx::PowerInfo> h_info(k);
+  TCX_HIP(hipMemcpyAsync(h_info.data(), info, sizeof(tcx::PowerInfo) * k, hipMemcpyDeviceToHost, st));
+  TCX_HIP(hipStreamSynchronize(st));
+  if (res) {
+    res->sum_class = nonempty;
+    for (int j = 0; j < k; ++j) {
+      res->steps_max = std::max(res->steps_max, h_info[j].steps);
+      res->fallbacks += h_info[j].fallback;
+    }
+  }
+  return TCX_OK;
+}
+
+}  // namespace
+
+// ======================================================================= ABI
+extern "C" {
+
+const char* tcx_last_error(void) { return tcx::g_err.c_str(); }
+
+int tcx_create(int device, size_t hbm_budget, tcx_ctx** out) {
+  if (!out) return fail(TCX_ERR_ARG, "null output pointer");
+  int ndev = 0;
+  TCX_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev)
+    return fail(TCX_ERR_ARG, "device " + std::to_string(device) + " not present (" +
+                                 std::to_string(ndev) + " devices)");
+  TCX_HIP(hipSetDevice(device));
+  // Raise the dynamic-LDS limit of the large-F kernels; if the device
+  // refuses, those launches report the error themselves.
+  (void)tcx::configure_sbd_kernels();
+  (void)tcx::configure_extract_kernels();
+  (void)hipGetLastError();
+  tcx_ctx* c = new tcx_ctx();
+  c->device = device;
+  c->budget = hbm_budget;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(TCX_ERR_HIP, "stream creation failed");
+  }
+  for (auto& e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      tcx_destroy(c);
+      return fail(TCX_ERR_HIP, "event creation failed");
+    }
+  *out = c;
+  tcx::clear_error();
+  return TCX_OK;
+}
+
+int tcx_destroy(tcx_ctx* c) {
+  if (!c) return TCX_OK;
+  // teardown: report the first failure but release everything
+  hipError_t e = hipSetDevice(c->device);
+  auto keep = [&e](hipError_t r) { if (e == hipSuccess) e = r; };
+  if (c->stream) keep(hipStreamSynchronize(c->stream));
+  for (int s = 0; s < kNumSlots; ++s)
+    if (c->slot_ptr[s]) keep(hipFree(c->slot_ptr[s]));
+  for (auto& kv : c->twiddles) keep(hipFree(kv.second));
+  for (auto& kv : c->user_allocs) keep(hipFree(kv.first));
+  for (auto& ev : c->ev)
+    if (ev) keep(hipEventDestroy(ev));
+  if (c->stream) keep(hipStreamDestroy(c->stream));
+  delete c;
+  if (e != hipSuccess) return fail(TCX_ERR_HIP, std::string("destroy: ") + hipGetErrorString(e));
+  return TCX_OK;
+}
+
+size_t tcx_workspace_bytes(const tcx_ctx* c) { return c ? c->ws_bytes : 0; }
+
+int tcx_malloc(tcx_ctx* c, void** p, size_t bytes) {
+  TCX_TRY(set_dev(c));
+  if (!p) return fail(TCX_ERR_ARG, "null pointer");
+  TCX_TRY(check_budget(c, bytes));
+  if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess)
+    return fail(TCX_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+  c->user_allocs[*p] = bytes;
+  c->user_bytes += bytes;
+  return TCX_OK;
+}
+
+int tcx_free(tcx_ctx* c, void* p) {
+  TCX_TRY(set_dev(c));
+  auto it = c->user_allocs.find(p);
+  if (it == c->user_allocs.end()) return fail(TCX_ERR_ARG, "pointer not allocated by tcx_malloc");
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  TCX_HIP(hipFree(p));
+  c->user_bytes -= it->second;
+  c->user_allocs.erase(it);
+  return TCX_OK;
+}
+
+int tcx_memcpy_h2d(tcx_ctx* c, void* d, const void* h, size_t bytes) {
+  TCX_TRY(set_dev(c));
+  TCX_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TCX_OK;
+}
+
+int tcx_memcpy_d2h(tcx_ctx* c, void* h, const void* d, size_t bytes) {
+  TCX_TRY(set_dev(c));
+  TCX_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TCX_OK;
+}
+
+int tcx_memset(tcx_ctx* c, void* d, int value, size_t bytes) {
+  TCX_TRY(set_dev(c));
+  TCX_HIP(hipMemsetAsync(d, value, bytes, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TCX_OK;
+}
+
+int tcx_synchronize(tcx_ctx* c) {
+  TCX_TRY(set_dev(c));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TCX_OK;
+}
+
+int tcx_znorm(tcx_ctx* c, float* d_x, int rows, int cols, int ddof) {
+  TCX_TRY(set_dev(c));
+  if (!d_x || rows < 0 || cols < 1 || ddof < 0 || ddof >= cols + 1)
+    return fail(TCX_ERR_ARG, "znorm: bad arguments");
+  TCX_HIP(tcx::launch_znorm(d_x, rows, cols, ddof, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TCX_OK;
+}
+
+int tcx_ncc(tcx_ctx* c, const float* d_x, const float* d_y, float* d_out, int x_rows,
+            int y_rows, int cols) {
+  TCX_TRY(set_dev(c));
+  TCX_TRY(check_series(x_rows, cols));
+  if (!d_x || !d_y || !d_out || y_rows < 1) return fail(TCX_ERR_ARG, "ncc: bad arguments");
+  float2* spec;
+  float* norm;
+  int* zero;
+  TCX_TRY(spectra(c, d_y, y_rows, cols, &spec, &norm, &zero));
+  int F = fft_size(cols);
+  const float2* tw;
+  TCX_TRY(twiddles(c, F, &tw));
+  tcx::SbdArgs a;
+  a.x = d_x; a.n = x_rows; a.m = cols; a.k = y_rows; a.cspec = spec; a.cnorm = norm;
+  a.tw = tw; a.ncc_out = d_out;
+  TCX_HIP(tcx::launch_sbd(F, tcx::kModeNcc, a, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TCX_OK;
+}
+
+int tcx_sbd_align(tcx_ctx* c, float* d_members, const float* d_center, int rows, int cols,
+                  int convention) {
+  TCX_TRY(set_dev(c));
+  TCX_TRY(check_series(rows, cols));
+  if (!d_members || !d_center || (convention != TCX_CONV_CPU && convention != TCX_CONV_KSHAPE))
+    return fail(TCX_ERR_ARG, "sbd_align: bad arguments");
+  float2* spec;
+  float* norm;
+  int* zero;
+  TCX_TRY(spectra(c, d_center, 1, cols, &spec, &norm, &zero));
+  int *lab, *sh;
+  TCX_TRY(ws(c, kSlotDummy, (size_t)rows, &lab));
+  TCX_TRY(ws(c, kSlotShift, (size_t)rows, &sh));
+  TCX_TRY(assign_core(c, d_members, rows, cols, 1, spec, norm, convention, lab, nullptr, sh,
+                      nullptr, nullptr));
+  TCX_HIP(tcx::launch_apply_shift(d_members, rows, cols, sh, zero, c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TCX_OK;
+}
+
+int tcx_assign(tcx_ctx* c, const float* d_x, const float* d_centroids, int n, int m, int k,
+               int* d_label, float* d_dist, int* d_shift, float* d_dmat) {
+  TCX_TRY(set_dev(c));
+  TCX_TRY(check_series(n, m));
+  if (!d_x || !d_centroids || !d_label || k < 1) return fail(TCX_ERR_ARG, "assign: bad arguments");
+  float2* spec;
+  float* norm;
+  int* zero;
+  TCX_TRY(spectra(c, d_centroids, k, m, &spec, &norm, &zero));
+  TCX_TRY(assign_core(c, d_x, n, m, k, spec, norm, 0, d_label, d_dist, d_shift, d_dmat, nullptr));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TCX_OK;
+}
+
+int tcx_extract_shape(tcx_ctx* c, const int* d_label, const float* d_x, const int* d_shift,
+                      float* d_centroids, int n, int m, int k, int iter, int* sum_class) {
+  TCX_TRY(set_dev(c));
+  TCX_TRY(check_series(n, m));
+  if (!d_label || !d_x || !d_centroids || k < 1)
+    return fail(TCX_ERR_ARG, "extract_shape: bad arguments");
+  float2* spec;
+  float* norm;
+  int* zero;
+  TCX_TRY(spectra(c, d_centroids, k, m, &spec, &norm, &zero));  // old-centroid zero flags
+  if (!d_shift) {
+    // shift of each series against its own cluster's old centroid
+    int *lab, *sh;
+    TCX_TRY(ws(c, kSlotDummy, (size_t)n, &lab));
+    TCX_TRY(ws(c, kSlotShift, (size_t)n, &sh));
+    int F = fft_size(m);
+    const float2* tw;
+    TCX_TRY(twiddles(c, F, &tw));
+    tcx::SbdArgs a;
+    a.x = d_x; a.n = n; a.m = m; a.k = k; a.cspec = spec; a.cnorm = norm; a.tw = tw;
+    a.label = lab; a.shift = sh; a.fixed_label = d_label;
+    TCX_HIP(tcx::launch_sbd(F, tcx::kModeAssign, a, c->stream));
+    d_shift = sh;
+  }
+  float* cnew;
+  TCX_TRY(ws(c, kSlotCentNew, (size_t)k * m, &cnew));
+  RefineResult rr;
+  TCX_TRY(refine(c, d_x, n, m, k, d_label, d_shift, zero, cnew, iter, 0, 1, 1e-12, &rr));
+  TCX_HIP(hipMemcpyAsync(d_centroids, cnew, sizeof(float) * k * m, hipMemcpyDeviceToDevice,
+                         c->stream));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  if (sum_class) *sum_class = rr.sum_class;
+  return TCX_OK;
+}
+
+void tcx_fit_opts_default(tcx_fit_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->max_iter = 10;
+  o->init_mode = 0;
+  o->seed = 0;
+  o->convention = TCX_CONV_CPU;
+  o->s_precision = TCX_S_SPLIT_BF16;
+  o->cal_select = 0;
+  o->power_tol = 1e-12;
+}
+
+int tcx_fit_begin(tcx_ctx* c, const float* d_x, int n, int m, int k, const tcx_fit_opts* opts) {
+  TCX_TRY(set_dev(c));
+  TCX_TRY(check_series(n, m));
+  FitState& f = c->fit;
+  f = FitState();
+  f.t_start = std::chrono::steady_clock::now();
+  tcx_fit_opts o;
+  if (opts) o = *opts; else tcx_fit_opts_default(&o);
+  if (!d_x) return fail(TCX_ERR_ARG, "fit: null data pointer");
+  if (k < 1) return fail(TCX_ERR_ARG, "fit: k must be >= 1");
+  if (o.max_iter < 1) return fail(TCX_ERR_ARG, "fit: max_iter must be >= 1");
+  if (o.init_mode != 0 && o.init_mode != 1) return fail(TCX_ERR_ARG, "fit: init_mode must be 0 or 1");
+  if (o.init_mode == 1 && k > n) return fail(TCX_ERR_ARG, "fit: k > n with centre initialisation");
+  if (o.convention != TCX_CONV_CPU && o.convention != TCX_CONV_KSHAPE)
+    return fail(TCX_ERR_ARG, "fit: unknown convention");
+  if (o.s_precision != TCX_S_SPLIT_BF16 && o.s_precision != TCX_S_BF16)
+    return fail(TCX_ERR_ARG, "fit: unknown s_precision");
+  if (o.cal_select != 0)
+    return fail(TCX_ERR_UNSUPPORTED, "cal_select=1 (eigen-solver path) is not supported yet");
+  hipStream_t st = c->stream;
+  TCX_TRY(ws(c, kSlotLabel, (size_t)n, &f.label));
+  TCX_TRY(ws(c, kSlotShift, (size_t)n, &f.shift));
+  TCX_TRY(ws(c, kSlotDist, (size_t)n, &f.dist));
+  TCX_TRY(ws(c, kSlotCent, (size_t)k * m, &f.cent));
+  TCX_TRY(ws(c, kSlotChanged, 1, &f.changed));
+  std::vector<int> init;
+  if (o.init_mode == 0) {
+    init.resize(n);
+    if (o.init_labels) {
+      for (int i = 0; i < n; ++i) {
+        if (o.init_labels[i] < 0 || o.init_labels[i] >= k)
+          return fail(TCX_ERR_ARG, "fit: initial label outside [0, k)");
+        init[i] = o.init_labels[i];
+      }
+    } else {
+      tcx_init_labels(o.seed, n, k, init.data());
+    }
+    TCX_HIP(hipMemcpyAsync(f.label, init.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    TCX_HIP(hipMemsetAsync(f.cent, 0, sizeof(float) * k * m, st));
+    TCX_HIP(hipMemsetAsync(f.shift, 0, sizeof(int) * n, st));
+  } else {
+    init.resize(k);
+    if (o.init_rows) {
+      for (int j = 0; j < k; ++j) {
+        if (o.init_rows[j] < 0 || o.init_rows[j] >= n)
+          return fail(TCX_ERR_ARG, "fit: initial centre row outside [0, n)");
+        init[j] = o.init_rows[j];
+      }
+    } else {
+      tcx_init_rows(o.seed, n, k, init.data());
+    }
+    for (int j = 0; j < k; ++j)
+      TCX_HIP(hipMemcpyAsync(f.cent + (size_t)j * m, d_x + (size_t)init[j] * m,
+                             sizeof(float) * m, hipMemcpyDeviceToDevice, st));
+    TCX_HIP(hipMemsetAsync(f.label, 0, sizeof(int) * n, st));
+  }
+  TCX_TRY(spectra(c, f.cent, k, m, &f.spec, &f.norm, &f.zero));
+  if (o.init_mode == 0) TCX_HIP(tcx::launch_fill_int(f.zero, k, 1, st));  // centroids start at 0
+  // the host copy of the initial labels must outlive the async upload
+  TCX_HIP(hipStreamSynchronize(st));
+  f.o = o;
+  f.o.init_labels = nullptr;
+  f.o.init_rows = nullptr;
+  f.tol = o.power_tol > 0 ? o.power_tol : 1e-12;
+  f.split = o.s_precision == TCX_S_SPLIT_BF16;
+  f.d_x = d_x;
+  f.n = n;
+  f.m = m;
+  f.k = k;
+  f.active = true;
+  return TCX_OK;
+}
+
+namespace {
+
+int fit_refine(tcx_ctx* c, FitState& f) {
+  hipStream_t st = c->stream;
+  float* cnew;
+  TCX_TRY(ws(c, kSlotCentNew, (size_t)f.k * f.m, &cnew));
+  TCX_HIP(hipEventRecord(c->ev[0], st));
+  RefineResult rr;
+  TCX_TRY(refine(c, f.d_x, f.n, f.m, f.k, f.label, f.shift, f.zero, cnew, f.it, f.o.seed,
+                 f.split, f.tol, &rr));
+  TCX_HIP(hipMemcpyAsync(f.cent, cnew, sizeof(float) * f.k * f.m, hipMemcpyDeviceToDevice, st));
+  TCX_TRY(spectra(c, f.cent, f.k, f.m, &f.spec, &f.norm, &f.zero));
+  TCX_HIP(hipEventRecord(c->ev[1], st));
+  TCX_HIP(hipEventSynchronize(c->ev[1]));
+  float ms = 0.f;
+  TCX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  f.ms_ref += ms;
+  f.sum_class = rr.sum_class;
+  f.steps_max = std::max(f.steps_max, rr.steps_max);
+  f.fallbacks += rr.fallbacks;
+  return TCX_OK;
+}
+
+int fit_assign(tcx_ctx* c, FitState& f, int* h_changed) {
+  hipStream_t st = c->stream;
+  TCX_HIP(tcx::launch_fill_int(f.changed, 1, 0, st));
+  TCX_HIP(hipEventRecord(c->ev[2], st));
+  TCX_TRY(assign_core(c, f.d_x, f.n, f.m, f.k, f.spec, f.norm, f.o.convention, f.label, f.dist,
+                      f.shift, nullptr, f.changed));
+  TCX_HIP(hipEventRecord(c->ev[3], st));
+  TCX_HIP(hipMemcpyAsync(h_changed, f.changed, sizeof(int), hipMemcpyDeviceToHost, st));
+  if (f.o.labels_per_iter && f.it < f.o.max_iter)
+    TCX_HIP(hipMemcpyAsync(f.o.labels_per_iter + (size_t)f.it * f.n, f.label,
+                           sizeof(int) * f.n, hipMemcpyDeviceToHost, st));
+  TCX_HIP(hipStreamSynchronize(st));
+  float ms = 0.f;
+  TCX_HIP(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+  f.ms_asg += ms;
+  f.ms_asg_last = ms;
+  return TCX_OK;
+}
+
+}  // namespace
+
+int tcx_fit_step(tcx_ctx* c, int* changed) {
+  TCX_TRY(set_dev(c));
+  FitState& f = c->fit;
+  if (!f.active) return fail(TCX_ERR_ARG, "fit_step: no fit in progress (call tcx_fit_begin)");
+  int h_changed = 0;
+  if (f.o.init_mode == 0) {
+    // test_timesC.py:223-264: refine, assign, stop when labels repeat
+    TCX_TRY(fit_refine(c, f));
+    TCX_TRY(fit_assign(c, f, &h_changed));
+    if (!h_changed) f.converged = 1;
+  } else {
+    // kshape.cuh:376-461 order: assign, stop test, refine
+    TCX_TRY(fit_assign(c, f, &h_changed));
+    if (f.it > 0 && !h_changed)
+      f.converged = 1;
+    else
+      TCX_TRY(fit_refine(c, f));
+  }
+  ++f.it;
+  if (changed) *changed = f.it == 1 && f.o.init_mode == 1 ? 1 : h_changed;
+  return TCX_OK;
+}
+
+int tcx_fit_query(tcx_ctx* c, tcx_fit_stats* stats) {
+  if (!c || !stats) return fail(TCX_ERR_ARG, "fit_query: null argument");
+  const FitState& f = c->fit;
+  if (!f.active) return fail(TCX_ERR_ARG, "fit_query: no fit in progress");
+  stats->iterations = f.it;
+  stats->converged = f.converged;
+  stats->sum_class = f.sum_class;
+  stats->power_steps_max = f.steps_max;
+  stats->fallbacks = f.fallbacks;
+  stats->ms_refine = f.ms_ref;
+  stats->ms_assign = f.ms_asg;
+  stats->ms_total = std::chrono::duration<double, std::milli>(
+                        std::chrono::steady_clock::now() - f.t_start).count();
+  return TCX_OK;
+}
+
+int tcx_fit_end(tcx_ctx* c, int* h_labels, float* h_centroids, tcx_fit_stats* stats) {
+  TCX_TRY(set_dev(c));
+  FitState& f = c->fit;
+  if (!f.active) return fail(TCX_ERR_ARG, "fit_end: no fit in progress");
+  hipStream_t st = c->stream;
+  if (h_labels)
+    TCX_HIP(hipMemcpyAsync(h_labels, f.label, sizeof(int) * f.n, hipMemcpyDeviceToHost, st));
+  if (h_centroids)
+    TCX_HIP(hipMemcpyAsync(h_centroids, f.cent, sizeof(float) * f.k * f.m,
+                           hipMemcpyDeviceToHost, st));
+  TCX_HIP(hipStreamSynchronize(st));
+  if (stats) {
+    stats->iterations = f.it;
+    stats->converged = f.converged;
+    stats->sum_class = f.sum_class;
+    stats->power_steps_max = f.steps_max;
+    stats->fallbacks = f.fallbacks;
+    stats->ms_refine = f.ms_ref;
+    stats->ms_assign = f.ms_asg;
+    stats->ms_total = std::chrono::duration<double, std::milli>(
+                          std::chrono::steady_clock::now() - f.t_start).count();
+  }
+  f.active = false;
+  return TCX_OK;
+}
+
+int tcx_kshape_fit(tcx_ctx* c, const float* d_x, int n, int m, int k, const tcx_fit_opts* opts,
+                   int* h_labels, float* h_centroids, tcx_fit_stats* stats) {
+  if (!h_labels) return fail(TCX_ERR_ARG, "fit: null label buffer");
+  TCX_TRY(tcx_fit_begin(c, d_x, n, m, k, opts));
+  FitState& f = c->fit;
+  while (f.it < f.o.max_iter && !f.converged) {
+    int rc = tcx_fit_step(c, nullptr);
+    if (rc != TCX_OK) {
+      f.active = false;
+      return rc;
+    }
+  }
+  return tcx_fit_end(c, h_labels, h_centroids, stats);
+}
+
+int tcx_kshape(tcx_ctx* c, const float* d_x, int* h_idx, int n, int m, int k, int* iter_num,
+               int* sum_class, int cal_select) {
+  if (!iter_num || !h_idx) return fail(TCX_ERR_ARG, "kshape: null pointer");
+  tcx_fit_opts o;
+  tcx_fit_opts_default(&o);
+  o.max_iter = *iter_num;
+  o.cal_select = cal_select;
+  tcx_fit_stats s;
+  TCX_TRY(tcx_kshape_fit(c, d_x, n, m, k, &o, h_idx, nullptr, &s));
+  for (int i = 0; i < n; ++i) h_idx[i] += 1;  // one-based, kshape.cuh:91
+  *iter_num = s.iterations;
+  if (sum_class) *sum_class = s.sum_class;
+  return TCX_OK;
+}
+
+int tcx_kshape_tsl(tcx_ctx* c, const float* d_x, const float* h_x, int* h_idx, int n, int m,
+                   int k, int* iter_num, int* sum_class, int cal_select) {
+  (void)h_x;
+  if (!iter_num || !h_idx) return fail(TCX_ERR_ARG, "kshape_tsl: null pointer");
+  tcx_fit_opts o;
+  tcx_fit_opts_default(&o);
+  o.max_iter = *iter_num;
+  o.init_mode = 1;
+  o.cal_select = cal_select;
+  tcx_fit_stats s;
+  TCX_TRY(tcx_kshape_fit(c, d_x, n, m, k, &o, h_idx, nullptr, &s));
+  for (int i = 0; i < n; ++i) h_idx[i] += 1;
+  *iter_num = s.iterations;
+  if (sum_class) *sum_class = s.sum_class;
+  return TCX_OK;
+}
+
+}  // extern "C"

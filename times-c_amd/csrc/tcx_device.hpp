@@ -1,0 +1,281 @@
+// Device-side building blocks for gfx950 (CDNA4, wave64): complex helpers,
+// lane-group reductions and the register/LDS Stockham FFT used by every SBD
+// kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tcx {
+
+constexpr int kWave = 64;
+
+// ------------------------------------------------------------- complex ops
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) {
+  return make_float2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ float2 csub(float2 a, float2 b) {
+  return make_float2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+// ------------------------------------------------------- synchronisation
+// LDS exchange inside one wavefront: a wave's LDS operations complete in
+// issue order, so only the compiler must be kept from reordering.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sync of an FFT lane group of L lanes: wave-local when the group lives in
+// one wave, a block barrier otherwise (then the group is the whole block).
+template <int L>
+__device__ __forceinline__ void group_sync() {
+  if constexpr (L <= kWave)
+    wave_lds_sync();
+  else
+    __syncthreads();
+}
+
+// ------------------------------------------------------ group reductions
+// Reductions over L lanes.  For L <= 64 the group is a contiguous lane
+// range of one wave; for L > 64 the group is the whole block and `scratch`
+// (at least 2*L/64 words of LDS) carries the per-wave partials.
+template <int L>
+__device__ __forceinline__ float group_sum(float v, float* scratch) {
+  constexpr int W = L < kWave ? L : kWave;
+#pragma unroll
+  for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
+  if constexpr (L > kWave) {
+    constexpr int NW = L / kWave;
+    int wid = threadIdx.x / kWave;
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) scratch[wid] = v;
+    __syncthreads();
+    float s = 0.f;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s += scratch[w];
+    v = s;
+  }
+  return v;
+}
+
+// Max with the smallest key on ties (NumPy argmax: first index wins).
+template <int L>
+__device__ __forceinline__ void group_argmax(float& v, int& key, float* scratch) {
+  constexpr int W = L < kWave ? L : kWave;
+#pragma unroll
+  for (int o = W / 2; o > 0; o >>= 1) {
+    float ov = __shfl_xor(v, o, W);
+    int ok = __shfl_xor(key, o, W);
+    if (ov > v || (ov == v && ok < key)) {
+      v = ov;
+      key = ok;
+    }
+  }
+  if constexpr (L > kWave) {
+    constexpr int NW = L / kWave;
+    int wid = threadIdx.x / kWave;
+    int* ks = reinterpret_cast<int*>(scratch + NW);
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+      scratch[wid] = v;
+      ks[wid] = key;
+    }
+    __syncthreads();
+    v = scratch[0];
+    key = ks[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) {
+      float ov = scratch[w];
+      int ok = ks[w];
+      if (ov > v || (ov == v && ok < key)) {
+        v = ov;
+        key = ok;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// ---------------------------------------------------------- in-register DFT
+// exp(-2 pi i q / 32): table of cos / sin used for compile-time butterflies.
+constexpr float kCos32[32] = {
+    1.0f, 0.98078528040323043f, 0.92387953251128674f, 0.83146961230254524f,
+    0.70710678118654757f, 0.55557023301960229f, 0.38268343236508984f, 0.19509032201612833f,
+    0.0f, -0.19509032201612819f, -0.38268343236508973f, -0.55557023301960196f,
+    -0.70710678118654746f, -0.83146961230254535f, -0.92387953251128674f, -0.98078528040323043f,
+    -1.0f, -0.98078528040323043f, -0.92387953251128685f, -0.83146961230254546f,
+    -0.70710678118654768f, -0.55557023301960218f, -0.38268343236509034f, -0.19509032201612866f,
+    0.0f, 0.1950903220161283f, 0.38268343236509f, 0.55557023301960184f,
+    0.70710678118654735f, 0.83146961230254524f, 0.92387953251128652f, 0.98078528040323032f,
+};
+constexpr float kSin32[32] = {
+    0.0f, 0.19509032201612825f, 0.38268343236508978f, 0.55557023301960218f,
+    0.70710678118654746f, 0.83146961230254524f, 0.92387953251128674f, 0.98078528040323043f,
+    1.0f, 0.98078528040323043f, 0.92387953251128674f, 0.83146961230254546f,
+    0.70710678118654757f, 0.55557023301960218f, 0.38268343236508989f, 0.19509032201612861f,
+    0.0f, -0.19509032201612836f, -0.38268343236508967f, -0.55557023301960196f,
+    -0.70710678118654746f, -0.83146961230254524f, -0.92387953251128652f, -0.98078528040323032f,
+    -1.0f, -0.98078528040323043f, -0.92387953251128663f, -0.83146961230254546f,
+    -0.70710678118654768f, -0.55557023301960218f, -0.38268343236509039f, -0.19509032201612872f,
+};
+
+// v * W_R^k, W_R = exp(-2 pi i / R) (forward) or its conjugate (inverse).
+template <int R, int K, bool INV>
+__device__ __forceinline__ float2 twmul(float2 v) {
+  constexpr int q = (K * (32 / R)) & 31;
+  if constexpr (q == 0) {
+    return v;
+  } else if constexpr (q == 8) {  // forward: -i
+    return INV ? make_float2(-v.y, v.x) : make_float2(v.y, -v.x);
+  } else if constexpr (q == 16) {
+    return make_float2(-v.x, -v.y);
+  } else if constexpr (q == 24) {  // forward: +i
+    return INV ? make_float2(v.y, -v.x) : make_float2(-v.y, v.x);
+  } else {
+    constexpr float c = kCos32[q];
+    constexpr float s = INV ? kSin32[q] : -kSin32[q];
+    return make_float2(v.x * c - v.y * s, v.x * s + v.y * c);
+  }
+}
+
+template <int R, bool INV, int K>
+__device__ __forceinline__ void dft_combine(float2* v, const float2* e, const float2* o) {
+  if constexpr (K < R / 2) {
+    float2 t = twmul<R, K, INV>(o[K]);
+    v[K] = cadd(e[K], t);
+    v[K + R / 2] = csub(e[K], t);
+    dft_combine<R, INV, K + 1>(v, e, o);
+  }
+}
+
+// Length-R DFT (R <= 32, power of two) of v[0..R-1], natural order in and
+// out, radix-2 decimation in time with compile-time twiddles.
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2* v) {
+  static_assert(R >= 1 && R <= 32 && (R & (R - 1)) == 0, "radix");
+  if constexpr (R == 2) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R > 2) {
+    float2 e[R / 2], o[R / 2];
+#pragma unroll
+    for (int i = 0; i < R / 2; ++i) {
+      e[i] = v[2 * i];
+      o[i] = v[2 * i + 1];
+    }
+    dft<R / 2, INV>(e);
+    dft<R / 2, INV>(o);
+    dft_combine<R, INV, 0>(v, e, o);
+  }
+}
+
+// --------------------------------------------------------- Stockham FFT
+// A length-F complex FFT held by a group of L lanes, P = F/L values per
+// lane, "lane-strided natural order" on input and output:
+//     v[s] of lane l  <->  element l + L*s.
+// Passes are radix P (the first reads registers directly, the last leaves
+// its output in registers); between passes the data goes through the
+// group's LDS buffer `buf`.  `tw` holds exp(-2 pi i q / F), q < F, in LDS.
+// The forward transform's output order equals the inverse transform's input
+// order, so spectra are multiplied in place without any reordering.
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 5); }
+template <int F>
+constexpr int fft_buf_len() {
+  return F + F / 32 + 1;
+}
+
+template <int F, int L, bool INV, int Ns>
+__device__ __forceinline__ void stockham_pass(float2 (&v)[F / L], float2* buf,
+                                              const float2* tw, int lane) {
+  constexpr int P = F / L;
+  constexpr int R = (F / Ns < P) ? F / Ns : P;
+  constexpr int NB = P / R;
+  constexpr bool kFirst = (Ns == 1);
+  constexpr bool kLast = (Ns * R == F);
+  static_assert(!kFirst || NB == 1, "first pass is one radix-P butterfly");
+  float2 u[P];
+  if constexpr (kFirst) {
+#pragma unroll
+    for (int r = 0; r < P; ++r) u[r] = v[r];
+  } else {
+#pragma unroll
+    for (int t = 0; t < NB; ++t)
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        u[t * R + r] = buf[lds_pad(lane + L * t + r * (F / R))];
+    if constexpr (L > kWave) __syncthreads();  // reads done before writes
+  }
+#pragma unroll
+  for (int t = 0; t < NB; ++t) {
+    const int j = lane + L * t;
+    const int kk = j & (Ns - 1);
+    float2* b = u + t * R;
+    if constexpr (!kFirst) {
+      const int qs = kk * (F / (Ns * R));
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = tw[r * qs];
+        b[r] = INV ? cmulc(b[r], w) : cmul(b[r], w);
+      }
+    }
+    dft<R, INV>(b);
+    if constexpr (!kLast) {
+      const int base = (j - kk) * R + kk;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[lds_pad(base + r * Ns)] = b[r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[t + r * NB] = b[r];
+    }
+  }
+  if constexpr (!kLast) {
+    group_sync<L>();
+    stockham_pass<F, L, INV, Ns * R>(v, buf, tw, lane);
+  }
+}
+
+// Unnormalized transform: forward uses exp(-2 pi i f t / F), inverse
+// exp(+2 pi i f t / F) (no 1/F).
+template <int F, int L, bool INV>
+__device__ __forceinline__ void fft(float2 (&v)[F / L], float2* buf,
+                                    const float2* tw, int lane) {
+  stockham_pass<F, L, INV, 1>(v, buf, tw, lane);
+}
+
+// Lane-group geometry per FFT length: L lanes, P = F/L values per lane.
+template <int F>
+struct FftGeom {
+  static constexpr int L = F <= 4 ? 1
+                         : F <= 64 ? F / 4
+                         : F <= 256 ? 16
+                         : F / 16;
+  static constexpr int P = F / L;
+  static constexpr int kBlock = L <= kWave ? 256 : L;  // threads per block
+  static constexpr int kGroups = kBlock / L;           // lane groups per block
+};
+
+// Lag key of circular correlation position tau (0 <= tau < F): the index in
+// NumPy's concatenation cc[-(m-1):] ++ cc[:m] (test_timesC.py:113), or -1
+// if tau is not one of the 2m-1 lags.  rev=1 gives the index in the
+// reversed array, i.e. the order of cc(centroid, member).
+__device__ __forceinline__ int lag_key(int tau, int m, int F, int rev) {
+  int idx = tau <= m - 1 ? tau + m - 1 : (tau >= F - (m - 1) ? tau - F + m - 1 : -1);
+  if (idx >= 0 && rev) idx = 2 * m - 2 - idx;
+  return idx;
+}
+
+}  // namespace tcx

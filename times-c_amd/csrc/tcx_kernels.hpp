@@ -1,0 +1,89 @@
+// Internal launchers of the HIP kernels (host-callable, stream-ordered).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tcx {
+
+constexpr int kModeAssign = 0;
+constexpr int kModeNcc = 1;
+
+struct SbdArgs {
+  const float* x = nullptr;      // [n][m] series
+  long n = 0;
+  int m = 0;
+  int k = 0;                     // centroids (assign) or y rows (ncc)
+  const float2* cspec = nullptr; // [k][F] lane-strided spectra
+  const float* cnorm = nullptr;  // [k]
+  const float2* tw = nullptr;    // [F] exp(-2 pi i q / F)
+  int rev = 0;                   // 1: k-Shape lag order
+  int* label = nullptr;
+  float* dist = nullptr;
+  int* shift = nullptr;
+  float* dmat = nullptr;
+  int* changed = nullptr;
+  float* ncc_out = nullptr;
+  const int* fixed_label = nullptr;  // shift against this centroid, no argmin
+};
+
+// largest supported FFT length (m <= 4096)
+constexpr int kMaxFft = 8192;
+
+hipError_t launch_znorm(float* x, long rows, int m, int ddof, hipStream_t st);
+hipError_t launch_spectra(int F, const float* y, int rows, int m, const float2* tw,
+                          float2* spec, float* norms, int* zero, hipStream_t st);
+hipError_t launch_sbd(int F, int mode, const SbdArgs& a, hipStream_t st);
+hipError_t launch_apply_shift(float* x, long rows, int m, const int* shift, const int* skip,
+                              hipStream_t st);
+hipError_t configure_sbd_kernels();
+
+// ------------------------------------------------------------ extraction
+constexpr int kGroupChunk = 2048;  // rows per grouping block
+constexpr int kPackPad = 64;       // cluster segments padded to this many columns
+constexpr int kSyrkTile = 128;     // S tile edge
+
+struct SyrkItem {
+  int cluster;
+  int ti, tj;         // tile row / column (ti <= tj)
+  int part;           // index of this item's partial tile
+  long p0, p1;        // packed column range [p0, p1), multiples of 16
+};
+
+struct SyrkTile {
+  int cluster;
+  int ti, tj;
+  int part0, nparts;  // partials part0 .. part0+nparts-1, summed in order
+};
+
+struct PowerInfo {
+  double lambda;
+  int steps;
+  int converged;
+  int fallback;
+  int flipped;
+  int zero;           // power iteration hit M v = 0 (NaN path, zero centroid)
+  int empty;
+};
+
+hipError_t launch_fill_int(int* p, long count, int value, hipStream_t st);
+hipError_t launch_group_count(const int* label, long n, int k, int* blk_count, int* bad,
+                              hipStream_t st);
+hipError_t launch_group_scan(int nblk, int k, const int* blk_count, int* blk_base,
+                             int* counts, long* offsets, hipStream_t st);
+hipError_t launch_group_place(const int* label, long n, int k, const int* blk_base,
+                              const long* offsets, int* perm, hipStream_t st);
+hipError_t launch_align_pack(const float* x, int m, const int* perm, long n_pad,
+                             const long* offsets, int k, const int* shift,
+                             const int* czero, uint16_t* yhi, uint16_t* ylo, float* a0,
+                             hipStream_t st);
+hipError_t launch_syrk(const SyrkItem* items, int nitems, const uint16_t* yhi,
+                       const uint16_t* ylo, long n_pad, int m, int split, float* part,
+                       hipStream_t st);
+hipError_t launch_syrk_reduce(const SyrkTile* tiles, int ntiles, const float* part, int m,
+                              float* S, hipStream_t st);
+hipError_t launch_power(const float* S, int m, int k, const int* counts, const float* a0,
+                        uint64_t seed, int iter, int max_steps, double tol, float* c_out,
+                        PowerInfo* info, hipStream_t st);
+hipError_t configure_extract_kernels();
+
+}  // namespace tcx

@@ -1,0 +1,392 @@
+"""ctypes binding of libtimesc_hip.so (the engine's C ABI, include/timesc.h).
+
+Used by the tests, bench.py and __graft_entry__.smoke().  Device memory is
+managed through the library's own helpers (tcx_malloc / tcx_memcpy_*), so no
+other GPU runtime is loaded into the process.  Loading fails loudly if the
+library has not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libtimesc_hip.so")
+CLI_PATH = os.path.join(PKG_ROOT, "bin", "test")
+
+OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_BUDGET, ERR_IO, ERR_UNSUPPORTED, ERR_NUMERIC = range(8)
+CONV_CPU, CONV_KSHAPE = 0, 1
+S_SPLIT_BF16, S_BF16 = 0, 1
+
+# every symbol include/timesc.h declares
+EXPORTS = [
+    "tcx_last_error", "tcx_create", "tcx_destroy", "tcx_workspace_bytes", "tcx_malloc",
+    "tcx_free", "tcx_memcpy_h2d", "tcx_memcpy_d2h", "tcx_memset", "tcx_synchronize",
+    "tcx_znorm", "tcx_ncc", "tcx_sbd_align", "tcx_assign", "tcx_extract_shape",
+    "tcx_fit_opts_default", "tcx_kshape_fit", "tcx_fit_begin", "tcx_fit_step", "tcx_fit_end",
+    "tcx_fit_query",
+    "tcx_kshape", "tcx_kshape_tsl",
+    "tcx_read_labeled_csv", "tcx_read_unlabeled_csv", "tcx_write_labels", "tcx_free_host",
+    "tcx_generate", "tcx_init_labels", "tcx_init_rows", "tcx_power_v0",
+]
+
+
+class TimescError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"timesc status {code}: {msg}")
+        self.code = code
+
+
+class FitOpts(ctypes.Structure):
+    _fields_ = [
+        ("max_iter", ctypes.c_int),
+        ("init_mode", ctypes.c_int),
+        ("seed", ctypes.c_uint64),
+        ("init_labels", ctypes.POINTER(ctypes.c_int)),
+        ("init_rows", ctypes.POINTER(ctypes.c_int)),
+        ("convention", ctypes.c_int),
+        ("s_precision", ctypes.c_int),
+        ("cal_select", ctypes.c_int),
+        ("power_tol", ctypes.c_double),
+        ("labels_per_iter", ctypes.POINTER(ctypes.c_int)),
+    ]
+
+
+class FitStats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int),
+        ("converged", ctypes.c_int),
+        ("sum_class", ctypes.c_int),
+        ("power_steps_max", ctypes.c_int),
+        ("fallbacks", ctypes.c_int),
+        ("ms_total", ctypes.c_double),
+        ("ms_refine", ctypes.c_double),
+        ("ms_assign", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_L = ctypes.c_long
+_Z = ctypes.c_size_t
+
+
+def load():
+    """Load the engine library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    lib = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "tcx_last_error": (ctypes.c_char_p, []),
+        "tcx_create": (_I, [_I, _Z, ctypes.POINTER(_P)]),
+        "tcx_destroy": (_I, [_P]),
+        "tcx_workspace_bytes": (_Z, [_P]),
+        "tcx_malloc": (_I, [_P, ctypes.POINTER(_P), _Z]),
+        "tcx_free": (_I, [_P, _P]),
+        "tcx_memcpy_h2d": (_I, [_P, _P, _P, _Z]),
+        "tcx_memcpy_d2h": (_I, [_P, _P, _P, _Z]),
+        "tcx_memset": (_I, [_P, _P, _I, _Z]),
+        "tcx_synchronize": (_I, [_P]),
+        "tcx_znorm": (_I, [_P, _P, _I, _I, _I]),
+        "tcx_ncc": (_I, [_P, _P, _P, _P, _I, _I, _I]),
+        "tcx_sbd_align": (_I, [_P, _P, _P, _I, _I, _I]),
+        "tcx_assign": (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
+        "tcx_extract_shape": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _I, ctypes.POINTER(_I)]),
+        "tcx_fit_opts_default": (None, [ctypes.POINTER(FitOpts)]),
+        "tcx_kshape_fit": (_I, [_P, _P, _I, _I, _I, ctypes.POINTER(FitOpts), _P, _P,
+                                ctypes.POINTER(FitStats)]),
+        "tcx_fit_begin": (_I, [_P, _P, _I, _I, _I, ctypes.POINTER(FitOpts)]),
+        "tcx_fit_step": (_I, [_P, ctypes.POINTER(_I)]),
+        "tcx_fit_end": (_I, [_P, _P, _P, ctypes.POINTER(FitStats)]),
+        "tcx_fit_query": (_I, [_P, ctypes.POINTER(FitStats)]),
+        "tcx_kshape": (_I, [_P, _P, _P, _I, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), _I]),
+        "tcx_kshape_tsl": (_I, [_P, _P, _P, _P, _I, _I, _I, ctypes.POINTER(_I),
+                                ctypes.POINTER(_I), _I]),
+        "tcx_read_labeled_csv": (_I, [ctypes.c_char_p, ctypes.POINTER(_P), ctypes.POINTER(_P),
+                                      ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+        "tcx_read_unlabeled_csv": (_I, [ctypes.c_char_p, ctypes.POINTER(_P),
+                                        ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+        "tcx_write_labels": (_I, [ctypes.c_char_p, _P, _I]),
+        "tcx_free_host": (None, [_P]),
+        "tcx_generate": (_I, [ctypes.c_uint64, _L, _I, _I, ctypes.c_double, _P, _P, _P]),
+        "tcx_init_labels": (_I, [ctypes.c_uint64, _L, _I, _P]),
+        "tcx_init_rows": (_I, [ctypes.c_uint64, _L, _I, _P]),
+        "tcx_power_v0": (None, [ctypes.c_uint64, _I, _I, _I, _P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def check(rc):
+    if rc != OK:
+        raise TimescError(rc, load().tcx_last_error().decode())
+
+
+# ------------------------------------------------------------- host utilities
+def generate(seed, n, m, K, sigma, class_frac=None):
+    """Seeded synthetic dataset: (X float32 [n,m], y int32 [n])."""
+    lib = load()
+    X = np.empty((n, m), np.float32)
+    y = np.empty(n, np.int32)
+    cf = None if class_frac is None else np.ascontiguousarray(class_frac, np.float64)
+    check(lib.tcx_generate(seed, n, m, K, sigma, _ptr(cf), _ptr(X), _ptr(y)))
+    return X, y
+
+
+def init_labels(seed, n, k):
+    out = np.empty(n, np.int32)
+    check(load().tcx_init_labels(seed, n, k, _ptr(out)))
+    return out
+
+
+def init_rows(seed, n, k):
+    out = np.empty(k, np.int32)
+    check(load().tcx_init_rows(seed, n, k, _ptr(out)))
+    return out
+
+
+def power_v0(seed, it, cluster, m):
+    out = np.empty(m, np.float64)
+    load().tcx_power_v0(seed, it, cluster, m, _ptr(out))
+    return out
+
+
+def read_labeled_csv(path):
+    lib = load()
+    d, l = ctypes.c_void_p(), ctypes.c_void_p()
+    r, c = ctypes.c_int(), ctypes.c_int()
+    check(lib.tcx_read_labeled_csv(path.encode(), ctypes.byref(d), ctypes.byref(l),
+                                   ctypes.byref(r), ctypes.byref(c)))
+    try:
+        X = np.ctypeslib.as_array((ctypes.c_float * (r.value * c.value)).from_address(d.value))
+        y = np.ctypeslib.as_array((ctypes.c_int * r.value).from_address(l.value))
+        return X.reshape(r.value, c.value).copy(), y.copy()
+    finally:
+        lib.tcx_free_host(d)
+        lib.tcx_free_host(l)
+
+
+def read_unlabeled_csv(path):
+    lib = load()
+    d = ctypes.c_void_p()
+    r, c = ctypes.c_int(), ctypes.c_int()
+    check(lib.tcx_read_unlabeled_csv(path.encode(), ctypes.byref(d), ctypes.byref(r),
+                                     ctypes.byref(c)))
+    try:
+        X = np.ctypeslib.as_array((ctypes.c_float * (r.value * c.value)).from_address(d.value))
+        return X.reshape(r.value, c.value).copy()
+    finally:
+        lib.tcx_free_host(d)
+
+
+def write_labels(path, labels):
+    lab = np.ascontiguousarray(labels, np.int32)
+    check(load().tcx_write_labels(path.encode(), _ptr(lab), lab.size))
+
+
+def write_labeled_csv(path, X, y):
+    """Labeled CSV in the reference's input format; values as %.9g of the
+    float32 value so every reader sees the same numbers."""
+    X = np.asarray(X, np.float32)
+    with open(path, "w") as f:
+        for i in range(X.shape[0]):
+            f.write(str(int(y[i])) + "," + ",".join("%.9g" % v for v in X[i]) + "\n")
+
+
+# --------------------------------------------------------------- device side
+class DeviceArray:
+    """A device buffer owned by a Context."""
+
+    def __init__(self, ctx, ptr, shape, dtype):
+        self.ctx, self.ptr, self.shape, self.dtype = ctx, ptr, tuple(shape), np.dtype(dtype)
+
+    @property
+    def nbytes(self):
+        return int(np.prod(self.shape)) * self.dtype.itemsize
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        self.ctx.d2h(out, self)
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx.free(self)
+
+
+class Context:
+    def __init__(self, device=0, budget=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        check(self.lib.tcx_create(device, budget, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            check(self.lib.tcx_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def workspace_bytes(self):
+        return self.lib.tcx_workspace_bytes(self.h)
+
+    def empty(self, shape, dtype):
+        shape = (shape,) if np.isscalar(shape) else tuple(shape)
+        nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        p = ctypes.c_void_p()
+        check(self.lib.tcx_malloc(self.h, ctypes.byref(p), nbytes))
+        return DeviceArray(self, p.value, shape, dtype)
+
+    def to_device(self, a):
+        a = np.ascontiguousarray(a)
+        d = self.empty(a.shape, a.dtype)
+        check(self.lib.tcx_memcpy_h2d(self.h, d.ptr, _ptr(a), a.nbytes))
+        return d
+
+    def h2d(self, d, a):
+        a = np.ascontiguousarray(a, d.dtype)
+        assert a.nbytes == d.nbytes
+        check(self.lib.tcx_memcpy_h2d(self.h, d.ptr, _ptr(a), a.nbytes))
+
+    def d2h(self, out, d):
+        assert out.flags.c_contiguous and out.nbytes == d.nbytes
+        check(self.lib.tcx_memcpy_d2h(self.h, _ptr(out), d.ptr, d.nbytes))
+
+    def free(self, d):
+        check(self.lib.tcx_free(self.h, d.ptr))
+        d.ptr = None
+
+    def sync(self):
+        check(self.lib.tcx_synchronize(self.h))
+
+    # primitives --------------------------------------------------------
+    def znorm(self, d, ddof=0):
+        rows, cols = d.shape
+        check(self.lib.tcx_znorm(self.h, d.ptr, rows, cols, ddof))
+
+    def ncc(self, dx, dy):
+        xr, m = dx.shape
+        yr = dy.shape[0]
+        out = self.empty((xr, yr, 2 * m - 1), np.float32)
+        try:
+            check(self.lib.tcx_ncc(self.h, dx.ptr, dy.ptr, out.ptr, xr, yr, m))
+            return out.numpy()
+        finally:
+            out.free()
+
+    def sbd_align(self, dmembers, dcenter, convention=CONV_CPU):
+        rows, m = dmembers.shape
+        check(self.lib.tcx_sbd_align(self.h, dmembers.ptr, dcenter.ptr, rows, m, convention))
+
+    def assign(self, dx, dc, dmat=False):
+        n, m = dx.shape
+        k = dc.shape[0]
+        lab = self.empty(n, np.int32)
+        dist = self.empty(n, np.float32)
+        sh = self.empty(n, np.int32)
+        dm = self.empty((n, k), np.float32) if dmat else None
+        try:
+            check(self.lib.tcx_assign(self.h, dx.ptr, dc.ptr, n, m, k, lab.ptr, dist.ptr,
+                                      sh.ptr, dm.ptr if dm else None))
+            out = (lab.numpy(), dist.numpy(), sh.numpy())
+            if dm:
+                out = out + (dm.numpy(),)
+            return out
+        finally:
+            for a in (lab, dist, sh, dm):
+                if a is not None:
+                    a.free()
+
+    def extract_shape(self, dlabel, dx, dshift, dcent, it=0):
+        n, m = dx.shape
+        k = dcent.shape[0]
+        sc = ctypes.c_int()
+        check(self.lib.tcx_extract_shape(self.h, dlabel.ptr, dx.ptr,
+                                         dshift.ptr if dshift is not None else None,
+                                         dcent.ptr, n, m, k, it, ctypes.byref(sc)))
+        return sc.value
+
+    def _opts(self, max_iter, init_mode, seed, convention, s_precision, power_tol):
+        o = FitOpts()
+        self.lib.tcx_fit_opts_default(ctypes.byref(o))
+        o.max_iter, o.init_mode, o.seed = max_iter, init_mode, seed
+        o.convention, o.s_precision, o.power_tol = convention, s_precision, power_tol
+        return o
+
+    def fit_begin(self, dx, k, max_iter=10, init_mode=0, seed=0, convention=CONV_CPU,
+                  s_precision=S_SPLIT_BF16, power_tol=1e-12):
+        n, m = dx.shape
+        o = self._opts(max_iter, init_mode, seed, convention, s_precision, power_tol)
+        check(self.lib.tcx_fit_begin(self.h, dx.ptr, n, m, k, ctypes.byref(o)))
+
+    def fit_step(self):
+        ch = ctypes.c_int()
+        check(self.lib.tcx_fit_step(self.h, ctypes.byref(ch)))
+        return ch.value
+
+    def fit_query(self):
+        st = FitStats()
+        check(self.lib.tcx_fit_query(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def fit_end(self, n, k, m):
+        labels = np.empty(n, np.int32)
+        cent = np.empty((k, m), np.float32)
+        st = FitStats()
+        check(self.lib.tcx_fit_end(self.h, _ptr(labels), _ptr(cent), ctypes.byref(st)))
+        return {"labels": labels, "centroids": cent, "stats": st.as_dict()}
+
+    def fit(self, dx, k, max_iter=10, init_mode=0, seed=0, init_labels=None, init_rows=None,
+            convention=CONV_CPU, s_precision=S_SPLIT_BF16, power_tol=1e-12,
+            labels_per_iter=False):
+        n, m = dx.shape
+        o = FitOpts()
+        self.lib.tcx_fit_opts_default(ctypes.byref(o))
+        o.max_iter, o.init_mode, o.seed = max_iter, init_mode, seed
+        o.convention, o.s_precision, o.power_tol = convention, s_precision, power_tol
+        keep = []
+        if init_labels is not None:
+            il = np.ascontiguousarray(init_labels, np.int32)
+            keep.append(il)
+            o.init_labels = il.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        if init_rows is not None:
+            ir = np.ascontiguousarray(init_rows, np.int32)
+            keep.append(ir)
+            o.init_rows = ir.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        hist = None
+        if labels_per_iter:
+            hist = np.zeros((max_iter, n), np.int32)
+            o.labels_per_iter = hist.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        labels = np.empty(n, np.int32)
+        cent = np.empty((k, m), np.float32)
+        st = FitStats()
+        check(self.lib.tcx_kshape_fit(self.h, dx.ptr, n, m, k, ctypes.byref(o), _ptr(labels),
+                                      _ptr(cent), ctypes.byref(st)))
+        res = {"labels": labels, "centroids": cent, "stats": st.as_dict()}
+        if hist is not None:
+            res["labels_per_iter"] = hist[: st.iterations]
+        return res
