@@ -283,9 +283,11 @@ __global__ __launch_bounds__(256) void k_syrk(const SyrkItem* __restrict__ items
 
 // Sum the K-chunk partials of each tile in a fixed order and write the tile
 // and its mirror, so S_j is exactly symmetric (diagonal tiles keep i <= j).
+// diagmax (may be NULL) receives max(0, max_i S_ii) per cluster as float bits.
 __global__ __launch_bounds__(256) void k_syrk_reduce(const SyrkTile* __restrict__ tiles,
                                                      const float* __restrict__ part, int m,
-                                                     float* __restrict__ S) {
+                                                     float* __restrict__ S,
+                                                     unsigned* __restrict__ diagmax) {
   const SyrkTile tl = tiles[blockIdx.x];
   float* Sj = S + (long)tl.cluster * m * m;
   for (int e = threadIdx.x; e < kSyrkTile * kSyrkTile; e += blockDim.x) {
@@ -297,6 +299,7 @@ __global__ __launch_bounds__(256) void k_syrk_reduce(const SyrkTile* __restrict_
     for (int c = 0; c < tl.nparts; ++c) s += part[(long)(tl.part0 + c) * kSyrkTile * kSyrkTile + e];
     Sj[(long)gi * m + gj] = s;
     Sj[(long)gj * m + gi] = s;
+    if (diagmax && gi == gj) atomicMax(&diagmax[tl.cluster], __float_as_uint(s > 0.f ? s : 0.f));
   }
 }
 
@@ -332,12 +335,48 @@ __device__ __forceinline__ int block_min_int(int v, int* scratch) {
   return s;
 }
 
+// Work split of a block-wide symmetric matvec: 64-column strips times
+// q_parts slices of the c range, q_parts = max(1, waves / strips).
+__device__ __forceinline__ int sym_qparts(int m) {
+  const int strips = (m + 63) / 64;
+  const int q = kPowerWaves / strips;
+  return q > 1 ? q : 1;
+}
+
+// y = S x for symmetric S (FP32) with FP64 x (LDS) into y (LDS); `part`
+// holds q_parts * m doubles of LDS.
+__device__ void sym_matvec_block(const float* __restrict__ Sj, int m, const double* x, double* y,
+                                 double* part) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int strips = (m + 63) / 64;
+  const int Q = sym_qparts(m);
+  const int per = (m + Q - 1) / Q;
+  for (int item = wid; item < strips * Q; item += kPowerWaves) {
+    const int st = item % strips, q = item / strips;
+    const int i = st * 64 + lane;
+    const int c0 = q * per, c1 = c0 + per < m ? c0 + per : m;
+    double acc = 0.0;
+    if (i < m) {
+#pragma unroll 8
+      for (int c = c0; c < c1; ++c) acc += (double)Sj[(long)c * m + i] * x[c];
+      part[(long)q * m + i] = acc;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < m; i += kPowerThreads) {
+    double t = 0.0;
+    for (int q = 0; q < Q; ++q) t += part[(long)q * m + i];
+    y[i] = t;
+  }
+  __syncthreads();
+}
+
 // w = M v with M = P S P = S - (r 1^T + 1 r^T)/m + (sigma/m^2) 1 1^T
 // (test_timesC.py:145-149 in rank-1 form; r = S 1, sigma = 1^T r), FP32 S,
 // FP64 v and accumulation.  Returns (|w|^2, v.w).
 __device__ void center_matvec(const float* __restrict__ Sj, int m, const double* v,
-                              const double* r, double sigma, double* w, double* scratch,
-                              double& nw2, double& vw) {
+                              const double* r, double sigma, double* w, double* part,
+                              double* scratch, double& nw2, double& vw) {
   double sv = 0.0, rv = 0.0;
   for (int i = threadIdx.x; i < m; i += kPowerThreads) {
     sv += v[i];
@@ -345,20 +384,14 @@ __device__ void center_matvec(const float* __restrict__ Sj, int m, const double*
   }
   sv = block_sum_f64(sv, scratch);
   rv = block_sum_f64(rv, scratch + kPowerWaves);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  sym_matvec_block(Sj, m, v, w, part);
   const double inv_m = 1.0 / (double)m;
-  for (int i = wid; i < m; i += kPowerWaves) {
-    const float* row = Sj + (long)i * m;
-    double acc = 0.0;
-    for (int c = lane; c < m; c += kWave) acc += (double)row[c] * v[c];
-    acc = wave_sum_f64(acc);
-    if (lane == 0) w[i] = acc - (r[i] * sv + rv) * inv_m + sigma * sv * inv_m * inv_m;
-  }
-  __syncthreads();
   double a = 0.0, b = 0.0;
   for (int i = threadIdx.x; i < m; i += kPowerThreads) {
-    a += w[i] * w[i];
-    b += v[i] * w[i];
+    const double wi = w[i] - (r[i] * sv + rv) * inv_m + sigma * sv * inv_m * inv_m;
+    w[i] = wi;
+    a += wi * wi;
+    b += v[i] * wi;
   }
   nw2 = block_sum_f64(a, scratch + 2 * kPowerWaves);
   vw = block_sum_f64(b, scratch + 3 * kPowerWaves);
@@ -367,12 +400,12 @@ __device__ void center_matvec(const float* __restrict__ Sj, int m, const double*
 // Runs power steps v <- M v / |M v| until the Rayleigh quotient changes by
 // <= tol relative (FP64) or `steps_cap` steps.  Returns 0, or 1 if M v = 0.
 __device__ int power_steps(const float* Sj, int m, double* v, const double* r, double sigma,
-                           double* w, double* scratch, int steps_cap, double tol, int& steps,
-                           int& converged) {
+                           double* w, double* part, double* scratch, int steps_cap, double tol,
+                           int& steps, int& converged) {
   double lam_prev = 0.0;
   for (int t = 0; t < steps_cap; ++t) {
     double nw2, vw;
-    center_matvec(Sj, m, v, r, sigma, w, scratch, nw2, vw);
+    center_matvec(Sj, m, v, r, sigma, w, part, scratch, nw2, vw);
     if (!(nw2 > 0.0)) return 1;
     const double nw = sqrt(nw2);
     for (int i = threadIdx.x; i < m; i += kPowerThreads) v[i] = w[i] / nw;
@@ -410,12 +443,16 @@ __device__ void sign_fix(const float* a0, double* c, double* w, int m, double* s
 // One block per cluster.  Steps of test_timesC.py:152-204: power iteration
 // (start vector = power_v0(seed, iter, cluster, .)), lambda = v.Mv, sign fix
 // by a0, the vecleft/vecright sign test and the signed-sum residual test,
-// then zscore(c, ddof=1).  The eigh fallback keeps iterating the power
-// method (the top eigenvector it converges to is eigh's last column up to
-// sign; the sign fix then picks the same sign).
+// then zscore(c, ddof=1).  With vin given, v = vin is the already iterated
+// vector (from k_power_coop) and no power steps run here.  The eigh
+// fallback keeps iterating the power method (the top eigenvector it
+// converges to is eigh's last column up to sign; the sign fix then picks
+// the same sign).
 __global__ __launch_bounds__(kPowerThreads) void k_power(
     const float* __restrict__ S, int m, const int* __restrict__ counts,
     const float* __restrict__ a0all, uint64_t seed, int iter, int max_steps, double tol,
+    const double* __restrict__ vin, const double* __restrict__ rsum,
+    const int* __restrict__ steps_in, const int* __restrict__ conv_in,
     float* __restrict__ c_out, PowerInfo* __restrict__ info) {
   extern __shared__ double sh[];
   double* v = sh;
@@ -423,6 +460,7 @@ __global__ __launch_bounds__(kPowerThreads) void k_power(
   double* r = sh + 2 * m;
   double* scratch = sh + 3 * m;  // 8 * kPowerWaves doubles
   int* iscratch = reinterpret_cast<int*>(scratch + 6 * kPowerWaves);
+  double* part = scratch + 8 * kPowerWaves;  // sym_qparts(m) * m doubles
   const int j = blockIdx.x;
   const float* Sj = S + (long)j * m * m;
   const float* a0 = a0all + (long)j * m;
@@ -436,31 +474,35 @@ __global__ __launch_bounds__(kPowerThreads) void k_power(
     }
     return;
   }
-  // r = S 1, sigma = 1^T S 1
-  {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int i = wid; i < m; i += kPowerWaves) {
-      const float* row = Sj + (long)i * m;
-      double acc = 0.0;
-      for (int c = lane; c < m; c += kWave) acc += (double)row[c];
-      acc = wave_sum_f64(acc);
-      if (lane == 0) r[i] = acc;
-    }
+  // r = S 1 (column sums = row sums, S symmetric), sigma = 1^T S 1
+  if (rsum) {
+    for (int i = threadIdx.x; i < m; i += kPowerThreads) r[i] = rsum[(long)j * m + i];
+  } else {
+    for (int i = threadIdx.x; i < m; i += kPowerThreads) w[i] = 1.0;
+    __syncthreads();
+    sym_matvec_block(Sj, m, w, r, part);
   }
-  for (int i = threadIdx.x; i < m; i += kPowerThreads) v[i] = power_v0(seed, iter, j, i);
-  __syncthreads();
-  double sigma = 0.0, n0 = 0.0;
+  const bool use_vin = vin && (!conv_in || conv_in[j]);
+  double n0 = 0.0;
   for (int i = threadIdx.x; i < m; i += kPowerThreads) {
-    sigma += r[i];
+    v[i] = use_vin ? vin[(long)j * m + i] : power_v0(seed, iter, j, i);
     n0 += v[i] * v[i];
   }
+  __syncthreads();
+  double sigma = 0.0;
+  for (int i = threadIdx.x; i < m; i += kPowerThreads) sigma += r[i];
   sigma = block_sum_f64(sigma, scratch);
   n0 = sqrt(block_sum_f64(n0, scratch + kPowerWaves));
+  const bool vzero = !(n0 > 0.0);
   for (int i = threadIdx.x; i < m; i += kPowerThreads) v[i] /= n0;
   __syncthreads();
 
-  int steps = 0, converged = 0;
-  if (power_steps(Sj, m, v, r, sigma, w, scratch, max_steps, tol, steps, converged)) {
+  int steps = use_vin ? (steps_in ? steps_in[j] : max_steps) : 0;
+  int converged = use_vin ? 1 : 0;
+  const bool zero_path =
+      vzero || (!use_vin && power_steps(Sj, m, v, r, sigma, w, part, scratch, max_steps, tol,
+                                        steps, converged));
+  if (zero_path) {
     // M v = 0: NumPy's v / 0 gives NaN, zscore(NaN) -> 0 (nan_to_num)
     for (int i = threadIdx.x; i < m; i += kPowerThreads) out[i] = 0.f;
     if (threadIdx.x == 0) {
@@ -471,7 +513,7 @@ __global__ __launch_bounds__(kPowerThreads) void k_power(
     return;
   }
   double nw2, vw;
-  center_matvec(Sj, m, v, r, sigma, w, scratch, nw2, vw);  // w = M v
+  center_matvec(Sj, m, v, r, sigma, w, part, scratch, nw2, vw);  // w = M v
   double vv = 0.0;
   for (int i = threadIdx.x; i < m; i += kPowerThreads) vv += v[i] * v[i];
   vv = block_sum_f64(vv, scratch + 4 * kPowerWaves);
@@ -502,12 +544,11 @@ __global__ __launch_bounds__(kPowerThreads) void k_power(
     if (lambda <= 0.0 || fabs(sd) > (double)m * 0.01) fallback = 1;  // :183
   }
   if (fallback) {  // :184-192
-    if (!converged) {
-      int more = 0;
-      power_steps(Sj, m, v, r, sigma, w, scratch, 8 * max_steps, tol, more, converged);
-      steps += more;
-    }
-    center_matvec(Sj, m, v, r, sigma, w, scratch, nw2, vw);
+    int more = 0, conv2 = 0;
+    power_steps(Sj, m, v, r, sigma, w, part, scratch, 8 * max_steps, tol, more, conv2);
+    steps += more;
+    converged = conv2;
+    center_matvec(Sj, m, v, r, sigma, w, part, scratch, nw2, vw);
     flipped = 0;
     sign_fix(a0, v, w, m, scratch, flipped);
   }
@@ -534,6 +575,178 @@ __global__ __launch_bounds__(kPowerThreads) void k_power(
     inf.fallback = fallback;
     inf.flipped = flipped;
     info[j] = inf;
+  }
+}
+
+// ----------------------------------------------------- Lanczos power result
+// The power iterate v_I = M^I v0 / |M^I v0| (test_timesC.py:30-38 with
+// I = min(m, 1000)) equals, in exact arithmetic, Q_j T_j^I e1 (normalized)
+// once the Krylov space K_j(M, v0) holds every eigen-direction whose weight
+// (lambda_i / lambda_1)^I is not negligible: Q_j = [q_1..q_j] orthonormal
+// Lanczos basis, T_j = Q_j^T M Q_j tridiagonal.  Lanczos with full
+// reorthogonalization in FP64 (one block per cluster, Q in global memory);
+// y = T_j^I e1 by I steps of the tridiagonal recurrence in one wave (FP64,
+// T scaled by its Gershgorin bound, renormalized every 32 steps).  Every 16
+// steps y is compared with the previous check; the result is accepted when
+// |y_j - y_prev| <= lz_tol or the Krylov space became invariant (breakdown).
+// Clusters that do not converge within kLzMax steps get conv_out = 0 and the
+// caller runs the plain power iteration for them.
+constexpr int kLzMax = 128;
+
+// y = T^I e1 / |T^I e1| for T = tridiag(beta[1..len-1], alpha[0..len-1],
+// beta[1..len-1]), len <= 128; executed by wave 0 only (lanes hold entries
+// lane and lane + 64).  Writes y[0..len) (zeros if T^I e1 = 0).
+__device__ void tridiag_power_wave(const double* alpha, const double* beta, int len, int I,
+                                   double* y) {
+  const int l = threadIdx.x & 63;
+  const int e0 = l, e1 = l + 64;
+  double g = 0.0;  // Gershgorin bound of T
+  for (int e = l; e < len; e += 64) {
+    double r = fabs(alpha[e]) + (e > 0 ? fabs(beta[e]) : 0.0) + (e + 1 < len ? fabs(beta[e + 1]) : 0.0);
+    g = r > g ? r : g;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double og = __shfl_xor(g, o, 64);
+    g = og > g ? og : g;
+  }
+  const double sc = g > 0.0 ? 1.0 / g : 0.0;
+  const double a0 = e0 < len ? alpha[e0] * sc : 0.0;
+  const double a1 = e1 < len ? alpha[e1] * sc : 0.0;
+  const double bl0 = (e0 > 0 && e0 < len) ? beta[e0] * sc : 0.0;      // T[e0][e0-1]
+  const double bl1 = (e1 < len) ? beta[e1] * sc : 0.0;                // T[e1][e1-1]
+  const double bu0 = (e0 + 1 < len) ? beta[e0 + 1] * sc : 0.0;        // T[e0][e0+1]
+  const double bu1 = (e1 + 1 < len) ? beta[e1 + 1] * sc : 0.0;        // T[e1][e1+1]
+  double y0 = (e0 == 0) ? 1.0 : 0.0, y1 = 0.0;
+  for (int s = 0; s < I; ++s) {
+    double p0 = __shfl_up(y0, 1, 64), p1 = __shfl_up(y1, 1, 64);
+    double n0 = __shfl_down(y0, 1, 64), n1 = __shfl_down(y1, 1, 64);
+    const double y63 = __shfl(y0, 63, 64), y64 = __shfl(y1, 0, 64);
+    if (l == 0) { p0 = 0.0; p1 = y63; }
+    if (l == 63) { n0 = y64; n1 = 0.0; }
+    const double z0 = bl0 * p0 + a0 * y0 + bu0 * n0;
+    const double z1 = bl1 * p1 + a1 * y1 + bu1 * n1;
+    y0 = z0;
+    y1 = z1;
+    if ((s & 31) == 31 || s == I - 1) {
+      double mx = fmax(fabs(y0), fabs(y1));
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+      if (mx > 0.0) { y0 /= mx; y1 /= mx; }
+    }
+  }
+  double nn = y0 * y0 + y1 * y1;
+  nn = wave_sum_f64(nn);
+  const double inv = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
+  if (e0 < len) y[e0] = y0 * inv;
+  if (e1 < len) y[e1] = y1 * inv;
+}
+
+__global__ __launch_bounds__(kPowerThreads) void k_power_lanczos(
+    const float* __restrict__ S, int m, const int* __restrict__ counts, uint64_t seed, int iter,
+    int I, double lz_tol, double* __restrict__ Qg, double* __restrict__ vout,
+    double* __restrict__ rout, int* __restrict__ steps_out, int* __restrict__ conv_out) {
+  extern __shared__ double lz_sh[];
+  double* q = lz_sh;                  // m
+  double* w = q + m;                  // m
+  double* r = w + m;                  // m
+  double* scratch = r + m;            // 8 * kPowerWaves
+  double* alpha = scratch + 8 * kPowerWaves;  // kLzMax + 1
+  double* beta = alpha + kLzMax + 1;  // kLzMax + 1
+  double* cf = beta + kLzMax + 1;     // kLzMax + 1
+  double* y = cf + kLzMax + 1;        // kLzMax
+  double* yp = y + kLzMax;            // kLzMax
+  int* flag = reinterpret_cast<int*>(yp + kLzMax);  // 2 ints (padded to 1 double)
+  double* part = yp + kLzMax + 1;     // sym_qparts(m) * m
+  const int j = blockIdx.x;
+  if (counts[j] == 0) return;
+  const float* Sj = S + (long)j * m * m;
+  double* Q = Qg + (long)j * (kLzMax + 1) * m;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < m; i += kPowerThreads) w[i] = 1.0;
+  __syncthreads();
+  sym_matvec_block(Sj, m, w, r, part);  // r = S 1
+  double sg = 0.0, n0 = 0.0;
+  for (int i = threadIdx.x; i < m; i += kPowerThreads) {
+    sg += r[i];
+    q[i] = power_v0(seed, iter, j, i);
+    n0 += q[i] * q[i];
+    rout[(long)j * m + i] = r[i];
+  }
+  const double sigma = block_sum_f64(sg, scratch);
+  n0 = sqrt(block_sum_f64(n0, scratch + kPowerWaves));
+  for (int i = threadIdx.x; i < m; i += kPowerThreads) {
+    q[i] /= n0;
+    Q[i] = q[i];
+  }
+  if (threadIdx.x == 0) beta[0] = 0.0;
+  __syncthreads();
+  const int jmax = m < kLzMax ? m : kLzMax;
+  int len = 0, conv = 0, prev_len = 0;
+  for (int it = 0; it < jmax; ++it) {
+    double nw2, qw;
+    center_matvec(Sj, m, q, r, sigma, w, part, scratch, nw2, qw);  // w = M q_it
+    if (threadIdx.x == 0) alpha[it] = qw;
+    // full reorthogonalization against q_0..q_it, classical Gram-Schmidt twice
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int l = wid; l <= it; l += kPowerWaves) {
+        const double* ql = Q + (long)l * m;
+        double acc = 0.0;
+        for (int e = lane; e < m; e += kWave) acc += ql[e] * w[e];
+        acc = wave_sum_f64(acc);
+        if (lane == 0) cf[l] = acc;
+      }
+      __syncthreads();
+      for (int e = threadIdx.x; e < m; e += kPowerThreads) {
+        double acc = w[e];
+        for (int l = 0; l <= it; ++l) acc -= cf[l] * Q[(long)l * m + e];
+        w[e] = acc;
+      }
+      __syncthreads();
+    }
+    double bb = 0.0;
+    for (int e = threadIdx.x; e < m; e += kPowerThreads) bb += w[e] * w[e];
+    const double b = sqrt(block_sum_f64(bb, scratch + 2 * kPowerWaves));
+    len = it + 1;
+    const bool breakdown = !(b > 1e-12 * sqrt(nw2));
+    if (breakdown || len == jmax || (len >= 16 && (len & 15) == 0)) {
+      if (wid == 0) {
+        tridiag_power_wave(alpha, beta, len, I, y);
+        if (lane == 0) flag[0] = 0;
+        double d = 0.0;
+        for (int e = lane; e < len; e += 64) {
+          const double pv = e < prev_len ? yp[e] : 0.0;  // previous y, zero padded
+          const double dd = y[e] - pv;
+          d += dd * dd;
+        }
+        d = wave_sum_f64(d);
+        if (lane == 0) flag[0] = breakdown || (prev_len > 0 && sqrt(d) <= lz_tol);
+        for (int e = lane; e < len; e += 64) yp[e] = y[e];
+      }
+      prev_len = len;
+      __syncthreads();
+      conv = flag[0];
+      __syncthreads();
+      if (conv || breakdown) break;
+    }
+    if (len == jmax) break;
+    if (threadIdx.x == 0) beta[it + 1] = b;
+    double* qn = Q + (long)(it + 1) * m;
+    for (int e = threadIdx.x; e < m; e += kPowerThreads) {
+      q[e] = w[e] / b;
+      qn[e] = q[e];
+    }
+    __syncthreads();
+  }
+  // v = Q y
+  for (int e = threadIdx.x; e < m; e += kPowerThreads) {
+    double acc = 0.0;
+    for (int l = 0; l < len; ++l) acc += y[l] * Q[(long)l * m + e];
+    vout[(long)j * m + e] = conv ? acc : 0.0;
+  }
+  if (threadIdx.x == 0) {
+    steps_out[j] = len;
+    conv_out[j] = conv;
   }
 }
 
@@ -589,27 +802,49 @@ hipError_t launch_syrk(const SyrkItem* items, int nitems, const uint16_t* yhi,
 }
 
 hipError_t launch_syrk_reduce(const SyrkTile* tiles, int ntiles, const float* part, int m,
-                              float* S, hipStream_t st) {
+                              float* S, unsigned* diagmax, hipStream_t st) {
   if (ntiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_syrk_reduce, dim3(ntiles), dim3(256), 0, st, tiles, part, m, S);
+  hipLaunchKernelGGL(k_syrk_reduce, dim3(ntiles), dim3(256), 0, st, tiles, part, m, S, diagmax);
   return hipGetLastError();
 }
 
 static size_t power_smem(int m) {
-  return sizeof(double) * (3 * (size_t)m + 8 * kPowerWaves) + sizeof(int) * kPowerWaves;
+  const int strips = (m + 63) / 64;
+  const int q = kPowerWaves / strips > 1 ? kPowerWaves / strips : 1;
+  return sizeof(double) * ((3 + (size_t)q) * m + 8 * kPowerWaves);
 }
 
 hipError_t launch_power(const float* S, int m, int k, const int* counts, const float* a0,
-                        uint64_t seed, int iter, int max_steps, double tol, float* c_out,
-                        PowerInfo* info, hipStream_t st) {
+                        uint64_t seed, int iter, int max_steps, double tol, const double* vin,
+                        const double* rsum, const int* steps_in, const int* conv_in,
+                        float* c_out, PowerInfo* info, hipStream_t st) {
   hipLaunchKernelGGL(k_power, dim3(k), dim3(kPowerThreads), power_smem(m), st, S, m, counts,
-                     a0, seed, iter, max_steps, tol, c_out, info);
+                     a0, seed, iter, max_steps, tol, vin, rsum, steps_in, conv_in, c_out, info);
+  return hipGetLastError();
+}
+
+static size_t lanczos_smem(int m) {
+  const int strips = (m + 63) / 64;
+  const int q = kPowerWaves / strips > 1 ? kPowerWaves / strips : 1;
+  return sizeof(double) * ((3 + (size_t)q) * m + 8 * kPowerWaves + 5 * (kLzMax + 1));
+}
+
+hipError_t launch_power_lanczos(const float* S, int m, int k, const int* counts, uint64_t seed,
+                                int iter, int I, double lz_tol, double* Q, double* vout,
+                                double* rout, int* steps_out, int* conv_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_power_lanczos, dim3(k), dim3(kPowerThreads), lanczos_smem(m), st, S, m,
+                     counts, seed, iter, I, lz_tol, Q, vout, rout, steps_out, conv_out);
   return hipGetLastError();
 }
 
 hipError_t configure_extract_kernels() {
-  return hipFuncSetAttribute((const void*)k_power, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)power_smem(4096));
+  hipError_t e = hipFuncSetAttribute((const void*)k_power,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)power_smem(4096));
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_power_lanczos,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lanczos_smem(4096));
+  return e;
 }
 
 }  // namespace tcx

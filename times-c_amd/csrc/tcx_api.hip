@@ -48,7 +48,8 @@ enum Slot {
   kSlotCspec, kSlotCnorm, kSlotCzero, kSlotLabel, kSlotShift, kSlotDist, kSlotCent,
   kSlotCentNew, kSlotChanged, kSlotBlkCount, kSlotBlkBase, kSlotCounts, kSlotOffsets,
   kSlotPerm, kSlotBad, kSlotYhi, kSlotYlo, kSlotA0, kSlotItems, kSlotTiles, kSlotPart,
-  kSlotS, kSlotInfo, kSlotDummy, kNumSlots
+  kSlotS, kSlotInfo, kSlotDummy, kSlotWbuf, kSlotRbuf, kSlotBar, kSlotVout, kSlotSteps,
+  kSlotConv, kSlotErr, kNumSlots
 };
 
 struct FitState {
@@ -81,6 +82,8 @@ struct tcx_ctx {
   // host copies kept alive until the stream has consumed them
   std::vector<tcx::SyrkItem> items;
   std::vector<tcx::SyrkTile> tiles;
+  int cus = 0;                 // compute units of the device
+  long long wall_khz = 0;      // wall_clock64() rate
 };
 
 namespace {
@@ -190,11 +193,66 @@ int assign_core(tcx_ctx* c, const float* d_x, long n, int m, int k, const float2
   return TCX_OK;
 }
 
+// Work list of a grouped S = Y^T Y: for each cluster with seg_len[j] > 0
+// columns starting at off[j], the upper triangle of ceil(m/128)^2 tiles,
+// each split along K so that the grid has at least ~2 blocks per CU.
+void plan_syrk(int m, int k, const long* off, const long* seg_len,
+               std::vector<tcx::SyrkItem>& items, std::vector<tcx::SyrkTile>& tiles) {
+  const int T = (m + tcx::kSyrkTile - 1) / tcx::kSyrkTile;
+  const int tiles_per = T * (T + 1) / 2;
+  int nonempty = 0;
+  for (int j = 0; j < k; ++j) nonempty += seg_len[j] > 0;
+  const long total_tiles = (long)tiles_per * nonempty;
+  const long want = 512;
+  const long split_f =
+      total_tiles >= want ? 1 : (want + total_tiles - 1) / std::max(1L, total_tiles);
+  items.clear();
+  tiles.clear();
+  for (int j = 0; j < k; ++j) {
+    const long K = seg_len[j];
+    if (K == 0) continue;
+    long chunk = (K + split_f - 1) / split_f;
+    chunk = (chunk + tcx::kPackPad - 1) / tcx::kPackPad * tcx::kPackPad;
+    for (int ti = 0; ti < T; ++ti)
+      for (int tj = ti; tj < T; ++tj) {
+        tcx::SyrkTile tl{j, ti, tj, (int)items.size(), 0};
+        for (long p = 0; p < K; p += chunk) {
+          tcx::SyrkItem it{j, ti, tj, (int)items.size(), off[j] + p, off[j] + std::min(K, p + chunk)};
+          items.push_back(it);
+          ++tl.nparts;
+        }
+        tiles.push_back(tl);
+      }
+  }
+}
+
 struct RefineResult {
   int sum_class = 0;
   int steps_max = 0;
   int fallbacks = 0;
 };
+
+// M^I v0 of every non-empty cluster by Lanczos (k_power_lanczos).  Returns
+// the vectors, the row sums r = S 1, the Lanczos step counts and the
+// per-cluster acceptance flags on device.
+int power_lanczos(tcx_ctx* c, const float* S, int m, int k, const int* counts, uint64_t seed,
+                  int iter, int steps, const double** vout, const double** rout,
+                  const int** steps_out, const int** conv_out) {
+  double *Q, *v, *r;
+  int *st_out, *cv_out;
+  TCX_TRY(ws(c, kSlotWbuf, (size_t)k * (tcx::kLanczosMax + 1) * m, &Q));
+  TCX_TRY(ws(c, kSlotVout, (size_t)k * m, &v));
+  TCX_TRY(ws(c, kSlotRbuf, (size_t)k * m, &r));
+  TCX_TRY(ws(c, kSlotSteps, (size_t)k, &st_out));
+  TCX_TRY(ws(c, kSlotConv, (size_t)k, &cv_out));
+  TCX_HIP(tcx::launch_power_lanczos(S, m, k, counts, seed, iter, steps, 1e-11, Q, v, r, st_out,
+                                    cv_out, c->stream));
+  *vout = v;
+  *rout = r;
+  *steps_out = st_out;
+  *conv_out = cv_out;
+  return TCX_OK;
+}
 
 // One refinement: group, align + pack, S, power iteration.  Reads the
 // labels, shifts and old-centroid zero flags; writes k*m new centroids.
@@ -232,34 +290,11 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
   TCX_TRY(ws(c, kSlotA0, (size_t)k * m, &a0));
   TCX_HIP(tcx::launch_align_pack(d_x, m, perm, n_pad, offsets, k, d_shift, d_czero, yhi, ylo,
                                  a0, st));
-  // S tiles: upper triangle of ceil(m/128)^2 tiles per non-empty cluster,
-  // K split so that the grid has at least ~2 blocks per CU.
-  const int T = (m + tcx::kSyrkTile - 1) / tcx::kSyrkTile;
-  const int tiles_per = T * (T + 1) / 2;
   int nonempty = 0;
   for (int j = 0; j < k; ++j) nonempty += h_counts[j] > 0;
-  const long total_tiles = (long)tiles_per * nonempty;
-  const long want = 512;
-  const long split_f = total_tiles >= want ? 1 : (want + total_tiles - 1) / std::max(1L, total_tiles);
-  c->items.clear();
-  c->tiles.clear();
-  for (int j = 0; j < k; ++j) {
-    if (h_counts[j] == 0) continue;
-    const long K = h_off[j + 1] - h_off[j];
-    long chunk = (K + split_f - 1) / split_f;
-    chunk = (chunk + tcx::kPackPad - 1) / tcx::kPackPad * tcx::kPackPad;
-    for (int ti = 0; ti < T; ++ti)
-      for (int tj = ti; tj < T; ++tj) {
-        tcx::SyrkTile tl{j, ti, tj, (int)c->items.size(), 0};
-        for (long p = 0; p < K; p += chunk) {
-          tcx::SyrkItem it{j, ti, tj, (int)c->items.size(), h_off[j] + p,
-                           h_off[j] + std::min(K, p + chunk)};
-          c->items.push_back(it);
-          ++tl.nparts;
-        }
-        c->tiles.push_back(tl);
-      }
-  }
+  std::vector<long> seg_len(k);
+  for (int j = 0; j < k; ++j) seg_len[j] = h_counts[j] > 0 ? h_off[j + 1] - h_off[j] : 0;
+  plan_syrk(m, k, h_off.data(), seg_len.data(), c->items, c->tiles);
   tcx::SyrkItem* d_items;
   tcx::SyrkTile* d_tiles;
   float *part, *S;
@@ -276,9 +311,17 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
                            hipMemcpyHostToDevice, st));
   }
   TCX_HIP(tcx::launch_syrk(d_items, (int)c->items.size(), yhi, ylo, n_pad, m, split, part, st));
-  TCX_HIP(tcx::launch_syrk_reduce(d_tiles, (int)c->tiles.size(), part, m, S, st));
+  TCX_HIP(tcx::launch_syrk_reduce(d_tiles, (int)c->tiles.size(), part, m, S, nullptr, st));
   const int max_steps = m > 1000 ? 1000 : m;  // test_timesC.py:152-156
-  TCX_HIP(tcx::launch_power(S, m, k, counts, a0, seed, iter, max_steps, tol, d_cout, info, st));
+  const double* vin = nullptr;
+  const double* rsum = nullptr;
+  const int* steps_in = nullptr;
+  const int* conv_in = nullptr;
+  if (nonempty > 0)
+    TCX_TRY(power_lanczos(c, S, m, k, counts, seed, iter, max_steps, &vin, &rsum, &steps_in,
+                          &conv_in));
+  TCX_HIP(tcx::launch_power(S, m, k, counts, a0, seed, iter, max_steps, tol, vin, rsum, steps_in,
+                            conv_in, d_cout, info, st));
   std::vector<tcx::PowerInfo> h_info(k);
   TCX_HIP(hipMemcpyAsync(h_info.data(), info, sizeof(tcx::PowerInfo) * k, hipMemcpyDeviceToHost, st));
   TCX_HIP(hipStreamSynchronize(st));
@@ -314,6 +357,14 @@ int tcx_create(int device, size_t hbm_budget, tcx_ctx** out) {
   (void)hipGetLastError();
   tcx_ctx* c = new tcx_ctx();
   c->device = device;
+  int cus = 0, wall = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      hipDeviceGetAttribute(&wall, hipDeviceAttributeWallClockRate, device) != hipSuccess) {
+    delete c;
+    return fail(TCX_ERR_HIP, "device attribute query failed");
+  }
+  c->cus = cus;
+  c->wall_khz = wall > 0 ? wall : 100000;
   c->budget = hbm_budget;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;

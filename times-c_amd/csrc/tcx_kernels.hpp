@@ -80,10 +80,17 @@ hipError_t launch_syrk(const SyrkItem* items, int nitems, const uint16_t* yhi,
                        const uint16_t* ylo, long n_pad, int m, int split, float* part,
                        hipStream_t st);
 hipError_t launch_syrk_reduce(const SyrkTile* tiles, int ntiles, const float* part, int m,
-                              float* S, hipStream_t st);
+                              float* S, unsigned* diagmax, hipStream_t st);
 hipError_t launch_power(const float* S, int m, int k, const int* counts, const float* a0,
-                        uint64_t seed, int iter, int max_steps, double tol, float* c_out,
-                        PowerInfo* info, hipStream_t st);
+                        uint64_t seed, int iter, int max_steps, double tol, const double* vin,
+                        const double* rsum, const int* steps_in, const int* conv_in,
+                        float* c_out, PowerInfo* info, hipStream_t st);
+
+// Lanczos evaluation of the power iterate M^I v0 (one block per cluster).
+constexpr int kLanczosMax = 128;
+hipError_t launch_power_lanczos(const float* S, int m, int k, const int* counts, uint64_t seed,
+                                int iter, int I, double lz_tol, double* Q, double* vout,
+                                double* rout, int* steps_out, int* conv_out, hipStream_t st);
 hipError_t configure_extract_kernels();
 
 }  // namespace tcx
