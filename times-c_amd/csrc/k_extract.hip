@@ -343,10 +343,52 @@ __device__ __forceinline__ int sym_qparts(int m) {
   return q > 1 ? q : 1;
 }
 
-// y = S x for symmetric S (FP32) with FP64 x (LDS) into y (LDS); `part`
-// holds q_parts * m doubles of LDS.
+// LDS doubles of `part`: sym_matvec_block needs at most 4096 (4 * 1024
+// float4 partials, or sym_qparts(m) * m <= 4096 for m <= 4096), the
+// Lanczos projection max(1024, m) <= 4096.
+__host__ __device__ constexpr int sym_part_len(int) { return 4096; }
+
+// y = S x for symmetric S (FP32) with FP64 x (LDS) into y (LDS): thread
+// <-> output, so every load is a coalesced row segment of S and x[c] a
+// broadcast.  m % 4 == 0: 16-byte loads of 4 outputs per thread, the c
+// range split over 1024 / (m/4) thread groups.  `part` holds
+// sym_part_len(m) doubles of LDS.
 __device__ void sym_matvec_block(const float* __restrict__ Sj, int m, const double* x, double* y,
                                  double* part) {
+  if (m % 4 == 0) {
+    const int nv = m / 4;  // float4 per row, <= 1024
+    const int P = kPowerThreads / nv > 0 ? kPowerThreads / nv : 1;
+    const int t = threadIdx.x;
+    const int g = t % nv, p = t / nv;
+    if (p < P) {
+      const int per = (m + P - 1) / P;
+      const int c0 = p * per, c1 = c0 + per < m ? c0 + per : m;
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+      const float4* col = reinterpret_cast<const float4*>(Sj) + g;
+#pragma unroll 8
+      for (int c = c0; c < c1; ++c) {
+        const float4 sv = col[(long)c * nv];
+        const double xc = x[c];
+        a0 += (double)sv.x * xc;
+        a1 += (double)sv.y * xc;
+        a2 += (double)sv.z * xc;
+        a3 += (double)sv.w * xc;
+      }
+      double* pp = part + (long)p * m + 4 * g;
+      pp[0] = a0;
+      pp[1] = a1;
+      pp[2] = a2;
+      pp[3] = a3;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += kPowerThreads) {
+      double s = 0.0;
+      for (int q = 0; q < P; ++q) s += part[(long)q * m + i];
+      y[i] = s;
+    }
+    __syncthreads();
+    return;
+  }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int strips = (m + 63) / 64;
   const int Q = sym_qparts(m);
@@ -460,7 +502,7 @@ __global__ __launch_bounds__(kPowerThreads) void k_power(
   double* r = sh + 2 * m;
   double* scratch = sh + 3 * m;  // 8 * kPowerWaves doubles
   int* iscratch = reinterpret_cast<int*>(scratch + 6 * kPowerWaves);
-  double* part = scratch + 8 * kPowerWaves;  // sym_qparts(m) * m doubles
+  double* part = scratch + 8 * kPowerWaves;  // sym_part_len(m) doubles
   const int j = blockIdx.x;
   const float* Sj = S + (long)j * m * m;
   const float* a0 = a0all + (long)j * m;
@@ -650,14 +692,15 @@ __global__ __launch_bounds__(kPowerThreads) void k_power_lanczos(
   double* q = lz_sh;                  // m
   double* w = q + m;                  // m
   double* r = w + m;                  // m
-  double* scratch = r + m;            // 8 * kPowerWaves
+  double* qp = r + m;                 // m: previous Lanczos vector
+  double* scratch = qp + m;           // 8 * kPowerWaves
   double* alpha = scratch + 8 * kPowerWaves;  // kLzMax + 1
   double* beta = alpha + kLzMax + 1;  // kLzMax + 1
   double* cf = beta + kLzMax + 1;     // kLzMax + 1
   double* y = cf + kLzMax + 1;        // kLzMax
   double* yp = y + kLzMax;            // kLzMax
   int* flag = reinterpret_cast<int*>(yp + kLzMax);  // 2 ints (padded to 1 double)
-  double* part = yp + kLzMax + 1;     // sym_qparts(m) * m
+  double* part = yp + kLzMax + 1;     // sym_part_len(m)
   const int j = blockIdx.x;
   if (counts[j] == 0) return;
   const float* Sj = S + (long)j * m * m;
@@ -670,6 +713,7 @@ __global__ __launch_bounds__(kPowerThreads) void k_power_lanczos(
   for (int i = threadIdx.x; i < m; i += kPowerThreads) {
     sg += r[i];
     q[i] = power_v0(seed, iter, j, i);
+    qp[i] = 0.0;
     n0 += q[i] * q[i];
     rout[(long)j * m + i] = r[i];
   }
@@ -682,37 +726,73 @@ __global__ __launch_bounds__(kPowerThreads) void k_power_lanczos(
   if (threadIdx.x == 0) beta[0] = 0.0;
   __syncthreads();
   const int jmax = m < kLzMax ? m : kLzMax;
+  // projection split: H groups of threads share the l range of w -= Q c
+  const int H = m <= kPowerThreads ? kPowerThreads / m : 1;
   int len = 0, conv = 0, prev_len = 0;
   for (int it = 0; it < jmax; ++it) {
     double nw2, qw;
     center_matvec(Sj, m, q, r, sigma, w, part, scratch, nw2, qw);  // w = M q_it
-    if (threadIdx.x == 0) alpha[it] = qw;
-    // full reorthogonalization against q_0..q_it, classical Gram-Schmidt twice
+    const double bprev = beta[it];
+    // three-term recurrence, then full reorthogonalization (classical
+    // Gram-Schmidt against q_0..q_it, repeated when it removed more than
+    // 1 - 1/sqrt(2) of the norm: Daniel-Gragg-Kaufman-Stewart)
+    double w1 = 0.0;
+    for (int e = threadIdx.x; e < m; e += kPowerThreads) {
+      const double x = w[e] - qw * q[e] - bprev * qp[e];
+      w[e] = x;
+      w1 += x * x;
+    }
+    w1 = block_sum_f64(w1, scratch + kPowerWaves);
+    double b = sqrt(w1);
     for (int pass = 0; pass < 2; ++pass) {
       for (int l = wid; l <= it; l += kPowerWaves) {
         const double* ql = Q + (long)l * m;
-        double acc = 0.0;
-        for (int e = lane; e < m; e += kWave) acc += ql[e] * w[e];
-        acc = wave_sum_f64(acc);
+        double a0 = 0.0, a1 = 0.0;
+        int e = lane;
+        for (; e + 64 < m; e += 128) {
+          a0 += ql[e] * w[e];
+          a1 += ql[e + 64] * w[e + 64];
+        }
+        if (e < m) a0 += ql[e] * w[e];
+        const double acc = wave_sum_f64(a0 + a1);
         if (lane == 0) cf[l] = acc;
       }
       __syncthreads();
-      for (int e = threadIdx.x; e < m; e += kPowerThreads) {
-        double acc = w[e];
-        for (int l = 0; l <= it; ++l) acc -= cf[l] * Q[(long)l * m + e];
-        w[e] = acc;
+      for (int t = threadIdx.x; t < m * H; t += kPowerThreads) {
+        const int e = t % m, g = t / m;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int l = g;
+        for (; l + 3 * H <= it; l += 4 * H) {
+          a0 += cf[l] * Q[(long)l * m + e];
+          a1 += cf[l + H] * Q[(long)(l + H) * m + e];
+          a2 += cf[l + 2 * H] * Q[(long)(l + 2 * H) * m + e];
+          a3 += cf[l + 3 * H] * Q[(long)(l + 3 * H) * m + e];
+        }
+        for (; l <= it; l += H) a0 += cf[l] * Q[(long)l * m + e];
+        part[(long)g * m + e] = (a0 + a1) + (a2 + a3);
       }
       __syncthreads();
+      double w2 = 0.0;
+      for (int e = threadIdx.x; e < m; e += kPowerThreads) {
+        double s2 = 0.0;
+        for (int g = 0; g < H; ++g) s2 += part[(long)g * m + e];
+        const double x = w[e] - s2;
+        w[e] = x;
+        w2 += x * x;
+      }
+      w2 = block_sum_f64(w2, scratch + 2 * kPowerWaves);
+      const bool again = w2 < 0.5 * w1;  // |w| fell below |w1| / sqrt(2)
+      b = sqrt(w2);
+      w1 = w2;
+      if (!again) break;
     }
-    double bb = 0.0;
-    for (int e = threadIdx.x; e < m; e += kPowerThreads) bb += w[e] * w[e];
-    const double b = sqrt(block_sum_f64(bb, scratch + 2 * kPowerWaves));
+    if (threadIdx.x == 0) alpha[it] = qw;
     len = it + 1;
     const bool breakdown = !(b > 1e-12 * sqrt(nw2));
     if (breakdown || len == jmax || (len >= 16 && (len & 15) == 0)) {
+      __syncthreads();
       if (wid == 0) {
         tridiag_power_wave(alpha, beta, len, I, y);
-        if (lane == 0) flag[0] = 0;
         double d = 0.0;
         for (int e = lane; e < len; e += 64) {
           const double pv = e < prev_len ? yp[e] : 0.0;  // previous y, zero padded
@@ -733,6 +813,7 @@ __global__ __launch_bounds__(kPowerThreads) void k_power_lanczos(
     if (threadIdx.x == 0) beta[it + 1] = b;
     double* qn = Q + (long)(it + 1) * m;
     for (int e = threadIdx.x; e < m; e += kPowerThreads) {
+      qp[e] = q[e];
       q[e] = w[e] / b;
       qn[e] = q[e];
     }
@@ -809,9 +890,7 @@ hipError_t launch_syrk_reduce(const SyrkTile* tiles, int ntiles, const float* pa
 }
 
 static size_t power_smem(int m) {
-  const int strips = (m + 63) / 64;
-  const int q = kPowerWaves / strips > 1 ? kPowerWaves / strips : 1;
-  return sizeof(double) * ((3 + (size_t)q) * m + 8 * kPowerWaves);
+  return sizeof(double) * (3 * (size_t)m + 8 * kPowerWaves + sym_part_len(m));
 }
 
 hipError_t launch_power(const float* S, int m, int k, const int* counts, const float* a0,
@@ -824,9 +903,7 @@ hipError_t launch_power(const float* S, int m, int k, const int* counts, const f
 }
 
 static size_t lanczos_smem(int m) {
-  const int strips = (m + 63) / 64;
-  const int q = kPowerWaves / strips > 1 ? kPowerWaves / strips : 1;
-  return sizeof(double) * ((3 + (size_t)q) * m + 8 * kPowerWaves + 5 * (kLzMax + 1));
+  return sizeof(double) * (4 * (size_t)m + 8 * kPowerWaves + 5 * (kLzMax + 1) + sym_part_len(m));
 }
 
 hipError_t launch_power_lanczos(const float* S, int m, int k, const int* counts, uint64_t seed,
