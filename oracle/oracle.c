@@ -5,6 +5,7 @@
  *
  * What pins each function (tests/test_oracle.py):
  *   orc_fft            numpy.fft.fft / ifft on random inputs, all F in 2..16384
+ *   orc_sum_pairwise   bit-exact against numpy's float64 sum
  *   orc_zscore         closed form mean/std (numpy), constant rows -> 0
  *   orc_roll_zeropad   closed form out[t] = a[t-s], |s| >= m -> zeros
  *   orc_ncc            O(m^2) direct correlation; ncc(x,x)[m-1] = 1; |ncc| <= 1;
@@ -146,6 +147,29 @@ void orc_roll_zeropad(const double* a, int m, int shift, double* out) {
     long src = (long)t - shift;
     out[t] = (src >= 0 && src < m) ? a[src] : 0.0;
   }
+}
+
+/* NumPy's float64 add.reduce of a contiguous array (pairwise summation,
+ * blocks of 8 accumulators up to 128 elements, halves above). */
+double orc_sum_pairwise(const double* a, long n) {
+  if (n < 8) {
+    double r = 0.0;
+    for (long i = 0; i < n; ++i) r += a[i];
+    return r;
+  }
+  if (n <= 128) {
+    double r[8];
+    for (int j = 0; j < 8; ++j) r[j] = a[j];
+    long i = 8;
+    for (; i < n - (n % 8); i += 8)
+      for (int j = 0; j < 8; ++j) r[j] += a[i + j];
+    double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    for (; i < n; ++i) res += a[i];
+    return res;
+  }
+  long n2 = n / 2;
+  n2 -= n2 % 8;
+  return orc_sum_pairwise(a, n2) + orc_sum_pairwise(a + n2, n - n2);
 }
 
 static double norm2(const double* a, int m) {
@@ -515,6 +539,7 @@ int orc_extract_shape(const double* A, long count, int m, const double* v0,
                       int exact_steps, double* centroid, orc_shape_info* info) {
   orc_shape_info inf;
   memset(&inf, 0, sizeof(inf));
+  if (m < 1) return 1;
   if (count == 0) { /* test_timesC.py:139-140 */
     memset(centroid, 0, sizeof(double) * m);
     inf.empty = 1;
@@ -623,7 +648,7 @@ int orc_extract_shape(const double* A, long count, int m, const double* v0,
 void orc_refine(const double* X, long n, int m, int k, const int* labels,
                 double* C, int convention, uint64_t seed, int iter,
                 int exact_steps, int* counts, orc_shape_info* infos,
-                int nthreads) {
+                int nthreads, int skip_rule) {
   set_threads(nthreads);
   int F = orc_fft_size(m);
   const double *tc, *ts;
@@ -647,9 +672,17 @@ void orc_refine(const double* X, long n, int m, int k, const int* labels,
     for (long p = 0; p < c; ++p)
       memcpy(A + p * (long)m, X + members[off[j] + p] * (long)m, sizeof(double) * m);
     const double* cj = C + (size_t)j * m;
-    double csum = 0.0; /* cur_center.sum() != 0 (test_timesC.py:128) */
-    for (int t = 0; t < m; ++t) csum += cj[t];
-    if (csum != 0.0 && c > 0) {
+    /* test_timesC.py:128 skips alignment when cur_center.sum() == 0.
+     * ORC_SKIP_ZERO: only an all-zero centroid skips (initial / empty
+     * cluster).  ORC_SKIP_NUMPY_SUM: the literal rule, NumPy's pairwise
+     * float64 sum compared with 0.0. */
+    int align = 0;
+    if (skip_rule == ORC_SKIP_NUMPY_SUM) {
+      align = orc_sum_pairwise(cj, m) != 0.0;
+    } else {
+      for (int t = 0; t < m; ++t) align |= cj[t] != 0.0;
+    }
+    if (align && c > 0) {
 #pragma omp parallel
       {
         double* tmp = (double*)malloc(sizeof(double) * m);
@@ -679,7 +712,7 @@ void orc_refine(const double* X, long n, int m, int k, const int* labels,
 int orc_kshape(const double* X, long n, int m, int k, int init_mode,
                const int* init_labels, const int* init_rows, int max_iter,
                int convention, uint64_t seed, int exact_steps,
-               int* labels_hist, double* centroids, int nthreads) {
+               int* labels_hist, double* centroids, int nthreads, int skip_rule) {
   int* lab = (int*)malloc(sizeof(int) * n);
   int* old = (int*)malloc(sizeof(int) * n);
   double* C = (double*)calloc((size_t)k * m, sizeof(double));
@@ -690,7 +723,7 @@ int orc_kshape(const double* X, long n, int m, int k, int init_mode,
     for (it = 0; it < max_iter; ++it) {
       memcpy(old, lab, sizeof(int) * n);
       orc_refine(X, n, m, k, lab, C, convention, seed, it, exact_steps, NULL,
-                 NULL, nthreads);
+                 NULL, nthreads, skip_rule);
       orc_assign(X, n, m, C, k, lab, NULL, NULL, NULL, nthreads);
       memcpy(labels_hist + (size_t)it * n, lab, sizeof(int) * n);
       if (memcmp(old, lab, sizeof(int) * n) == 0) { ++it; break; }
@@ -706,7 +739,7 @@ int orc_kshape(const double* X, long n, int m, int k, int init_mode,
       if (it > 0 && memcmp(old, lab, sizeof(int) * n) == 0) { ++it; break; }
       memcpy(old, lab, sizeof(int) * n);
       orc_refine(X, n, m, k, lab, C, convention, seed, it, exact_steps, NULL,
-                 NULL, nthreads);
+                 NULL, nthreads, skip_rule);
     }
   }
   if (centroids) memcpy(centroids, C, sizeof(double) * (size_t)k * m);

@@ -28,6 +28,17 @@ extern "C" {
 #define ORC_CONV_CPU 0
 #define ORC_CONV_KSHAPE 1
 
+/* Alignment-skip rules for `cur_center.sum() != 0` (test_timesC.py:128).
+ * ORC_SKIP_ZERO (default): skip only for an all-zero centroid.
+ * ORC_SKIP_NUMPY_SUM: literal -- NumPy's pairwise float64 sum == 0.0, which
+ * also fires for non-zero z-scored centroids whose rounded sum is exactly 0
+ * (about 6% of centroids at m=512, 14% at m=32). */
+#define ORC_SKIP_ZERO 0
+#define ORC_SKIP_NUMPY_SUM 1
+
+/* NumPy's float64 sum of a contiguous array (pairwise summation). */
+double orc_sum_pairwise(const double* a, long n);
+
 typedef struct {
   double lambda;      /* Rayleigh quotient returned by the power iteration   */
   int steps;          /* power-iteration steps actually run                  */
@@ -82,7 +93,7 @@ int orc_extract_shape(const double* A, long count, int m, const double* v0,
 void orc_refine(const double* X, long n, int m, int k, const int* labels,
                 double* C, int convention, uint64_t seed, int iter,
                 int exact_steps, int* counts, orc_shape_info* infos,
-                int nthreads);
+                int nthreads, int skip_rule);
 
 /* Full k-Shape loop (test_timesC.py:213-265) on z-normalized X.
  * init_mode 0: init_labels (n) are the starting labels, centroids start at 0.
@@ -94,7 +105,8 @@ void orc_refine(const double* X, long n, int m, int k, const int* labels,
 int orc_kshape(const double* X, long n, int m, int k, int init_mode,
                const int* init_labels, const int* init_rows, int max_iter,
                int convention, uint64_t seed, int exact_steps,
-               int* labels_hist, double* centroids, int nthreads);
+               int* labels_hist, double* centroids, int nthreads,
+               int skip_rule);
 
 /* Top eigenpair of a symmetric m x m matrix (FP64): Householder
  * tridiagonalization, bisection for the largest eigenvalue, inverse

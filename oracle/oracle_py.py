@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 CONV_CPU, CONV_KSHAPE = 0, 1
+SKIP_ZERO, SKIP_NUMPY_SUM = 0, 1
 
 
 class ShapeInfo(ctypes.Structure):
@@ -50,8 +51,9 @@ def load():
         "orc_assign": (None, [_P, L, I, _P, I, _P, _P, _P, _P, I]),
         "orc_power_v0": (None, [U, I, I, I, _P]),
         "orc_extract_shape": (I, [_P, L, I, _P, I, _P, ctypes.POINTER(ShapeInfo)]),
-        "orc_refine": (None, [_P, L, I, I, _P, _P, I, U, I, I, _P, _P, I]),
-        "orc_kshape": (I, [_P, L, I, I, I, _P, _P, I, I, U, I, _P, _P, I]),
+        "orc_refine": (None, [_P, L, I, I, _P, _P, I, U, I, I, _P, _P, I, I]),
+        "orc_kshape": (I, [_P, L, I, I, I, _P, _P, I, I, U, I, _P, _P, I, I]),
+        "orc_sum_pairwise": (D, [_P, L]),
         "orc_top_eig": (I, [_P, I, ctypes.POINTER(D), _P]),
     }
     for name, (res, args) in sig.items():
@@ -138,7 +140,13 @@ def extract_shape(A, v0, exact_steps=False):
     return out, {name: getattr(info, name) for name, _ in info._fields_}
 
 
-def refine(X, labels, C, convention=CONV_CPU, seed=0, it=0, exact_steps=False, nthreads=0):
+def sum_pairwise(a):
+    a = _f64(a)
+    return load().orc_sum_pairwise(_p(a), a.size)
+
+
+def refine(X, labels, C, convention=CONV_CPU, seed=0, it=0, exact_steps=False, nthreads=0,
+           skip_rule=SKIP_ZERO):
     X = _f64(X)
     n, m = X.shape
     C = _f64(C).copy()
@@ -147,12 +155,12 @@ def refine(X, labels, C, convention=CONV_CPU, seed=0, it=0, exact_steps=False, n
     counts = np.empty(k, np.int32)
     infos = (ShapeInfo * k)()
     load().orc_refine(_p(X), n, m, k, _p(lab), _p(C), convention, seed, it, int(exact_steps),
-                      _p(counts), ctypes.cast(infos, ctypes.c_void_p), nthreads)
+                      _p(counts), ctypes.cast(infos, ctypes.c_void_p), nthreads, skip_rule)
     return C, counts, [{name: getattr(i, name) for name, _ in i._fields_} for i in infos]
 
 
 def kshape(X, k, init_mode=0, init_labels=None, init_rows=None, max_iter=10,
-           convention=CONV_CPU, seed=0, exact_steps=False, nthreads=0):
+           convention=CONV_CPU, seed=0, exact_steps=False, nthreads=0, skip_rule=SKIP_ZERO):
     X = _f64(X)
     n, m = X.shape
     hist = np.zeros((max_iter, n), np.int32)
@@ -160,7 +168,7 @@ def kshape(X, k, init_mode=0, init_labels=None, init_rows=None, max_iter=10,
     il = None if init_labels is None else np.ascontiguousarray(init_labels, np.int32)
     ir = None if init_rows is None else np.ascontiguousarray(init_rows, np.int32)
     it = load().orc_kshape(_p(X), n, m, k, init_mode, _p(il), _p(ir), max_iter, convention,
-                           seed, int(exact_steps), _p(hist), _p(cent), nthreads)
+                           seed, int(exact_steps), _p(hist), _p(cent), nthreads, skip_rule)
     return hist[:it], cent
 
 
