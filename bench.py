@@ -30,13 +30,22 @@ def fft_size(m):
     return 1 << (2 * m - 1).bit_length()
 
 
-def sbd_assign_flops(n, m, k):
-    """Algorithmic FP32 flops of one assignment (FFTW convention 2.5 F log2 F
-    per real transform): forward transform + norm per series, and per pair a
-    conj-multiply of F/2+1 bins, an inverse transform and the 2m-1 lag max."""
+def sbd_assign_flops(n, m, k, evals=None):
+    """FP32 flops of one assignment, FFTW convention (5 F log2 F per complex
+    transform of length F; two real series share one transform).
+
+    Full search: per series a half forward transform and its norm, per
+    (series, centroid) a half inverse transform, the conj-multiply of F/2+1
+    bins and the 2m-1 lag max.  Pruned search (`evals` inverse transforms of
+    series pairs were run): per pair the forward transform, the magnitude
+    split (~12 F) and the bound of every centroid (4 F), per evaluation the
+    conj-multiply (6 F), the inverse transform and both lag maxima."""
     F = fft_size(m)
-    t = 2.5 * F * math.log2(F)
-    return n * (t + 2 * m) + n * k * (t + 6 * (F // 2 + 1) + 2 * m - 1)
+    t = 5.0 * F * math.log2(F)
+    if evals is None:
+        return n * (t / 2 + 2 * m) + n * k * (t / 2 + 6 * (F // 2 + 1) + 2 * m - 1)
+    pairs = (n + 1) // 2
+    return pairs * (t + 12 * F + 4 * m) + pairs * k * 4 * F + evals * (t + 6 * F + 4 * m)
 
 
 def iteration_bytes(n, m, k):
@@ -120,7 +129,8 @@ def main():
     ms_step = dt / args.steps * 1e3
     ms_assign = (s1["ms_assign"] - s0["ms_assign"]) / args.steps
     ms_refine = (s1["ms_refine"] - s0["ms_refine"]) / args.steps
-    flops = sbd_assign_flops(n, m, k)
+    evals = (s1["sbd_evals"] - s0["sbd_evals"]) / args.steps
+    flops = sbd_assign_flops(n, m, k, evals)
     achieved = flops / (ms_assign * 1e-3) / 1e12
     out = {
         "metric": METRIC,
@@ -146,11 +156,13 @@ def main():
         "assign_pairs_per_s": n * k / (ms_assign * 1e-3),
         "hbm_fraction_of_iteration": iteration_bytes(n, m, k) / (PEAK_HBM_GBS * 1e9) / (ms_step * 1e-3),
         "power_steps_max": s1["power_steps_max"],
+        "assign_eval_fraction": evals / (((n + 1) // 2) * k),
+        "assign_full_search_equiv_tflops": sbd_assign_flops(n, m, k) / (ms_assign * 1e-3) / 1e12,
         "p50_job_ms": None,
         "p99_job_ms": None,
         "roofline": {
-            "bound": "fp32",
-            "kernel": "k_sbd (fused FFT SBD assignment)",
+            "bound": "mfma",
+            "kernel": "k_assign_prune (fused FFT SBD assignment, FP32 VALU)",
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",

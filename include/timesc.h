@@ -92,6 +92,8 @@ int tcx_sbd_align(tcx_ctx* ctx, float* d_members, const float* d_center,
                   int rows, int cols, int convention);
 
 /* Fused SBD assignment: dist[i] = min_j (1 - max_lag NCC(x_i, c_j)),
+ * computed exactly; without d_dmat, centroids whose spectral upper bound
+ * on the NCC cannot beat the best distance found are skipped,
  * label[i] = first argmin (zero-based), shift[i] = lag of the maximum for
  * the chosen centroid (the shift the next refinement applies).  d_dmat
  * (n*k, may be NULL) receives all distances; d_dist and d_shift may be NULL.
@@ -144,6 +146,9 @@ typedef struct {
   double ms_total;       /* wall time of the fit (host clock)               */
   double ms_refine;      /* summed refinement time (HIP events)             */
   double ms_assign;      /* summed assignment time (HIP events)             */
+  long long sbd_evals;   /* inverse transforms run by the pruned assignment
+                            (each covers up to two series x one centroid)   */
+  long long sbd_pairs;   /* series x centroid pairs assigned                */
 } tcx_fit_stats;
 
 /* Fill opts with defaults (max_iter 10, seed 0, CPU convention,

@@ -86,6 +86,64 @@ def test_assign_matches_oracle(ctx, m, n, k):
         assert abs(cc[sh[i] + m - 1] - cc[osh[i] + m - 1]) <= SBD_TIE, (i, sh[i], osh[i])
 
 
+@pytest.mark.parametrize("m,n,k", [(2, 31, 3), (2, 40, 90), (17, 101, 4), (60, 333, 40),
+                                   (128, 512, 4), (512, 1001, 16), (512, 600, 130),
+                                   (1000, 77, 5), (2709, 40, 7), (4096, 9, 3)])
+def test_pruned_assign_matches_oracle(ctx, m, n, k):
+    """The assignment without a distance matrix skips centroids by their
+    spectral bound; labels, distances and shifts stay those of the full
+    search (k=90 at m=2 exceeds the pruned kernel's LDS and takes the full
+    search)."""
+    X, _ = _data(2 * m + n, n, m, min(k, 8))
+    X[3] = 0.0  # an all-zero series: distance 1 to everything, label 0
+    d = ctx.to_device(X)
+    try:
+        ctx.znorm(d, ddof=0)
+        Z = d.numpy()
+        rng = np.random.default_rng(m * k)
+        rows = rng.choice(n, size=k, replace=k > n)
+        C = Z[rows].copy()
+        if k > 2:
+            C[1] = 0.0          # zero centroid: distance exactly 1
+            C[k - 1] = C[k - 2]  # duplicate centroids: the smaller index wins
+        dc = ctx.to_device(C)
+        lab, dist, sh = ctx.assign(d, dc)
+        dc.free()
+    finally:
+        d.free()
+    olab, odist, osh, odm = O.assign(Z.astype(np.float64), C.astype(np.float64), dmat=True)
+    assert_labels_match(lab, olab, odm)
+    assert lab[3] == 0 and dist[3] == 1.0 and sh[3] == 1 - m
+    if k > 2:
+        assert not np.any(lab == k - 1)
+    same = lab == olab
+    assert np.abs(dist[same] - odist[same]).max() <= 2e-6
+    for i in np.nonzero(same & (sh != osh))[0]:
+        cc = O.ncc(Z[i].astype(np.float64), C[lab[i]].astype(np.float64))
+        assert abs(cc[sh[i] + m - 1] - cc[osh[i] + m - 1]) <= SBD_TIE, (i, sh[i], osh[i])
+
+
+@pytest.mark.parametrize("m,n,k", [(96, 3001, 12), (2048, 257, 6)])
+def test_pruned_assign_equals_full_search(ctx, m, n, k):
+    """Pruned and full assignment of the same inputs agree bit for bit on
+    labels and distances (both pair rows 2p, 2p+1 in one transform)."""
+    X, _ = _data(5 * m + k, n, m, k)
+    d = ctx.to_device(X)
+    try:
+        ctx.znorm(d, ddof=0)
+        Z = d.numpy()
+        C = Z[T.init_rows(7, n, k)].copy()
+        dc = ctx.to_device(C)
+        lab, dist, sh = ctx.assign(d, dc)
+        flab, fdist, fsh, _ = ctx.assign(d, dc, dmat=True)
+        dc.free()
+    finally:
+        d.free()
+    assert np.array_equal(lab, flab)
+    assert np.array_equal(dist, fdist)
+    assert np.array_equal(sh, fsh)
+
+
 @pytest.mark.parametrize("convention", [T.CONV_CPU, T.CONV_KSHAPE])
 @pytest.mark.parametrize("m", [16, 128, 700])
 def test_sbd_align_matches_oracle(ctx, convention, m):

@@ -50,11 +50,13 @@ __global__ __launch_bounds__(256) void k_znorm(float* __restrict__ x, long rows,
 
 // ------------------------------------------------------------- spectra
 // Forward FFT of each row (zero padded to F, imaginary part 0), stored
-// lane-strided, plus the row's L2 norm and an all-zero flag.
+// lane-strided, plus the row's L2 norm, an all-zero flag and (mag != NULL)
+// the spectrum's magnitudes in the same layout.
 template <int F>
 __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_spectra(
     const float* __restrict__ y, int rows, int m, const float2* __restrict__ twg,
-    float2* __restrict__ spec, float* __restrict__ norms, int* __restrict__ zero) {
+    float2* __restrict__ spec, float* __restrict__ norms, int* __restrict__ zero,
+    float* __restrict__ mag) {
   using G = FftGeom<F>;
   constexpr int L = G::L, P = G::P;
   extern __shared__ float2 smem[];
@@ -83,6 +85,11 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_spectra(
     float2* o = spec + (long)row * F;
 #pragma unroll
     for (int s = 0; s < P; ++s) o[s * L + lane] = v[s];
+    if (mag) {
+      float* mo = mag + (long)row * F;
+#pragma unroll
+      for (int s = 0; s < P; ++s) mo[s * L + lane] = sqrtf(v[s].x * v[s].x + v[s].y * v[s].y);
+    }
     if (lane == 0) {
       norms[row] = sqrtf(ss);
       if (zero) zero[row] = fnz == 0.f;
@@ -214,6 +221,202 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_sbd(
   }
 }
 
+// ------------------------------------------- pruned assignment (exact)
+// For every centroid j the lag maximum of the correlation is bounded by the
+// spectra:  max_l cc[l] = max_l (1/F) |sum_f X_f conj(C_f) e^{2 pi i f l/F}|
+// <= (1/F) sum_f |X_f| |C_f|,  so  dist_j >= lb_j = 1 - ub_j  with
+// ub_j = sum_f |X_f||C_f| / (F |x| |c_j|).  Per pair of series the
+// centroids are evaluated (inverse FFT, exact distance) in decreasing
+// order of their bound; a centroid whose lower bound (with a rounding
+// margin) exceeds the best exact distance found cannot be the argmin and
+// is skipped.  Ties keep NumPy's first-index rule.  The result equals the
+// dense kernel's; the pair's two series share each evaluated transform.
+// |X_a|, |X_b| come from the packed transform Z = FFT(x_a + i x_b):
+// X_a = (Z_f + conj(Z_{F-f}))/2, X_b = (Z_f - conj(Z_{F-f}))/(2i).
+// order (may be NULL): row indices to pair up, -1 = hole; rows with the
+// same previous label are adjacent there, so both series of a pair tend to
+// share their candidates.
+constexpr float kUbRel = 1e-4f;   // rounding margin of the bound
+constexpr float kUbAbs = 1e-5f;
+
+template <int F>
+__global__ __launch_bounds__(FftGeom<F>::kBlock) void k_assign_prune(
+    const float* __restrict__ x, long n, long npairs, int m, int k,
+    const float2* __restrict__ cspec, const float* __restrict__ cmag,
+    const float* __restrict__ cnorm, const float2* __restrict__ twg, int rev,
+    const int* __restrict__ order, int* __restrict__ label, float* __restrict__ dist,
+    int* __restrict__ shift, int* __restrict__ changed, unsigned long long* __restrict__ evals) {
+  using G = FftGeom<F>;
+  constexpr int L = G::L, P = G::P;
+  extern __shared__ float2 smem[];
+  float2* tw = smem;
+  float* red = reinterpret_cast<float*>(smem + F + G::kGroups * fft_buf_len<F>());
+  float* ubs_all = red + 64;
+  for (int i = threadIdx.x; i < F; i += blockDim.x) tw[i] = twg[i];
+  __syncthreads();
+  const int g = threadIdx.x / L, lane = threadIdx.x % L;
+  float2* buf = smem + F + g * fft_buf_len<F>();
+  float* ubs = ubs_all + (long)g * 2 * k;  // [0,k): series a, [k,2k): series b
+  unsigned long long my_evals = 0;
+  for (long pr = (long)blockIdx.x * G::kGroups + g; pr < npairs;
+       pr += (long)gridDim.x * G::kGroups) {
+    long ia = order ? order[2 * pr] : 2 * pr;
+    long ib = order ? (2 * pr + 1 < 2 * npairs ? order[2 * pr + 1] : -1) : 2 * pr + 1;
+    if (ia >= n) ia = -1;
+    if (ib >= n) ib = -1;
+    if (ia < 0) {
+      ia = ib;
+      ib = -1;
+    }
+    if (ia < 0) continue;  // hole pair (uniform for the group)
+    const bool hasb = ib >= 0;
+    const float* xa = x + ia * (long)m;
+    const float* xb = x + (hasb ? ib : ia) * (long)m;
+    float2 zx[P];
+    float sa = 0.f, sb = 0.f;
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+      const int t = lane + L * s;
+      const float a = t < m ? xa[t] : 0.f;
+      const float b = (hasb && t < m) ? xb[t] : 0.f;
+      zx[s] = make_float2(a, b);
+      sa += a * a;
+      sb += b * b;
+    }
+    sa = group_sum<L>(sa, red);
+    sb = group_sum<L>(sb, red);
+    const float na = sqrtf(sa), nb = sqrtf(sb);
+    fft<F, L, false>(zx, buf, tw, lane);
+    // magnitudes |X_a[f]|, |X_b[f]| of this lane's frequencies f = lane + L*s;
+    // partner F - f is (lane' = (L - lane) % L, s' = P-1-s) for lane != 0 and
+    // (lane 0, s' = (P - s) % P) for lane 0
+    float ma[P], mb[P];
+    const int pl = (L - lane) & (L - 1);
+    if constexpr (L > kWave) {
+      // partner lanes sit in other waves: exchange through LDS (the
+      // transform's last pass ends with a barrier after its LDS reads)
+#pragma unroll
+      for (int s = 0; s < P; ++s) buf[lds_pad(lane + L * s)] = zx[s];
+      __syncthreads();
+    }
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+      float px, py;
+      if constexpr (L > kWave) {
+        const float2 q = buf[lds_pad(pl + L * (P - 1 - s))];
+        px = q.x;
+        py = q.y;
+      } else {
+        px = __shfl(zx[P - 1 - s].x, pl, L);
+        py = __shfl(zx[P - 1 - s].y, pl, L);
+      }
+      if (lane == 0) {
+        px = zx[(P - s) & (P - 1)].x;
+        py = zx[(P - s) & (P - 1)].y;
+      }
+      const float ar = zx[s].x + px, ai = zx[s].y - py;
+      const float br = zx[s].x - px, bi = zx[s].y + py;
+      ma[s] = 0.5f * sqrtf(ar * ar + ai * ai);
+      mb[s] = 0.5f * sqrtf(br * br + bi * bi);
+    }
+    if constexpr (L > kWave) __syncthreads();  // exchange reads done before buf is reused
+    // bounds of every centroid
+    for (int j = 0; j < k; ++j) {
+      const float* cm = cmag + (long)j * F;
+      float aa = 0.f, ab = 0.f;
+#pragma unroll
+      for (int s = 0; s < P; ++s) {
+        const float c = cm[s * L + lane];
+        aa += ma[s] * c;
+        ab += mb[s] * c;
+      }
+      aa = group_sum<L>(aa, red);
+      ab = group_sum<L>(ab, red);
+      if (lane == 0) {
+        const float cn = cnorm[j];
+        const float dena = na * cn, denb = nb * cn;
+        ubs[j] = dena != 0.f ? (aa / (float)F) / dena : 0.f;
+        ubs[k + j] = denb != 0.f ? (ab / (float)F) / denb : 0.f;
+      }
+    }
+    group_sync<L>();
+    float bestA = INFINITY, bestB = hasb ? INFINITY : -INFINITY;
+    int labA = 0, labB = 0, keyA = 0, keyB = 0;
+    for (int round = 0; round < k; ++round) {
+      // candidate with the largest bound among centroids that can still win
+      float sc = -INFINITY;
+      int cj = 0x7fffffff;
+      for (int j = lane; j < k; j += L) {
+        const float ua = ubs[j], ub = ubs[k + j];
+        float v = -INFINITY;
+        if (ua >= 0.f && 1.f - (ua * (1.f + kUbRel) + kUbAbs) <= bestA) v = ua;
+        if (ub >= 0.f && 1.f - (ub * (1.f + kUbRel) + kUbAbs) <= bestB) v = fmaxf(v, ub);
+        if (v > sc || (v == sc && j < cj)) {
+          sc = v;
+          cj = j;
+        }
+      }
+      group_argmax<L>(sc, cj, red);
+      if (sc == -INFINITY) break;
+      const int j = cj;
+      group_sync<L>();
+      if (lane == 0) {
+        ubs[j] = -1.f;  // evaluated
+        ubs[k + j] = -1.f;
+      }
+      const float cn = cnorm[j];
+      const float dena = na * cn, denb = nb * cn;
+      float da = 1.f, db = 1.f;
+      int ka = 0, kb = 0;
+      if (cn != 0.f) {
+        ++my_evals;
+        const float2* cjs = cspec + (long)j * F;
+        float2 z[P];
+#pragma unroll
+        for (int s = 0; s < P; ++s) z[s] = cmulc(zx[s], cjs[s * L + lane]);
+        fft<F, L, true>(z, buf, tw, lane);
+        float va = -INFINITY, vb = -INFINITY;
+        ka = 0x7fffffff;
+        kb = 0x7fffffff;
+#pragma unroll
+        for (int s = 0; s < P; ++s) {
+          const int key = lag_key(lane + L * s, m, F, rev);
+          if (key >= 0) {
+            if (z[s].x > va || (z[s].x == va && key < ka)) {
+              va = z[s].x;
+              ka = key;
+            }
+            if (z[s].y > vb || (z[s].y == vb && key < kb)) {
+              vb = z[s].y;
+              kb = key;
+            }
+          }
+        }
+        group_argmax<L>(va, ka, red);
+        group_argmax<L>(vb, kb, red);
+        if (dena != 0.f) da = 1.f - (va / (float)F) / dena; else ka = 0;
+        if (denb != 0.f) db = 1.f - (vb / (float)F) / denb; else kb = 0;
+      }
+      if (da < bestA || (da == bestA && j < labA)) { bestA = da; labA = j; keyA = ka; }
+      if (hasb && (db < bestB || (db == bestB && j < labB))) { bestB = db; labB = j; keyB = kb; }
+      group_sync<L>();
+    }
+    if (lane == 0) {
+      if (changed && label[ia] != labA) *changed = 1;
+      label[ia] = labA;
+      if (dist) dist[ia] = bestA;
+      if (shift) shift[ia] = keyA + 1 - m;
+      if (hasb) {
+        if (changed && label[ib] != labB) *changed = 1;
+        label[ib] = labB;
+        if (dist) dist[ib] = bestB;
+        if (shift) shift[ib] = keyB + 1 - m;
+      }
+    }
+  }
+  if (evals && lane == 0 && my_evals) atomicAdd(evals, my_evals);
+}
+
 // ---------------------------------------------------------- zero-fill shift
 // out[t] = in[t - s] (0 outside), s = shift[row]; skipped when *skip != 0.
 // roll_zeropad, test_timesC.py:55-77.  In place: one block per row, the row
@@ -245,13 +448,14 @@ static size_t sbd_smem() {
 
 template <int F>
 static hipError_t launch_spectra_F(const float* y, int rows, int m, const float2* tw,
-                                   float2* spec, float* norms, int* zero, hipStream_t st) {
+                                   float2* spec, float* norms, int* zero, float* mag,
+                                   hipStream_t st) {
   using G = FftGeom<F>;
   int blocks = (rows + G::kGroups - 1) / G::kGroups;
   if (blocks < 1) blocks = 1;
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL((k_spectra<F>), dim3(blocks), dim3(G::kBlock), sbd_smem<F>(), st, y,
-                     rows, m, tw, spec, norms, zero);
+                     rows, m, tw, spec, norms, zero, mag);
   return hipGetLastError();
 }
 
@@ -268,15 +472,66 @@ static hipError_t launch_sbd_F(const SbdArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
+template <int F>
+static size_t prune_smem(int k) {
+  return sbd_smem<F>() + sizeof(float) * 2 * (size_t)k * FftGeom<F>::kGroups;
+}
+
+// Largest k whose per-group bound lists fit the LDS of one workgroup.
+template <int F>
+static int prune_max_k_F() {
+  const size_t fixed = sbd_smem<F>();
+  const size_t per_k = sizeof(float) * 2 * FftGeom<F>::kGroups;
+  const size_t fit = fixed < (size_t)kLdsBytes ? ((size_t)kLdsBytes - fixed) / per_k : 0;
+  return fit < (size_t)kPruneMaxK ? (int)fit : kPruneMaxK;
+}
+
+template <int F>
+static hipError_t launch_prune_F(const PruneArgs& a, hipStream_t st) {
+  using G = FftGeom<F>;
+  long blocks = (a.npairs + G::kGroups - 1) / G::kGroups;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL((k_assign_prune<F>), dim3((unsigned)blocks), dim3(G::kBlock),
+                     prune_smem<F>(a.k), st, a.x, a.n, a.npairs, a.m, a.k, a.cspec, a.cmag,
+                     a.cnorm, a.tw, a.rev, a.order, a.label, a.dist, a.shift, a.changed, a.evals);
+  return hipGetLastError();
+}
+
 #define TCX_FOR_EACH_F(X) \
   X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
 
 hipError_t launch_spectra(int F, const float* y, int rows, int m, const float2* tw,
-                          float2* spec, float* norms, int* zero, hipStream_t st) {
+                          float2* spec, float* norms, int* zero, float* mag, hipStream_t st) {
   switch (F) {
 #define X(FF) \
   case FF:    \
-    return launch_spectra_F<FF>(y, rows, m, tw, spec, norms, zero, st);
+    return launch_spectra_F<FF>(y, rows, m, tw, spec, norms, zero, mag, st);
+    TCX_FOR_EACH_F(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+int prune_max_k(int F) {
+  switch (F) {
+#define X(FF) \
+  case FF:    \
+    return prune_max_k_F<FF>();
+    TCX_FOR_EACH_F(X)
+#undef X
+    default:
+      return 0;
+  }
+}
+
+hipError_t launch_assign_prune(int F, const PruneArgs& a, hipStream_t st) {
+  if (a.k > prune_max_k(F)) return hipErrorInvalidValue;
+  switch (F) {
+#define X(FF) \
+  case FF:    \
+    return launch_prune_F<FF>(a, st);
     TCX_FOR_EACH_F(X)
 #undef X
     default:
@@ -309,7 +564,11 @@ hipError_t configure_sbd_kernels() {
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sbd_smem<FF>()); \
   if (e == hipSuccess)                                                                        \
     e = hipFuncSetAttribute((const void*)k_spectra<FF>,                                       \
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sbd_smem<FF>());
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sbd_smem<FF>()); \
+  if (e == hipSuccess)                                                                        \
+    e = hipFuncSetAttribute((const void*)k_assign_prune<FF>,                                  \
+                            hipFuncAttributeMaxDynamicSharedMemorySize,                       \
+                            (int)prune_smem<FF>(prune_max_k_F<FF>()));
   TCX_FOR_EACH_F(X)
 #undef X
   return e;

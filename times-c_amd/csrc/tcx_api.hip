@@ -49,7 +49,7 @@ enum Slot {
   kSlotCentNew, kSlotChanged, kSlotBlkCount, kSlotBlkBase, kSlotCounts, kSlotOffsets,
   kSlotPerm, kSlotBad, kSlotYhi, kSlotYlo, kSlotA0, kSlotItems, kSlotTiles, kSlotPart,
   kSlotS, kSlotInfo, kSlotDummy, kSlotWbuf, kSlotRbuf, kSlotBar, kSlotVout, kSlotSteps,
-  kSlotConv, kSlotErr, kNumSlots
+  kSlotConv, kSlotErr, kSlotCmag, kSlotEvals, kNumSlots
 };
 
 struct FitState {
@@ -60,6 +60,9 @@ struct FitState {
   double tol = 1e-12;
   int split = 1;
   int it = 0, converged = 0, sum_class = 0, steps_max = 0, fallbacks = 0;
+  long n_pad = 0;
+  bool have_order = false;   // the refine's cluster-sorted rows are valid
+  long long evals = 0, pairs = 0;
   float ms_ref = 0.f, ms_asg = 0.f, ms_asg_last = 0.f;
   int *label = nullptr, *shift = nullptr, *changed = nullptr, *zero = nullptr;
   float *dist = nullptr, *cent = nullptr, *norm = nullptr;
@@ -166,25 +169,41 @@ int check_series(int n, int m) {
   return TCX_OK;
 }
 
-// Spectra, norms and all-zero flags of k rows.
+// Spectra, spectrum magnitudes, norms and all-zero flags of k rows.
 int spectra(tcx_ctx* c, const float* d_rows, int rows, int m, float2** spec, float** norm,
             int** zero) {
   int F = fft_size(m);
   const float2* tw;
+  float* mag;
   TCX_TRY(twiddles(c, F, &tw));
   TCX_TRY(ws(c, kSlotCspec, (size_t)rows * F, spec));
+  TCX_TRY(ws(c, kSlotCmag, (size_t)rows * F, &mag));
   TCX_TRY(ws(c, kSlotCnorm, (size_t)rows, norm));
   TCX_TRY(ws(c, kSlotCzero, (size_t)rows, zero));
-  TCX_HIP(tcx::launch_spectra(F, d_rows, rows, m, tw, *spec, *norm, *zero, c->stream));
+  TCX_HIP(tcx::launch_spectra(F, d_rows, rows, m, tw, *spec, *norm, *zero, mag, c->stream));
   return TCX_OK;
 }
 
+// Assignment.  Without a distance matrix the exact pruned kernel runs,
+// pairing the rows of `order` (npairs pairs, -1 = hole) or rows 2p, 2p+1.
 int assign_core(tcx_ctx* c, const float* d_x, long n, int m, int k, const float2* cspec,
                 const float* cnorm, int rev, int* label, float* dist, int* shift, float* dmat,
-                int* changed) {
+                int* changed, const int* order = nullptr, long npairs = 0,
+                unsigned long long* evals = nullptr) {
   int F = fft_size(m);
   const float2* tw;
   TCX_TRY(twiddles(c, F, &tw));
+  if (!dmat && k <= tcx::prune_max_k(F)) {
+    tcx::PruneArgs p;
+    p.x = d_x; p.n = n; p.m = m; p.k = k; p.cspec = cspec; p.cnorm = cnorm; p.tw = tw;
+    p.cmag = reinterpret_cast<const float*>(c->slot_ptr[kSlotCmag]);
+    p.rev = rev; p.label = label; p.dist = dist; p.shift = shift; p.changed = changed;
+    p.order = order;
+    p.npairs = order ? npairs : (n + 1) / 2;
+    p.evals = evals;
+    TCX_HIP(tcx::launch_assign_prune(F, p, c->stream));
+    return TCX_OK;
+  }
   tcx::SbdArgs a;
   a.x = d_x; a.n = n; a.m = m; a.k = k; a.cspec = cspec; a.cnorm = cnorm; a.tw = tw;
   a.rev = rev; a.label = label; a.dist = dist; a.shift = shift; a.dmat = dmat;
@@ -227,6 +246,7 @@ void plan_syrk(int m, int k, const long* off, const long* seg_len,
 }
 
 struct RefineResult {
+  long n_pad = 0;
   int sum_class = 0;
   int steps_max = 0;
   int fallbacks = 0;
@@ -260,6 +280,7 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
            const int* d_shift, const int* d_czero, float* d_cout, int iter, uint64_t seed,
            int split, double tol, RefineResult* res) {
   hipStream_t st = c->stream;
+  c->fit.have_order = false;  // the row order is rewritten below
   long nblk = (n + tcx::kGroupChunk - 1) / tcx::kGroupChunk;
   int *blk_count, *blk_base, *counts, *perm, *bad;
   long* offsets;
@@ -326,6 +347,7 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
   TCX_HIP(hipMemcpyAsync(h_info.data(), info, sizeof(tcx::PowerInfo) * k, hipMemcpyDeviceToHost, st));
   TCX_HIP(hipStreamSynchronize(st));
   if (res) {
+    res->n_pad = n_pad;
     res->sum_class = nonempty;
     for (int j = 0; j < k; ++j) {
       res->steps_max = std::max(res->steps_max, h_info[j].steps);
@@ -650,6 +672,8 @@ int fit_refine(tcx_ctx* c, FitState& f) {
   TCX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
   f.ms_ref += ms;
   f.sum_class = rr.sum_class;
+  f.n_pad = rr.n_pad;
+  f.have_order = true;
   f.steps_max = std::max(f.steps_max, rr.steps_max);
   f.fallbacks += rr.fallbacks;
   return TCX_OK;
@@ -658,10 +682,16 @@ int fit_refine(tcx_ctx* c, FitState& f) {
 int fit_assign(tcx_ctx* c, FitState& f, int* h_changed) {
   hipStream_t st = c->stream;
   TCX_HIP(tcx::launch_fill_int(f.changed, 1, 0, st));
+  unsigned long long* evals;
+  TCX_TRY(ws(c, kSlotEvals, 1, &evals));
+  TCX_HIP(tcx::launch_fill_int(reinterpret_cast<int*>(evals), 2, 0, st));
   TCX_HIP(hipEventRecord(c->ev[2], st));
+  const int* order = f.have_order ? reinterpret_cast<const int*>(c->slot_ptr[kSlotPerm]) : nullptr;
   TCX_TRY(assign_core(c, f.d_x, f.n, f.m, f.k, f.spec, f.norm, f.o.convention, f.label, f.dist,
-                      f.shift, nullptr, f.changed));
+                      f.shift, nullptr, f.changed, order, f.n_pad / 2, evals));
   TCX_HIP(hipEventRecord(c->ev[3], st));
+  unsigned long long h_evals = 0;
+  TCX_HIP(hipMemcpyAsync(&h_evals, evals, sizeof(h_evals), hipMemcpyDeviceToHost, st));
   TCX_HIP(hipMemcpyAsync(h_changed, f.changed, sizeof(int), hipMemcpyDeviceToHost, st));
   if (f.o.labels_per_iter && f.it < f.o.max_iter)
     TCX_HIP(hipMemcpyAsync(f.o.labels_per_iter + (size_t)f.it * f.n, f.label,
@@ -671,6 +701,8 @@ int fit_assign(tcx_ctx* c, FitState& f, int* h_changed) {
   TCX_HIP(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
   f.ms_asg += ms;
   f.ms_asg_last = ms;
+  f.evals += (long long)h_evals;
+  f.pairs += (long long)f.n * f.k;
   return TCX_OK;
 }
 
@@ -710,6 +742,8 @@ int tcx_fit_query(tcx_ctx* c, tcx_fit_stats* stats) {
   stats->fallbacks = f.fallbacks;
   stats->ms_refine = f.ms_ref;
   stats->ms_assign = f.ms_asg;
+  stats->sbd_evals = f.evals;
+  stats->sbd_pairs = f.pairs;
   stats->ms_total = std::chrono::duration<double, std::milli>(
                         std::chrono::steady_clock::now() - f.t_start).count();
   return TCX_OK;
@@ -734,6 +768,8 @@ int tcx_fit_end(tcx_ctx* c, int* h_labels, float* h_centroids, tcx_fit_stats* st
     stats->fallbacks = f.fallbacks;
     stats->ms_refine = f.ms_ref;
     stats->ms_assign = f.ms_asg;
+    stats->sbd_evals = f.evals;
+    stats->sbd_pairs = f.pairs;
     stats->ms_total = std::chrono::duration<double, std::milli>(
                           std::chrono::steady_clock::now() - f.t_start).count();
   }

@@ -31,7 +31,31 @@ constexpr int kMaxFft = 8192;
 
 hipError_t launch_znorm(float* x, long rows, int m, int ddof, hipStream_t st);
 hipError_t launch_spectra(int F, const float* y, int rows, int m, const float2* tw,
-                          float2* spec, float* norms, int* zero, hipStream_t st);
+                          float2* spec, float* norms, int* zero, float* mag, hipStream_t st);
+
+// Exact pruned assignment (bound-and-prune on the spectra), k <= kPruneMaxK.
+constexpr int kPruneMaxK = 1024;
+constexpr int kLdsBytes = 160 * 1024;  // LDS of one workgroup on gfx950
+// Largest k the pruned assignment takes at FFT length F (LDS-bound).
+int prune_max_k(int F);
+struct PruneArgs {
+  const float* x = nullptr;
+  long n = 0;
+  long npairs = 0;               // pairs of order[] (or ceil(n/2) without order)
+  int m = 0, k = 0;
+  const float2* cspec = nullptr;
+  const float* cmag = nullptr;   // |C| in the spectrum layout
+  const float* cnorm = nullptr;
+  const float2* tw = nullptr;
+  int rev = 0;
+  const int* order = nullptr;    // 2*npairs row indices, -1 = hole (may be NULL)
+  int* label = nullptr;
+  float* dist = nullptr;
+  int* shift = nullptr;
+  int* changed = nullptr;
+  unsigned long long* evals = nullptr;  // inverse transforms run (may be NULL)
+};
+hipError_t launch_assign_prune(int F, const PruneArgs& a, hipStream_t st);
 hipError_t launch_sbd(int F, int mode, const SbdArgs& a, hipStream_t st);
 hipError_t launch_apply_shift(float* x, long rows, int m, const int* shift, const int* skip,
                               hipStream_t st);

@@ -65,6 +65,8 @@ class FitStats(ctypes.Structure):
         ("ms_total", ctypes.c_double),
         ("ms_refine", ctypes.c_double),
         ("ms_assign", ctypes.c_double),
+        ("sbd_evals", ctypes.c_longlong),
+        ("sbd_pairs", ctypes.c_longlong),
     ]
 
     def as_dict(self):
