@@ -38,14 +38,15 @@ def sbd_assign_flops(n, m, k, evals=None):
     (series, centroid) a half inverse transform, the conj-multiply of F/2+1
     bins and the 2m-1 lag max.  Pruned search (`evals` inverse transforms of
     series pairs were run): per pair the forward transform, the magnitude
-    split (~12 F) and the bound of every centroid (4 F), per evaluation the
+    split of the half spectrum (~6 F) and the bound of every centroid (2 F:
+    F/2 + 1 bins, two series, one multiply-add each), per evaluation the
     conj-multiply (6 F), the inverse transform and both lag maxima."""
     F = fft_size(m)
     t = 5.0 * F * math.log2(F)
     if evals is None:
         return n * (t / 2 + 2 * m) + n * k * (t / 2 + 6 * (F // 2 + 1) + 2 * m - 1)
     pairs = (n + 1) // 2
-    return pairs * (t + 12 * F + 4 * m) + pairs * k * 4 * F + evals * (t + 6 * F + 4 * m)
+    return pairs * (t + 6 * F + 4 * m) + pairs * k * 2 * F + evals * (t + 6 * F + 4 * m)
 
 
 def iteration_bytes(n, m, k):
@@ -89,6 +90,19 @@ def cpu_baseline(X, k, sample, seed):
     }
 
 
+def copy_calibration(mib):
+    """A plain copy of `mib` MiB (read once, written once), three launches."""
+    import torch
+
+    x = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": x.numel() * 4, "launches": 3}))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,10 +114,15 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=100000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB three times "
+                         "(the reference kernel for the HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; run with --gpus 1", file=sys.stderr)
         return 2
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     import timesc
 
     n, m, k = args.n, args.m, args.k

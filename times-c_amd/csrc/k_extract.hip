@@ -628,9 +628,10 @@ __global__ __launch_bounds__(kPowerThreads) void k_power(
 // Lanczos basis, T_j = Q_j^T M Q_j tridiagonal.  Lanczos with full
 // reorthogonalization in FP64 (one block per cluster, Q in global memory);
 // y = T_j^I e1 by I steps of the tridiagonal recurrence in one wave (FP64,
-// T scaled by its Gershgorin bound, renormalized every 32 steps).  Every 16
-// steps y is compared with the previous check; the result is accepted when
-// |y_j - y_prev| <= lz_tol or the Krylov space became invariant (breakdown).
+// T scaled by its Gershgorin bound, renormalized every 32 steps).  From 16
+// steps on, every 8 steps y is compared with the previous check; the result
+// is accepted when |y_j - y_prev| <= lz_tol or the Krylov space became
+// invariant (breakdown).
 // Clusters that do not converge within kLzMax steps get conv_out = 0 and the
 // caller runs the plain power iteration for them.
 constexpr int kLzMax = 128;
@@ -789,7 +790,7 @@ __global__ __launch_bounds__(kPowerThreads) void k_power_lanczos(
     if (threadIdx.x == 0) alpha[it] = qw;
     len = it + 1;
     const bool breakdown = !(b > 1e-12 * sqrt(nw2));
-    if (breakdown || len == jmax || (len >= 16 && (len & 15) == 0)) {
+    if (breakdown || len == jmax || (len >= 16 && (len & 7) == 0)) {
       __syncthreads();
       if (wid == 0) {
         tridiag_power_wave(alpha, beta, len, I, y);

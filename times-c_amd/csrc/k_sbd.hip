@@ -239,15 +239,50 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_sbd(
 constexpr float kUbRel = 1e-4f;   // rounding margin of the bound
 constexpr float kUbAbs = 1e-5f;
 
+// Reduce-scatter of the CNT values v[0..CNT) of every lane over a group of
+// L lanes (one wave or less): at lane-bit O a lane keeps the lower or upper
+// half of its values (by that bit) plus the partner's matching half; once a
+// single value is left the remaining levels are plain butterfly sums.
+// Afterwards a lane holds the group sums of values base .. base + CNT/L - 1
+// (one value when CNT <= L).
+template <int CNT, int O, int L, int N>
+__device__ __forceinline__ void reduce_scatter(float (&v)[N], int lane, int& base) {
+  if constexpr (O >= 1) {
+    const bool up = (lane & O) != 0;
+    if constexpr (CNT > 1) {
+      constexpr int HF = CNT / 2;
+#pragma unroll
+      for (int i = 0; i < HF; ++i) {
+        const float send = up ? v[i] : v[i + HF];
+        const float keep = up ? v[i + HF] : v[i];
+        v[i] = keep + __shfl_xor(send, O, L);
+      }
+      if (up) base += HF;
+      reduce_scatter<HF, O / 2, L>(v, lane, base);
+    } else {
+      v[0] += __shfl_xor(v[0], O, L);
+      reduce_scatter<1, O / 2, L>(v, lane, base);
+    }
+  }
+}
+
+// Launch geometry of the pruned assignment (that of the dense kernel).
 template <int F>
-__global__ __launch_bounds__(FftGeom<F>::kBlock) void k_assign_prune(
+struct PruneGeom {
+  static constexpr int L = FftGeom<F>::L;
+  static constexpr int kBlock = FftGeom<F>::kBlock;
+  static constexpr int kGroups = kBlock / L;
+};
+
+template <int F>
+__global__ __launch_bounds__(PruneGeom<F>::kBlock) void k_assign_prune(
     const float* __restrict__ x, long n, long npairs, int m, int k,
     const float2* __restrict__ cspec, const float* __restrict__ cmag,
     const float* __restrict__ cnorm, const float2* __restrict__ twg, int rev,
     const int* __restrict__ order, int* __restrict__ label, float* __restrict__ dist,
     int* __restrict__ shift, int* __restrict__ changed, unsigned long long* __restrict__ evals) {
-  using G = FftGeom<F>;
-  constexpr int L = G::L, P = G::P;
+  using G = PruneGeom<F>;
+  constexpr int L = G::L, P = FftGeom<F>::P;
   extern __shared__ float2 smem[];
   float2* tw = smem;
   float* red = reinterpret_cast<float*>(smem + F + G::kGroups * fft_buf_len<F>());
@@ -290,7 +325,14 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_assign_prune(
     // magnitudes |X_a[f]|, |X_b[f]| of this lane's frequencies f = lane + L*s;
     // partner F - f is (lane' = (L - lane) % L, s' = P-1-s) for lane != 0 and
     // (lane 0, s' = (P - s) % P) for lane 0
-    float ma[P], mb[P];
+    // weighted magnitudes of the half spectrum f <= F/2 held by this lane,
+    // f = lane + L*s for s < P/2 and f = F/2 (lane 0, s = P/2):
+    // |X_a[f]| = |Z[f] + conj(Z[F-f])| / 2, |X_b[f]| = |Z[f] - conj(Z[F-f])| / 2,
+    // weight 2 for 0 < f < F/2 (f and F-f have equal magnitudes), 1 for f = 0
+    // and f = F/2.  The partner F-f sits at (lane' = (L - lane) % L,
+    // s' = P-1-s) for lane != 0 and at (lane 0, s' = (P - s) % P) for lane 0.
+    constexpr int H = P / 2;
+    float ma[H + 1], mb[H + 1];
     const int pl = (L - lane) & (L - 1);
     if constexpr (L > kWave) {
       // partner lanes sit in other waves: exchange through LDS (the
@@ -300,7 +342,7 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_assign_prune(
       __syncthreads();
     }
 #pragma unroll
-    for (int s = 0; s < P; ++s) {
+    for (int s = 0; s <= H; ++s) {
       float px, py;
       if constexpr (L > kWave) {
         const float2 q = buf[lds_pad(pl + L * (P - 1 - s))];
@@ -316,27 +358,64 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_assign_prune(
       }
       const float ar = zx[s].x + px, ai = zx[s].y - py;
       const float br = zx[s].x - px, bi = zx[s].y + py;
-      ma[s] = 0.5f * sqrtf(ar * ar + ai * ai);
-      mb[s] = 0.5f * sqrtf(br * br + bi * bi);
+      const float wgt = s == H ? (lane == 0 ? 0.5f : 0.f) : ((s == 0 && lane == 0) ? 0.5f : 1.f);
+      ma[s] = wgt * sqrtf(ar * ar + ai * ai);
+      mb[s] = wgt * sqrtf(br * br + bi * bi);
     }
     if constexpr (L > kWave) __syncthreads();  // exchange reads done before buf is reused
-    // bounds of every centroid
-    for (int j = 0; j < k; ++j) {
-      const float* cm = cmag + (long)j * F;
-      float aa = 0.f, ab = 0.f;
+    // bounds of every centroid: ub = sum_f |X_f| |C_f| / (F |x| |c|)
+    if constexpr (L <= kWave) {
+      // KC centroids at a time; the 2*KC per-lane partial sums are reduced
+      // by a reduce-scatter over the L lanes (each level halves the values
+      // a lane keeps)
+      constexpr int KC = 8;
+      for (int j0 = 0; j0 < k; j0 += KC) {
+        float v[2 * KC];
 #pragma unroll
-      for (int s = 0; s < P; ++s) {
-        const float c = cm[s * L + lane];
-        aa += ma[s] * c;
-        ab += mb[s] * c;
+        for (int c = 0; c < KC; ++c) {
+          float aa = 0.f, ab = 0.f;
+          if (j0 + c < k) {
+            const float* cm = cmag + (long)(j0 + c) * F;
+#pragma unroll
+            for (int s = 0; s <= H; ++s) {
+              const float cv = cm[s * L + lane];
+              aa += ma[s] * cv;
+              ab += mb[s] * cv;
+            }
+          }
+          v[c] = aa;
+          v[KC + c] = ab;
+        }
+        int base = 0;
+        reduce_scatter<2 * KC, L / 2, L>(v, lane, base);
+        constexpr int kHeld = 2 * KC >= L ? 2 * KC / L : 1;
+#pragma unroll
+        for (int i = 0; i < kHeld; ++i) {
+          const int idx = base + i, ser = idx / KC, jj = j0 + idx % KC;
+          if (jj < k) {
+            const float den = (ser ? nb : na) * cnorm[jj];
+            ubs[ser * k + jj] = den != 0.f ? (v[i] / (float)F) / den : 0.f;
+          }
+        }
       }
-      aa = group_sum<L>(aa, red);
-      ab = group_sum<L>(ab, red);
-      if (lane == 0) {
-        const float cn = cnorm[j];
-        const float dena = na * cn, denb = nb * cn;
-        ubs[j] = dena != 0.f ? (aa / (float)F) / dena : 0.f;
-        ubs[k + j] = denb != 0.f ? (ab / (float)F) / denb : 0.f;
+    } else {
+      for (int j = 0; j < k; ++j) {
+        const float* cm = cmag + (long)j * F;
+        float aa = 0.f, ab = 0.f;
+#pragma unroll
+        for (int s = 0; s <= H; ++s) {
+          const float cv = cm[s * L + lane];
+          aa += ma[s] * cv;
+          ab += mb[s] * cv;
+        }
+        aa = group_sum<L>(aa, red);
+        ab = group_sum<L>(ab, red);
+        if (lane == 0) {
+          const float cn = cnorm[j];
+          const float dena = na * cn, denb = nb * cn;
+          ubs[j] = dena != 0.f ? (aa / (float)F) / dena : 0.f;
+          ubs[k + j] = denb != 0.f ? (ab / (float)F) / denb : 0.f;
+        }
       }
     }
     group_sync<L>();
@@ -474,21 +553,23 @@ static hipError_t launch_sbd_F(const SbdArgs& a, hipStream_t st) {
 
 template <int F>
 static size_t prune_smem(int k) {
-  return sbd_smem<F>() + sizeof(float) * 2 * (size_t)k * FftGeom<F>::kGroups;
+  using G = PruneGeom<F>;
+  return sizeof(float2) * (F + G::kGroups * fft_buf_len<F>()) + sizeof(float) * 64 +
+         sizeof(float) * 2 * (size_t)k * G::kGroups;
 }
 
 // Largest k whose per-group bound lists fit the LDS of one workgroup.
 template <int F>
 static int prune_max_k_F() {
-  const size_t fixed = sbd_smem<F>();
-  const size_t per_k = sizeof(float) * 2 * FftGeom<F>::kGroups;
+  const size_t fixed = prune_smem<F>(0);
+  const size_t per_k = sizeof(float) * 2 * PruneGeom<F>::kGroups;
   const size_t fit = fixed < (size_t)kLdsBytes ? ((size_t)kLdsBytes - fixed) / per_k : 0;
   return fit < (size_t)kPruneMaxK ? (int)fit : kPruneMaxK;
 }
 
 template <int F>
 static hipError_t launch_prune_F(const PruneArgs& a, hipStream_t st) {
-  using G = FftGeom<F>;
+  using G = PruneGeom<F>;
   long blocks = (a.npairs + G::kGroups - 1) / G::kGroups;
   if (blocks < 1) blocks = 1;
   if (blocks > 8192) blocks = 8192;
