@@ -24,6 +24,10 @@ sys.path.insert(0, os.path.join(ROOT, "times-c_amd"))
 METRIC = "SBD pairs/s + k-Shape iter/s at n=100k,m=512,k=16; p50/p99 job latency"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per k_assign_prune launch at the default workload (n=100000,
+# m=512, k=16), from rocprofv3 FETCH_SIZE (x2) and WRITE_SIZE (x1) counter
+# runs calibrated on a plain 256 MiB copy: profiles/kshape_iteration_n100k_m512_k16.md
+ASSIGN_TRAFFIC_BYTES = {(100000, 512, 16): (208588 + 8369) * 1024}
 
 
 def fft_size(m):
@@ -186,7 +190,10 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": ASSIGN_TRAFFIC_BYTES.get((n, m, k)),
+            "algorithmic_bytes": n * m * 4 + 5 * n * 4,
+            "note": "compute-bound FP32 VALU (FFT butterflies, no matrix product); the FP32 "
+                    "peak counts packed FP32, which this build disables (78.6 TFLOP/s unpacked)",
         },
         "cpu_baseline": None,
     }
