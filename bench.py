@@ -94,6 +94,54 @@ def cpu_baseline(X, k, sample, seed):
     }
 
 
+def batch_workload(timesc, njobs):
+    """Uneven independent fits: job j has n ~ U{100..5000}, m = round(exp(U[ln 64,
+    ln 2048])), k ~ U{2..32}, the benchmark generator at sigma 0.3 with seed
+    1000 + j, random initial labels, at most 10 iterations."""
+    jobs = []
+    for j in range(njobs):
+        rng = np.random.default_rng(1000 + j)
+        n = int(rng.integers(100, 5001))
+        m = int(round(math.exp(rng.uniform(math.log(64), math.log(2048)))))
+        k = int(rng.integers(2, 33))
+        X, _ = timesc.generate(1000 + j, n, m, k, 0.3)
+        jobs.append((X, k, 1000 + j))
+    return jobs
+
+
+def run_batch(timesc, ctx, jobs, workers):
+    """Runs the uneven batch twice (the first run sizes every worker's
+    workspace) and reports the second: per-job latency from batch start to
+    job completion, per-job run time, jobs/s."""
+    ds = [ctx.to_device(X) for X, _, _ in jobs]
+    specs = [{"d": d, "k": k, "seed": seed, "znorm_ddof": 0, "max_iter": 10}
+             for d, (_, k, seed) in zip(ds, jobs)]
+    try:
+        with timesc.Batch(0, workers=workers) as b:
+            b.run(specs, order=timesc.ORDER_LONGEST_FIRST)
+            for d, (X, _, _) in zip(ds, jobs):  # the first run normalized in place
+                ctx.h2d(d, X)
+            t0 = time.perf_counter()
+            res = b.run(specs, order=timesc.ORDER_LONGEST_FIRST)
+            wall = time.perf_counter() - t0
+    finally:
+        for d in ds:
+            d.free()
+    lat = np.array([r["ms_latency"] for r in res])
+    run = np.array([r["ms_run"] for r in res])
+    iters = np.array([r["stats"]["iterations"] for r in res])
+    return {
+        "jobs": len(jobs), "workers": workers, "makespan_ms": wall * 1e3,
+        "jobs_per_s": len(jobs) / wall,
+        "p50_latency_ms": float(np.percentile(lat, 50)),
+        "p99_latency_ms": float(np.percentile(lat, 99)),
+        "p50_run_ms": float(np.percentile(run, 50)), "p99_run_ms": float(np.percentile(run, 99)),
+        "mean_iterations": float(iters.mean()),
+        "workload": "n~U{100..5000}, m=round(exp U[ln64,ln2048]), k~U{2..32}, sigma 0.3, "
+                    "seeds 1000+j, <=10 iterations, z-norm included, all jobs submitted at once",
+    }
+
+
 def copy_calibration(mib):
     """A plain copy of `mib` MiB (read once, written once), three launches."""
     import torch
@@ -118,6 +166,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=100000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-jobs", type=int, default=256)
+    ap.add_argument("--batch-workers", type=int, default=8)
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times "
                          "(the reference kernel for the HBM byte counters) and exit")
@@ -198,6 +249,12 @@ def main():
         "cpu_baseline": None,
     }
     dx.free()
+    if not args.no_batch and args.batch_jobs > 0:
+        print(f"# batch of {args.batch_jobs} uneven fits", file=sys.stderr, flush=True)
+        b = run_batch(timesc, ctx, batch_workload(timesc, args.batch_jobs), args.batch_workers)
+        out["batch"] = b
+        out["p50_job_ms"] = b["p50_latency_ms"]
+        out["p99_job_ms"] = b["p99_latency_ms"]
     ctx.close()
     if not args.no_cpu_baseline:
         print("# cpu baseline", file=sys.stderr, flush=True)

@@ -190,6 +190,46 @@ int tcx_kshape_tsl(tcx_ctx* ctx, const float* d_x, const float* h_x,
                    int* h_idx, int n, int m, int k, int* iter_num,
                    int* sum_class, int cal_select);
 
+/* ------------------------------------------------------ batched fits */
+/* A pool of worker threads, each owning one context (stream + workspace)
+ * on the same device, that runs many independent fits concurrently.
+ * No reference counterpart (the reference runs one fit per process,
+ * test_kshape.cu:13-80). */
+typedef struct tcx_batch tcx_batch;
+
+typedef struct {
+  /* in */
+  float* d_x;            /* device [n][m] FP32 series                        */
+  int n, m, k;
+  int znorm_ddof;        /* >= 0: z-normalize d_x in place first (ddof);
+                            < 0: d_x is already normalized                  */
+  tcx_fit_opts opts;     /* as for tcx_kshape_fit                            */
+  int* h_labels;         /* out: n zero-based labels (required)              */
+  float* h_centroids;    /* out: k*m final centroids (may be NULL)           */
+  /* out */
+  int status;            /* TCX_OK or the job's error code                   */
+  int worker;            /* worker that ran the job                          */
+  tcx_fit_stats stats;
+  double ms_queued;      /* batch start -> job start (host clock)            */
+  double ms_run;         /* job start -> job end                             */
+  double ms_latency;     /* batch start -> job end                           */
+  char message[160];     /* error message of a failed job ("" if none)       */
+} tcx_job;
+
+/* Job order of a batch run. */
+#define TCX_ORDER_GIVEN 0
+#define TCX_ORDER_SHORTEST_FIRST 1 /* by n*k*F log F, smallest first      */
+#define TCX_ORDER_LONGEST_FIRST 2  /* largest first (shortest makespan)   */
+
+/* workers >= 1 contexts on `device`, each limited to hbm_budget bytes of
+ * workspace (0 = no limit). */
+int tcx_batch_create(int device, int workers, size_t hbm_budget, tcx_batch** out);
+/* Runs all jobs and returns when every job has finished.  Returns TCX_OK
+ * when every job succeeded, else the status of the first failed job (each
+ * job's own status is in jobs[i].status).  One batch runs at a time. */
+int tcx_batch_run(tcx_batch* b, tcx_job* jobs, int njobs, int order);
+int tcx_batch_destroy(tcx_batch* b);
+
 /* ------------------------------------------------------------- host I/O */
 
 /* Labeled CSV "label,v1,...,vm" per line.  *data and *labels are

@@ -493,9 +493,9 @@ __device__ void sign_fix(const float* a0, double* c, double* w, int m, double* s
 __global__ __launch_bounds__(kPowerThreads) void k_power(
     const float* __restrict__ S, int m, const int* __restrict__ counts,
     const float* __restrict__ a0all, uint64_t seed, int iter, int max_steps, double tol,
-    const double* __restrict__ vin, const double* __restrict__ rsum,
-    const int* __restrict__ steps_in, const int* __restrict__ conv_in,
-    float* __restrict__ c_out, PowerInfo* __restrict__ info) {
+    const double* __restrict__ vin, const double* __restrict__ svin,
+    const double* __restrict__ rsum, const int* __restrict__ steps_in,
+    const int* __restrict__ conv_in, float* __restrict__ c_out, PowerInfo* __restrict__ info) {
   extern __shared__ double sh[];
   double* v = sh;
   double* w = sh + m;
@@ -555,7 +555,29 @@ __global__ __launch_bounds__(kPowerThreads) void k_power(
     return;
   }
   double nw2, vw;
-  center_matvec(Sj, m, v, r, sigma, w, part, scratch, nw2, vw);  // w = M v
+  if (use_vin && svin) {
+    // w = M v from S vin (computed grid-wide): S v = S vin / |vin|
+    double sv = 0.0, rv = 0.0;
+    for (int i = threadIdx.x; i < m; i += kPowerThreads) {
+      sv += v[i];
+      rv += r[i] * v[i];
+    }
+    sv = block_sum_f64(sv, scratch);
+    rv = block_sum_f64(rv, scratch + kPowerWaves);
+    const double inv_m = 1.0 / (double)m;
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < m; i += kPowerThreads) {
+      const double wi = svin[(long)j * m + i] / n0 - (r[i] * sv + rv) * inv_m +
+                        sigma * sv * inv_m * inv_m;
+      w[i] = wi;
+      a += wi * wi;
+      b += v[i] * wi;
+    }
+    nw2 = block_sum_f64(a, scratch + 2 * kPowerWaves);
+    vw = block_sum_f64(b, scratch + 3 * kPowerWaves);
+  } else {
+    center_matvec(Sj, m, v, r, sigma, w, part, scratch, nw2, vw);  // w = M v
+  }
   double vv = 0.0;
   for (int i = threadIdx.x; i < m; i += kPowerThreads) vv += v[i] * v[i];
   vv = block_sum_f64(vv, scratch + 4 * kPowerWaves);
@@ -626,7 +648,7 @@ __global__ __launch_bounds__(kPowerThreads) void k_power(
 // once the Krylov space K_j(M, v0) holds every eigen-direction whose weight
 // (lambda_i / lambda_1)^I is not negligible: Q_j = [q_1..q_j] orthonormal
 // Lanczos basis, T_j = Q_j^T M Q_j tridiagonal.  Lanczos with full
-// reorthogonalization in FP64 (one block per cluster, Q in global memory);
+// reorthogonalization in FP64 (Q in global memory, steps below);
 // y = T_j^I e1 by I steps of the tridiagonal recurrence in one wave (FP64,
 // T scaled by its Gershgorin bound, renormalized every 32 steps).  From 16
 // steps on, every 8 steps y is compared with the previous check; the result
@@ -634,7 +656,7 @@ __global__ __launch_bounds__(kPowerThreads) void k_power(
 // invariant (breakdown).
 // Clusters that do not converge within kLzMax steps get conv_out = 0 and the
 // caller runs the plain power iteration for them.
-constexpr int kLzMax = 128;
+constexpr int kLzMax = kLanczosMax;
 
 // y = T^I e1 / |T^I e1| for T = tridiag(beta[1..len-1], alpha[0..len-1],
 // beta[1..len-1]), len <= 128; executed by wave 0 only (lanes hold entries
@@ -685,6 +707,8 @@ __device__ void tridiag_power_wave(const double* alpha, const double* beta, int 
   if (e1 < len) y[e1] = y1 * inv;
 }
 
+// Fused form for small m (S_j resident in L2): one block per cluster runs
+// every step, with the symmetric matvec done by the block itself.
 __global__ __launch_bounds__(kPowerThreads) void k_power_lanczos(
     const float* __restrict__ S, int m, const int* __restrict__ counts, uint64_t seed, int iter,
     int I, double lz_tol, double* __restrict__ Qg, double* __restrict__ vout,
@@ -832,6 +856,256 @@ __global__ __launch_bounds__(kPowerThreads) void k_power_lanczos(
   }
 }
 
+// --------------------------------------------- Lanczos, one step per launch
+// The Lanczos stage runs as a host-driven sequence of two launches per step
+// so that the symmetric matvec S_j q_j of every cluster spreads over the
+// whole GPU instead of one CU per cluster:
+//   k_cluster_matvec  y_j = S_j x_j for all active clusters (grid-wide,
+//                     one wave per row of S, FP64 accumulation)
+//   k_lz_step         one block per cluster: centring, three-term
+//                     recurrence, full reorthogonalization, the next basis
+//                     vector and, every 8 steps from 16 on, the check of
+//                     y = T^I e1 against the previous check.
+// k_lz_setup starts every cluster (r = S 1 from a matvec of ones, sigma,
+// q_0 = power_v0 / |power_v0|) and k_lz_finish forms v = Q y.  State that
+// lives across launches (alpha, beta, y, step counters) is in global
+// memory, LzState per cluster.
+constexpr int kMvThreads = 256;
+constexpr int kMvRowsPerWave = 4;
+constexpr int kMvRows = kMvRowsPerWave * (kMvThreads / kWave);  // rows per block
+
+__global__ __launch_bounds__(kMvThreads) void k_cluster_matvec(
+    const float* __restrict__ S, int m, int k, const double* __restrict__ x, long xstride,
+    double* __restrict__ y, const LzState* __restrict__ state) {
+  const int panels = (m + kMvRows - 1) / kMvRows;
+  const int j = blockIdx.x / panels, p = blockIdx.x % panels;
+  if (j >= k || (state && state[j].done)) return;
+  extern __shared__ double xs[];  // m doubles
+  const double* xj = x ? x + (long)j * xstride : nullptr;
+  for (int i = threadIdx.x; i < m; i += kMvThreads) xs[i] = xj ? xj[i] : 1.0;
+  __syncthreads();
+  const float* Sj = S + (long)j * m * m;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int r0 = p * kMvRows + wid * kMvRowsPerWave;
+  double acc[kMvRowsPerWave];
+#pragma unroll
+  for (int q = 0; q < kMvRowsPerWave; ++q) acc[q] = 0.0;
+  if ((m & 3) == 0) {
+    const int nv = m / 4;
+    for (int c = lane; c < nv; c += kWave) {
+      const double x0 = xs[4 * c], x1 = xs[4 * c + 1], x2 = xs[4 * c + 2], x3 = xs[4 * c + 3];
+#pragma unroll
+      for (int q = 0; q < kMvRowsPerWave; ++q) {
+        const int r = r0 + q < m ? r0 + q : m - 1;  // clamped rows are discarded below
+        const float4 sv = reinterpret_cast<const float4*>(Sj + (long)r * m)[c];
+        acc[q] += (double)sv.x * x0 + (double)sv.y * x1 + (double)sv.z * x2 + (double)sv.w * x3;
+      }
+    }
+  } else {
+    for (int c = lane; c < m; c += kWave) {
+      const double xc = xs[c];
+#pragma unroll
+      for (int q = 0; q < kMvRowsPerWave; ++q) {
+        const int r = r0 + q < m ? r0 + q : m - 1;
+        acc[q] += (double)Sj[(long)r * m + c] * xc;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kMvRowsPerWave; ++q) {
+    const double t = wave_sum_f64(acc[q]);
+    if (lane == 0 && r0 + q < m) y[(long)j * m + r0 + q] = t;
+  }
+}
+
+// Starts the Lanczos process of every cluster from r = S 1 (in rbuf).
+__global__ __launch_bounds__(kPowerThreads) void k_lz_setup(
+    int m, const int* __restrict__ counts, uint64_t seed, int iter, const double* __restrict__ rbuf,
+    double* __restrict__ Qg, LzState* __restrict__ state) {
+  __shared__ double scratch[2 * kPowerWaves];
+  const int j = blockIdx.x;
+  double* Q = Qg + (long)j * (kLzMax + 1) * m;
+  double sg = 0.0, n0 = 0.0;
+  for (int i = threadIdx.x; i < m; i += kPowerThreads) {
+    sg += rbuf[(long)j * m + i];
+    const double v = power_v0(seed, iter, j, i);
+    n0 += v * v;
+  }
+  const double sigma = block_sum_f64(sg, scratch);
+  n0 = sqrt(block_sum_f64(n0, scratch + kPowerWaves));
+  for (int i = threadIdx.x; i < m; i += kPowerThreads) Q[i] = power_v0(seed, iter, j, i) / n0;
+  if (threadIdx.x == 0) {
+    LzState z = {};
+    z.sigma = sigma;
+    z.done = counts[j] == 0;
+    state[j] = z;
+  }
+}
+
+// One Lanczos step of every active cluster, given w = S q_it in wbuf.
+__global__ __launch_bounds__(kPowerThreads) void k_lz_step(
+    int m, int I, double lz_tol, const double* __restrict__ rbuf, const double* __restrict__ wbuf,
+    double* __restrict__ Qg, LzState* __restrict__ state) {
+  extern __shared__ double lz_sh[];
+  double* q = lz_sh;              // m
+  double* w = q + m;              // m
+  double* r = w + m;              // m
+  double* qp = r + m;             // m: previous Lanczos vector
+  double* scratch = qp + m;       // 8 * kPowerWaves
+  double* cf = scratch + 8 * kPowerWaves;  // kLzMax + 1
+  double* part = cf + kLzMax + 1;          // sym_part_len(m)
+  const int j = blockIdx.x;
+  LzState* stp = state + j;
+  if (stp->done) return;
+  const int it = stp->it;
+  const double sigma = stp->sigma;
+  double* Q = Qg + (long)j * (kLzMax + 1) * m;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double sv = 0.0, rv = 0.0;
+  for (int e = threadIdx.x; e < m; e += kPowerThreads) {
+    const double qe = Q[(long)it * m + e];
+    q[e] = qe;
+    qp[e] = it > 0 ? Q[(long)(it - 1) * m + e] : 0.0;
+    r[e] = rbuf[(long)j * m + e];
+    w[e] = wbuf[(long)j * m + e];
+    sv += qe;
+    rv += r[e] * qe;
+  }
+  sv = block_sum_f64(sv, scratch);
+  rv = block_sum_f64(rv, scratch + kPowerWaves);
+  // w = M q = S q - (r 1^T q + 1 r^T q)/m + sigma (1^T q)/m^2 1
+  const double inv_m = 1.0 / (double)m;
+  double a = 0.0, b0 = 0.0;
+  for (int e = threadIdx.x; e < m; e += kPowerThreads) {
+    const double we = w[e] - (r[e] * sv + rv) * inv_m + sigma * sv * inv_m * inv_m;
+    w[e] = we;
+    a += we * we;
+    b0 += q[e] * we;
+  }
+  const double nw2 = block_sum_f64(a, scratch + 2 * kPowerWaves);
+  const double qw = block_sum_f64(b0, scratch + 3 * kPowerWaves);
+  const double bprev = it > 0 ? stp->beta[it] : 0.0;
+  // three-term recurrence, then full reorthogonalization (classical
+  // Gram-Schmidt against q_0..q_it, repeated when it removed more than
+  // 1 - 1/sqrt(2) of the norm: Daniel-Gragg-Kaufman-Stewart)
+  double w1 = 0.0;
+  for (int e = threadIdx.x; e < m; e += kPowerThreads) {
+    const double xv = w[e] - qw * q[e] - bprev * qp[e];
+    w[e] = xv;
+    w1 += xv * xv;
+  }
+  w1 = block_sum_f64(w1, scratch + 4 * kPowerWaves);
+  double b = sqrt(w1);
+  const int H = m <= kPowerThreads ? kPowerThreads / m : 1;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int l = wid; l <= it; l += kPowerWaves) {
+      const double* ql = Q + (long)l * m;
+      double a0 = 0.0, a1 = 0.0;
+      int e = lane;
+      for (; e + 64 < m; e += 128) {
+        a0 += ql[e] * w[e];
+        a1 += ql[e + 64] * w[e + 64];
+      }
+      if (e < m) a0 += ql[e] * w[e];
+      const double acc = wave_sum_f64(a0 + a1);
+      if (lane == 0) cf[l] = acc;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < m * H; t += kPowerThreads) {
+      const int e = t % m, g = t / m;
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+      int l = g;
+      for (; l + 3 * H <= it; l += 4 * H) {
+        a0 += cf[l] * Q[(long)l * m + e];
+        a1 += cf[l + H] * Q[(long)(l + H) * m + e];
+        a2 += cf[l + 2 * H] * Q[(long)(l + 2 * H) * m + e];
+        a3 += cf[l + 3 * H] * Q[(long)(l + 3 * H) * m + e];
+      }
+      for (; l <= it; l += H) a0 += cf[l] * Q[(long)l * m + e];
+      part[(long)g * m + e] = (a0 + a1) + (a2 + a3);
+    }
+    __syncthreads();
+    double w2 = 0.0;
+    for (int e = threadIdx.x; e < m; e += kPowerThreads) {
+      double s2 = 0.0;
+      for (int g = 0; g < H; ++g) s2 += part[(long)g * m + e];
+      const double xv = w[e] - s2;
+      w[e] = xv;
+      w2 += xv * xv;
+    }
+    w2 = block_sum_f64(w2, scratch + 5 * kPowerWaves);
+    const bool again = w2 < 0.5 * w1;  // |w| fell below |w1| / sqrt(2)
+    b = sqrt(w2);
+    w1 = w2;
+    if (!again) break;
+  }
+  if (threadIdx.x == 0) stp->alpha[it] = qw;
+  const int len = it + 1;
+  const int jmax = m < kLzMax ? m : kLzMax;
+  const bool breakdown = !(b > 1e-12 * sqrt(nw2));
+  int finished = breakdown || len == jmax;
+  if (breakdown || len == jmax || (len >= 16 && (len & 7) == 0)) {
+    __syncthreads();  // alpha[it] visible to wave 0
+    if (wid == 0) {
+      double* y = cf;  // kLzMax doubles of LDS
+      tridiag_power_wave(stp->alpha, stp->beta, len, I, y);
+      const int prev_len = stp->prev_len;
+      double d = 0.0;
+      for (int e = lane; e < len; e += 64) {
+        const double pv = e < prev_len ? stp->y[e] : 0.0;  // previous y, zero padded
+        const double dd = y[e] - pv;
+        d += dd * dd;
+      }
+      d = wave_sum_f64(d);
+      const int conv = breakdown || (prev_len > 0 && sqrt(d) <= lz_tol);
+      for (int e = lane; e < len; e += 64) stp->y[e] = y[e];
+      if (lane == 0) {
+        stp->prev_len = len;
+        stp->conv = conv;
+        scratch[0] = conv;
+      }
+    }
+    __syncthreads();
+    if (scratch[0] != 0.0) finished = 1;
+  }
+  if (finished) {
+    if (threadIdx.x == 0) {
+      stp->len = len;
+      stp->done = 1;
+    }
+    return;
+  }
+  double* qn = Q + (long)(it + 1) * m;
+  for (int e = threadIdx.x; e < m; e += kPowerThreads) qn[e] = w[e] / b;
+  if (threadIdx.x == 0) {
+    stp->beta[it + 1] = b;
+    stp->it = it + 1;
+    stp->len = len;
+  }
+}
+
+// v = Q y of every cluster (zeros when its Lanczos run was not accepted).
+__global__ __launch_bounds__(kPowerThreads) void k_lz_finish(
+    int m, const int* __restrict__ counts, const double* __restrict__ Qg,
+    const LzState* __restrict__ state, double* __restrict__ vout, int* __restrict__ steps_out,
+    int* __restrict__ conv_out) {
+  const int j = blockIdx.x;
+  const LzState& z = state[j];
+  const int len = counts[j] == 0 ? 0 : z.len;
+  const int conv = counts[j] != 0 && z.conv;
+  const double* Q = Qg + (long)j * (kLzMax + 1) * m;
+  for (int e = threadIdx.x; e < m; e += kPowerThreads) {
+    double acc = 0.0;
+    if (conv)
+      for (int l = 0; l < len; ++l) acc += z.y[l] * Q[(long)l * m + e];
+    vout[(long)j * m + e] = acc;
+  }
+  if (threadIdx.x == 0) {
+    steps_out[j] = len;
+    conv_out[j] = conv;
+  }
+}
+
 // ---------------------------------------------------------------- launchers
 hipError_t launch_fill_int(int* p, long count, int value, hipStream_t st) {
   if (count <= 0) return hipSuccess;
@@ -896,10 +1170,11 @@ static size_t power_smem(int m) {
 
 hipError_t launch_power(const float* S, int m, int k, const int* counts, const float* a0,
                         uint64_t seed, int iter, int max_steps, double tol, const double* vin,
-                        const double* rsum, const int* steps_in, const int* conv_in,
-                        float* c_out, PowerInfo* info, hipStream_t st) {
+                        const double* svin, const double* rsum, const int* steps_in,
+                        const int* conv_in, float* c_out, PowerInfo* info, hipStream_t st) {
   hipLaunchKernelGGL(k_power, dim3(k), dim3(kPowerThreads), power_smem(m), st, S, m, counts,
-                     a0, seed, iter, max_steps, tol, vin, rsum, steps_in, conv_in, c_out, info);
+                     a0, seed, iter, max_steps, tol, vin, svin, rsum, steps_in, conv_in, c_out,
+                     info);
   return hipGetLastError();
 }
 
@@ -915,6 +1190,41 @@ hipError_t launch_power_lanczos(const float* S, int m, int k, const int* counts,
   return hipGetLastError();
 }
 
+static size_t lz_step_smem(int m) {
+  return sizeof(double) * (4 * (size_t)m + 8 * kPowerWaves + (kLzMax + 1) + sym_part_len(m));
+}
+
+hipError_t launch_cluster_matvec(const float* S, int m, int k, const double* x, long xstride,
+                                 double* y, const LzState* state, hipStream_t st) {
+  if (k <= 0) return hipSuccess;
+  const long panels = (m + kMvRows - 1) / kMvRows;
+  hipLaunchKernelGGL(k_cluster_matvec, dim3((unsigned)(panels * k)), dim3(kMvThreads),
+                     sizeof(double) * m, st, S, m, k, x, xstride, y, state);
+  return hipGetLastError();
+}
+
+hipError_t launch_lz_setup(int m, int k, const int* counts, uint64_t seed, int iter,
+                           const double* rbuf, double* Q, LzState* state, hipStream_t st) {
+  hipLaunchKernelGGL(k_lz_setup, dim3(k), dim3(kPowerThreads), 0, st, m, counts, seed, iter,
+                     rbuf, Q, state);
+  return hipGetLastError();
+}
+
+hipError_t launch_lz_step(int m, int k, int I, double lz_tol, const double* rbuf,
+                          const double* wbuf, double* Q, LzState* state, hipStream_t st) {
+  hipLaunchKernelGGL(k_lz_step, dim3(k), dim3(kPowerThreads), lz_step_smem(m), st, m, I, lz_tol,
+                     rbuf, wbuf, Q, state);
+  return hipGetLastError();
+}
+
+hipError_t launch_lz_finish(int m, int k, const int* counts, const double* Q,
+                            const LzState* state, double* vout, int* steps_out, int* conv_out,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_lz_finish, dim3(k), dim3(kPowerThreads), 0, st, m, counts, Q, state, vout,
+                     steps_out, conv_out);
+  return hipGetLastError();
+}
+
 hipError_t configure_extract_kernels() {
   hipError_t e = hipFuncSetAttribute((const void*)k_power,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -922,6 +1232,13 @@ hipError_t configure_extract_kernels() {
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void*)k_power_lanczos,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lanczos_smem(4096));
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_lz_step, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lz_step_smem(4096));
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_cluster_matvec,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(double) * 4096));
   return e;
 }
 

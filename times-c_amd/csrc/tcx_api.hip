@@ -49,7 +49,7 @@ enum Slot {
   kSlotCentNew, kSlotChanged, kSlotBlkCount, kSlotBlkBase, kSlotCounts, kSlotOffsets,
   kSlotPerm, kSlotBad, kSlotYhi, kSlotYlo, kSlotA0, kSlotItems, kSlotTiles, kSlotPart,
   kSlotS, kSlotInfo, kSlotDummy, kSlotWbuf, kSlotRbuf, kSlotBar, kSlotVout, kSlotSteps,
-  kSlotConv, kSlotErr, kSlotCmag, kSlotEvals, kNumSlots
+  kSlotConv, kSlotErr, kSlotCmag, kSlotEvals, kSlotSvout, kSlotLzW, kSlotLzState, kNumSlots
 };
 
 struct FitState {
@@ -252,22 +252,62 @@ struct RefineResult {
   int fallbacks = 0;
 };
 
-// M^I v0 of every non-empty cluster by Lanczos (k_power_lanczos).  Returns
-// the vectors, the row sums r = S 1, the Lanczos step counts and the
-// per-cluster acceptance flags on device.
+// M^I v0 of every non-empty cluster by Lanczos.  For m <= 768 one fused
+// launch (one block per cluster, S_j read from L2 every step); for larger m
+// per step one grid-wide matvec launch and one per-cluster update launch,
+// and every 8 steps from 16 on the host reads the clusters' done flags and
+// stops once all are done.
+// Returns on device the vectors v, S v, the row sums r = S 1, the Lanczos
+// step counts and the per-cluster acceptance flags.
 int power_lanczos(tcx_ctx* c, const float* S, int m, int k, const int* counts, uint64_t seed,
-                  int iter, int steps, const double** vout, const double** rout,
-                  const int** steps_out, const int** conv_out) {
-  double *Q, *v, *r;
+                  int iter, int steps, const double** vout, const double** svout,
+                  const double** rout, const int** steps_out, const int** conv_out) {
+  hipStream_t st = c->stream;
+  double *Q, *v, *sv, *r, *w;
   int *st_out, *cv_out;
+  tcx::LzState* state;
   TCX_TRY(ws(c, kSlotWbuf, (size_t)k * (tcx::kLanczosMax + 1) * m, &Q));
   TCX_TRY(ws(c, kSlotVout, (size_t)k * m, &v));
+  TCX_TRY(ws(c, kSlotSvout, (size_t)k * m, &sv));
   TCX_TRY(ws(c, kSlotRbuf, (size_t)k * m, &r));
+  TCX_TRY(ws(c, kSlotLzW, (size_t)k * m, &w));
   TCX_TRY(ws(c, kSlotSteps, (size_t)k, &st_out));
   TCX_TRY(ws(c, kSlotConv, (size_t)k, &cv_out));
-  TCX_HIP(tcx::launch_power_lanczos(S, m, k, counts, seed, iter, steps, 1e-11, Q, v, r, st_out,
-                                    cv_out, c->stream));
+  TCX_TRY(ws(c, kSlotLzState, (size_t)k, &state));
+  if (m <= tcx::kFusedLanczosMaxM) {
+    // S_j fits in L2: one block per cluster runs every step in one launch
+    TCX_HIP(tcx::launch_power_lanczos(S, m, k, counts, seed, iter, steps, 1e-11, Q, v, r, st_out,
+                                      cv_out, st));
+    TCX_HIP(tcx::launch_cluster_matvec(S, m, k, v, m, sv, nullptr, st));  // S v
+    *vout = v;
+    *svout = sv;
+    *rout = r;
+    *steps_out = st_out;
+    *conv_out = cv_out;
+    return TCX_OK;
+  }
+  TCX_HIP(tcx::launch_cluster_matvec(S, m, k, nullptr, 0, r, nullptr, st));  // r = S 1
+  TCX_HIP(tcx::launch_lz_setup(m, k, counts, seed, iter, r, Q, state, st));
+  const int jmax = m < tcx::kLanczosMax ? m : tcx::kLanczosMax;
+  const long qstride = (long)(tcx::kLanczosMax + 1) * m;
+  std::vector<int> done(k);
+  for (int it = 0; it < jmax; ++it) {
+    TCX_HIP(tcx::launch_cluster_matvec(S, m, k, Q + (long)it * m, qstride, w, state, st));
+    TCX_HIP(tcx::launch_lz_step(m, k, steps, 1e-11, r, w, Q, state, st));
+    const int len = it + 1;
+    if (len == jmax || (len >= 16 && (len & 7) == 0)) {
+      TCX_HIP(hipMemcpy2DAsync(done.data(), sizeof(int), &state[0].done, sizeof(tcx::LzState),
+                               sizeof(int), k, hipMemcpyDeviceToHost, st));
+      TCX_HIP(hipStreamSynchronize(st));
+      bool all = true;
+      for (int x : done) all = all && x;
+      if (all) break;
+    }
+  }
+  TCX_HIP(tcx::launch_lz_finish(m, k, counts, Q, state, v, st_out, cv_out, st));
+  TCX_HIP(tcx::launch_cluster_matvec(S, m, k, v, m, sv, nullptr, st));  // S v
   *vout = v;
+  *svout = sv;
   *rout = r;
   *steps_out = st_out;
   *conv_out = cv_out;
@@ -335,14 +375,15 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
   TCX_HIP(tcx::launch_syrk_reduce(d_tiles, (int)c->tiles.size(), part, m, S, nullptr, st));
   const int max_steps = m > 1000 ? 1000 : m;  // test_timesC.py:152-156
   const double* vin = nullptr;
+  const double* svin = nullptr;
   const double* rsum = nullptr;
   const int* steps_in = nullptr;
   const int* conv_in = nullptr;
   if (nonempty > 0)
-    TCX_TRY(power_lanczos(c, S, m, k, counts, seed, iter, max_steps, &vin, &rsum, &steps_in,
-                          &conv_in));
-  TCX_HIP(tcx::launch_power(S, m, k, counts, a0, seed, iter, max_steps, tol, vin, rsum, steps_in,
-                            conv_in, d_cout, info, st));
+    TCX_TRY(power_lanczos(c, S, m, k, counts, seed, iter, max_steps, &vin, &svin, &rsum,
+                          &steps_in, &conv_in));
+  TCX_HIP(tcx::launch_power(S, m, k, counts, a0, seed, iter, max_steps, tol, vin, svin, rsum,
+                            steps_in, conv_in, d_cout, info, st));
   std::vector<tcx::PowerInfo> h_info(k);
   TCX_HIP(hipMemcpyAsync(h_info.data(), info, sizeof(tcx::PowerInfo) * k, hipMemcpyDeviceToHost, st));
   TCX_HIP(hipStreamSynchronize(st));

@@ -107,14 +107,37 @@ hipError_t launch_syrk_reduce(const SyrkTile* tiles, int ntiles, const float* pa
                               float* S, unsigned* diagmax, hipStream_t st);
 hipError_t launch_power(const float* S, int m, int k, const int* counts, const float* a0,
                         uint64_t seed, int iter, int max_steps, double tol, const double* vin,
-                        const double* rsum, const int* steps_in, const int* conv_in,
-                        float* c_out, PowerInfo* info, hipStream_t st);
+                        const double* svin, const double* rsum, const int* steps_in,
+                        const int* conv_in, float* c_out, PowerInfo* info, hipStream_t st);
 
 // Lanczos evaluation of the power iterate M^I v0 (one block per cluster).
+// Lanczos evaluation of the power iterate (one step = two launches).
 constexpr int kLanczosMax = 128;
+struct LzState {
+  double sigma;                   // 1^T S 1
+  double alpha[kLanczosMax + 1];  // tridiagonal T: diagonal
+  double beta[kLanczosMax + 1];   // off-diagonal, beta[l] = T[l][l-1]
+  double y[kLanczosMax];          // T^I e1 / |T^I e1| at the last check
+  int it;                         // index of the newest basis vector
+  int len;                        // steps done
+  int prev_len;                   // len at the last check
+  int conv;                       // accepted
+  int done;                       // finished (accepted, breakdown or cap)
+};
+// Fused single-launch form (one block per cluster, for small m).
+constexpr int kFusedLanczosMaxM = 768;
 hipError_t launch_power_lanczos(const float* S, int m, int k, const int* counts, uint64_t seed,
                                 int iter, int I, double lz_tol, double* Q, double* vout,
                                 double* rout, int* steps_out, int* conv_out, hipStream_t st);
+hipError_t launch_cluster_matvec(const float* S, int m, int k, const double* x, long xstride,
+                                 double* y, const LzState* state, hipStream_t st);
+hipError_t launch_lz_setup(int m, int k, const int* counts, uint64_t seed, int iter,
+                           const double* rbuf, double* Q, LzState* state, hipStream_t st);
+hipError_t launch_lz_step(int m, int k, int I, double lz_tol, const double* rbuf,
+                          const double* wbuf, double* Q, LzState* state, hipStream_t st);
+hipError_t launch_lz_finish(int m, int k, const int* counts, const double* Q,
+                            const LzState* state, double* vout, int* steps_out, int* conv_out,
+                            hipStream_t st);
 hipError_t configure_extract_kernels();
 
 }  // namespace tcx

@@ -28,7 +28,7 @@ EXPORTS = [
     "tcx_znorm", "tcx_ncc", "tcx_sbd_align", "tcx_assign", "tcx_extract_shape",
     "tcx_fit_opts_default", "tcx_kshape_fit", "tcx_fit_begin", "tcx_fit_step", "tcx_fit_end",
     "tcx_fit_query",
-    "tcx_kshape", "tcx_kshape_tsl",
+    "tcx_kshape", "tcx_kshape_tsl", "tcx_batch_create", "tcx_batch_run", "tcx_batch_destroy",
     "tcx_read_labeled_csv", "tcx_read_unlabeled_csv", "tcx_write_labels", "tcx_free_host",
     "tcx_generate", "tcx_init_labels", "tcx_init_rows", "tcx_power_v0",
 ]
@@ -80,6 +80,29 @@ _I = ctypes.c_int
 _L = ctypes.c_long
 _Z = ctypes.c_size_t
 
+ORDER_GIVEN, ORDER_SHORTEST_FIRST, ORDER_LONGEST_FIRST = 0, 1, 2
+
+
+class Job(ctypes.Structure):
+    """tcx_job: one fit of a batch."""
+    _fields_ = [
+        ("d_x", ctypes.c_void_p),
+        ("n", ctypes.c_int),
+        ("m", ctypes.c_int),
+        ("k", ctypes.c_int),
+        ("znorm_ddof", ctypes.c_int),
+        ("opts", FitOpts),
+        ("h_labels", ctypes.c_void_p),
+        ("h_centroids", ctypes.c_void_p),
+        ("status", ctypes.c_int),
+        ("worker", ctypes.c_int),
+        ("stats", FitStats),
+        ("ms_queued", ctypes.c_double),
+        ("ms_run", ctypes.c_double),
+        ("ms_latency", ctypes.c_double),
+        ("message", ctypes.c_char * 160),
+    ]
+
 
 def load():
     """Load the engine library (raises if it is not built)."""
@@ -115,6 +138,9 @@ def load():
         "tcx_kshape": (_I, [_P, _P, _P, _I, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), _I]),
         "tcx_kshape_tsl": (_I, [_P, _P, _P, _P, _I, _I, _I, ctypes.POINTER(_I),
                                 ctypes.POINTER(_I), _I]),
+        "tcx_batch_create": (_I, [_I, _I, _Z, ctypes.POINTER(_P)]),
+        "tcx_batch_run": (_I, [_P, ctypes.POINTER(Job), _I, _I]),
+        "tcx_batch_destroy": (_I, [_P]),
         "tcx_read_labeled_csv": (_I, [ctypes.c_char_p, ctypes.POINTER(_P), ctypes.POINTER(_P),
                                       ctypes.POINTER(_I), ctypes.POINTER(_I)]),
         "tcx_read_unlabeled_csv": (_I, [ctypes.c_char_p, ctypes.POINTER(_P),
@@ -391,4 +417,66 @@ class Context:
         res = {"labels": labels, "centroids": cent, "stats": st.as_dict()}
         if hist is not None:
             res["labels_per_iter"] = hist[: st.iterations]
+        return res
+
+
+class Batch:
+    """A pool of worker contexts running independent fits concurrently."""
+
+    def __init__(self, device=0, workers=4, budget=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        check(self.lib.tcx_batch_create(device, workers, budget, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            check(self.lib.tcx_batch_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def run(self, specs, order=ORDER_GIVEN, raise_on_error=True):
+        """specs: dicts with d (DeviceArray [n, m]), k and optional
+        znorm_ddof (default -1 = already normalized), max_iter, init_mode,
+        seed, init_labels, init_rows, convention.  Returns one dict per job
+        (labels, centroids, stats, status, message, worker, ms_queued,
+        ms_run, ms_latency)."""
+        jobs = (Job * len(specs))()
+        keep = []
+        outs = []
+        for j, sp in zip(jobs, specs):
+            d = sp["d"]
+            n, m = d.shape
+            k = sp["k"]
+            j.d_x, j.n, j.m, j.k = d.ptr, n, m, k
+            j.znorm_ddof = sp.get("znorm_ddof", -1)
+            self.lib.tcx_fit_opts_default(ctypes.byref(j.opts))
+            j.opts.max_iter = sp.get("max_iter", 10)
+            j.opts.init_mode = sp.get("init_mode", 0)
+            j.opts.seed = sp.get("seed", 0)
+            j.opts.convention = sp.get("convention", CONV_CPU)
+            for key, field in (("init_labels", "init_labels"), ("init_rows", "init_rows")):
+                if sp.get(key) is not None:
+                    a = np.ascontiguousarray(sp[key], np.int32)
+                    keep.append(a)
+                    setattr(j.opts, field, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+            lab = np.empty(n, np.int32)
+            cent = np.empty((k, m), np.float32)
+            keep += [lab, cent]
+            j.h_labels, j.h_centroids = _ptr(lab), _ptr(cent)
+            outs.append((lab, cent))
+        rc = self.lib.tcx_batch_run(self.h, jobs, len(specs), order)
+        if rc != OK and raise_on_error:
+            check(rc)
+        res = []
+        for j, (lab, cent) in zip(jobs, outs):
+            res.append({"labels": lab, "centroids": cent, "stats": j.stats.as_dict(),
+                        "status": j.status, "message": j.message.decode(), "worker": j.worker,
+                        "ms_queued": j.ms_queued, "ms_run": j.ms_run,
+                        "ms_latency": j.ms_latency})
         return res
