@@ -113,13 +113,17 @@ int tcx_assign(tcx_ctx* ctx, const float* d_x, const float* d_centroids,
  * sign fix by the lowest-index member, z-normalize (ddof=1).
  * d_centroids: in = old centroids (k*m), out = new centroids; an empty
  * cluster gets a zero centroid.  `iter` selects the power-iteration start
- * vectors.  *sum_class (may be NULL) = number of non-empty clusters.
+ * vectors.  cal_select 0: power iteration (I = min(m, 1000) steps,
+ * evaluated by Lanczos), with clusters that fail its tests taking the top
+ * eigenvector; 1: the top eigenvector of every M_j (the eigh branch,
+ * test_timesC.py:184-192; shapeExtract.cuh:366-468).  *sum_class (may be
+ * NULL) = number of non-empty clusters.
  * Replaces extract_shape<float> (Code/include/shapeExtract.cuh:1057-1181);
  * semantics of _preSBD_extract_shape + _extract_shape
  * (Code/test/test_timesC.py:127-204). */
 int tcx_extract_shape(tcx_ctx* ctx, const int* d_label, const float* d_x,
                       const int* d_shift, float* d_centroids, int n, int m,
-                      int k, int iter, int* sum_class);
+                      int k, int iter, int cal_select, int* sum_class);
 
 /* ------------------------------------------------------------------- fit */
 
@@ -131,7 +135,7 @@ typedef struct {
   const int* init_rows;  /* optional host row indices (k), mode 1            */
   int convention;        /* TCX_CONV_CPU (default) or TCX_CONV_KSHAPE         */
   int s_precision;       /* TCX_S_SPLIT_BF16 (default) or TCX_S_BF16          */
-  int cal_select;        /* 0: power iteration (1 is not supported yet)       */
+  int cal_select;        /* 0: power iteration, 1: top eigenvector (eigh)      */
   double power_tol;      /* relative Rayleigh-quotient tolerance, 0 -> 1e-12 */
   int* labels_per_iter;  /* optional host buffer (max_iter*n): labels after
                             each iteration's assignment                      */

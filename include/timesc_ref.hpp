@@ -92,7 +92,6 @@ inline void extract_shape<float>(int* d_idx, const float* d_mat, int* d_k_count,
                                  const int cal_select) {
   (void)mat_row;
   tcx_ctx* c = timesc_ref::context();
-  if (cal_select != 0) timesc_ref::check(TCX_ERR_UNSUPPORTED, "extract_shape(cal_select=1)");
   std::vector<int> lab(idx_len), cnt(k, 0);
   timesc_ref::check(tcx_memcpy_d2h(c, lab.data(), d_idx, sizeof(int) * idx_len), "extract_shape");
   for (int& l : lab) {
@@ -103,7 +102,7 @@ inline void extract_shape<float>(int* d_idx, const float* d_mat, int* d_k_count,
   timesc_ref::check(tcx_malloc(c, (void**)&d_lab, sizeof(int) * idx_len), "extract_shape");
   timesc_ref::check(tcx_memcpy_h2d(c, d_lab, lab.data(), sizeof(int) * idx_len), "extract_shape");
   timesc_ref::check(tcx_extract_shape(c, d_lab, d_mat, nullptr, d_centers, idx_len, mat_col, k,
-                                      0, &sum_class),
+                                      0, cal_select, &sum_class),
                     "extract_shape");
   timesc_ref::check(tcx_free(c, d_lab), "extract_shape");
   if (d_k_count)

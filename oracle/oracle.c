@@ -536,7 +536,8 @@ static void sign_fix(const double* a0, double* c, int m, int* flipped) {
 }
 
 int orc_extract_shape(const double* A, long count, int m, const double* v0,
-                      int exact_steps, double* centroid, orc_shape_info* info) {
+                      int exact_steps, double* centroid, orc_shape_info* info,
+                      int cal_select) {
   orc_shape_info inf;
   memset(&inf, 0, sizeof(inf));
   if (m < 1) return 1;
@@ -567,8 +568,11 @@ int orc_extract_shape(const double* A, long count, int m, const double* v0,
     for (int j = 0; j < m; ++j)
       M[(size_t)i * m + j] += -(r[i] + r[j]) * inv_m + sigma * inv_m * inv_m;
   free(r);
-  /* :152-157 power_iteration(M, min(m, 1000)) (:30-38) */
-  int iters = m > 1000 ? 1000 : m;
+  /* :152-157 power_iteration(M, min(m, 1000)) (:30-38); cal_select = 1
+   * skips it and takes the eigh branch directly (the CUDA path's
+   * get_class_centers_a, shapeExtract.cuh:366-468; the commented-out block
+   * test_timesC.py:194-201) */
+  int iters = cal_select == 1 ? 0 : (m > 1000 ? 1000 : m);
   double* v = (double*)malloc(sizeof(double) * m);
   double* w = (double*)malloc(sizeof(double) * m);
   double nv = 0.0;
@@ -599,7 +603,9 @@ int orc_extract_shape(const double* A, long count, int m, const double* v0,
   const double* a0 = A; /* a[0]: first aligned member, before z-norm */
   int fallback = 0;
   double sumdiff = 0.0;
-  if (lambda == 0.0) {
+  if (cal_select == 1) {
+    fallback = 1;
+  } else if (lambda == 0.0) {
     /* :159-161 flag=False leaves vecright unbound in the reference (an
      * UnboundLocalError); treated here as the eigh fallback. */
     fallback = 1;
@@ -648,7 +654,7 @@ int orc_extract_shape(const double* A, long count, int m, const double* v0,
 void orc_refine(const double* X, long n, int m, int k, const int* labels,
                 double* C, int convention, uint64_t seed, int iter,
                 int exact_steps, int* counts, orc_shape_info* infos,
-                int nthreads, int skip_rule) {
+                int nthreads, int skip_rule, int cal_select) {
   set_threads(nthreads);
   int F = orc_fft_size(m);
   const double *tc, *ts;
@@ -698,7 +704,7 @@ void orc_refine(const double* X, long n, int m, int k, const int* labels,
     }
     orc_power_v0(seed, iter, j, m, v0);
     orc_shape_info info;
-    orc_extract_shape(A, c, m, v0, exact_steps, newC + (size_t)j * m, &info);
+    orc_extract_shape(A, c, m, v0, exact_steps, newC + (size_t)j * m, &info, cal_select);
     if (infos) infos[j] = info;
     if (counts) counts[j] = (int)c;
     free(A);
@@ -712,7 +718,8 @@ void orc_refine(const double* X, long n, int m, int k, const int* labels,
 int orc_kshape(const double* X, long n, int m, int k, int init_mode,
                const int* init_labels, const int* init_rows, int max_iter,
                int convention, uint64_t seed, int exact_steps,
-               int* labels_hist, double* centroids, int nthreads, int skip_rule) {
+               int* labels_hist, double* centroids, int nthreads, int skip_rule,
+               int cal_select) {
   int* lab = (int*)malloc(sizeof(int) * n);
   int* old = (int*)malloc(sizeof(int) * n);
   double* C = (double*)calloc((size_t)k * m, sizeof(double));
@@ -723,7 +730,7 @@ int orc_kshape(const double* X, long n, int m, int k, int init_mode,
     for (it = 0; it < max_iter; ++it) {
       memcpy(old, lab, sizeof(int) * n);
       orc_refine(X, n, m, k, lab, C, convention, seed, it, exact_steps, NULL,
-                 NULL, nthreads, skip_rule);
+                 NULL, nthreads, skip_rule, cal_select);
       orc_assign(X, n, m, C, k, lab, NULL, NULL, NULL, nthreads);
       memcpy(labels_hist + (size_t)it * n, lab, sizeof(int) * n);
       if (memcmp(old, lab, sizeof(int) * n) == 0) { ++it; break; }
@@ -739,7 +746,7 @@ int orc_kshape(const double* X, long n, int m, int k, int init_mode,
       if (it > 0 && memcmp(old, lab, sizeof(int) * n) == 0) { ++it; break; }
       memcpy(old, lab, sizeof(int) * n);
       orc_refine(X, n, m, k, lab, C, convention, seed, it, exact_steps, NULL,
-                 NULL, nthreads, skip_rule);
+                 NULL, nthreads, skip_rule, cal_select);
     }
   }
   if (centroids) memcpy(centroids, C, sizeof(double) * (size_t)k * m);

@@ -82,9 +82,12 @@ void orc_power_v0(uint64_t seed, int iter, int cluster, int m, double* v0);
 /* _extract_shape (test_timesC.py:136-204) on `count` aligned members A
  * (count x m).  v0 is the start vector (m values, not yet normalized).
  * exact_steps != 0 runs exactly min(m,1000) steps; otherwise it stops once a
- * step changes v by <= 1e-14 (infinity norm).  Returns 0 on success. */
+ * step changes v by <= 1e-14 (infinity norm).  cal_select = 1 takes the
+ * top eigenvector (the eigh branch) without power iteration.  Returns 0 on
+ * success. */
 int orc_extract_shape(const double* A, long count, int m, const double* v0,
-                      int exact_steps, double* centroid, orc_shape_info* info);
+                      int exact_steps, double* centroid, orc_shape_info* info,
+                      int cal_select);
 
 /* One refinement (test_timesC.py:228-255): group rows by label in index
  * order, align each cluster to C[j] unless C[j] sums to 0, extract shapes.
@@ -93,7 +96,7 @@ int orc_extract_shape(const double* A, long count, int m, const double* v0,
 void orc_refine(const double* X, long n, int m, int k, const int* labels,
                 double* C, int convention, uint64_t seed, int iter,
                 int exact_steps, int* counts, orc_shape_info* infos,
-                int nthreads, int skip_rule);
+                int nthreads, int skip_rule, int cal_select);
 
 /* Full k-Shape loop (test_timesC.py:213-265) on z-normalized X.
  * init_mode 0: init_labels (n) are the starting labels, centroids start at 0.
@@ -106,7 +109,7 @@ int orc_kshape(const double* X, long n, int m, int k, int init_mode,
                const int* init_labels, const int* init_rows, int max_iter,
                int convention, uint64_t seed, int exact_steps,
                int* labels_hist, double* centroids, int nthreads,
-               int skip_rule);
+               int skip_rule, int cal_select);
 
 /* Top eigenpair of a symmetric m x m matrix (FP64): Householder
  * tridiagonalization, bisection for the largest eigenvalue, inverse

@@ -50,9 +50,9 @@ def load():
         "orc_align_shift": (I, [_P, _P, I, I]),
         "orc_assign": (None, [_P, L, I, _P, I, _P, _P, _P, _P, I]),
         "orc_power_v0": (None, [U, I, I, I, _P]),
-        "orc_extract_shape": (I, [_P, L, I, _P, I, _P, ctypes.POINTER(ShapeInfo)]),
-        "orc_refine": (None, [_P, L, I, I, _P, _P, I, U, I, I, _P, _P, I, I]),
-        "orc_kshape": (I, [_P, L, I, I, I, _P, _P, I, I, U, I, _P, _P, I, I]),
+        "orc_extract_shape": (I, [_P, L, I, _P, I, _P, ctypes.POINTER(ShapeInfo), I]),
+        "orc_refine": (None, [_P, L, I, I, _P, _P, I, U, I, I, _P, _P, I, I, I]),
+        "orc_kshape": (I, [_P, L, I, I, I, _P, _P, I, I, U, I, _P, _P, I, I, I]),
         "orc_sum_pairwise": (D, [_P, L]),
         "orc_top_eig": (I, [_P, I, ctypes.POINTER(D), _P]),
     }
@@ -127,14 +127,14 @@ def power_v0(seed, it, cluster, m):
     return out
 
 
-def extract_shape(A, v0, exact_steps=False):
+def extract_shape(A, v0, exact_steps=False, cal_select=0):
     A = _f64(A).reshape(-1, np.asarray(v0).size)
     v0 = _f64(v0)
     m = v0.size
     out = np.empty(m)
     info = ShapeInfo()
     rc = load().orc_extract_shape(_p(A), A.shape[0], m, _p(v0), int(exact_steps), _p(out),
-                                  ctypes.byref(info))
+                                  ctypes.byref(info), cal_select)
     if rc:
         raise RuntimeError(f"orc_extract_shape failed ({rc})")
     return out, {name: getattr(info, name) for name, _ in info._fields_}
@@ -146,7 +146,7 @@ def sum_pairwise(a):
 
 
 def refine(X, labels, C, convention=CONV_CPU, seed=0, it=0, exact_steps=False, nthreads=0,
-           skip_rule=SKIP_ZERO):
+           skip_rule=SKIP_ZERO, cal_select=0):
     X = _f64(X)
     n, m = X.shape
     C = _f64(C).copy()
@@ -155,12 +155,14 @@ def refine(X, labels, C, convention=CONV_CPU, seed=0, it=0, exact_steps=False, n
     counts = np.empty(k, np.int32)
     infos = (ShapeInfo * k)()
     load().orc_refine(_p(X), n, m, k, _p(lab), _p(C), convention, seed, it, int(exact_steps),
-                      _p(counts), ctypes.cast(infos, ctypes.c_void_p), nthreads, skip_rule)
+                      _p(counts), ctypes.cast(infos, ctypes.c_void_p), nthreads, skip_rule,
+                      cal_select)
     return C, counts, [{name: getattr(i, name) for name, _ in i._fields_} for i in infos]
 
 
 def kshape(X, k, init_mode=0, init_labels=None, init_rows=None, max_iter=10,
-           convention=CONV_CPU, seed=0, exact_steps=False, nthreads=0, skip_rule=SKIP_ZERO):
+           convention=CONV_CPU, seed=0, exact_steps=False, nthreads=0, skip_rule=SKIP_ZERO,
+           cal_select=0):
     X = _f64(X)
     n, m = X.shape
     hist = np.zeros((max_iter, n), np.int32)
@@ -168,7 +170,8 @@ def kshape(X, k, init_mode=0, init_labels=None, init_rows=None, max_iter=10,
     il = None if init_labels is None else np.ascontiguousarray(init_labels, np.int32)
     ir = None if init_rows is None else np.ascontiguousarray(init_rows, np.int32)
     it = load().orc_kshape(_p(X), n, m, k, init_mode, _p(il), _p(ir), max_iter, convention,
-                           seed, int(exact_steps), _p(hist), _p(cent), nthreads, skip_rule)
+                           seed, int(exact_steps), _p(hist), _p(cent), nthreads, skip_rule,
+                           cal_select)
     return hist[:it], cent
 
 

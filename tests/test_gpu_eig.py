@@ -1,0 +1,107 @@
+"""The eigenvector path: cal_select = 1 (top eigenvector of M for every
+cluster, shapeExtract.cuh:366-468; test_timesC.py:184-192), the clusters on
+which the power result takes the eigh branch, and the CLI's rerun rule
+(test_kshape.cu:54-59), against the FP64 oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+import timesc as T
+from parity import CENT_RTOL, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _extract(ctx, A, cal_select, it=0):
+    """One cluster holding the rows of A, zero old centroid (no alignment)."""
+    c, m = A.shape
+    dx = ctx.to_device(A.astype(np.float32))
+    dl = ctx.to_device(np.zeros(c, np.int32))
+    ds = ctx.to_device(np.zeros(c, np.int32))
+    dc = ctx.to_device(np.zeros((1, m), np.float32))
+    try:
+        ctx.extract_shape(dl, dx, ds, dc, it=it, cal_select=cal_select)
+        return dc.numpy()[0]
+    finally:
+        for d in (dx, dl, ds, dc):
+            d.free()
+
+
+@pytest.mark.parametrize("m,count", [(16, 5), (64, 40), (300, 120), (800, 60), (1500, 30)])
+def test_eigenvector_centroid_matches_oracle(ctx, m, count):
+    X, _ = T.generate(m + count, count, m, 1, 0.3)
+    got = _extract(ctx, X, 1)
+    ref, info = O.extract_shape(X.astype(np.float32).astype(np.float64), T.power_v0(0, 0, 0, m),
+                                cal_select=1)
+    assert info["fallback"] == 1
+    assert rel_l2(got, ref) <= CENT_RTOL / 10, rel_l2(got, ref)
+
+
+# clusters (seed, m, count) on which the oracle's power result, started
+# from the engine's start vector power_v0(0, 0, 0), fails the sign/residual
+# tests and takes the eigh branch (found by a CPU search over small random
+# clusters built as below)
+ORACLE_FALLBACK_CASES = [(879, 12, 3), (1587, 31, 4), (1619, 3, 5), (3529, 7, 3)]
+
+
+@pytest.mark.parametrize("seed,m,count", ORACLE_FALLBACK_CASES)
+def test_clusters_taking_the_eigh_branch(ctx, seed, m, count):
+    rng = np.random.default_rng(seed)
+    assert int(rng.integers(3, 40)) == m and int(rng.integers(1, 6)) == count
+    A = rng.standard_normal((count, m))
+    if seed % 3 == 0:
+        A = np.round(A)
+    A = A.astype(np.float32)
+    ref, info = O.extract_shape(A.astype(np.float64), T.power_v0(0, 0, 0, m), exact_steps=True)
+    assert info["fallback"] == 1
+    got = _extract(ctx, A, 0)
+    assert rel_l2(got, ref) <= CENT_RTOL, rel_l2(got, ref)
+
+
+@pytest.mark.parametrize("init_mode", [0, 1])
+def test_fit_with_eigenvector_centroids_matches_oracle(ctx, init_mode):
+    n, m, k, seed = 400, 96, 4, 11
+    X, _ = T.generate(seed, n, m, k, 0.3)
+    d = ctx.to_device(X)
+    try:
+        ctx.znorm(d, ddof=0)
+        Z = d.numpy()
+        kw = ({"init_labels": T.init_labels(seed, n, k)} if init_mode == 0
+              else {"init_rows": T.init_rows(seed, n, k)})
+        res = ctx.fit(d, k, max_iter=20, init_mode=init_mode, seed=seed, cal_select=1,
+                      labels_per_iter=True, **kw)
+    finally:
+        d.free()
+    hist, cent = O.kshape(Z.astype(np.float64), k, init_mode=init_mode, max_iter=20, seed=seed,
+                          cal_select=1, **kw)
+    assert np.array_equal(res["labels_per_iter"], hist)
+    errs = [rel_l2(a, b) for a, b in zip(res["centroids"], cent) if np.any(b)]
+    assert max(errs) <= CENT_RTOL, errs
+
+
+def test_cli_reruns_with_eigenvector_centroids(ctx, tmp_path):
+    """A labeled file with k = 8 whose data has 2 classes: the first fit ends
+    with fewer than k/2 non-empty clusters, so the CLI reruns k_shape with
+    cal_select = 1 and writes that run's labels."""
+    n, m, k = 200, 48, 8
+    X, _ = T.generate(0, n, m, 2, 0.1)
+    csv = str(tmp_path / "data.csv")
+    T.write_labeled_csv(csv, X, np.arange(n, dtype=np.int32) % k)
+    out = str(tmp_path / "result.txt")
+    r = subprocess.run([T.CLI_PATH, "-d", "0", csv, "0", "0", "0", "--out=" + out],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "rerun k_shape with cal_select=1" in r.stdout, r.stdout
+    cli = np.array([int(v) for v in open(out).read()[:-1].split(",")])
+    d = ctx.to_device(X)
+    try:
+        ctx.znorm(d, ddof=0)
+        first = ctx.fit(d, k, max_iter=10, seed=0)
+        second = ctx.fit(d, k, max_iter=10, seed=0, cal_select=1)
+    finally:
+        d.free()
+    assert first["stats"]["sum_class"] < k // 2
+    assert np.array_equal(cli, second["labels"] + 1)

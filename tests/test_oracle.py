@@ -178,6 +178,20 @@ def test_extract_shape_matches_eigh():
     assert A[0] @ got > 0 and info["fallback"] == 0
 
 
+def test_extract_shape_eigenvector_path_is_the_eigh_branch():
+    """cal_select = 1: no power iteration, the top eigenvector of PSP with the
+    sign fix (test_timesC.py:184-192), even where the power iterate after
+    min(m, 1000) steps is still far from it."""
+    rng = np.random.default_rng(8)
+    t = np.arange(48)
+    p1, p2 = np.sin(2 * np.pi * 3 * t / 48), np.cos(2 * np.pi * 4 * t / 48)
+    A = np.stack([rng.normal(1, 0.3) * p1 + rng.normal(0, 0.98) * p2 + 0.05 * rng.standard_normal(48)
+                  for _ in range(30)])
+    got, info = O.extract_shape(A, O.power_v0(0, 0, 0, 48), cal_select=1)
+    assert np.abs(got - closed_form_centroid(A)).max() <= 1e-9
+    assert info["fallback"] == 1 and info["steps"] == 0
+
+
 def test_extract_shape_one_member_and_degenerate():
     rng = np.random.default_rng(7)
     a = rng.standard_normal((1, 32))

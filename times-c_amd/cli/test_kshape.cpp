@@ -6,7 +6,10 @@
 // CLI k ignored, test_kshape.cu:24-30), 10 iterations (:21);
 // flag 1: unlabeled CSV, k from the command line, 20 iterations (:90).
 // init_chose 0: random initial labels (k_shape); 1: random initial centres
-// (k_shape_tsl).  Output: one-based labels "l,l,...,l," (readfile.cuh:120).
+// (k_shape_tsl).  When the fit ends with one non-empty cluster or fewer
+// than k/2, it is rerun as k_shape with eigenvector centroids (cal_select
+// 1), as test_kshape.cu:54-59 does.  Output: one-based labels "l,l,...,l,"
+// (readfile.cuh:120).
 //
 // Optional trailing flags (no reference counterpart):
 //   --out=PATH          label file (default ../out/cukshape/result.txt)
@@ -148,11 +151,20 @@ int main(int argc, char** argv) {
   std::printf("refine %.3f ms assign %.3f ms\n", st.ms_refine, st.ms_assign);
   std::printf("total time run on GPU: %f msecs.\n",
               std::chrono::duration<double, std::milli>(t2 - t0).count());
-  if (st.sum_class == 1 || st.sum_class < k / 2)
-    std::printf("note: %d non-empty clusters; the reference CLI would re-run with the "
-                "eigen-solver centroids (test_kshape.cu:54-59), which the converged power "
-                "iteration already gives\n",
+  if (st.sum_class == 1 || st.sum_class < k / 2) {
+    // test_kshape.cu:54-59, 112-117: too few non-empty clusters -> rerun
+    // k_shape (random initial labels) with eigenvector centroids
+    std::printf("sum_class %d: rerun k_shape with cal_select=1\n", st.sum_class);
+    tcx_fit_opts o2 = o;
+    o2.init_mode = 0;
+    o2.cal_select = 1;
+    CHECKRC(tcx_kshape_fit(ctx, d, rows, cols, k, &o2, labels.data(), cent.data(), &st));
+    t2 = std::chrono::steady_clock::now();
+    std::printf("iterations %d converged %d sum_class %d\n", st.iterations, st.converged,
                 st.sum_class);
+    std::printf("total time run on GPU: %f msecs.\n",
+                std::chrono::duration<double, std::milli>(t2 - t0).count());
+  }
   for (int& l : labels) l += 1;  // one-based output, kshape.cuh:91
   CHECKRC(tcx_write_labels(out.c_str(), labels.data(), rows));
   if (!dump.empty()) {

@@ -4,6 +4,7 @@
 // rank-1 centering, FP64 Rayleigh-quotient test, sign fix and final z-norm.
 // Semantics: _preSBD_extract_shape + _extract_shape,
 // Code/test/test_timesC.py:127-204 (grouping order :228-243).
+#include <cfloat>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -486,16 +487,20 @@ __device__ void sign_fix(const float* a0, double* c, double* w, int m, double* s
 // (start vector = power_v0(seed, iter, cluster, .)), lambda = v.Mv, sign fix
 // by a0, the vecleft/vecright sign test and the signed-sum residual test,
 // then zscore(c, ddof=1).  With vin given, v = vin is the already iterated
-// vector (from k_power_coop) and no power steps run here.  The eigh
-// fallback keeps iterating the power method (the top eigenvector it
-// converges to is eigh's last column up to sign; the sign fix then picks
-// the same sign).
+// vector (from the Lanczos stage) and no power steps run here.
+// eig_mode: vin is the top eigenvector of M (the eigh branch, used for
+// cal_select = 1 and for clusters whose power result failed the tests):
+// sign fix and zscore only.  defer_fallback: a cluster that fails the tests
+// only reports fallback = 1 (the caller reruns it in eig_mode); otherwise,
+// and when no eigenvector converged, the power method keeps iterating
+// towards the top eigenvector.  mask (may be NULL): clusters to process.
 __global__ __launch_bounds__(kPowerThreads) void k_power(
     const float* __restrict__ S, int m, const int* __restrict__ counts,
     const float* __restrict__ a0all, uint64_t seed, int iter, int max_steps, double tol,
     const double* __restrict__ vin, const double* __restrict__ svin,
     const double* __restrict__ rsum, const int* __restrict__ steps_in,
-    const int* __restrict__ conv_in, float* __restrict__ c_out, PowerInfo* __restrict__ info) {
+    const int* __restrict__ conv_in, int eig_mode, const int* __restrict__ mask,
+    int defer_fallback, float* __restrict__ c_out, PowerInfo* __restrict__ info) {
   extern __shared__ double sh[];
   double* v = sh;
   double* w = sh + m;
@@ -504,6 +509,7 @@ __global__ __launch_bounds__(kPowerThreads) void k_power(
   int* iscratch = reinterpret_cast<int*>(scratch + 6 * kPowerWaves);
   double* part = scratch + 8 * kPowerWaves;  // sym_part_len(m) doubles
   const int j = blockIdx.x;
+  if (mask && !mask[j]) return;
   const float* Sj = S + (long)j * m * m;
   const float* a0 = a0all + (long)j * m;
   float* out = c_out + (long)j * m;
@@ -583,7 +589,16 @@ __global__ __launch_bounds__(kPowerThreads) void k_power(
   vv = block_sum_f64(vv, scratch + 4 * kPowerWaves);
   double lambda = vw / vv;  // :37
   int flipped = 0, fallback = 0;
-  if (lambda == 0.0) {
+  bool eig_done = false;
+  if (eig_mode) {
+    // v is the top eigenvector of M (eigh's last column): sign fix only
+    // (test_timesC.py:184-192)
+    fallback = 1;
+    if (use_vin) {
+      sign_fix(a0, v, w, m, scratch, flipped);
+      eig_done = true;
+    }
+  } else if (lambda == 0.0) {
     fallback = 1;
   } else {
     sign_fix(a0, v, w, m, scratch, flipped);  // :163-166
@@ -607,7 +622,7 @@ __global__ __launch_bounds__(kPowerThreads) void k_power(
     sd = block_sum_f64(sd, scratch + 5 * kPowerWaves);
     if (lambda <= 0.0 || fabs(sd) > (double)m * 0.01) fallback = 1;  // :183
   }
-  if (fallback) {  // :184-192
+  if (fallback && !eig_done && !(defer_fallback && !eig_mode)) {  // :184-192
     int more = 0, conv2 = 0;
     power_steps(Sj, m, v, r, sigma, w, part, scratch, 8 * max_steps, tol, more, conv2);
     steps += more;
@@ -705,6 +720,130 @@ __device__ void tridiag_power_wave(const double* alpha, const double* beta, int 
   const double inv = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
   if (e0 < len) y[e0] = y0 * inv;
   if (e1 < len) y[e1] = y1 * inv;
+}
+
+// Number of eigenvalues of T = tridiag(beta, alpha, beta) (len n) below x:
+// sign count of the LDL^T pivots (Sturm sequence), pivots guarded away
+// from zero by pivmin.
+__device__ int sturm_count(const double* alpha, const double* beta, int n, double x,
+                           double pivmin) {
+  double q = alpha[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  int c = q < 0.0;
+  for (int i = 1; i < n; ++i) {
+    q = (alpha[i] - x) - beta[i] * beta[i] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    c += q < 0.0;
+  }
+  return c;
+}
+
+// Largest eigenvalue theta of T and its unit eigenvector s (len n <= 128),
+// executed by wave 0: multisection on the Sturm count (64 points per
+// round) to full FP64 resolution, then inverse iteration with Gaussian
+// elimination with partial pivoting on T - theta I (two solves, lane 0).
+// wk holds 5 * kLzMax doubles of LDS.  Returns theta.
+__device__ double tridiag_top_eig_wave(const double* alpha, const double* beta, int n,
+                                       double* wk, double* s) {
+  const int l = threadIdx.x & 63;
+  double lo = 0.0, hi = 0.0, g = 0.0;
+  for (int e = l; e < n; e += 64) {
+    const double r = (e > 0 ? fabs(beta[e]) : 0.0) + (e + 1 < n ? fabs(beta[e + 1]) : 0.0);
+    const double a = alpha[e];
+    lo = (e == l) ? a - r : fmin(lo, a - r);
+    hi = (e == l) ? a + r : fmax(hi, a + r);
+    g = fmax(g, fabs(a) + r);
+  }
+  if (l >= n) {  // lanes without entries take lane 0's values below
+    lo = INFINITY;
+    hi = -INFINITY;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, o, 64));
+    hi = fmax(hi, __shfl_xor(hi, o, 64));
+    g = fmax(g, __shfl_xor(g, o, 64));
+  }
+  const double pivmin = fmax(g, 1e-300) * 1e-300;
+  double L = lo, H = hi;  // all eigenvalues lie in [L, H]
+  for (int round = 0; round < 16; ++round) {
+    if (!(H - L > 4.0 * DBL_EPSILON * fmax(fabs(L), fabs(H)))) break;
+    const double x = L + (H - L) * (double)(l + 1) / 65.0;
+    const bool above = sturm_count(alpha, beta, n, x, pivmin) >= n;  // x above the top
+    const unsigned long long mask = __ballot(above);
+    const int first = mask ? __builtin_ctzll(mask) : 64;  // first point above the top
+    const double nH = first < 64 ? L + (H - L) * (double)(first + 1) / 65.0 : H;
+    const double nL = first > 0 ? L + (H - L) * (double)first / 65.0 : L;
+    L = nL;
+    H = nH;
+  }
+  const double theta = 0.5 * (L + H);
+  if (l == 0) {
+    // inverse iteration: (T - theta I) s = rhs, LU with partial pivoting
+    double* dl = wk;               // sub-diagonal, then multipliers
+    double* d = wk + kLzMax;       // diagonal of U
+    double* du = wk + 2 * kLzMax;  // first super-diagonal of U
+    double* du2 = wk + 3 * kLzMax; // second super-diagonal of U
+    int* piv = reinterpret_cast<int*>(wk + 4 * kLzMax);  // 1: rows i, i+1 swapped
+    const double tiny = fmax(g, 1e-300) * DBL_EPSILON;
+    for (int i = 0; i < n; ++i) {
+      d[i] = alpha[i] - theta;
+      if (i + 1 < n) {
+        dl[i] = beta[i + 1];
+        du[i] = beta[i + 1];
+      }
+      du2[i] = 0.0;
+      piv[i] = 0;
+    }
+    for (int i = 0; i + 1 < n; ++i) {
+      if (fabs(d[i]) >= fabs(dl[i])) {
+        if (d[i] == 0.0) d[i] = tiny;
+        const double f = dl[i] / d[i];
+        dl[i] = f;
+        d[i + 1] -= f * du[i];
+      } else {
+        const double f = d[i] / dl[i];
+        d[i] = dl[i];
+        dl[i] = f;
+        const double t = du[i];
+        du[i] = d[i + 1];
+        d[i + 1] = t - f * d[i + 1];
+        if (i + 2 < n) {
+          du2[i] = du[i + 1];
+          du[i + 1] = -f * du[i + 1];
+        }
+        piv[i] = 1;
+      }
+    }
+    if (d[n - 1] == 0.0) d[n - 1] = tiny;
+    for (int i = 0; i < n; ++i) s[i] = 1.0;
+    for (int rep = 0; rep < 3; ++rep) {
+      for (int i = 0; i + 1 < n; ++i) {  // L y = P rhs
+        if (piv[i]) {
+          const double t = s[i];
+          s[i] = s[i + 1];
+          s[i + 1] = t - dl[i] * s[i + 1];
+        } else {
+          s[i + 1] -= dl[i] * s[i];
+        }
+      }
+      for (int i = n - 1; i >= 0; --i) {  // U x = y
+        double t = s[i];
+        if (i + 1 < n) t -= du[i] * s[i + 1];
+        if (i + 2 < n) t -= du2[i] * s[i + 2];
+        s[i] = t / d[i];
+      }
+      double nn = 0.0, mx = 0.0;
+      for (int i = 0; i < n; ++i) mx = fmax(mx, fabs(s[i]));
+      for (int i = 0; i < n; ++i) {
+        s[i] /= mx;
+        nn += s[i] * s[i];
+      }
+      const double inv = 1.0 / sqrt(nn);
+      for (int i = 0; i < n; ++i) s[i] *= inv;
+    }
+  }
+  return theta;
 }
 
 // Fused form for small m (S_j resident in L2): one block per cluster runs
@@ -870,6 +1009,7 @@ __global__ __launch_bounds__(kPowerThreads) void k_power_lanczos(
 // q_0 = power_v0 / |power_v0|) and k_lz_finish forms v = Q y.  State that
 // lives across launches (alpha, beta, y, step counters) is in global
 // memory, LzState per cluster.
+constexpr int kLzPower = 0;  // mode 0: M^I v0 (power iterate); mode 1: top eigenvector of M
 constexpr int kMvThreads = 256;
 constexpr int kMvRowsPerWave = 4;
 constexpr int kMvRows = kMvRowsPerWave * (kMvThreads / kWave);  // rows per block
@@ -920,8 +1060,8 @@ __global__ __launch_bounds__(kMvThreads) void k_cluster_matvec(
 
 // Starts the Lanczos process of every cluster from r = S 1 (in rbuf).
 __global__ __launch_bounds__(kPowerThreads) void k_lz_setup(
-    int m, const int* __restrict__ counts, uint64_t seed, int iter, const double* __restrict__ rbuf,
-    double* __restrict__ Qg, LzState* __restrict__ state) {
+    int m, const int* __restrict__ counts, const int* __restrict__ mask, uint64_t seed, int iter,
+    const double* __restrict__ rbuf, double* __restrict__ Qg, LzState* __restrict__ state) {
   __shared__ double scratch[2 * kPowerWaves];
   const int j = blockIdx.x;
   double* Q = Qg + (long)j * (kLzMax + 1) * m;
@@ -937,15 +1077,15 @@ __global__ __launch_bounds__(kPowerThreads) void k_lz_setup(
   if (threadIdx.x == 0) {
     LzState z = {};
     z.sigma = sigma;
-    z.done = counts[j] == 0;
+    z.done = counts[j] == 0 || (mask && !mask[j]);
     state[j] = z;
   }
 }
 
 // One Lanczos step of every active cluster, given w = S q_it in wbuf.
 __global__ __launch_bounds__(kPowerThreads) void k_lz_step(
-    int m, int I, double lz_tol, const double* __restrict__ rbuf, const double* __restrict__ wbuf,
-    double* __restrict__ Qg, LzState* __restrict__ state) {
+    int m, int mode, int I, double lz_tol, const double* __restrict__ rbuf,
+    const double* __restrict__ wbuf, double* __restrict__ Qg, LzState* __restrict__ state) {
   extern __shared__ double lz_sh[];
   double* q = lz_sh;              // m
   double* w = q + m;              // m
@@ -1044,20 +1184,30 @@ __global__ __launch_bounds__(kPowerThreads) void k_lz_step(
   const int jmax = m < kLzMax ? m : kLzMax;
   const bool breakdown = !(b > 1e-12 * sqrt(nw2));
   int finished = breakdown || len == jmax;
-  if (breakdown || len == jmax || (len >= 16 && (len & 7) == 0)) {
+  const bool check = mode == kLzPower ? (len >= 16 && (len & 7) == 0) : (len >= 4 && (len & 3) == 0);
+  if (breakdown || len == jmax || check) {
     __syncthreads();  // alpha[it] visible to wave 0
     if (wid == 0) {
       double* y = cf;  // kLzMax doubles of LDS
-      tridiag_power_wave(stp->alpha, stp->beta, len, I, y);
-      const int prev_len = stp->prev_len;
-      double d = 0.0;
-      for (int e = lane; e < len; e += 64) {
-        const double pv = e < prev_len ? stp->y[e] : 0.0;  // previous y, zero padded
-        const double dd = y[e] - pv;
-        d += dd * dd;
+      int conv;
+      if (mode == kLzPower) {
+        tridiag_power_wave(stp->alpha, stp->beta, len, I, y);
+        const int prev_len = stp->prev_len;
+        double d = 0.0;
+        for (int e = lane; e < len; e += 64) {
+          const double pv = e < prev_len ? stp->y[e] : 0.0;  // previous y, zero padded
+          const double dd = y[e] - pv;
+          d += dd * dd;
+        }
+        d = wave_sum_f64(d);
+        conv = breakdown || (prev_len > 0 && sqrt(d) <= lz_tol);
+      } else {
+        // top Ritz pair (theta, Q y); its residual |M Q y - theta Q y| is
+        // b |y[len-1]| (b = norm of the next Lanczos residual)
+        const double theta = tridiag_top_eig_wave(stp->alpha, stp->beta, len, part, y);
+        wave_lds_sync();
+        conv = breakdown || b * fabs(y[len - 1]) <= lz_tol * fabs(theta);
       }
-      d = wave_sum_f64(d);
-      const int conv = breakdown || (prev_len > 0 && sqrt(d) <= lz_tol);
       for (int e = lane; e < len; e += 64) stp->y[e] = y[e];
       if (lane == 0) {
         stp->prev_len = len;
@@ -1171,10 +1321,11 @@ static size_t power_smem(int m) {
 hipError_t launch_power(const float* S, int m, int k, const int* counts, const float* a0,
                         uint64_t seed, int iter, int max_steps, double tol, const double* vin,
                         const double* svin, const double* rsum, const int* steps_in,
-                        const int* conv_in, float* c_out, PowerInfo* info, hipStream_t st) {
+                        const int* conv_in, int eig_mode, const int* mask, int defer_fallback,
+                        float* c_out, PowerInfo* info, hipStream_t st) {
   hipLaunchKernelGGL(k_power, dim3(k), dim3(kPowerThreads), power_smem(m), st, S, m, counts,
-                     a0, seed, iter, max_steps, tol, vin, svin, rsum, steps_in, conv_in, c_out,
-                     info);
+                     a0, seed, iter, max_steps, tol, vin, svin, rsum, steps_in, conv_in, eig_mode,
+                     mask, defer_fallback, c_out, info);
   return hipGetLastError();
 }
 
@@ -1203,17 +1354,18 @@ hipError_t launch_cluster_matvec(const float* S, int m, int k, const double* x, 
   return hipGetLastError();
 }
 
-hipError_t launch_lz_setup(int m, int k, const int* counts, uint64_t seed, int iter,
-                           const double* rbuf, double* Q, LzState* state, hipStream_t st) {
-  hipLaunchKernelGGL(k_lz_setup, dim3(k), dim3(kPowerThreads), 0, st, m, counts, seed, iter,
+hipError_t launch_lz_setup(int m, int k, const int* counts, const int* mask, uint64_t seed,
+                           int iter, const double* rbuf, double* Q, LzState* state,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(k_lz_setup, dim3(k), dim3(kPowerThreads), 0, st, m, counts, mask, seed, iter,
                      rbuf, Q, state);
   return hipGetLastError();
 }
 
-hipError_t launch_lz_step(int m, int k, int I, double lz_tol, const double* rbuf,
+hipError_t launch_lz_step(int m, int k, int mode, int I, double lz_tol, const double* rbuf,
                           const double* wbuf, double* Q, LzState* state, hipStream_t st) {
-  hipLaunchKernelGGL(k_lz_step, dim3(k), dim3(kPowerThreads), lz_step_smem(m), st, m, I, lz_tol,
-                     rbuf, wbuf, Q, state);
+  hipLaunchKernelGGL(k_lz_step, dim3(k), dim3(kPowerThreads), lz_step_smem(m), st, m, mode, I,
+                     lz_tol, rbuf, wbuf, Q, state);
   return hipGetLastError();
 }
 

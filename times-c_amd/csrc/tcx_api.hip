@@ -49,7 +49,8 @@ enum Slot {
   kSlotCentNew, kSlotChanged, kSlotBlkCount, kSlotBlkBase, kSlotCounts, kSlotOffsets,
   kSlotPerm, kSlotBad, kSlotYhi, kSlotYlo, kSlotA0, kSlotItems, kSlotTiles, kSlotPart,
   kSlotS, kSlotInfo, kSlotDummy, kSlotWbuf, kSlotRbuf, kSlotBar, kSlotVout, kSlotSteps,
-  kSlotConv, kSlotErr, kSlotCmag, kSlotEvals, kSlotSvout, kSlotLzW, kSlotLzState, kNumSlots
+  kSlotConv, kSlotErr, kSlotCmag, kSlotEvals, kSlotSvout, kSlotLzW, kSlotLzState, kSlotMask,
+  kNumSlots
 };
 
 struct FitState {
@@ -260,8 +261,9 @@ struct RefineResult {
 // Returns on device the vectors v, S v, the row sums r = S 1, the Lanczos
 // step counts and the per-cluster acceptance flags.
 int power_lanczos(tcx_ctx* c, const float* S, int m, int k, const int* counts, uint64_t seed,
-                  int iter, int steps, const double** vout, const double** svout,
-                  const double** rout, const int** steps_out, const int** conv_out) {
+                  int iter, int steps, int mode, const int* mask, const double** vout,
+                  const double** svout, const double** rout, const int** steps_out,
+                  const int** conv_out) {
   hipStream_t st = c->stream;
   double *Q, *v, *sv, *r, *w;
   int *st_out, *cv_out;
@@ -274,7 +276,7 @@ int power_lanczos(tcx_ctx* c, const float* S, int m, int k, const int* counts, u
   TCX_TRY(ws(c, kSlotSteps, (size_t)k, &st_out));
   TCX_TRY(ws(c, kSlotConv, (size_t)k, &cv_out));
   TCX_TRY(ws(c, kSlotLzState, (size_t)k, &state));
-  if (m <= tcx::kFusedLanczosMaxM) {
+  if (mode == 0 && !mask && m <= tcx::kFusedLanczosMaxM) {
     // S_j fits in L2: one block per cluster runs every step in one launch
     TCX_HIP(tcx::launch_power_lanczos(S, m, k, counts, seed, iter, steps, 1e-11, Q, v, r, st_out,
                                       cv_out, st));
@@ -287,15 +289,17 @@ int power_lanczos(tcx_ctx* c, const float* S, int m, int k, const int* counts, u
     return TCX_OK;
   }
   TCX_HIP(tcx::launch_cluster_matvec(S, m, k, nullptr, 0, r, nullptr, st));  // r = S 1
-  TCX_HIP(tcx::launch_lz_setup(m, k, counts, seed, iter, r, Q, state, st));
+  TCX_HIP(tcx::launch_lz_setup(m, k, counts, mask, seed, iter, r, Q, state, st));
   const int jmax = m < tcx::kLanczosMax ? m : tcx::kLanczosMax;
   const long qstride = (long)(tcx::kLanczosMax + 1) * m;
   std::vector<int> done(k);
   for (int it = 0; it < jmax; ++it) {
     TCX_HIP(tcx::launch_cluster_matvec(S, m, k, Q + (long)it * m, qstride, w, state, st));
-    TCX_HIP(tcx::launch_lz_step(m, k, steps, 1e-11, r, w, Q, state, st));
+    TCX_HIP(tcx::launch_lz_step(m, k, mode, steps, mode == 0 ? 1e-11 : 1e-12, r, w, Q, state,
+                                st));
     const int len = it + 1;
-    if (len == jmax || (len >= 16 && (len & 7) == 0)) {
+    const bool check = mode == 0 ? (len >= 16 && (len & 7) == 0) : (len >= 8 && (len & 7) == 0);
+    if (len == jmax || check) {
       TCX_HIP(hipMemcpy2DAsync(done.data(), sizeof(int), &state[0].done, sizeof(tcx::LzState),
                                sizeof(int), k, hipMemcpyDeviceToHost, st));
       TCX_HIP(hipStreamSynchronize(st));
@@ -318,7 +322,7 @@ int power_lanczos(tcx_ctx* c, const float* S, int m, int k, const int* counts, u
 // labels, shifts and old-centroid zero flags; writes k*m new centroids.
 int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_label,
            const int* d_shift, const int* d_czero, float* d_cout, int iter, uint64_t seed,
-           int split, double tol, RefineResult* res) {
+           int split, double tol, int cal_select, RefineResult* res) {
   hipStream_t st = c->stream;
   c->fit.have_order = false;  // the row order is rewritten below
   long nblk = (n + tcx::kGroupChunk - 1) / tcx::kGroupChunk;
@@ -379,14 +383,39 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
   const double* rsum = nullptr;
   const int* steps_in = nullptr;
   const int* conv_in = nullptr;
+  // cal_select = 1: the top eigenvector of every M_j (eigh, shapeExtract.cuh:366-468 and
+  // test_timesC.py:184-192); 0: the power iterate, with the clusters that fail the
+  // power result's tests rerun through the eigenvector path.
+  const int mode = cal_select == 1 ? 1 : 0;
   if (nonempty > 0)
-    TCX_TRY(power_lanczos(c, S, m, k, counts, seed, iter, max_steps, &vin, &svin, &rsum,
-                          &steps_in, &conv_in));
+    TCX_TRY(power_lanczos(c, S, m, k, counts, seed, iter, max_steps, mode, nullptr, &vin, &svin,
+                          &rsum, &steps_in, &conv_in));
   TCX_HIP(tcx::launch_power(S, m, k, counts, a0, seed, iter, max_steps, tol, vin, svin, rsum,
-                            steps_in, conv_in, d_cout, info, st));
+                            steps_in, conv_in, mode, nullptr, 1, d_cout, info, st));
   std::vector<tcx::PowerInfo> h_info(k);
   TCX_HIP(hipMemcpyAsync(h_info.data(), info, sizeof(tcx::PowerInfo) * k, hipMemcpyDeviceToHost, st));
   TCX_HIP(hipStreamSynchronize(st));
+  if (mode == 0) {
+    std::vector<int> redo(k, 0);
+    bool any = false;
+    for (int j = 0; j < k; ++j)
+      if (h_info[j].fallback && !h_info[j].empty && !h_info[j].zero) {
+        redo[j] = 1;
+        any = true;
+      }
+    if (any) {
+      int* d_mask;
+      TCX_TRY(ws(c, kSlotMask, (size_t)k, &d_mask));
+      TCX_HIP(hipMemcpyAsync(d_mask, redo.data(), sizeof(int) * k, hipMemcpyHostToDevice, st));
+      TCX_TRY(power_lanczos(c, S, m, k, counts, seed, iter, max_steps, 1, d_mask, &vin, &svin,
+                            &rsum, &steps_in, &conv_in));
+      TCX_HIP(tcx::launch_power(S, m, k, counts, a0, seed, iter, max_steps, tol, vin, svin, rsum,
+                                steps_in, conv_in, 1, d_mask, 0, d_cout, info, st));
+      TCX_HIP(hipMemcpyAsync(h_info.data(), info, sizeof(tcx::PowerInfo) * k,
+                             hipMemcpyDeviceToHost, st));
+      TCX_HIP(hipStreamSynchronize(st));
+    }
+  }
   if (res) {
     res->n_pad = n_pad;
     res->sum_class = nonempty;
@@ -576,11 +605,14 @@ int tcx_assign(tcx_ctx* c, const float* d_x, const float* d_centroids, int n, in
 }
 
 int tcx_extract_shape(tcx_ctx* c, const int* d_label, const float* d_x, const int* d_shift,
-                      float* d_centroids, int n, int m, int k, int iter, int* sum_class) {
+                      float* d_centroids, int n, int m, int k, int iter, int cal_select,
+                      int* sum_class) {
   TCX_TRY(set_dev(c));
   TCX_TRY(check_series(n, m));
   if (!d_label || !d_x || !d_centroids || k < 1)
     return fail(TCX_ERR_ARG, "extract_shape: bad arguments");
+  if (cal_select != 0 && cal_select != 1)
+    return fail(TCX_ERR_ARG, "extract_shape: cal_select must be 0 or 1");
   float2* spec;
   float* norm;
   int* zero;
@@ -602,7 +634,8 @@ int tcx_extract_shape(tcx_ctx* c, const int* d_label, const float* d_x, const in
   float* cnew;
   TCX_TRY(ws(c, kSlotCentNew, (size_t)k * m, &cnew));
   RefineResult rr;
-  TCX_TRY(refine(c, d_x, n, m, k, d_label, d_shift, zero, cnew, iter, 0, 1, 1e-12, &rr));
+  TCX_TRY(refine(c, d_x, n, m, k, d_label, d_shift, zero, cnew, iter, 0, 1, 1e-12, cal_select,
+                 &rr));
   TCX_HIP(hipMemcpyAsync(d_centroids, cnew, sizeof(float) * k * m, hipMemcpyDeviceToDevice,
                          c->stream));
   TCX_HIP(hipStreamSynchronize(c->stream));
@@ -639,8 +672,8 @@ int tcx_fit_begin(tcx_ctx* c, const float* d_x, int n, int m, int k, const tcx_f
     return fail(TCX_ERR_ARG, "fit: unknown convention");
   if (o.s_precision != TCX_S_SPLIT_BF16 && o.s_precision != TCX_S_BF16)
     return fail(TCX_ERR_ARG, "fit: unknown s_precision");
-  if (o.cal_select != 0)
-    return fail(TCX_ERR_UNSUPPORTED, "cal_select=1 (eigen-solver path) is not supported yet");
+  if (o.cal_select != 0 && o.cal_select != 1)
+    return fail(TCX_ERR_ARG, "fit: cal_select must be 0 (power iteration) or 1 (eigenvector)");
   hipStream_t st = c->stream;
   TCX_TRY(ws(c, kSlotLabel, (size_t)n, &f.label));
   TCX_TRY(ws(c, kSlotShift, (size_t)n, &f.shift));
@@ -704,7 +737,7 @@ int fit_refine(tcx_ctx* c, FitState& f) {
   TCX_HIP(hipEventRecord(c->ev[0], st));
   RefineResult rr;
   TCX_TRY(refine(c, f.d_x, f.n, f.m, f.k, f.label, f.shift, f.zero, cnew, f.it, f.o.seed,
-                 f.split, f.tol, &rr));
+                 f.split, f.tol, f.o.cal_select, &rr));
   TCX_HIP(hipMemcpyAsync(f.cent, cnew, sizeof(float) * f.k * f.m, hipMemcpyDeviceToDevice, st));
   TCX_TRY(spectra(c, f.cent, f.k, f.m, &f.spec, &f.norm, &f.zero));
   TCX_HIP(hipEventRecord(c->ev[1], st));

@@ -108,7 +108,8 @@ hipError_t launch_syrk_reduce(const SyrkTile* tiles, int ntiles, const float* pa
 hipError_t launch_power(const float* S, int m, int k, const int* counts, const float* a0,
                         uint64_t seed, int iter, int max_steps, double tol, const double* vin,
                         const double* svin, const double* rsum, const int* steps_in,
-                        const int* conv_in, float* c_out, PowerInfo* info, hipStream_t st);
+                        const int* conv_in, int eig_mode, const int* mask, int defer_fallback,
+                        float* c_out, PowerInfo* info, hipStream_t st);
 
 // Lanczos evaluation of the power iterate M^I v0 (one block per cluster).
 // Lanczos evaluation of the power iterate (one step = two launches).
@@ -131,9 +132,13 @@ hipError_t launch_power_lanczos(const float* S, int m, int k, const int* counts,
                                 double* rout, int* steps_out, int* conv_out, hipStream_t st);
 hipError_t launch_cluster_matvec(const float* S, int m, int k, const double* x, long xstride,
                                  double* y, const LzState* state, hipStream_t st);
-hipError_t launch_lz_setup(int m, int k, const int* counts, uint64_t seed, int iter,
-                           const double* rbuf, double* Q, LzState* state, hipStream_t st);
-hipError_t launch_lz_step(int m, int k, int I, double lz_tol, const double* rbuf,
+// mask (may be NULL): clusters to run; others are left done.
+hipError_t launch_lz_setup(int m, int k, const int* counts, const int* mask, uint64_t seed,
+                           int iter, const double* rbuf, double* Q, LzState* state,
+                           hipStream_t st);
+// mode 0: power iterate M^I v0; mode 1: top eigenvector of M (tol on the
+// Ritz residual relative to the Ritz value).
+hipError_t launch_lz_step(int m, int k, int mode, int I, double lz_tol, const double* rbuf,
                           const double* wbuf, double* Q, LzState* state, hipStream_t st);
 hipError_t launch_lz_finish(int m, int k, const int* counts, const double* Q,
                             const LzState* state, double* vout, int* steps_out, int* conv_out,

@@ -127,7 +127,7 @@ def load():
         "tcx_ncc": (_I, [_P, _P, _P, _P, _I, _I, _I]),
         "tcx_sbd_align": (_I, [_P, _P, _P, _I, _I, _I]),
         "tcx_assign": (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
-        "tcx_extract_shape": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _I, ctypes.POINTER(_I)]),
+        "tcx_extract_shape": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, ctypes.POINTER(_I)]),
         "tcx_fit_opts_default": (None, [ctypes.POINTER(FitOpts)]),
         "tcx_kshape_fit": (_I, [_P, _P, _I, _I, _I, ctypes.POINTER(FitOpts), _P, _P,
                                 ctypes.POINTER(FitStats)]),
@@ -349,13 +349,13 @@ class Context:
                 if a is not None:
                     a.free()
 
-    def extract_shape(self, dlabel, dx, dshift, dcent, it=0):
+    def extract_shape(self, dlabel, dx, dshift, dcent, it=0, cal_select=0):
         n, m = dx.shape
         k = dcent.shape[0]
         sc = ctypes.c_int()
         check(self.lib.tcx_extract_shape(self.h, dlabel.ptr, dx.ptr,
                                          dshift.ptr if dshift is not None else None,
-                                         dcent.ptr, n, m, k, it, ctypes.byref(sc)))
+                                         dcent.ptr, n, m, k, it, cal_select, ctypes.byref(sc)))
         return sc.value
 
     def _opts(self, max_iter, init_mode, seed, convention, s_precision, power_tol):
@@ -390,11 +390,12 @@ class Context:
 
     def fit(self, dx, k, max_iter=10, init_mode=0, seed=0, init_labels=None, init_rows=None,
             convention=CONV_CPU, s_precision=S_SPLIT_BF16, power_tol=1e-12,
-            labels_per_iter=False):
+            labels_per_iter=False, cal_select=0):
         n, m = dx.shape
         o = FitOpts()
         self.lib.tcx_fit_opts_default(ctypes.byref(o))
         o.max_iter, o.init_mode, o.seed = max_iter, init_mode, seed
+        o.cal_select = cal_select
         o.convention, o.s_precision, o.power_tol = convention, s_precision, power_tol
         keep = []
         if init_labels is not None:
