@@ -142,6 +142,72 @@ def run_batch(timesc, ctx, jobs, workers):
     }
 
 
+def serve_workload(timesc, nreq, qps, seed=7):
+    """Poisson arrivals at `qps`: 30% fits (sizes of the uneven batch), 70%
+    assignment queries of 1..1024 series against one of 8 resident models
+    (m = round(exp(U[ln 64, ln 1024])), k ~ U{2..64})."""
+    rng = np.random.default_rng(seed)
+    models = []
+    for j in range(8):
+        m = int(round(math.exp(rng.uniform(math.log(64), math.log(1024)))))
+        k = int(rng.integers(2, 65))
+        C, _ = timesc.generate(2000 + j, k, m, k, 0.3)
+        C = (C - C.mean(1, keepdims=True)) / C.std(1, keepdims=True)
+        models.append(np.ascontiguousarray(C, np.float32))
+    reqs = []
+    t = 0.0
+    for i in range(nreq):
+        t += rng.exponential(1.0 / qps)
+        if rng.random() < 0.3:
+            n = int(rng.integers(100, 5001))
+            m = int(round(math.exp(rng.uniform(math.log(64), math.log(2048)))))
+            k = int(rng.integers(2, 33))
+            reqs.append((t, "fit", timesc.generate(3000 + i, n, m, k, 0.3)[0], k))
+        else:
+            j = int(rng.integers(0, len(models)))
+            n = int(rng.integers(1, 1025))
+            reqs.append((t, "assign", timesc.generate(3000 + i, n, models[j].shape[1], 4, 0.3)[0],
+                         j))
+    return models, reqs
+
+
+def run_serve(timesc, models, reqs, qps, fit_workers, query_workers):
+    """Replays the request stream twice on one server: once submitted at
+    once (sizes every worker's workspace), then at its arrival times.
+    Latency = submit -> result, per request kind."""
+    def submit(srv, mids, r):
+        _, kind, X, a = r
+        return srv.submit_fit(X, a) if kind == "fit" else srv.submit_assign(mids[a], X)
+
+    with timesc.Server(0, fit_workers=fit_workers, query_workers=query_workers) as srv:
+        mids = [srv.add_model(C) for C in models]
+        for h in [submit(srv, mids, r) for r in reqs]:
+            srv.wait(h)
+        hs = []
+        t0 = time.perf_counter()
+        for r in reqs:
+            delay = t0 + r[0] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            hs.append((r[1], submit(srv, mids, r)))
+        res = [(kind, srv.wait(h)) for kind, h in hs]
+        wall = time.perf_counter() - t0
+
+    def pct(kind):
+        lat = np.array([r["ms_latency"] for k, r in res if kind in (None, k)])
+        return {"count": int(lat.size), "p50_ms": float(np.percentile(lat, 50)),
+                "p99_ms": float(np.percentile(lat, 99))}
+
+    return {
+        "requests": len(reqs), "qps_offered": qps, "qps_served": len(reqs) / wall,
+        "fit_workers": fit_workers, "query_workers": query_workers,
+        "fit": pct("fit"), "assign": pct("assign"), "all": pct(None),
+        "workload": "Poisson arrivals; 30% fits (n~U{100..5000}, m=round(exp U[ln64,ln2048]), "
+                    "k~U{2..32}, <=10 iterations), 70% assignment queries (1..1024 series) "
+                    "against 8 resident models (m<=1024, k~U{2..64}); z-norm included",
+    }
+
+
 def copy_calibration(mib):
     """A plain copy of `mib` MiB (read once, written once), three launches."""
     import torch
@@ -169,6 +235,9 @@ def main():
     ap.add_argument("--batch-jobs", type=int, default=256)
     ap.add_argument("--batch-workers", type=int, default=8)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-qps", type=float, default=100.0)
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times "
                          "(the reference kernel for the HBM byte counters) and exit")
@@ -255,6 +324,11 @@ def main():
         out["batch"] = b
         out["p50_job_ms"] = b["p50_latency_ms"]
         out["p99_job_ms"] = b["p99_latency_ms"]
+    if not args.no_serve and args.serve_requests > 0:
+        print(f"# serving {args.serve_requests} requests at {args.serve_qps} qps", file=sys.stderr,
+              flush=True)
+        models, reqs = serve_workload(timesc, args.serve_requests, args.serve_qps)
+        out["serve"] = run_serve(timesc, models, reqs, args.serve_qps, 8, 4)
     ctx.close()
     if not args.no_cpu_baseline:
         print("# cpu baseline", file=sys.stderr, flush=True)

@@ -234,6 +234,60 @@ int tcx_batch_create(int device, int workers, size_t hbm_budget, tcx_batch** out
 int tcx_batch_run(tcx_batch* b, tcx_job* jobs, int njobs, int order);
 int tcx_batch_destroy(tcx_batch* b);
 
+/* -------------------------------------------------------- serving loop */
+/* A long-running server on one device: fit requests and assignment
+ * queries arrive asynchronously (tcx_server_submit), are admitted under an
+ * HBM budget and run on worker contexts -- fit workers take fits (and
+ * queries when no fit waits), query workers take only queries, so short
+ * queries are not stuck behind long fits.  Queries assign series to a
+ * resident model: centroids whose spectra stay in HBM.  No reference
+ * counterpart. */
+typedef struct tcx_server tcx_server;
+
+#define TCX_REQ_FIT 0
+#define TCX_REQ_ASSIGN 1
+
+typedef struct {
+  int kind;              /* TCX_REQ_FIT or TCX_REQ_ASSIGN                     */
+  const float* h_x;      /* host [n][m] series (copied at admission)          */
+  int n, m;
+  int znorm_ddof;        /* >= 0: z-normalize the series first; < 0: as is    */
+  /* fit */
+  int k;
+  tcx_fit_opts opts;
+  int publish_model;     /* 1: register the fitted centroids as a model       */
+  /* assign */
+  int model;             /* model id to assign against                        */
+  /* outputs (host buffers owned by the caller) */
+  int* h_labels;         /* n labels (required)                               */
+  float* h_centroids;    /* fit: k*m centroids (may be NULL)                  */
+  float* h_dist;         /* assign: n SBD distances (may be NULL)             */
+  int* h_shift;          /* assign: n lags of the maxima (may be NULL)        */
+  int model_out;         /* fit with publish_model: the new model id          */
+  tcx_fit_stats stats;   /* fit statistics                                    */
+  int status;            /* TCX_OK or the request's error code                */
+  int done;              /* set to 1 when the request has finished            */
+  double ms_queued;      /* submit -> start (host clock)                      */
+  double ms_run;         /* start -> end                                      */
+  double ms_latency;     /* submit -> end                                     */
+  char message[160];     /* error message ("" if none)                        */
+} tcx_request;
+
+/* fit_workers + query_workers >= 1 worker contexts on `device`.
+ * hbm_budget (0 = none) bounds the bytes of admitted requests' series and
+ * results plus resident models. */
+int tcx_server_create(int device, int fit_workers, int query_workers, size_t hbm_budget,
+                      tcx_server** out);
+/* Uploads k centroids (k*m, host) as a resident model; *model_id out. */
+int tcx_server_add_model(tcx_server* s, const float* h_centroids, int k, int m, int* model_id);
+/* Queues a request (the struct and its buffers must stay valid until it is
+ * done).  Returns at once; errors found at submission are returned here. */
+int tcx_server_submit(tcx_server* s, tcx_request* req);
+/* Blocks until req is done; returns its status. */
+int tcx_server_wait(tcx_server* s, tcx_request* req);
+/* Finishes queued requests, stops the workers and frees the models. */
+int tcx_server_destroy(tcx_server* s);
+
 /* ------------------------------------------------------------- host I/O */
 
 /* Labeled CSV "label,v1,...,vm" per line.  *data and *labels are

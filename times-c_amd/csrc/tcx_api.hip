@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "tcx_host.hpp"
+#include "tcx_internal.hpp"
 #include "tcx_kernels.hpp"
 #include "timesc.h"
 
@@ -190,14 +191,14 @@ int spectra(tcx_ctx* c, const float* d_rows, int rows, int m, float2** spec, flo
 int assign_core(tcx_ctx* c, const float* d_x, long n, int m, int k, const float2* cspec,
                 const float* cnorm, int rev, int* label, float* dist, int* shift, float* dmat,
                 int* changed, const int* order = nullptr, long npairs = 0,
-                unsigned long long* evals = nullptr) {
+                unsigned long long* evals = nullptr, const float* cmag = nullptr) {
   int F = fft_size(m);
   const float2* tw;
   TCX_TRY(twiddles(c, F, &tw));
   if (!dmat && k <= tcx::prune_max_k(F)) {
     tcx::PruneArgs p;
     p.x = d_x; p.n = n; p.m = m; p.k = k; p.cspec = cspec; p.cnorm = cnorm; p.tw = tw;
-    p.cmag = reinterpret_cast<const float*>(c->slot_ptr[kSlotCmag]);
+    p.cmag = cmag ? cmag : reinterpret_cast<const float*>(c->slot_ptr[kSlotCmag]);
     p.rev = rev; p.label = label; p.dist = dist; p.shift = shift; p.changed = changed;
     p.order = order;
     p.npairs = order ? npairs : (n + 1) / 2;
@@ -428,6 +429,39 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
 }
 
 }  // namespace
+
+// ============================================================== internal
+namespace tcx {
+
+int fft_length(int m) { return fft_size(m); }
+
+int model_spectra(tcx_ctx* c, const float* d_cent, int k, int m, void* spec, float* mag,
+                  float* norm, int* zero) {
+  TCX_TRY(set_dev(c));
+  TCX_TRY(check_series(k, m));
+  const int F = fft_size(m);
+  const float2* tw;
+  TCX_TRY(twiddles(c, F, &tw));
+  TCX_HIP(launch_spectra(F, d_cent, k, m, tw, static_cast<float2*>(spec), norm, zero, mag,
+                         c->stream));
+  return TCX_OK;
+}
+
+int assign_resident(tcx_ctx* c, const float* d_x, long n, int m, int k, const void* spec,
+                    const float* mag, const float* norm, int convention, int* label, float* dist,
+                    int* shift) {
+  TCX_TRY(set_dev(c));
+  TCX_TRY(check_series(n, m));
+  return assign_core(c, d_x, n, m, k, static_cast<const float2*>(spec), norm, convention, label,
+                     dist, shift, nullptr, nullptr, nullptr, 0, nullptr, mag);
+}
+
+int stream_sync(tcx_ctx* c) {
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TCX_OK;
+}
+
+}  // namespace tcx
 
 // ======================================================================= ABI
 extern "C" {

@@ -1,0 +1,25 @@
+// Engine entry points shared between the library's own translation units
+// (not part of the C ABI): resident centroid spectra for the serving loop.
+#pragma once
+#include <stddef.h>
+
+#include "timesc.h"
+
+namespace tcx {
+
+// FFT length of series of length m: 1 << (2m-1).bit_length().
+int fft_length(int m);
+// Spectra of k centroid rows d_cent (k*m) into caller-owned device buffers:
+// spec k*F complex FP32 (2 floats each), mag k*F, norm k, zero k.
+// Asynchronous on the context's stream.
+int model_spectra(tcx_ctx* c, const float* d_cent, int k, int m, void* spec, float* mag,
+                  float* norm, int* zero);
+// Pruned (or, beyond the pruned kernel's k limit, full) SBD assignment of
+// n series against resident spectra; asynchronous on the context's stream.
+int assign_resident(tcx_ctx* c, const float* d_x, long n, int m, int k, const void* spec,
+                    const float* mag, const float* norm, int convention, int* label, float* dist,
+                    int* shift);
+// Waits for the context's stream.
+int stream_sync(tcx_ctx* c);
+
+}  // namespace tcx

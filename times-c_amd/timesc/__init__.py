@@ -29,6 +29,8 @@ EXPORTS = [
     "tcx_fit_opts_default", "tcx_kshape_fit", "tcx_fit_begin", "tcx_fit_step", "tcx_fit_end",
     "tcx_fit_query",
     "tcx_kshape", "tcx_kshape_tsl", "tcx_batch_create", "tcx_batch_run", "tcx_batch_destroy",
+    "tcx_server_create", "tcx_server_add_model", "tcx_server_submit", "tcx_server_wait",
+    "tcx_server_destroy",
     "tcx_read_labeled_csv", "tcx_read_unlabeled_csv", "tcx_write_labels", "tcx_free_host",
     "tcx_generate", "tcx_init_labels", "tcx_init_rows", "tcx_power_v0",
 ]
@@ -71,6 +73,36 @@ class FitStats(ctypes.Structure):
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+REQ_FIT, REQ_ASSIGN = 0, 1
+
+
+class Request(ctypes.Structure):
+    """tcx_request: one fit or assignment query of a server."""
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("h_x", ctypes.c_void_p),
+        ("n", ctypes.c_int),
+        ("m", ctypes.c_int),
+        ("znorm_ddof", ctypes.c_int),
+        ("k", ctypes.c_int),
+        ("opts", FitOpts),
+        ("publish_model", ctypes.c_int),
+        ("model", ctypes.c_int),
+        ("h_labels", ctypes.c_void_p),
+        ("h_centroids", ctypes.c_void_p),
+        ("h_dist", ctypes.c_void_p),
+        ("h_shift", ctypes.c_void_p),
+        ("model_out", ctypes.c_int),
+        ("stats", FitStats),
+        ("status", ctypes.c_int),
+        ("done", ctypes.c_int),
+        ("ms_queued", ctypes.c_double),
+        ("ms_run", ctypes.c_double),
+        ("ms_latency", ctypes.c_double),
+        ("message", ctypes.c_char * 160),
+    ]
 
 
 _lib = None
@@ -141,6 +173,11 @@ def load():
         "tcx_batch_create": (_I, [_I, _I, _Z, ctypes.POINTER(_P)]),
         "tcx_batch_run": (_I, [_P, ctypes.POINTER(Job), _I, _I]),
         "tcx_batch_destroy": (_I, [_P]),
+        "tcx_server_create": (_I, [_I, _I, _I, _Z, ctypes.POINTER(_P)]),
+        "tcx_server_add_model": (_I, [_P, _P, _I, _I, ctypes.POINTER(_I)]),
+        "tcx_server_submit": (_I, [_P, ctypes.POINTER(Request)]),
+        "tcx_server_wait": (_I, [_P, ctypes.POINTER(Request)]),
+        "tcx_server_destroy": (_I, [_P]),
         "tcx_read_labeled_csv": (_I, [ctypes.c_char_p, ctypes.POINTER(_P), ctypes.POINTER(_P),
                                       ctypes.POINTER(_I), ctypes.POINTER(_I)]),
         "tcx_read_unlabeled_csv": (_I, [ctypes.c_char_p, ctypes.POINTER(_P),
@@ -481,3 +518,74 @@ class Batch:
                         "ms_queued": j.ms_queued, "ms_run": j.ms_run,
                         "ms_latency": j.ms_latency})
         return res
+
+
+class Server:
+    """Serving loop: asynchronous fits and assignment queries against
+    resident models (tcx_server_*)."""
+
+    def __init__(self, device=0, fit_workers=4, query_workers=2, budget=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        check(self.lib.tcx_server_create(device, fit_workers, query_workers, budget,
+                                         ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            check(self.lib.tcx_server_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add_model(self, centroids):
+        c = np.ascontiguousarray(centroids, np.float32)
+        mid = ctypes.c_int()
+        check(self.lib.tcx_server_add_model(self.h, _ptr(c), c.shape[0], c.shape[1],
+                                            ctypes.byref(mid)))
+        return mid.value
+
+    def _submit(self, req, keep):
+        check(self.lib.tcx_server_submit(self.h, ctypes.byref(req)))
+        return {"req": req, "keep": keep}
+
+    def submit_fit(self, X, k, znorm_ddof=0, publish=False, max_iter=10, seed=0, init_mode=0,
+                   cal_select=0):
+        X = np.ascontiguousarray(X, np.float32)
+        n, m = X.shape
+        r = Request()
+        r.kind, r.h_x, r.n, r.m, r.znorm_ddof = REQ_FIT, _ptr(X), n, m, znorm_ddof
+        r.k, r.publish_model = k, int(publish)
+        self.lib.tcx_fit_opts_default(ctypes.byref(r.opts))
+        r.opts.max_iter, r.opts.seed, r.opts.init_mode = max_iter, seed, init_mode
+        r.opts.cal_select = cal_select
+        lab = np.empty(n, np.int32)
+        cent = np.empty((k, m), np.float32)
+        r.h_labels, r.h_centroids = _ptr(lab), _ptr(cent)
+        return self._submit(r, {"X": X, "labels": lab, "centroids": cent})
+
+    def submit_assign(self, model, X, znorm_ddof=0):
+        X = np.ascontiguousarray(X, np.float32)
+        n, m = X.shape
+        r = Request()
+        r.kind, r.h_x, r.n, r.m, r.znorm_ddof, r.model = REQ_ASSIGN, _ptr(X), n, m, znorm_ddof, model
+        lab = np.empty(n, np.int32)
+        dist = np.empty(n, np.float32)
+        sh = np.empty(n, np.int32)
+        r.h_labels, r.h_dist, r.h_shift = _ptr(lab), _ptr(dist), _ptr(sh)
+        return self._submit(r, {"X": X, "labels": lab, "dist": dist, "shift": sh})
+
+    def wait(self, handle, raise_on_error=True):
+        r = handle["req"]
+        rc = self.lib.tcx_server_wait(self.h, ctypes.byref(r))
+        if rc != OK and raise_on_error:
+            check(rc)
+        out = {key: v for key, v in handle["keep"].items() if key != "X"}
+        out.update({"status": r.status, "message": r.message.decode(), "ms_queued": r.ms_queued,
+                    "ms_run": r.ms_run, "ms_latency": r.ms_latency,
+                    "model_out": r.model_out, "stats": r.stats.as_dict()})
+        return out
