@@ -290,7 +290,8 @@ int tcx_server_destroy(tcx_server* s);
 
 /* ------------------------------------------------------------- host I/O */
 
-/* Labeled CSV "label,v1,...,vm" per line.  *data and *labels are
+/* Labeled CSV "label,v1,...,vm" per line, parsed by several threads
+ * (same values as a sequential atof/stoi parse).  *data and *labels are
  * allocated with malloc (free with tcx_free_host).
  * Replaces readFile<float> (Code/include/readfile.cuh:12-67). */
 int tcx_read_labeled_csv(const char* path, float** data, int** labels,
@@ -302,6 +303,16 @@ int tcx_read_unlabeled_csv(const char* path, float** data, int* rows,
 /* "l,l,...,l," on one line.
  * Replaces writeFile<int> (Code/include/readfile.cuh:120-129). */
 int tcx_write_labels(const char* path, const int* labels, int n);
+/* Zero-based labels one per line, the CPU path's output
+ * (np.savetxt(..., fmt='%.0f'), test_timesC.py:324-325, 342-343). */
+int tcx_write_labels_lines(const char* path, const int* labels, int n);
+/* Binary matrix: "TCXB0001", int64 rows, cols, labeled flag, then int32
+ * labels (when labeled) and float32 values, little-endian.  labels may be
+ * NULL on write (unlabeled) and on read (labels ignored); *labels is set to
+ * NULL for an unlabeled file.  No reference counterpart (I/O fast path). */
+int tcx_write_binary(const char* path, const float* data, const int* labels, int rows,
+                     int cols);
+int tcx_read_binary(const char* path, float** data, int** labels, int* rows, int* cols);
 void tcx_free_host(void* p);
 
 /* --------------------------------------------- synthetic data (host) */

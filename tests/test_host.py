@@ -138,6 +138,72 @@ def test_missing_file_is_an_error():
     assert e.value.code == T.ERR_IO
 
 
+def test_value_parsing_like_atof(tmp_path):
+    """Each value as atof then float: leading blanks, '+', exponents,
+    inf/nan, hex floats, a valid prefix before junk, no digits -> 0,
+    out-of-range magnitudes."""
+    fields = ["1.5", "  -2.25", "+3", "1e-3", "6.02E23", "inf", "-Infinity", "0x1p3", "-0x1.8p1",
+              "3.5abc", "abc", "", "-", "+-1", "1e400", "-1e400", "1e-400", "0.1", "-0",
+              "123456789.123456789"]
+    expect = [1.5, -2.25, 3.0, 1e-3, 6.02e23, np.inf, -np.inf, 8.0, -3.0, 3.5, 0.0, 0.0, 0.0, 0.0,
+              np.inf, -np.inf, 0.0, 0.1, -0.0, 123456789.123456789]
+    p = tmp_path / "v.csv"
+    p.write_text("7," + ",".join(fields) + "\n" + "8," + ",".join(["nan"] * len(fields)) + "\n")
+    X, y = T.read_labeled_csv(str(p))
+    assert list(y) == [7, 8]
+    assert np.array_equal(X[0], np.array(expect, np.float32))
+    assert np.signbit(X[0][18]) and np.all(np.isnan(X[1]))
+
+
+def test_large_csv_parsed_in_parallel_chunks(tmp_path):
+    """A file of several 4 MB chunks: values and labels equal a plain
+    sequential parse (Python float -> float32), and a ragged line in a later
+    chunk is reported with its file line number."""
+    rng = np.random.default_rng(1)
+    n, m = 9000, 120
+    X = rng.standard_normal((n, m)) * 10.0 ** rng.integers(-3, 4, (n, 1))
+    y = rng.integers(-5, 100, n)
+    lines = [f"{y[i]}," + ",".join(repr(float(v)) for v in X[i]) for i in range(n)]
+    p = tmp_path / "big.csv"
+    p.write_text("\n".join(lines) + "\n")
+    assert p.stat().st_size > 3 * (4 << 20)
+    got, lab = T.read_labeled_csv(str(p))
+    assert np.array_equal(lab, y.astype(np.int32))
+    assert np.array_equal(got, X.astype(np.float32))
+    lines[7000] = lines[7000] + ",1.0"
+    p.write_text("\n".join(lines) + "\n")
+    with pytest.raises(T.TimescError) as e:
+        T.read_labeled_csv(str(p))
+    assert e.value.code == T.ERR_IO and "line 7001" in str(e.value)
+
+
+def test_binary_matrix_roundtrip_and_errors(tmp_path):
+    X, y = T.generate(2, 50, 33, 3, 0.3)
+    p = str(tmp_path / "d.tcxb")
+    T.write_binary(p, X, y)
+    X2, y2 = T.read_binary(p)
+    assert np.array_equal(X, X2) and np.array_equal(y, y2)
+    T.write_binary(p, X)
+    X3, y3 = T.read_binary(p)
+    assert np.array_equal(X, X3) and y3 is None
+    raw = open(p, "rb").read()
+    open(p, "wb").write(raw[:-4])
+    with pytest.raises(T.TimescError) as e:
+        T.read_binary(p)
+    assert e.value.code == T.ERR_IO and "truncated" in str(e.value)
+    open(p, "wb").write(b"NOTTCXB!" + raw[8:])
+    with pytest.raises(T.TimescError):
+        T.read_binary(p)
+
+
+def test_write_labels_lines_format(tmp_path):
+    """The CPU path's output: zero-based, one per line (np.savetxt fmt %.0f)."""
+    p = str(tmp_path / "t.txt")
+    T.write_labels_lines(p, [0, 3, 11])
+    assert open(p).read() == "0\n3\n11\n"
+    assert np.array_equal(np.loadtxt(p), [0, 3, 11])
+
+
 def test_write_labels_format(tmp_path):
     p = str(tmp_path / "r.txt")
     T.write_labels(p, [1, 2, 10])

@@ -23,6 +23,10 @@
 //   --dump-iters=PATH   labels after every iteration, one line each
 //   --centroids=PATH    final centroids, one CSV line each
 //   --budget=BYTES      HBM budget of the context
+//   --cpu-format        write zero-based labels one per line, the CPU
+//                       path's output (test_timesC.py:324-325)
+// <file> may also be a binary matrix (tcx_write_binary, "TCXB0001" magic);
+// flag 0 then needs the file's labels.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -64,7 +68,7 @@ int main(int argc, char** argv) {
                  "usage: %s -d <dev> <file> <k> <flag> <init_chose> [--out=PATH] "
                  "[--max-iter=N] [--seed=S] [--ddof=D] [--convention=cpu|kshape] "
                  "[--precision=split|bf16] [--dump-iters=PATH] [--centroids=PATH] "
-                 "[--budget=BYTES]\n",
+                 "[--budget=BYTES] [--cpu-format]\n",
                  argv[0]);
     return 2;
   }
@@ -77,6 +81,7 @@ int main(int argc, char** argv) {
   int max_iter = -1, ddof = 0;
   uint64_t seed = 0;
   size_t budget = 0;
+  bool cpu_format = false;
   tcx_fit_opts o;
   tcx_fit_opts_default(&o);
   for (int i = 7; i < argc; ++i) {
@@ -90,6 +95,7 @@ int main(int argc, char** argv) {
     else if (opt(argv[i], "--dump-iters", &v)) dump = v;
     else if (opt(argv[i], "--centroids", &v)) cent_path = v;
     else if (opt(argv[i], "--budget", &v)) budget = std::strtoull(v.c_str(), nullptr, 10);
+    else if (std::strcmp(argv[i], "--cpu-format") == 0) cpu_format = true;
     else {
       std::fprintf(stderr, "unknown option %s\n", argv[i]);
       return 2;
@@ -101,8 +107,21 @@ int main(int argc, char** argv) {
   float* h = nullptr;
   int* lab = nullptr;
   int rows = 0, cols = 0, iters_default;
+  bool binary = false;
+  if (FILE* f = std::fopen(file.c_str(), "rb")) {
+    char magic[8] = {0};
+    binary = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "TCXB0001", 8) == 0;
+    std::fclose(f);
+  }
+  if (binary) {
+    CHECKRC(tcx_read_binary(file.c_str(), &h, &lab, &rows, &cols));
+    if (flag == 0 && !lab) {
+      std::fprintf(stderr, "flag 0 needs labels; %s has none\n", file.c_str());
+      return 2;
+    }
+  }
   if (flag == 0) {
-    CHECKRC(tcx_read_labeled_csv(file.c_str(), &h, &lab, &rows, &cols));
+    if (!binary) CHECKRC(tcx_read_labeled_csv(file.c_str(), &h, &lab, &rows, &cols));
     int mx = lab[0], mn = lab[0];
     for (int i = 1; i < rows; ++i) {
       mx = lab[i] > mx ? lab[i] : mx;
@@ -113,7 +132,7 @@ int main(int argc, char** argv) {
     else if (mn == -1) k = 2;
     iters_default = 10;
   } else {
-    CHECKRC(tcx_read_unlabeled_csv(file.c_str(), &h, &rows, &cols));
+    if (!binary) CHECKRC(tcx_read_unlabeled_csv(file.c_str(), &h, &rows, &cols));
     iters_default = 20;
   }
   std::printf("mat_rows%d mat_cols%d\n", rows, cols);
@@ -165,8 +184,12 @@ int main(int argc, char** argv) {
     std::printf("total time run on GPU: %f msecs.\n",
                 std::chrono::duration<double, std::milli>(t2 - t0).count());
   }
-  for (int& l : labels) l += 1;  // one-based output, kshape.cuh:91
-  CHECKRC(tcx_write_labels(out.c_str(), labels.data(), rows));
+  if (cpu_format) {
+    CHECKRC(tcx_write_labels_lines(out.c_str(), labels.data(), rows));  // test_timesC.py:324-325
+  } else {
+    for (int& l : labels) l += 1;  // one-based output, kshape.cuh:91
+    CHECKRC(tcx_write_labels(out.c_str(), labels.data(), rows));
+  }
   if (!dump.empty()) {
     FILE* f = std::fopen(dump.c_str(), "wb");
     if (!f) die("open --dump-iters file", TCX_ERR_IO);

@@ -31,7 +31,8 @@ EXPORTS = [
     "tcx_kshape", "tcx_kshape_tsl", "tcx_batch_create", "tcx_batch_run", "tcx_batch_destroy",
     "tcx_server_create", "tcx_server_add_model", "tcx_server_submit", "tcx_server_wait",
     "tcx_server_destroy",
-    "tcx_read_labeled_csv", "tcx_read_unlabeled_csv", "tcx_write_labels", "tcx_free_host",
+    "tcx_read_labeled_csv", "tcx_read_unlabeled_csv", "tcx_write_labels",
+    "tcx_write_labels_lines", "tcx_write_binary", "tcx_read_binary", "tcx_free_host",
     "tcx_generate", "tcx_init_labels", "tcx_init_rows", "tcx_power_v0",
 ]
 
@@ -183,6 +184,10 @@ def load():
         "tcx_read_unlabeled_csv": (_I, [ctypes.c_char_p, ctypes.POINTER(_P),
                                         ctypes.POINTER(_I), ctypes.POINTER(_I)]),
         "tcx_write_labels": (_I, [ctypes.c_char_p, _P, _I]),
+        "tcx_write_labels_lines": (_I, [ctypes.c_char_p, _P, _I]),
+        "tcx_write_binary": (_I, [ctypes.c_char_p, _P, _P, _I, _I]),
+        "tcx_read_binary": (_I, [ctypes.c_char_p, ctypes.POINTER(_P), ctypes.POINTER(_P),
+                                 ctypes.POINTER(_I), ctypes.POINTER(_I)]),
         "tcx_free_host": (None, [_P]),
         "tcx_generate": (_I, [ctypes.c_uint64, _L, _I, _I, ctypes.c_double, _P, _P, _P]),
         "tcx_init_labels": (_I, [ctypes.c_uint64, _L, _I, _P]),
@@ -248,6 +253,37 @@ def read_labeled_csv(path):
     finally:
         lib.tcx_free_host(d)
         lib.tcx_free_host(l)
+
+
+def write_labels_lines(path, labels):
+    lab = np.ascontiguousarray(labels, np.int32)
+    check(load().tcx_write_labels_lines(path.encode(), _ptr(lab), lab.size))
+
+
+def write_binary(path, X, labels=None):
+    X = np.ascontiguousarray(X, np.float32)
+    lab = None if labels is None else np.ascontiguousarray(labels, np.int32)
+    check(load().tcx_write_binary(path.encode(), _ptr(X), _ptr(lab), X.shape[0], X.shape[1]))
+
+
+def read_binary(path):
+    """(X float32 [rows, cols], labels int32 [rows] or None)."""
+    lib = load()
+    d, l = ctypes.c_void_p(), ctypes.c_void_p()
+    r, c = ctypes.c_int(), ctypes.c_int()
+    check(lib.tcx_read_binary(path.encode(), ctypes.byref(d), ctypes.byref(l), ctypes.byref(r),
+                              ctypes.byref(c)))
+    try:
+        X = np.ctypeslib.as_array((ctypes.c_float * (r.value * c.value)).from_address(d.value))
+        X = X.reshape(r.value, c.value).copy()
+        y = None
+        if l.value:
+            y = np.ctypeslib.as_array((ctypes.c_int * r.value).from_address(l.value)).copy()
+        return X, y
+    finally:
+        lib.tcx_free_host(d)
+        if l.value:
+            lib.tcx_free_host(l)
 
 
 def read_unlabeled_csv(path):
