@@ -89,3 +89,23 @@ def test_cli_writes_the_fit_labels(ctx, tmp_path, init_chose):
     assert np.array_equal(cli, res["labels"] + 1)
     rows = [list(map(int, line.split(","))) for line in open(dump).read().split()]
     assert np.array_equal(np.array(rows), res["labels_per_iter"])
+
+
+def test_cli_binary_input_and_cpu_label_format(ctx, tmp_path):
+    """The same data as a binary matrix gives the same labels as the CSV;
+    --cpu-format writes them zero-based, one per line."""
+    X, y = T.generate(5, 240, 64, 3, 0.3)
+    csv, binf = str(tmp_path / "d.csv"), str(tmp_path / "d.tcxb")
+    T.write_labeled_csv(csv, X, y)
+    T.write_binary(binf, X, y)
+    outs = []
+    for path, extra in ((csv, []), (binf, []), (binf, ["--cpu-format"])):
+        out = str(tmp_path / f"r{len(outs)}.txt")
+        r = subprocess.run([T.CLI_PATH, "-d", "0", path, "3", "0", "0", "--out=" + out] + extra,
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs.append(open(out).read())
+    assert outs[0] == outs[1]
+    one_based = np.array([int(v) for v in outs[0][:-1].split(",")])
+    zero_based = np.array([int(v) for v in outs[2].split()])
+    assert outs[2].endswith("\n") and np.array_equal(zero_based + 1, one_based)
