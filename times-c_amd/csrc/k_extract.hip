@@ -53,29 +53,62 @@ __global__ __launch_bounds__(256) void k_group_count(const int* __restrict__ lab
   for (int j = threadIdx.x; j < k; j += blockDim.x) blk_count[(long)blockIdx.x * k + j] = cnt[j];
 }
 
-// Exclusive scan over blocks per cluster; cluster sizes; padded segment
-// offsets (each cluster's columns start at a multiple of kPackPad).
+// Exclusive scan over blocks for one cluster per block (blockIdx.x = j):
+// blk_base[b][j] = sum of blk_count[b'][j] for b' < b; counts[j] = total.
+// Each of the 256 threads scans a contiguous run of blocks, then the run
+// totals are scanned in LDS.
 __global__ __launch_bounds__(256) void k_group_scan(int nblk, int k,
                                                     const int* __restrict__ blk_count,
                                                     int* __restrict__ blk_base,
-                                                    int* __restrict__ counts,
-                                                    long* __restrict__ offsets) {
-  for (int j = threadIdx.x; j < k; j += blockDim.x) {
-    int run = 0;
-    for (int b = 0; b < nblk; ++b) {
-      blk_base[(long)b * k + j] = run;
-      run += blk_count[(long)b * k + j];
-    }
-    counts[j] = run;
+                                                    int* __restrict__ counts) {
+  __shared__ int tot[256];
+  const int j = blockIdx.x, t = threadIdx.x;
+  const int per = (nblk + 255) / 256;
+  const int b0 = t * per, b1 = b0 + per < nblk ? b0 + per : nblk;
+  int run = 0;
+  for (int b = b0; b < b1; ++b) run += blk_count[(long)b * k + j];
+  tot[t] = run;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {  // inclusive Hillis-Steele scan
+    const int v = t >= o ? tot[t - o] : 0;
+    __syncthreads();
+    tot[t] += v;
+    __syncthreads();
+  }
+  run = t > 0 ? tot[t - 1] : 0;
+  for (int b = b0; b < b1; ++b) {
+    blk_base[(long)b * k + j] = run;
+    run += blk_count[(long)b * k + j];
+  }
+  if (t == 255) counts[j] = tot[255];
+}
+
+// Padded segment offsets: each cluster's columns start at a multiple of
+// kPackPad.  One block; the k cluster sizes are scanned 1024 at a time.
+__global__ __launch_bounds__(1024) void k_group_offsets(int k, const int* __restrict__ counts,
+                                                        long* __restrict__ offsets) {
+  __shared__ long tot[1024];
+  __shared__ long carry;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    carry = 0;
+    offsets[0] = 0;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    long o = 0;
-    offsets[0] = 0;
-    for (int j = 0; j < k; ++j) {
-      o += ((long)counts[j] + kPackPad - 1) / kPackPad * kPackPad;
-      offsets[j + 1] = o;
+  for (int j0 = 0; j0 < k; j0 += 1024) {
+    const int j = j0 + t;
+    tot[t] = j < k ? ((long)counts[j] + kPackPad - 1) / kPackPad * kPackPad : 0;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      const long v = t >= o ? tot[t - o] : 0;
+      __syncthreads();
+      tot[t] += v;
+      __syncthreads();
     }
+    if (j < k) offsets[j + 1] = carry + tot[t];
+    __syncthreads();
+    if (t == 0) carry += tot[1023];
+    __syncthreads();
   }
 }
 
@@ -126,58 +159,131 @@ __device__ __forceinline__ int cluster_of(long p, const long* offsets, int k) {
   return lo;
 }
 
+// FP64 sum and sum of squares of every row of x (one wave per row).  The
+// shape extraction derives each aligned member's mean and variance from
+// these and the few values its shift drops, so it reads x once per refine.
+__global__ __launch_bounds__(256) void k_row_stats(const float* __restrict__ x, long rows, int m,
+                                                   double2* __restrict__ stats) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= rows) return;
+  const float* r = x + row * (long)m;
+  double s = 0.0, q = 0.0;
+  for (int t = lane; t < m; t += kWave) {
+    const double v = (double)r[t];
+    s += v;
+    q += v * v;
+  }
+  s = wave_sum_f64(s);
+  q = wave_sum_f64(q);
+  if (lane == 0) stats[row] = make_double2(s, q);
+}
+
 // Block of 64 packed columns.  For each column (a member of cluster cl):
 // a = roll_zeropad(x_row, s) with s = shift[row] unless the old centroid of
 // cl is all zero (test_timesC.py:128-132); y = zscore(a, ddof=1) with FP64
 // statistics (:142); y -> (hi, lo) BF16 with y ~= hi + lo, written
 // transposed: Y[t][p].  The first column of a cluster (its lowest-index
 // member) also writes its aligned row to a0 for the sign fix (:163).
+// The statistics of a come from the row's sum and sum of squares (stats)
+// minus the |s| values the shift drops (or, when most values are dropped,
+// from the kept values); where the one-pass form would lose precision
+// (variance below 1e-8 of the mean square, or the kept sum of squares below
+// 1e-8 of the row's) the row is summed again in two passes.  x is then read
+// once, to pack.
 __global__ __launch_bounds__(256) void k_align_pack(
     const float* __restrict__ x, int m, const int* __restrict__ perm, long n_pad,
     const long* __restrict__ offsets, int k, const int* __restrict__ shift,
-    const int* __restrict__ czero, uint16_t* __restrict__ yhi, uint16_t* __restrict__ ylo,
-    float* __restrict__ a0) {
-  __shared__ double s_mean[64], s_sd[64];
-  __shared__ int s_row[64], s_sh[64];
+    const int* __restrict__ czero, const double2* __restrict__ stats,
+    uint16_t* __restrict__ yhi, uint16_t* __restrict__ ylo, float* __restrict__ a0) {
+  __shared__ double s_mean[64], s_isd[64];
+  __shared__ int s_row[64], s_sh[64], s_first[64];
   __shared__ uint16_t t_hi[64][66], t_lo[64][66];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const long p0 = (long)blockIdx.x * 64;
-  for (int q = wid; q < 64; q += 4) {
-    const long p = p0 + q;
+  // each column's row, shift and (for a cluster's first member) cluster,
+  // resolved by the 64 lanes of wave 0 at once
+  if (wid == 0) {
+    const long p = p0 + lane;
     const int row = p < n_pad ? perm[p] : -1;
-    int s = 0, cl = 0;
+    int sh = 0, first = -1;
     if (row >= 0) {
-      cl = cluster_of(p, offsets, k);
-      s = czero[cl] ? 0 : shift[row];
+      const int cl = cluster_of(p, offsets, k);
+      sh = czero[cl] ? 0 : shift[row];
+      if (p == offsets[cl]) first = cl;
     }
-    const float* xr = x + (long)(row >= 0 ? row : 0) * m;
-    double sum = 0.0;
-    if (row >= 0)
-      for (int t = lane; t < m; t += kWave) {
-        int src = t - s;
-        sum += (src >= 0 && src < m) ? (double)xr[src] : 0.0;
+    s_row[lane] = row;
+    s_sh[lane] = sh;
+    s_first[lane] = first;
+  }
+  __syncthreads();
+  // statistics of 16 columns per wave, 4 at a time so that their loads
+  // overlap
+  for (int q0 = wid * 16; q0 < wid * 16 + 16; q0 += 4) {
+    double ps[4], pq[4], tq[4];
+    double2 tot[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int row = s_row[q0 + u], sh = s_sh[q0 + u];
+      const float* xr = x + (long)(row >= 0 ? row : 0) * m;
+      // kept source range [k0, k1) of roll_zeropad(x, sh); sum the kept
+      // values when they are few, else the dropped ones
+      const int k0 = sh > 0 ? 0 : (-sh < m ? -sh : m);
+      const int k1 = sh > 0 ? (sh < m ? m - sh : 0) : m;
+      const bool direct = 2 * (k1 - k0) <= m;
+      const int a1 = direct ? k0 : 0, n1 = direct ? k1 - k0 : k0, n2 = direct ? 0 : m - k1;
+      const int cnt = row >= 0 ? n1 + n2 : 0;
+      double a = 0.0, b = 0.0;
+      for (int i = lane; i < cnt; i += kWave) {
+        const double v = (double)xr[i < n1 ? a1 + i : k1 + (i - n1)];
+        a += v;
+        b += v * v;
       }
-    sum = wave_sum_f64(sum);
-    const double mean = sum / (double)m;
-    double ss = 0.0;
-    if (row >= 0)
-      for (int t = lane; t < m; t += kWave) {
-        int src = t - s;
-        double d = ((src >= 0 && src < m) ? (double)xr[src] : 0.0) - mean;
-        ss += d * d;
-      }
-    ss = wave_sum_f64(ss);
-    if (lane == 0) {
-      s_row[q] = row;
-      s_sh[q] = s;
-      s_mean[q] = mean;
-      s_sd[q] = sqrt(ss / (double)(m - 1));
+      ps[u] = a;
+      pq[u] = b;
+      tot[u] = row >= 0 ? stats[row] : make_double2(0.0, 0.0);
+      tq[u] = direct ? 0.0 : 1.0;  // 1: subtract from the row's totals
     }
-    if (row >= 0 && p == offsets[cl]) {
-      float* o = a0 + (long)cl * m;
-      for (int t = lane; t < m; t += kWave) {
-        int src = t - s;
-        o[t] = (src >= 0 && src < m) ? xr[src] : 0.f;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = q0 + u;
+      const int row = s_row[q], sh = s_sh[q];
+      const double a = wave_sum_f64(ps[u]), b = wave_sum_f64(pq[u]);
+      const bool sub = tq[u] != 0.0;
+      const double sum = sub ? tot[u].x - a : a;
+      const double sq = sub ? tot[u].y - b : b;
+      const double tsq = sub ? tot[u].y : b;
+      double mean = sum / (double)m;
+      double ss = sq - (double)m * mean * mean;
+      if (row >= 0 && !(ss > 1e-8 * sq && sq > 1e-8 * tsq)) {
+        // ill-conditioned, constant or empty aligned row: two passes
+        const float* xr = x + (long)row * m;
+        double s1 = 0.0;
+        for (int t = lane; t < m; t += kWave) {
+          const int src = t - sh;
+          s1 += (src >= 0 && src < m) ? (double)xr[src] : 0.0;
+        }
+        mean = wave_sum_f64(s1) / (double)m;
+        double s2 = 0.0;
+        for (int t = lane; t < m; t += kWave) {
+          const int src = t - sh;
+          const double d = ((src >= 0 && src < m) ? (double)xr[src] : 0.0) - mean;
+          s2 += d * d;
+        }
+        ss = wave_sum_f64(s2);
+      }
+      if (lane == 0) {
+        s_mean[q] = mean;
+        s_isd[q] = 1.0 / sqrt(ss / (double)(m - 1));
+      }
+      const int first = s_first[q];
+      if (first >= 0) {  // the cluster's lowest-index member: its aligned row
+        const float* xr = x + (long)row * m;
+        float* o = a0 + (long)first * m;
+        for (int t = lane; t < m; t += kWave) {
+          const int src = t - sh;
+          o[t] = (src >= 0 && src < m) ? xr[src] : 0.f;
+        }
       }
     }
   }
@@ -193,7 +299,7 @@ __global__ __launch_bounds__(256) void k_align_pack(
       if (row >= 0 && t < m) {
         const int src = t - s_sh[q];
         const double a = (src >= 0 && src < m) ? (double)x[(long)row * m + src] : 0.0;
-        const double y = (a - s_mean[q]) / s_sd[q];
+        const double y = (a - s_mean[q]) * s_isd[q];
         f = isnan(y) ? 0.f : (isinf(y) ? (y > 0 ? 3.402823466e38f : -3.402823466e38f) : (float)y);
       }
       const uint16_t h = f2bf_rne(f);
@@ -320,6 +426,27 @@ __device__ __forceinline__ double block_sum_f64(double v, double* scratch) {
   return s;
 }
 
+// Two block sums with one pair of barriers (scratch: 2 * kPowerWaves).
+__device__ __forceinline__ void block_sum2_f64(double& a, double& b, double* scratch) {
+  a = wave_sum_f64(a);
+  b = wave_sum_f64(b);
+  const int wid = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    scratch[wid] = a;
+    scratch[kPowerWaves + wid] = b;
+  }
+  __syncthreads();
+  double sa = 0.0, sb = 0.0;
+#pragma unroll
+  for (int w = 0; w < kPowerWaves; ++w) {
+    sa += scratch[w];
+    sb += scratch[kPowerWaves + w];
+  }
+  a = sa;
+  b = sb;
+}
+
 __device__ __forceinline__ int block_min_int(int v, int* scratch) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -425,8 +552,7 @@ __device__ void center_matvec(const float* __restrict__ Sj, int m, const double*
     sv += v[i];
     rv += r[i] * v[i];
   }
-  sv = block_sum_f64(sv, scratch);
-  rv = block_sum_f64(rv, scratch + kPowerWaves);
+  block_sum2_f64(sv, rv, scratch);
   sym_matvec_block(Sj, m, v, w, part);
   const double inv_m = 1.0 / (double)m;
   double a = 0.0, b = 0.0;
@@ -436,8 +562,9 @@ __device__ void center_matvec(const float* __restrict__ Sj, int m, const double*
     a += wi * wi;
     b += v[i] * wi;
   }
-  nw2 = block_sum_f64(a, scratch + 2 * kPowerWaves);
-  vw = block_sum_f64(b, scratch + 3 * kPowerWaves);
+  block_sum2_f64(a, b, scratch + 2 * kPowerWaves);
+  nw2 = a;
+  vw = b;
 }
 
 // Runs power steps v <- M v / |M v| until the Rayleigh quotient changes by
@@ -685,11 +812,7 @@ __device__ void tridiag_power_wave(const double* alpha, const double* beta, int 
     double r = fabs(alpha[e]) + (e > 0 ? fabs(beta[e]) : 0.0) + (e + 1 < len ? fabs(beta[e + 1]) : 0.0);
     g = r > g ? r : g;
   }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    double og = __shfl_xor(g, o, 64);
-    g = og > g ? og : g;
-  }
+  g = wave_max_f64(g);
   const double sc = g > 0.0 ? 1.0 / g : 0.0;
   const double a0 = e0 < len ? alpha[e0] * sc : 0.0;
   const double a1 = e1 < len ? alpha[e1] * sc : 0.0;
@@ -699,20 +822,21 @@ __device__ void tridiag_power_wave(const double* alpha, const double* beta, int 
   const double bu1 = (e1 + 1 < len) ? beta[e1 + 1] * sc : 0.0;        // T[e1][e1+1]
   double y0 = (e0 == 0) ? 1.0 : 0.0, y1 = 0.0;
   for (int s = 0; s < I; ++s) {
-    double p0 = __shfl_up(y0, 1, 64), p1 = __shfl_up(y1, 1, 64);
-    double n0 = __shfl_down(y0, 1, 64), n1 = __shfl_down(y1, 1, 64);
-    const double y63 = __shfl(y0, 63, 64), y64 = __shfl(y1, 0, 64);
-    if (l == 0) { p0 = 0.0; p1 = y63; }
-    if (l == 63) { n0 = y64; n1 = 0.0; }
+    // neighbours through DPP lane shifts: entries l-1, l+1 and l+63, l+65
+    const double y63 = readlane_f64(y0, 63), y64 = readlane_f64(y1, 0);
+    const double p0 = wave_shr1_f64(y0, 0.0), p1 = wave_shr1_f64(y1, y63);
+    const double n0 = wave_shl1_f64(y0, y64), n1 = wave_shl1_f64(y1, 0.0);
     const double z0 = bl0 * p0 + a0 * y0 + bu0 * n0;
     const double z1 = bl1 * p1 + a1 * y1 + bu1 * n1;
     y0 = z0;
     y1 = z1;
     if ((s & 31) == 31 || s == I - 1) {
-      double mx = fmax(fabs(y0), fabs(y1));
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
-      if (mx > 0.0) { y0 /= mx; y1 /= mx; }
+      const double mx = wave_max_f64(fmax(fabs(y0), fabs(y1)));
+      if (mx > 0.0) {
+        const double inv = 1.0 / mx;
+        y0 *= inv;
+        y1 *= inv;
+      }
     }
   }
   double nn = y0 * y0 + y1 * y1;
@@ -976,9 +1100,10 @@ __global__ __launch_bounds__(kPowerThreads) void k_power_lanczos(
     if (len == jmax) break;
     if (threadIdx.x == 0) beta[it + 1] = b;
     double* qn = Q + (long)(it + 1) * m;
+    const double inv_b = 1.0 / b;
     for (int e = threadIdx.x; e < m; e += kPowerThreads) {
       qp[e] = q[e];
-      q[e] = w[e] / b;
+      q[e] = w[e] * inv_b;
       qn[e] = q[e];
     }
     __syncthreads();
@@ -1275,8 +1400,11 @@ hipError_t launch_group_count(const int* label, long n, int k, int* blk_count, i
 
 hipError_t launch_group_scan(int nblk, int k, const int* blk_count, int* blk_base, int* counts,
                              long* offsets, hipStream_t st) {
-  hipLaunchKernelGGL(k_group_scan, dim3(1), dim3(256), 0, st, nblk, k, blk_count, blk_base,
-                     counts, offsets);
+  hipLaunchKernelGGL(k_group_scan, dim3(k), dim3(256), 0, st, nblk, k, blk_count, blk_base,
+                     counts);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_group_offsets, dim3(1), dim3(1024), 0, st, k, counts, offsets);
   return hipGetLastError();
 }
 
@@ -1288,13 +1416,21 @@ hipError_t launch_group_place(const int* label, long n, int k, const int* blk_ba
   return hipGetLastError();
 }
 
+hipError_t launch_row_stats(const float* x, long rows, int m, double2* stats, hipStream_t st) {
+  if (rows <= 0) return hipSuccess;
+  const long blocks = (rows + 3) / 4;
+  hipLaunchKernelGGL(k_row_stats, dim3((unsigned)blocks), dim3(256), 0, st, x, rows, m, stats);
+  return hipGetLastError();
+}
+
 hipError_t launch_align_pack(const float* x, int m, const int* perm, long n_pad,
                              const long* offsets, int k, const int* shift, const int* czero,
-                             uint16_t* yhi, uint16_t* ylo, float* a0, hipStream_t st) {
+                             const double2* stats, uint16_t* yhi, uint16_t* ylo, float* a0,
+                             hipStream_t st) {
   if (n_pad <= 0) return hipSuccess;
   long blocks = n_pad / 64;
   hipLaunchKernelGGL(k_align_pack, dim3((unsigned)blocks), dim3(256), 0, st, x, m, perm, n_pad,
-                     offsets, k, shift, czero, yhi, ylo, a0);
+                     offsets, k, shift, czero, stats, yhi, ylo, a0);
   return hipGetLastError();
 }
 

@@ -51,7 +51,7 @@ enum Slot {
   kSlotPerm, kSlotBad, kSlotYhi, kSlotYlo, kSlotA0, kSlotItems, kSlotTiles, kSlotPart,
   kSlotS, kSlotInfo, kSlotDummy, kSlotWbuf, kSlotRbuf, kSlotBar, kSlotVout, kSlotSteps,
   kSlotConv, kSlotErr, kSlotCmag, kSlotEvals, kSlotSvout, kSlotLzW, kSlotLzState, kSlotMask,
-  kNumSlots
+  kSlotRowStats, kSlotFitRowStats, kNumSlots
 };
 
 struct FitState {
@@ -64,6 +64,7 @@ struct FitState {
   int it = 0, converged = 0, sum_class = 0, steps_max = 0, fallbacks = 0;
   long n_pad = 0;
   bool have_order = false;   // the refine's cluster-sorted rows are valid
+  const double2* row_stats = nullptr;  // per-row sums of d_x (fixed during the fit)
   long long evals = 0, pairs = 0;
   float ms_ref = 0.f, ms_asg = 0.f, ms_asg_last = 0.f;
   int *label = nullptr, *shift = nullptr, *changed = nullptr, *zero = nullptr;
@@ -279,7 +280,7 @@ int power_lanczos(tcx_ctx* c, const float* S, int m, int k, const int* counts, u
   TCX_TRY(ws(c, kSlotLzState, (size_t)k, &state));
   if (mode == 0 && !mask && m <= tcx::kFusedLanczosMaxM) {
     // S_j fits in L2: one block per cluster runs every step in one launch
-    TCX_HIP(tcx::launch_power_lanczos(S, m, k, counts, seed, iter, steps, 1e-11, Q, v, r, st_out,
+    TCX_HIP(tcx::launch_power_lanczos(S, m, k, counts, seed, iter, steps, 1e-9, Q, v, r, st_out,
                                       cv_out, st));
     TCX_HIP(tcx::launch_cluster_matvec(S, m, k, v, m, sv, nullptr, st));  // S v
     *vout = v;
@@ -296,7 +297,7 @@ int power_lanczos(tcx_ctx* c, const float* S, int m, int k, const int* counts, u
   std::vector<int> done(k);
   for (int it = 0; it < jmax; ++it) {
     TCX_HIP(tcx::launch_cluster_matvec(S, m, k, Q + (long)it * m, qstride, w, state, st));
-    TCX_HIP(tcx::launch_lz_step(m, k, mode, steps, mode == 0 ? 1e-11 : 1e-12, r, w, Q, state,
+    TCX_HIP(tcx::launch_lz_step(m, k, mode, steps, mode == 0 ? 1e-9 : 1e-12, r, w, Q, state,
                                 st));
     const int len = it + 1;
     const bool check = mode == 0 ? (len >= 16 && (len & 7) == 0) : (len >= 8 && (len & 7) == 0);
@@ -323,8 +324,14 @@ int power_lanczos(tcx_ctx* c, const float* S, int m, int k, const int* counts, u
 // labels, shifts and old-centroid zero flags; writes k*m new centroids.
 int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_label,
            const int* d_shift, const int* d_czero, float* d_cout, int iter, uint64_t seed,
-           int split, double tol, int cal_select, RefineResult* res) {
+           int split, double tol, int cal_select, const double2* row_stats, RefineResult* res) {
   hipStream_t st = c->stream;
+  if (!row_stats) {
+    double2* rs;
+    TCX_TRY(ws(c, kSlotRowStats, (size_t)n, &rs));
+    TCX_HIP(tcx::launch_row_stats(d_x, n, m, rs, st));
+    row_stats = rs;
+  }
   c->fit.have_order = false;  // the row order is rewritten below
   long nblk = (n + tcx::kGroupChunk - 1) / tcx::kGroupChunk;
   int *blk_count, *blk_base, *counts, *perm, *bad;
@@ -354,7 +361,8 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
   TCX_TRY(ws(c, kSlotYhi, (size_t)m * (n_pad > 0 ? n_pad : 1), &yhi));
   TCX_TRY(ws(c, kSlotYlo, (size_t)m * (n_pad > 0 ? n_pad : 1), &ylo));
   TCX_TRY(ws(c, kSlotA0, (size_t)k * m, &a0));
-  TCX_HIP(tcx::launch_align_pack(d_x, m, perm, n_pad, offsets, k, d_shift, d_czero, yhi, ylo,
+  TCX_HIP(tcx::launch_align_pack(d_x, m, perm, n_pad, offsets, k, d_shift, d_czero, row_stats,
+                                 yhi, ylo,
                                  a0, st));
   int nonempty = 0;
   for (int j = 0; j < k; ++j) nonempty += h_counts[j] > 0;
@@ -669,7 +677,7 @@ int tcx_extract_shape(tcx_ctx* c, const int* d_label, const float* d_x, const in
   TCX_TRY(ws(c, kSlotCentNew, (size_t)k * m, &cnew));
   RefineResult rr;
   TCX_TRY(refine(c, d_x, n, m, k, d_label, d_shift, zero, cnew, iter, 0, 1, 1e-12, cal_select,
-                 &rr));
+                 nullptr, &rr));
   TCX_HIP(hipMemcpyAsync(d_centroids, cnew, sizeof(float) * k * m, hipMemcpyDeviceToDevice,
                          c->stream));
   TCX_HIP(hipStreamSynchronize(c->stream));
@@ -747,6 +755,10 @@ int tcx_fit_begin(tcx_ctx* c, const float* d_x, int n, int m, int k, const tcx_f
   }
   TCX_TRY(spectra(c, f.cent, k, m, &f.spec, &f.norm, &f.zero));
   if (o.init_mode == 0) TCX_HIP(tcx::launch_fill_int(f.zero, k, 1, st));  // centroids start at 0
+  double2* rs;
+  TCX_TRY(ws(c, kSlotFitRowStats, (size_t)n, &rs));
+  TCX_HIP(tcx::launch_row_stats(d_x, n, m, rs, st));
+  f.row_stats = rs;
   // the host copy of the initial labels must outlive the async upload
   TCX_HIP(hipStreamSynchronize(st));
   f.o = o;
@@ -771,7 +783,7 @@ int fit_refine(tcx_ctx* c, FitState& f) {
   TCX_HIP(hipEventRecord(c->ev[0], st));
   RefineResult rr;
   TCX_TRY(refine(c, f.d_x, f.n, f.m, f.k, f.label, f.shift, f.zero, cnew, f.it, f.o.seed,
-                 f.split, f.tol, f.o.cal_select, &rr));
+                 f.split, f.tol, f.o.cal_select, f.row_stats, &rr));
   TCX_HIP(hipMemcpyAsync(f.cent, cnew, sizeof(float) * f.k * f.m, hipMemcpyDeviceToDevice, st));
   TCX_TRY(spectra(c, f.cent, f.k, f.m, &f.spec, &f.norm, &f.zero));
   TCX_HIP(hipEventRecord(c->ev[1], st));

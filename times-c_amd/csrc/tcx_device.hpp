@@ -103,10 +103,51 @@ __device__ __forceinline__ void group_argmax(float& v, int& key, float* scratch)
   }
 }
 
+// Cross-lane moves of a double through DPP (VALU data-parallel primitives,
+// no LDS round trip): CTRL is the DPP control word.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, false),
+                          __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Whole-wave shifts by one lane (DPP wave_shr:1 / wave_shl:1): lane l gets
+// v of lane l-1 (resp. l+1); lane 0 (resp. 63) gets fill.
+__device__ __forceinline__ double wave_shr1_f64(double v, double fill) {
+  return __hiloint2double(
+      __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), 0x138, 0xF, 0xF, false),
+      __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double wave_shl1_f64(double v, double fill) {
+  return __hiloint2double(
+      __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), 0x130, 0xF, 0xF, false),
+      __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), 0x130, 0xF, 0xF, false));
+}
+
+// Maximum over the 64 lanes of a wave (all lanes active), same pattern as
+// wave_sum_f64.
+__device__ __forceinline__ double wave_max_f64(double v) {
+  v = fmax(v, dpp_f64<0xB1>(v));
+  v = fmax(v, dpp_f64<0x4E>(v));
+  v = fmax(v, dpp_f64<0x124>(v));
+  v = fmax(v, dpp_f64<0x128>(v));
+  return fmax(fmax(readlane_f64(v, 0), readlane_f64(v, 16)),
+              fmax(readlane_f64(v, 32), readlane_f64(v, 48)));
+}
+
+// Sum over the 64 lanes of a wave (all lanes active); every lane gets the
+// same value (fixed order: quads, rows of 16, then the four rows).
 __device__ __forceinline__ double wave_sum_f64(double v) {
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-  return v;
+  v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]: lane ^ 1
+  v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]: lane ^ 2
+  v += dpp_f64<0x124>(v);  // row_ror:4
+  v += dpp_f64<0x128>(v);  // row_ror:8 -> each lane holds its row's sum
+  return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 
 // ---------------------------------------------------------- in-register DFT

@@ -62,7 +62,7 @@ hipError_t launch_apply_shift(float* x, long rows, int m, const int* shift, cons
 hipError_t configure_sbd_kernels();
 
 // ------------------------------------------------------------ extraction
-constexpr int kGroupChunk = 2048;  // rows per grouping block
+constexpr int kGroupChunk = 512;  // rows per grouping block
 constexpr int kPackPad = 64;       // cluster segments padded to this many columns
 constexpr int kSyrkTile = 128;     // S tile edge
 
@@ -96,9 +96,11 @@ hipError_t launch_group_scan(int nblk, int k, const int* blk_count, int* blk_bas
                              int* counts, long* offsets, hipStream_t st);
 hipError_t launch_group_place(const int* label, long n, int k, const int* blk_base,
                               const long* offsets, int* perm, hipStream_t st);
+// FP64 (sum, sum of squares) of every row.
+hipError_t launch_row_stats(const float* x, long rows, int m, double2* stats, hipStream_t st);
 hipError_t launch_align_pack(const float* x, int m, const int* perm, long n_pad,
-                             const long* offsets, int k, const int* shift,
-                             const int* czero, uint16_t* yhi, uint16_t* ylo, float* a0,
+                             const long* offsets, int k, const int* shift, const int* czero,
+                             const double2* stats, uint16_t* yhi, uint16_t* ylo, float* a0,
                              hipStream_t st);
 hipError_t launch_syrk(const SyrkItem* items, int nitems, const uint16_t* yhi,
                        const uint16_t* ylo, long n_pad, int m, int split, float* part,
