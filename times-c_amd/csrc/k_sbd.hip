@@ -579,6 +579,25 @@ static hipError_t launch_prune_F(const PruneArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Host: the twiddle table of the length-F transform in the layout the
+// Stockham passes read (tw_offset).
+void fft_twiddle_table(int F, float2* out) {
+  const int P = F / fft_lanes(F);
+  const double two_pi = 6.283185307179586476925286766559;
+  for (int ns = 1; ns < F;) {
+    const int R = F / ns < P ? F / ns : P;
+    if (ns > 1) {
+      const int off = tw_offset(F, P, ns);
+      for (int r = 1; r < R; ++r)
+        for (int kk = 0; kk < ns; ++kk) {
+          const double a = two_pi * (double)r * (double)kk / ((double)ns * (double)R);
+          out[off + (r - 1) * ns + kk] = make_float2((float)std::cos(a), (float)-std::sin(a));
+        }
+    }
+    ns *= R;
+  }
+}
+
 #define TCX_FOR_EACH_F(X) \
   X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
 

@@ -148,12 +148,8 @@ int twiddles(tcx_ctx* c, int F, const float2** out) {
     *out = it->second;
     return TCX_OK;
   }
-  std::vector<float2> h(F);
-  const double two_pi = 6.283185307179586476925286766559;
-  for (int q = 0; q < F; ++q) {
-    double a = two_pi * (double)q / (double)F;
-    h[q] = make_float2((float)std::cos(a), (float)-std::sin(a));
-  }
+  std::vector<float2> h(F, make_float2(1.f, 0.f));
+  tcx::fft_twiddle_table(F, h.data());
   TCX_TRY(check_budget(c, sizeof(float2) * F));
   float2* d = nullptr;
   if (hipMalloc(&d, sizeof(float2) * F) != hipSuccess)

@@ -47,11 +47,53 @@ __device__ __forceinline__ void group_sync() {
 // Reductions over L lanes.  For L <= 64 the group is a contiguous lane
 // range of one wave; for L > 64 the group is the whole block and `scratch`
 // (at least 2*L/64 words of LDS) carries the per-wave partials.
+// Inside a wave the partners come from DPP (lane ^ 1, lane ^ 2, the mirror
+// in 8 lanes, the mirror in a row of 16), then the four row results are
+// read out with readlane: VALU/SALU only, no LDS.  Every lane of a group
+// ends with the same value.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float readlane_f32(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+template <int W, int CTRL>
+__device__ __forceinline__ void argmax_step(float& v, int& key) {
+  const float ov = dpp_f32<CTRL>(v);
+  const int ok = dpp_i32<CTRL>(key);
+  if (ov > v || (ov == v && ok < key)) {
+    v = ov;
+    key = ok;
+  }
+}
+
+__device__ __forceinline__ void argmax_merge(float& v, int& key, float ov, int ok) {
+  if (ov > v || (ov == v && ok < key)) {
+    v = ov;
+    key = ok;
+  }
+}
+
 template <int L>
 __device__ __forceinline__ float group_sum(float v, float* scratch) {
   constexpr int W = L < kWave ? L : kWave;
-#pragma unroll
-  for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
+  if constexpr (W >= 2) v += dpp_f32<0xB1>(v);   // lane ^ 1
+  if constexpr (W >= 4) v += dpp_f32<0x4E>(v);   // lane ^ 2
+  if constexpr (W >= 8) v += dpp_f32<0x141>(v);  // mirror in 8 lanes
+  if constexpr (W >= 16) v += dpp_f32<0x140>(v); // mirror in 16 lanes
+  if constexpr (W == 32) {
+    const float a = readlane_f32(v, 0) + readlane_f32(v, 16);
+    const float b = readlane_f32(v, 32) + readlane_f32(v, 48);
+    v = (threadIdx.x & 32) ? b : a;
+  } else if constexpr (W == 64) {
+    v = (readlane_f32(v, 0) + readlane_f32(v, 16)) + (readlane_f32(v, 32) + readlane_f32(v, 48));
+  }
   if constexpr (L > kWave) {
     constexpr int NW = L / kWave;
     int wid = threadIdx.x / kWave;
@@ -70,13 +112,25 @@ __device__ __forceinline__ float group_sum(float v, float* scratch) {
 template <int L>
 __device__ __forceinline__ void group_argmax(float& v, int& key, float* scratch) {
   constexpr int W = L < kWave ? L : kWave;
-#pragma unroll
-  for (int o = W / 2; o > 0; o >>= 1) {
-    float ov = __shfl_xor(v, o, W);
-    int ok = __shfl_xor(key, o, W);
-    if (ov > v || (ov == v && ok < key)) {
-      v = ov;
-      key = ok;
+  if constexpr (W >= 2) argmax_step<W, 0xB1>(v, key);
+  if constexpr (W >= 4) argmax_step<W, 0x4E>(v, key);
+  if constexpr (W >= 8) argmax_step<W, 0x141>(v, key);
+  if constexpr (W >= 16) argmax_step<W, 0x140>(v, key);
+  if constexpr (W >= 32) {
+    float v0 = readlane_f32(v, 0), v1 = readlane_f32(v, 16);
+    int k0 = __builtin_amdgcn_readlane(key, 0), k1 = __builtin_amdgcn_readlane(key, 16);
+    argmax_merge(v0, k0, v1, k1);
+    float v2 = readlane_f32(v, 32), v3 = readlane_f32(v, 48);
+    int k2 = __builtin_amdgcn_readlane(key, 32), k3 = __builtin_amdgcn_readlane(key, 48);
+    argmax_merge(v2, k2, v3, k3);
+    if constexpr (W == 32) {
+      const bool hi = (threadIdx.x & 32) != 0;
+      v = hi ? v2 : v0;
+      key = hi ? k2 : k0;
+    } else {
+      argmax_merge(v0, k0, v2, k2);
+      v = v0;
+      key = k0;
     }
   }
   if constexpr (L > kWave) {
@@ -239,6 +293,27 @@ constexpr int fft_buf_len() {
   return F + F / 32 + 1;
 }
 
+// Lanes of an FFT group of length F (see FftGeom).
+__host__ __device__ constexpr int fft_lanes(int F) {
+  return F <= 4 ? 1 : F <= 64 ? F / 4 : F <= 256 ? 16 : F / 16;
+}
+
+// Twiddle table layout: the passes of a length-F transform with P values
+// per lane have strides Ns = 1, R_1, R_1 R_2, ... (R = min(F / Ns, P)); the
+// pass with stride Ns > 1 reads entries off(Ns) + (r - 1) Ns + kk =
+// exp(-2 pi i r kk / (Ns R)), r = 1..R-1, kk = 0..Ns-1, so the lanes of a
+// wave read consecutive entries (no LDS bank conflicts).  The table has
+// F - R_1 < F entries.
+__host__ __device__ constexpr int tw_offset(int F, int P, int Ns) {
+  int off = 0;
+  for (int ns = 1; ns < Ns;) {
+    const int R = F / ns < P ? F / ns : P;
+    if (ns > 1) off += (R - 1) * ns;
+    ns *= R;
+  }
+  return off;
+}
+
 template <int F, int L, bool INV, int Ns>
 __device__ __forceinline__ void stockham_pass(float2 (&v)[F / L], float2* buf,
                                               const float2* tw, int lane) {
@@ -266,10 +341,10 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[F / L], float2* buf,
     const int kk = j & (Ns - 1);
     float2* b = u + t * R;
     if constexpr (!kFirst) {
-      const int qs = kk * (F / (Ns * R));
+      constexpr int kOff = tw_offset(F, P, Ns);
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        float2 w = tw[r * qs];
+        const float2 w = tw[kOff + (r - 1) * Ns + kk];
         b[r] = INV ? cmulc(b[r], w) : cmul(b[r], w);
       }
     }
@@ -300,10 +375,7 @@ __device__ __forceinline__ void fft(float2 (&v)[F / L], float2* buf,
 // Lane-group geometry per FFT length: L lanes, P = F/L values per lane.
 template <int F>
 struct FftGeom {
-  static constexpr int L = F <= 4 ? 1
-                         : F <= 64 ? F / 4
-                         : F <= 256 ? 16
-                         : F / 16;
+  static constexpr int L = fft_lanes(F);
   static constexpr int P = F / L;
   static constexpr int kBlock = L <= kWave ? 256 : L;  // threads per block
   static constexpr int kGroups = kBlock / L;           // lane groups per block

@@ -30,6 +30,8 @@ struct SbdArgs {
 constexpr int kMaxFft = 8192;
 
 hipError_t launch_znorm(float* x, long rows, int m, int ddof, hipStream_t st);
+// Twiddle table (F entries) of the length-F transform, host side.
+void fft_twiddle_table(int F, float2* out);
 hipError_t launch_spectra(int F, const float* y, int rows, int m, const float2* tw,
                           float2* spec, float* norms, int* zero, float* mag, hipStream_t st);
 
