@@ -198,7 +198,7 @@ __global__ __launch_bounds__(256) void k_align_pack(
     uint16_t* __restrict__ yhi, uint16_t* __restrict__ ylo, float* __restrict__ a0) {
   __shared__ double s_mean[64], s_isd[64];
   __shared__ int s_row[64], s_sh[64], s_first[64];
-  __shared__ uint16_t t_hi[64][66], t_lo[64][66];
+  __shared__ __align__(16) uint16_t t_hi[64][72], t_lo[64][72];  // 144-byte rows
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const long p0 = (long)blockIdx.x * 64;
   // each column's row, shift and (for a cluster's first member) cluster,
@@ -307,16 +307,22 @@ __global__ __launch_bounds__(256) void k_align_pack(
       t_lo[tt][q] = f2bf_rne(f - bf2f(h));
     }
     __syncthreads();
+    // eight 1 KB fragment chunks (4 column chunks x 2 row chunks) per array,
+    // one 16-byte store per lane
+    const int tm = panel_blocks(m);
 #pragma unroll
-    for (int pass = 0; pass < 8; ++pass) {
-      const int tt = pass * 8 + (tid >> 5);
-      const int qq = (tid & 31) * 2;
-      const int t = t0 + tt;
-      if (t < m) {
-        uint32_t h = (uint32_t)t_hi[tt][qq] | ((uint32_t)t_hi[tt][qq + 1] << 16);
-        uint32_t l = (uint32_t)t_lo[tt][qq] | ((uint32_t)t_lo[tt][qq + 1] << 16);
-        *reinterpret_cast<uint32_t*>(yhi + (long)t * n_pad + p0 + qq) = h;
-        *reinterpret_cast<uint32_t*>(ylo + (long)t * n_pad + p0 + qq) = l;
+    for (int pass = 0; pass < 2; ++pass) {
+      const int sidx = pass * 256 + tid;
+      const int ch = sidx >> 6, ln = sidx & 63;
+      const int cc = ch >> 1, rr = ch & 1;
+      const int r32 = (t0 >> 5) + rr;
+      if (r32 < tm) {
+        const int tt = rr * 32 + (ln & 31), q = cc * 16 + 8 * (ln >> 5);
+        const uint4 h = *reinterpret_cast<const uint4*>(&t_hi[tt][q]);
+        const uint4 l = *reinterpret_cast<const uint4*>(&t_lo[tt][q]);
+        const long off = (((p0 >> 4) + cc) * (long)tm + r32) * 512 + ln * 8;
+        *reinterpret_cast<uint4*>(yhi + off) = h;
+        *reinterpret_cast<uint4*>(ylo + off) = l;
       }
     }
     __syncthreads();
@@ -324,89 +330,125 @@ __global__ __launch_bounds__(256) void k_align_pack(
 }
 
 // --------------------------------------------------------------- S = Y^T Y
-// One 128x128 tile of S_j over packed columns [p0, p1) per block; four waves
-// in a 2x2 arrangement, each 2x2 MFMA 32x32x16 BF16 blocks with FP32
-// accumulation.  split != 0 adds hi*lo + lo*hi (split-BF16).  Operands come
-// straight from the transposed panel: lane l supplies row (l % 32) and
-// columns p + 8*(l / 32) .. +7, one 16-byte load per fragment.
+// One 256x256 tile of S_j over packed columns [p0, p1) per block: four waves
+// in a 2x2 arrangement, each a 128x128 quarter as 4x4 MFMA 32x32x16 BF16
+// blocks with FP32 accumulation.  split != 0 adds hi*lo + lo*hi
+// (split-BF16).  The panel is in fragment order (panel_blocks), so every
+// operand is one coalesced 1 KB load per wave, 16 bytes per lane; the next
+// 16 columns' operands load while the current ones multiply.  On diagonal
+// tiles the lower-left quarter (the mirror of the upper-right one) is not
+// computed.
+template <bool kSplit>
 __global__ __launch_bounds__(256) void k_syrk(const SyrkItem* __restrict__ items,
                                               const uint16_t* __restrict__ yhi,
-                                              const uint16_t* __restrict__ ylo, long n_pad,
-                                              int m, int split, float* __restrict__ part) {
+                                              const uint16_t* __restrict__ ylo, int m,
+                                              float* __restrict__ part) {
   const SyrkItem it = items[blockIdx.x];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (it.cluster < 0) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wi = wave >> 1, wj = wave & 1;
-  const int kofs = 8 * (lane >> 5);
-  int ra[2], rb[2];
+  if (it.ti == it.tj && wi > wj) return;
+  const int tm = panel_blocks(m);
+  const int ra0 = it.ti * 8 + wi * 4, rb0 = it.tj * 8 + wj * 4;  // first 32-row blocks
+  const int na = tm - ra0 < 4 ? tm - ra0 : 4, nb = tm - rb0 < 4 ? tm - rb0 : 4;
+  if (na <= 0 || nb <= 0) return;  // quarter entirely past row m
+  const bf16x8* H = reinterpret_cast<const bf16x8*>(yhi) + lane;
+  const bf16x8* Lo = reinterpret_cast<const bf16x8*>(ylo) + lane;
+  const long c0 = it.p0 >> 4, c1 = it.p1 >> 4;
+  f32x16 acc[4][4];
 #pragma unroll
-  for (int a = 0; a < 2; ++a) {
-    ra[a] = it.ti * kSyrkTile + wi * 64 + a * 32 + (lane & 31);
-    rb[a] = it.tj * kSyrkTile + wj * 64 + a * 32 + (lane & 31);
-  }
-  f32x16 acc[2][2];
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < 4; ++b)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
   const bf16x8 zero8 = {};
-  for (long p = it.p0; p < it.p1; p += 16) {
-    bf16x8 ah[2], al[2], bh[2], bl[2];
+  bf16x8 ah[4], al[4], bh[4], bl[4];
+  auto load = [&](long c, bf16x8* xh, bf16x8* xl, int r0, int nr) {
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      const long oa = (long)ra[a] * n_pad + p + kofs;
-      const long ob = (long)rb[a] * n_pad + p + kofs;
-      ah[a] = ra[a] < m ? *reinterpret_cast<const bf16x8*>(yhi + oa) : zero8;
-      bh[a] = rb[a] < m ? *reinterpret_cast<const bf16x8*>(yhi + ob) : zero8;
-      if (split) {
-        al[a] = ra[a] < m ? *reinterpret_cast<const bf16x8*>(ylo + oa) : zero8;
-        bl[a] = rb[a] < m ? *reinterpret_cast<const bf16x8*>(ylo + ob) : zero8;
-      }
+    for (int a = 0; a < 4; ++a) {
+      const long o = (c * tm + r0 + a) * 64;
+      xh[a] = a < nr ? H[o] : zero8;
+      if constexpr (kSplit) xl[a] = a < nr ? Lo[o] : zero8;
+    }
+  };
+  load(c0, ah, al, ra0, na);
+  load(c0, bh, bl, rb0, nb);
+  for (long c = c0; c < c1; ++c) {
+    const long cn = c + 1 < c1 ? c + 1 : c;  // the last step reloads its own chunk
+    bf16x8 nah[4], nal[4], nbh[4], nbl[4];
+    load(cn, nah, nal, ra0, na);
+    load(cn, nbh, nbl, rb0, nb);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+    if constexpr (kSplit) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
     }
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
-        if (split) {
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
-        }
-      }
+    for (int a = 0; a < 4; ++a) {
+      ah[a] = nah[a];
+      al[a] = nal[a];
+      bh[a] = nbh[a];
+      bl[a] = nbl[a];
+    }
   }
   float* o = part + (long)it.part * kSyrkTile * kSyrkTile;
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < 4; ++b)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const int i = wi * 64 + a * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-        const int j = wj * 64 + b * 32 + (lane & 31);
+        const int i = wi * 128 + a * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+        const int j = wj * 128 + b * 32 + (lane & 31);
         o[i * kSyrkTile + j] = acc[a][b][r];
       }
 }
 
-// Sum the K-chunk partials of each tile in a fixed order and write the tile
-// and its mirror, so S_j is exactly symmetric (diagonal tiles keep i <= j).
-// diagmax (may be NULL) receives max(0, max_i S_ii) per cluster as float bits.
+// S_j from the K-chunk partials of each tile, summed in a fixed order.  One
+// block per 64x64 sub-block (16 per tile, the lower ones of diagonal tiles
+// skipped) writes the entries with i <= j and, through an LDS transpose,
+// their mirrors, so S_j is exactly symmetric and every store is a
+// coalesced row segment.
 __global__ __launch_bounds__(256) void k_syrk_reduce(const SyrkTile* __restrict__ tiles,
                                                      const float* __restrict__ part, int m,
-                                                     float* __restrict__ S,
-                                                     unsigned* __restrict__ diagmax) {
-  const SyrkTile tl = tiles[blockIdx.x];
+                                                     float* __restrict__ S) {
+  constexpr int kSub = kSyrkTile / 64;
+  const SyrkTile tl = tiles[blockIdx.x / (kSub * kSub)];
+  const int sb = blockIdx.x % (kSub * kSub), si = sb / kSub, sj = sb % kSub;
+  if (tl.ti == tl.tj && si > sj) return;
+  const int gi0 = tl.ti * kSyrkTile + si * 64, gj0 = tl.tj * kSyrkTile + sj * 64;
+  if (gi0 >= m || gj0 >= m) return;
+  __shared__ float tr[64][65];
   float* Sj = S + (long)tl.cluster * m * m;
-  for (int e = threadIdx.x; e < kSyrkTile * kSyrkTile; e += blockDim.x) {
-    const int i = e / kSyrkTile, jj = e % kSyrkTile;
-    const int gi = tl.ti * kSyrkTile + i, gj = tl.tj * kSyrkTile + jj;
-    if (gi >= m || gj >= m) continue;
-    if (tl.ti == tl.tj && jj < i) continue;
+  const int col = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+  const float* pp = part + (long)tl.part0 * kSyrkTile * kSyrkTile;
+#pragma unroll 4
+  for (int r = r0; r < 64; r += 4) {
+    const long e = (long)(si * 64 + r) * kSyrkTile + sj * 64 + col;
     float s = 0.f;
-    for (int c = 0; c < tl.nparts; ++c) s += part[(long)(tl.part0 + c) * kSyrkTile * kSyrkTile + e];
-    Sj[(long)gi * m + gj] = s;
-    Sj[(long)gj * m + gi] = s;
-    if (diagmax && gi == gj) atomicMax(&diagmax[tl.cluster], __float_as_uint(s > 0.f ? s : 0.f));
+    for (int c = 0; c < tl.nparts; ++c) s += pp[(long)c * kSyrkTile * kSyrkTile + e];
+    tr[r][col] = s;
+    const int gi = gi0 + r, gj = gj0 + col;
+    if (gi < m && gj < m && gi <= gj) Sj[(long)gi * m + gj] = s;
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int r = r0; r < 64; r += 4) {
+    const int gi = gi0 + col, gj = gj0 + r;  // mirror of entry (gi, gj)
+    if (gi < m && gj < m && gi <= gj) Sj[(long)gj * m + gi] = tr[col][r];
   }
 }
 
@@ -1435,18 +1477,21 @@ hipError_t launch_align_pack(const float* x, int m, const int* perm, long n_pad,
 }
 
 hipError_t launch_syrk(const SyrkItem* items, int nitems, const uint16_t* yhi,
-                       const uint16_t* ylo, long n_pad, int m, int split, float* part,
-                       hipStream_t st) {
+                       const uint16_t* ylo, int m, int split, float* part, hipStream_t st) {
   if (nitems <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_syrk, dim3(nitems), dim3(256), 0, st, items, yhi, ylo, n_pad, m, split,
-                     part);
+  if (split)
+    hipLaunchKernelGGL(k_syrk<true>, dim3(nitems), dim3(256), 0, st, items, yhi, ylo, m, part);
+  else
+    hipLaunchKernelGGL(k_syrk<false>, dim3(nitems), dim3(256), 0, st, items, yhi, ylo, m, part);
   return hipGetLastError();
 }
 
 hipError_t launch_syrk_reduce(const SyrkTile* tiles, int ntiles, const float* part, int m,
-                              float* S, unsigned* diagmax, hipStream_t st) {
+                              float* S, hipStream_t st) {
   if (ntiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_syrk_reduce, dim3(ntiles), dim3(256), 0, st, tiles, part, m, S, diagmax);
+  constexpr int kSub = kSyrkTile / 64;
+  hipLaunchKernelGGL(k_syrk_reduce, dim3((unsigned)ntiles * kSub * kSub), dim3(256), 0, st, tiles,
+                     part, m, S);
   return hipGetLastError();
 }
 

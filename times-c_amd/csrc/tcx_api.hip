@@ -212,35 +212,59 @@ int assign_core(tcx_ctx* c, const float* d_x, long n, int m, int k, const float2
 }
 
 // Work list of a grouped S = Y^T Y: for each cluster with seg_len[j] > 0
-// columns starting at off[j], the upper triangle of ceil(m/128)^2 tiles,
-// each split along K so that the grid has at least ~2 blocks per CU.
-void plan_syrk(int m, int k, const long* off, const long* seg_len,
+// packed columns starting at off[j], the upper triangle of
+// ceil(m/256)^2 tiles, each split along K into chunks of one length, the
+// shortest for which the grid is at most one block per CU (one 4-wave block
+// fills a CU's register file).  The tiles of one (cluster, chunk) read the
+// same panel rows, so they are placed on one XCD (blocks b with equal
+// b % 8 run on one XCD) at adjacent dispatch slots; free slots are holes.
+void plan_syrk(int m, int k, const long* off, const long* seg_len, int cus,
                std::vector<tcx::SyrkItem>& items, std::vector<tcx::SyrkTile>& tiles) {
+  constexpr int kXcds = 8;
   const int T = (m + tcx::kSyrkTile - 1) / tcx::kSyrkTile;
   const int tiles_per = T * (T + 1) / 2;
-  int nonempty = 0;
-  for (int j = 0; j < k; ++j) nonempty += seg_len[j] > 0;
-  const long total_tiles = (long)tiles_per * nonempty;
-  const long want = 512;
-  const long split_f =
-      total_tiles >= want ? 1 : (want + total_tiles - 1) / std::max(1L, total_tiles);
   items.clear();
   tiles.clear();
+  long max_steps = 0;
+  for (int j = 0; j < k; ++j) max_steps = std::max(max_steps, (seg_len[j] + 15) / 16);
+  if (max_steps == 0) return;
+  auto blocks_for = [&](long cs) {
+    long b = 0;
+    for (int j = 0; j < k; ++j) b += (seg_len[j] + 16 * cs - 1) / (16 * cs) * tiles_per;
+    return b;
+  };
+  long lo = 1, hi = max_steps;  // smallest chunk (in 16-column steps) with blocks <= cus
+  while (lo < hi) {
+    const long mid = (lo + hi) / 2;
+    if (blocks_for(mid) <= std::max(cus, 1)) hi = mid; else lo = mid + 1;
+  }
+  const long cs = std::min(max_steps, std::max(lo, 16L)) * 16;  // columns per chunk
+  struct Group { int j; long p0, p1; int c; int tile0; };
+  std::vector<Group> groups;
   for (int j = 0; j < k; ++j) {
     const long K = seg_len[j];
     if (K == 0) continue;
-    long chunk = (K + split_f - 1) / split_f;
-    chunk = (chunk + tcx::kPackPad - 1) / tcx::kPackPad * tcx::kPackPad;
+    const int nc = (int)((K + cs - 1) / cs);
+    const int tile0 = (int)tiles.size();
+    int part = tiles.empty() ? 0 : tiles.back().part0 + tiles.back().nparts;
     for (int ti = 0; ti < T; ++ti)
       for (int tj = ti; tj < T; ++tj) {
-        tcx::SyrkTile tl{j, ti, tj, (int)items.size(), 0};
-        for (long p = 0; p < K; p += chunk) {
-          tcx::SyrkItem it{j, ti, tj, (int)items.size(), off[j] + p, off[j] + std::min(K, p + chunk)};
-          items.push_back(it);
-          ++tl.nparts;
-        }
-        tiles.push_back(tl);
+        tiles.push_back(tcx::SyrkTile{j, ti, tj, part, nc});
+        part += nc;
       }
+    for (int c = 0; c < nc; ++c)
+      groups.push_back(Group{j, off[j] + c * cs, off[j] + std::min(K, (c + 1) * cs), c, tile0});
+  }
+  const long per_xcd = ((long)groups.size() + kXcds - 1) / kXcds;
+  items.assign((size_t)(kXcds * tiles_per * per_xcd), tcx::SyrkItem{-1, 0, 0, 0, 0, 0});
+  for (long g = 0; g < (long)groups.size(); ++g) {
+    const Group& gr = groups[g];
+    const long x = g % kXcds, gx = g / kXcds;
+    for (int r = 0; r < tiles_per; ++r) {
+      const tcx::SyrkTile& tl = tiles[gr.tile0 + r];
+      items[kXcds * (tiles_per * gx + r) + x] =
+          tcx::SyrkItem{gr.j, tl.ti, tl.tj, tl.part0 + gr.c, gr.p0, gr.p1};
+    }
   }
 }
 
@@ -354,8 +378,9 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
   TCX_HIP(tcx::launch_group_place(d_label, n, k, blk_base, offsets, perm, st));
   uint16_t *yhi, *ylo;
   float* a0;
-  TCX_TRY(ws(c, kSlotYhi, (size_t)m * (n_pad > 0 ? n_pad : 1), &yhi));
-  TCX_TRY(ws(c, kSlotYlo, (size_t)m * (n_pad > 0 ? n_pad : 1), &ylo));
+  const size_t panel = (size_t)tcx::panel_blocks(m) * 32 * (n_pad > 0 ? n_pad : 1);
+  TCX_TRY(ws(c, kSlotYhi, panel, &yhi));
+  TCX_TRY(ws(c, kSlotYlo, panel, &ylo));
   TCX_TRY(ws(c, kSlotA0, (size_t)k * m, &a0));
   TCX_HIP(tcx::launch_align_pack(d_x, m, perm, n_pad, offsets, k, d_shift, d_czero, row_stats,
                                  yhi, ylo,
@@ -364,14 +389,15 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
   for (int j = 0; j < k; ++j) nonempty += h_counts[j] > 0;
   std::vector<long> seg_len(k);
   for (int j = 0; j < k; ++j) seg_len[j] = h_counts[j] > 0 ? h_off[j + 1] - h_off[j] : 0;
-  plan_syrk(m, k, h_off.data(), seg_len.data(), c->items, c->tiles);
+  plan_syrk(m, k, h_off.data(), seg_len.data(), c->cus, c->items, c->tiles);
   tcx::SyrkItem* d_items;
   tcx::SyrkTile* d_tiles;
   float *part, *S;
   tcx::PowerInfo* info;
   TCX_TRY(ws(c, kSlotItems, c->items.size(), &d_items));
   TCX_TRY(ws(c, kSlotTiles, c->tiles.size(), &d_tiles));
-  TCX_TRY(ws(c, kSlotPart, c->items.size() * tcx::kSyrkTile * tcx::kSyrkTile, &part));
+  const size_t nparts = c->tiles.empty() ? 1 : c->tiles.back().part0 + c->tiles.back().nparts;
+  TCX_TRY(ws(c, kSlotPart, nparts * tcx::kSyrkTile * tcx::kSyrkTile, &part));
   TCX_TRY(ws(c, kSlotS, (size_t)k * m * m, &S));
   TCX_TRY(ws(c, kSlotInfo, (size_t)k, &info));
   if (!c->items.empty()) {
@@ -380,8 +406,8 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
     TCX_HIP(hipMemcpyAsync(d_tiles, c->tiles.data(), sizeof(tcx::SyrkTile) * c->tiles.size(),
                            hipMemcpyHostToDevice, st));
   }
-  TCX_HIP(tcx::launch_syrk(d_items, (int)c->items.size(), yhi, ylo, n_pad, m, split, part, st));
-  TCX_HIP(tcx::launch_syrk_reduce(d_tiles, (int)c->tiles.size(), part, m, S, nullptr, st));
+  TCX_HIP(tcx::launch_syrk(d_items, (int)c->items.size(), yhi, ylo, m, split, part, st));
+  TCX_HIP(tcx::launch_syrk_reduce(d_tiles, (int)c->tiles.size(), part, m, S, st));
   const int max_steps = m > 1000 ? 1000 : m;  // test_timesC.py:152-156
   const double* vin = nullptr;
   const double* svin = nullptr;

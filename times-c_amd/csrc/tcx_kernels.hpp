@@ -66,10 +66,16 @@ hipError_t configure_sbd_kernels();
 // ------------------------------------------------------------ extraction
 constexpr int kGroupChunk = 512;  // rows per grouping block
 constexpr int kPackPad = 64;       // cluster segments padded to this many columns
-constexpr int kSyrkTile = 128;     // S tile edge
+constexpr int kSyrkTile = 256;     // S tile edge (one block)
+
+// The packed panel Y (time t x packed column p, BF16 hi and lo parts) is
+// stored in MFMA operand order: chunk (c = p / 16, r = t / 32) holds 512
+// values at element offset (c * panel_blocks(m) + r) * 512, value (t, p) at
+// ((t % 32) + 32 * ((p / 8) % 2)) * 8 + p % 8 inside it, rows t >= m zero.
+__host__ __device__ constexpr int panel_blocks(int m) { return (m + 31) / 32; }
 
 struct SyrkItem {
-  int cluster;
+  int cluster;        // < 0: no work (keeps the grid's XCD grouping)
   int ti, tj;         // tile row / column (ti <= tj)
   int part;           // index of this item's partial tile
   long p0, p1;        // packed column range [p0, p1), multiples of 16
@@ -105,10 +111,9 @@ hipError_t launch_align_pack(const float* x, int m, const int* perm, long n_pad,
                              const double2* stats, uint16_t* yhi, uint16_t* ylo, float* a0,
                              hipStream_t st);
 hipError_t launch_syrk(const SyrkItem* items, int nitems, const uint16_t* yhi,
-                       const uint16_t* ylo, long n_pad, int m, int split, float* part,
-                       hipStream_t st);
+                       const uint16_t* ylo, int m, int split, float* part, hipStream_t st);
 hipError_t launch_syrk_reduce(const SyrkTile* tiles, int ntiles, const float* part, int m,
-                              float* S, unsigned* diagmax, hipStream_t st);
+                              float* S, hipStream_t st);
 hipError_t launch_power(const float* S, int m, int k, const int* counts, const float* a0,
                         uint64_t seed, int iter, int max_steps, double tol, const double* vin,
                         const double* svin, const double* rsum, const int* steps_in,
