@@ -288,7 +288,8 @@ __global__ __launch_bounds__(256) void k_align_pack(
     }
   }
   __syncthreads();
-  for (int t0 = 0; t0 < m; t0 += 64) {
+  const int tm = panel_blocks(m);
+  for (int t0 = 0; t0 < tm * 32; t0 += 64) {
 #pragma unroll 4
     for (int it = 0; it < 16; ++it) {
       const int idx = it * 256 + tid;
@@ -308,22 +309,19 @@ __global__ __launch_bounds__(256) void k_align_pack(
     }
     __syncthreads();
     // eight 1 KB fragment chunks (4 column chunks x 2 row chunks) per array,
-    // one 16-byte store per lane
-    const int tm = panel_blocks(m);
+    // one 16-byte store per lane (tm is a multiple of 8: both row chunks exist)
 #pragma unroll
     for (int pass = 0; pass < 2; ++pass) {
       const int sidx = pass * 256 + tid;
       const int ch = sidx >> 6, ln = sidx & 63;
       const int cc = ch >> 1, rr = ch & 1;
       const int r32 = (t0 >> 5) + rr;
-      if (r32 < tm) {
-        const int tt = rr * 32 + (ln & 31), q = cc * 16 + 8 * (ln >> 5);
-        const uint4 h = *reinterpret_cast<const uint4*>(&t_hi[tt][q]);
-        const uint4 l = *reinterpret_cast<const uint4*>(&t_lo[tt][q]);
-        const long off = (((p0 >> 4) + cc) * (long)tm + r32) * 512 + ln * 8;
-        *reinterpret_cast<uint4*>(yhi + off) = h;
-        *reinterpret_cast<uint4*>(ylo + off) = l;
-      }
+      const int tt = rr * 32 + (ln & 31), q = cc * 16 + 8 * (ln >> 5);
+      const uint4 h = *reinterpret_cast<const uint4*>(&t_hi[tt][q]);
+      const uint4 l = *reinterpret_cast<const uint4*>(&t_lo[tt][q]);
+      const long off = (((p0 >> 4) + cc) * (long)tm + r32) * 512 + ln * 8;
+      *reinterpret_cast<uint4*>(yhi + off) = h;
+      *reinterpret_cast<uint4*>(ylo + off) = l;
     }
     __syncthreads();
   }
@@ -348,10 +346,9 @@ __global__ __launch_bounds__(256) void k_syrk(const SyrkItem* __restrict__ items
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wi = wave >> 1, wj = wave & 1;
   if (it.ti == it.tj && wi > wj) return;
+  if (it.ti * kSyrkTile + wi * 128 >= m || it.tj * kSyrkTile + wj * 128 >= m) return;
   const int tm = panel_blocks(m);
   const int ra0 = it.ti * 8 + wi * 4, rb0 = it.tj * 8 + wj * 4;  // first 32-row blocks
-  const int na = tm - ra0 < 4 ? tm - ra0 : 4, nb = tm - rb0 < 4 ? tm - rb0 : 4;
-  if (na <= 0 || nb <= 0) return;  // quarter entirely past row m
   const bf16x8* H = reinterpret_cast<const bf16x8*>(yhi) + lane;
   const bf16x8* Lo = reinterpret_cast<const bf16x8*>(ylo) + lane;
   const long c0 = it.p0 >> 4, c1 = it.p1 >> 4;
@@ -362,48 +359,58 @@ __global__ __launch_bounds__(256) void k_syrk(const SyrkItem* __restrict__ items
     for (int b = 0; b < 4; ++b)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
-  const bf16x8 zero8 = {};
-  bf16x8 ah[4], al[4], bh[4], bl[4];
-  auto load = [&](long c, bf16x8* xh, bf16x8* xl, int r0, int nr) {
+  // two operand sets used in turn (no register copies between steps, so a
+  // step waits only for its own set's loads)
+  struct Ops {
+    bf16x8 ah[4], al[4], bh[4], bl[4];
+  };
+  auto load = [&](long c, Ops& x) {
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-      const long o = (c * tm + r0 + a) * 64;
-      xh[a] = a < nr ? H[o] : zero8;
-      if constexpr (kSplit) xl[a] = a < nr ? Lo[o] : zero8;
+      const long oa = (c * tm + ra0 + a) * 64, ob = (c * tm + rb0 + a) * 64;
+      x.ah[a] = H[oa];
+      x.bh[a] = H[ob];
+      if constexpr (kSplit) {
+        x.al[a] = Lo[oa];
+        x.bl[a] = Lo[ob];
+      }
     }
   };
-  load(c0, ah, al, ra0, na);
-  load(c0, bh, bl, rb0, nb);
-  for (long c = c0; c < c1; ++c) {
-    const long cn = c + 1 < c1 ? c + 1 : c;  // the last step reloads its own chunk
-    bf16x8 nah[4], nal[4], nbh[4], nbl[4];
-    load(cn, nah, nal, ra0, na);
-    load(cn, nbh, nbl, rb0, nb);
+  auto mma = [&](const Ops& x) {
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b)
-        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x.ah[a], x.bh[b], acc[a][b], 0, 0, 0);
     if constexpr (kSplit) {
 #pragma unroll
       for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b)
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x.ah[a], x.bl[b], acc[a][b], 0, 0, 0);
 #pragma unroll
       for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b)
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x.al[a], x.bh[b], acc[a][b], 0, 0, 0);
     }
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      ah[a] = nah[a];
-      al[a] = nal[a];
-      bh[a] = nbh[a];
-      bl[a] = nbl[a];
-    }
+  };
+  Ops X, Y;
+  load(c0, X);
+  long c = c0;
+  // the scheduling barriers keep each set's loads ahead of the other set's
+  // multiplies, so the loads have a whole block of MFMAs to land
+  for (; c + 1 < c1; c += 2) {
+    load(c + 1, Y);
+    __builtin_amdgcn_sched_barrier(0);
+    mma(X);
+    __builtin_amdgcn_sched_barrier(0);
+    load(c + 2 < c1 ? c + 2 : c + 1, X);  // past the end: a harmless reload
+    __builtin_amdgcn_sched_barrier(0);
+    mma(Y);
+    __builtin_amdgcn_sched_barrier(0);
   }
+  if (c < c1) mma(X);
   float* o = part + (long)it.part * kSyrkTile * kSyrkTile;
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -419,9 +426,9 @@ __global__ __launch_bounds__(256) void k_syrk(const SyrkItem* __restrict__ items
 
 // S_j from the K-chunk partials of each tile, summed in a fixed order.  One
 // block per 64x64 sub-block (16 per tile, the lower ones of diagonal tiles
-// skipped) writes the entries with i <= j and, through an LDS transpose,
-// their mirrors, so S_j is exactly symmetric and every store is a
-// coalesced row segment.
+// skipped): 16-byte partial loads, four rows per thread in flight, the sums
+// staged in LDS; then the entries with i <= j and their mirrors are stored
+// as coalesced row segments, so S_j is exactly symmetric.
 __global__ __launch_bounds__(256) void k_syrk_reduce(const SyrkTile* __restrict__ tiles,
                                                      const float* __restrict__ part, int m,
                                                      float* __restrict__ S) {
@@ -432,23 +439,41 @@ __global__ __launch_bounds__(256) void k_syrk_reduce(const SyrkTile* __restrict_
   const int gi0 = tl.ti * kSyrkTile + si * 64, gj0 = tl.tj * kSyrkTile + sj * 64;
   if (gi0 >= m || gj0 >= m) return;
   __shared__ float tr[64][65];
-  float* Sj = S + (long)tl.cluster * m * m;
-  const int col = threadIdx.x & 63, r0 = threadIdx.x >> 6;
-  const float* pp = part + (long)tl.part0 * kSyrkTile * kSyrkTile;
-#pragma unroll 4
-  for (int r = r0; r < 64; r += 4) {
-    const long e = (long)(si * 64 + r) * kSyrkTile + sj * 64 + col;
-    float s = 0.f;
-    for (int c = 0; c < tl.nparts; ++c) s += pp[(long)c * kSyrkTile * kSyrkTile + e];
-    tr[r][col] = s;
-    const int gi = gi0 + r, gj = gj0 + col;
-    if (gi < m && gj < m && gi <= gj) Sj[(long)gi * m + gj] = s;
+  const int c4 = (threadIdx.x & 15) * 4, rr = threadIdx.x >> 4;
+  const float* pp = part + (long)tl.part0 * kSyrkTile * kSyrkTile + (long)(si * 64 + rr) * kSyrkTile +
+                    sj * 64 + c4;
+  float4 acc[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) acc[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll 2
+  for (int c = 0; c < tl.nparts; ++c) {
+    const float* pc = pp + (long)c * kSyrkTile * kSyrkTile;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float4 v = *reinterpret_cast<const float4*>(pc + (long)16 * u * kSyrkTile);
+      acc[u].x += v.x;
+      acc[u].y += v.y;
+      acc[u].z += v.z;
+      acc[u].w += v.w;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    float* t = &tr[rr + 16 * u][c4];
+    t[0] = acc[u].x;
+    t[1] = acc[u].y;
+    t[2] = acc[u].z;
+    t[3] = acc[u].w;
   }
   __syncthreads();
+  float* Sj = S + (long)tl.cluster * m * m;
+  const int col = threadIdx.x & 63, r0 = threadIdx.x >> 6;
 #pragma unroll 4
   for (int r = r0; r < 64; r += 4) {
-    const int gi = gi0 + col, gj = gj0 + r;  // mirror of entry (gi, gj)
-    if (gi < m && gj < m && gi <= gj) Sj[(long)gj * m + gi] = tr[col][r];
+    const int gi = gi0 + r, gj = gj0 + col;
+    if (gi < m && gj < m && gi <= gj) Sj[(long)gi * m + gj] = tr[r][col];
+    const int hi = gi0 + col, hj = gj0 + r;  // mirror of entry (hi, hj)
+    if (hi < m && hj < m && hi <= hj) Sj[(long)hj * m + hi] = tr[col][r];
   }
 }
 

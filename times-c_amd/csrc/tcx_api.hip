@@ -213,11 +213,12 @@ int assign_core(tcx_ctx* c, const float* d_x, long n, int m, int k, const float2
 
 // Work list of a grouped S = Y^T Y: for each cluster with seg_len[j] > 0
 // packed columns starting at off[j], the upper triangle of
-// ceil(m/256)^2 tiles, each split along K into chunks of one length, the
-// shortest for which the grid is at most one block per CU (one 4-wave block
-// fills a CU's register file).  The tiles of one (cluster, chunk) read the
-// same panel rows, so they are placed on one XCD (blocks b with equal
-// b % 8 run on one XCD) at adjacent dispatch slots; free slots are holes.
+// ceil(m/256)^2 tiles, each split along K into chunks of one length.  The
+// tiles of one (cluster, chunk) read the same panel rows, so they are
+// placed on one XCD (block b runs on XCD b % 8) at adjacent dispatch
+// slots; free slots are holes.  The chunk is the shortest for which every
+// XCD holds at most one block per CU (one 4-wave block fills a CU's
+// register file), so all blocks run in one round.
 void plan_syrk(int m, int k, const long* off, const long* seg_len, int cus,
                std::vector<tcx::SyrkItem>& items, std::vector<tcx::SyrkTile>& tiles) {
   constexpr int kXcds = 8;
@@ -228,15 +229,16 @@ void plan_syrk(int m, int k, const long* off, const long* seg_len, int cus,
   long max_steps = 0;
   for (int j = 0; j < k; ++j) max_steps = std::max(max_steps, (seg_len[j] + 15) / 16);
   if (max_steps == 0) return;
-  auto blocks_for = [&](long cs) {
-    long b = 0;
-    for (int j = 0; j < k; ++j) b += (seg_len[j] + 16 * cs - 1) / (16 * cs) * tiles_per;
-    return b;
+  const long slots = std::max(1, cus / kXcds);  // blocks per XCD in one round
+  auto fits = [&](long cs) {
+    long g = 0;
+    for (int j = 0; j < k; ++j) g += (seg_len[j] + 16 * cs - 1) / (16 * cs);
+    return (g + kXcds - 1) / kXcds * tiles_per <= slots;
   };
-  long lo = 1, hi = max_steps;  // smallest chunk (in 16-column steps) with blocks <= cus
+  long lo = 1, hi = max_steps;  // smallest chunk (in 16-column steps) that fits
   while (lo < hi) {
     const long mid = (lo + hi) / 2;
-    if (blocks_for(mid) <= std::max(cus, 1)) hi = mid; else lo = mid + 1;
+    if (fits(mid)) hi = mid; else lo = mid + 1;
   }
   const long cs = std::min(max_steps, std::max(lo, 16L)) * 16;  // columns per chunk
   struct Group { int j; long p0, p1; int c; int tile0; };

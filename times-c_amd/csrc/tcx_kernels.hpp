@@ -71,8 +71,11 @@ constexpr int kSyrkTile = 256;     // S tile edge (one block)
 // The packed panel Y (time t x packed column p, BF16 hi and lo parts) is
 // stored in MFMA operand order: chunk (c = p / 16, r = t / 32) holds 512
 // values at element offset (c * panel_blocks(m) + r) * 512, value (t, p) at
-// ((t % 32) + 32 * ((p / 8) % 2)) * 8 + p % 8 inside it, rows t >= m zero.
-__host__ __device__ constexpr int panel_blocks(int m) { return (m + 31) / 32; }
+// ((t % 32) + 32 * ((p / 8) % 2)) * 8 + p % 8 inside it.  Rows run to a
+// whole number of S tiles, rows t >= m are zero.
+__host__ __device__ constexpr int panel_blocks(int m) {
+  return (m + kSyrkTile - 1) / kSyrkTile * (kSyrkTile / 32);
+}
 
 struct SyrkItem {
   int cluster;        // < 0: no work (keeps the grid's XCD grouping)
