@@ -125,6 +125,8 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_sbd(
   float2* buf = smem + F + g * fft_buf_len<F>();
   const long npairs = (n + 1) / 2;
   const int nlag = 2 * m - 1;
+  int s_lo, s_hi;
+  lag_rows(lane, L, m, F, s_lo, s_hi);
   for (long pr = (long)blockIdx.x * G::kGroups + g; pr < npairs;
        pr += (long)gridDim.x * G::kGroups) {
     const long ia = 2 * pr, ib = 2 * pr + 1;
@@ -161,22 +163,9 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_sbd(
       const float cn = cnorm[j];
       const float dena = na * cn, denb = nb * cn;
       if constexpr (MODE == kModeAssign) {
-        float va = -INFINITY, vb = -INFINITY;
-        int ka = 0x7fffffff, kb = 0x7fffffff;
-#pragma unroll
-        for (int s = 0; s < P; ++s) {
-          int key = lag_key(lane + L * s, m, F, rev);
-          if (key >= 0) {
-            if (z[s].x > va || (z[s].x == va && key < ka)) {
-              va = z[s].x;
-              ka = key;
-            }
-            if (z[s].y > vb || (z[s].y == vb && key < kb)) {
-              vb = z[s].y;
-              kb = key;
-            }
-          }
-        }
+        float va, vb;
+        int ka, kb;
+        lag_argmax2<L, P>(z, s_lo, s_hi, rev, lane, m, F, va, ka, vb, kb);
         group_argmax<L>(va, ka, red);
         group_argmax<L>(vb, kb, red);
         // den == 0: NumPy divides by inf, every lag is 0, argmax is index 0
@@ -293,6 +282,8 @@ __global__ __launch_bounds__(PruneGeom<F>::kBlock) void k_assign_prune(
   float2* buf = smem + F + g * fft_buf_len<F>();
   float* ubs = ubs_all + (long)g * 2 * k;  // [0,k): series a, [k,2k): series b
   unsigned long long my_evals = 0;
+  int s_lo, s_hi;
+  lag_rows(lane, L, m, F, s_lo, s_hi);
   for (long pr = (long)blockIdx.x * G::kGroups + g; pr < npairs;
        pr += (long)gridDim.x * G::kGroups) {
     long ia = order ? order[2 * pr] : 2 * pr;
@@ -425,15 +416,15 @@ __global__ __launch_bounds__(PruneGeom<F>::kBlock) void k_assign_prune(
       // candidate with the largest bound among centroids that can still win
       float sc = -INFINITY;
       int cj = 0x7fffffff;
-      for (int j = lane; j < k; j += L) {
+      for (int j = lane; j < k; j += L) {  // j increases: strict '>' keeps the first
         const float ua = ubs[j], ub = ubs[k + j];
-        float v = -INFINITY;
-        if (ua >= 0.f && 1.f - (ua * (1.f + kUbRel) + kUbAbs) <= bestA) v = ua;
-        if (ub >= 0.f && 1.f - (ub * (1.f + kUbRel) + kUbAbs) <= bestB) v = fmaxf(v, ub);
-        if (v > sc || (v == sc && j < cj)) {
-          sc = v;
-          cj = j;
-        }
+        const bool oka = (ua >= 0.f) & (1.f - (ua * (1.f + kUbRel) + kUbAbs) <= bestA);
+        const bool okb = (ub >= 0.f) & (1.f - (ub * (1.f + kUbRel) + kUbAbs) <= bestB);
+        float v = oka ? ua : -INFINITY;
+        v = okb ? fmaxf(v, ub) : v;
+        const bool take = v > sc;
+        sc = take ? v : sc;
+        cj = take ? j : cj;
       }
       group_argmax<L>(sc, cj, red);
       if (sc == -INFINITY) break;
@@ -454,23 +445,8 @@ __global__ __launch_bounds__(PruneGeom<F>::kBlock) void k_assign_prune(
 #pragma unroll
         for (int s = 0; s < P; ++s) z[s] = cmulc(zx[s], cjs[s * L + lane]);
         fft<F, L, true>(z, buf, tw, lane);
-        float va = -INFINITY, vb = -INFINITY;
-        ka = 0x7fffffff;
-        kb = 0x7fffffff;
-#pragma unroll
-        for (int s = 0; s < P; ++s) {
-          const int key = lag_key(lane + L * s, m, F, rev);
-          if (key >= 0) {
-            if (z[s].x > va || (z[s].x == va && key < ka)) {
-              va = z[s].x;
-              ka = key;
-            }
-            if (z[s].y > vb || (z[s].y == vb && key < kb)) {
-              vb = z[s].y;
-              kb = key;
-            }
-          }
-        }
+        float va, vb;
+        lag_argmax2<L, P>(z, s_lo, s_hi, rev, lane, m, F, va, ka, vb, kb);
         group_argmax<L>(va, ka, red);
         group_argmax<L>(vb, kb, red);
         if (dena != 0.f) da = 1.f - (va / (float)F) / dena; else ka = 0;

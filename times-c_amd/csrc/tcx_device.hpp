@@ -63,21 +63,17 @@ __device__ __forceinline__ float readlane_f32(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-template <int W, int CTRL>
-__device__ __forceinline__ void argmax_step(float& v, int& key) {
-  const float ov = dpp_f32<CTRL>(v);
-  const int ok = dpp_i32<CTRL>(key);
-  if (ov > v || (ov == v && ok < key)) {
-    v = ov;
-    key = ok;
-  }
+// (v, key) <- the larger value, the smaller key among equal values; written
+// as selects so that no lane mask is saved or restored
+__device__ __forceinline__ void argmax_merge(float& v, int& key, float ov, int ok) {
+  const bool take = (ov > v) | ((ov == v) & (ok < key));
+  v = take ? ov : v;
+  key = take ? ok : key;
 }
 
-__device__ __forceinline__ void argmax_merge(float& v, int& key, float ov, int ok) {
-  if (ov > v || (ov == v && ok < key)) {
-    v = ov;
-    key = ok;
-  }
+template <int W, int CTRL>
+__device__ __forceinline__ void argmax_step(float& v, int& key) {
+  argmax_merge(v, key, dpp_f32<CTRL>(v), dpp_i32<CTRL>(key));
 }
 
 template <int L>
@@ -146,14 +142,7 @@ __device__ __forceinline__ void group_argmax(float& v, int& key, float* scratch)
     v = scratch[0];
     key = ks[0];
 #pragma unroll
-    for (int w = 1; w < NW; ++w) {
-      float ov = scratch[w];
-      int ok = ks[w];
-      if (ov > v || (ov == v && ok < key)) {
-        v = ov;
-        key = ok;
-      }
-    }
+    for (int w = 1; w < NW; ++w) argmax_merge(v, key, scratch[w], ks[w]);
   }
 }
 
@@ -389,6 +378,55 @@ __device__ __forceinline__ int lag_key(int tau, int m, int F, int rev) {
   int idx = tau <= m - 1 ? tau + m - 1 : (tau >= F - (m - 1) ? tau - F + m - 1 : -1);
   if (idx >= 0 && rev) idx = 2 * m - 2 - idx;
   return idx;
+}
+
+// Per-lane argmax over the lags of two inverse transforms packed as
+// z[s].x, z[s].y at tau = lane + L*s (P values per lane, L*P = F).  tau is
+// a lag when tau <= m-1 (s <= s_lo) or tau >= F-m+1 (s >= s_hi); lag_rows
+// gives this lane's s_lo and s_hi.  tau < F/2 holds the lags 0..m-1 (keys
+// m-1+tau) and tau >= F/2 the negative lags (keys tau-F+m-1, all smaller),
+// so visiting s = P/2..P-1 and then 0..P/2-1 meets the keys in increasing
+// order (the reverse order when rev, whose keys are mirrored); a strict '>'
+// then keeps the smallest key among equal values.  No lag taken (all -inf
+// or NaN): key INT_MAX.
+__device__ __forceinline__ void lag_rows(int lane, int L, int m, int F, int& s_lo, int& s_hi) {
+  const int a = m - 1 - lane;                 // tau <= m-1  <=>  L*s <= a
+  s_lo = a >= 0 ? a / L : -1;
+  const int b = F - (m - 1) - lane;           // tau >= F-m+1  <=>  L*s >= b
+  s_hi = b <= 0 ? 0 : (b + L - 1) / L;
+}
+
+template <int L, int P>
+__device__ __forceinline__ void lag_argmax2(const float2* z, int s_lo, int s_hi, int rev, int lane,
+                                            int m, int F, float& va, int& ka, float& vb,
+                                            int& kb) {
+  float bx = -INFINITY, by = -INFINITY;
+  int sx = 0, sy = 0;
+  auto visit = [&](int s) {
+    const bool ok = (s <= s_lo) | (s >= s_hi);
+    const float zx = ok ? z[s].x : -INFINITY;
+    const float zy = ok ? z[s].y : -INFINITY;
+    const bool tx = zx > bx, ty = zy > by;
+    bx = tx ? zx : bx;
+    sx = tx ? s : sx;
+    by = ty ? zy : by;
+    sy = ty ? s : sy;
+  };
+  if (!rev) {
+#pragma unroll
+    for (int s = P / 2; s < P; ++s) visit(s);
+#pragma unroll
+    for (int s = 0; s < P / 2; ++s) visit(s);
+  } else {
+#pragma unroll
+    for (int s = P / 2 - 1; s >= 0; --s) visit(s);
+#pragma unroll
+    for (int s = P - 1; s >= P / 2; --s) visit(s);
+  }
+  va = bx;
+  vb = by;
+  ka = bx > -INFINITY ? lag_key(lane + L * sx, m, F, rev) : 0x7fffffff;
+  kb = by > -INFINITY ? lag_key(lane + L * sy, m, F, rev) : 0x7fffffff;
 }
 
 }  // namespace tcx
