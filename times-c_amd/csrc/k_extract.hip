@@ -327,6 +327,165 @@ __global__ __launch_bounds__(256) void k_align_pack(
   }
 }
 
+// The same result as k_align_pack for m <= kPackLdsMaxM, from R whole rows
+// staged in LDS (R = 16, 32 or 64 packed columns per block, R * (m + 4)
+// floats): every gathered row is read from HBM once, the statistics of the
+// aligned row a = roll_zeropad(x_row, s) are two-pass FP64 sums over LDS
+// (mean, then the centred sum of squares with the |s| zeros included), and
+// the panel chunks are formed from LDS.
+constexpr int kPackLdsMaxM = 1024;
+constexpr int kPackLdsBytes = 36 * 1024;     // R = 64 or 32 within this: four blocks per CU
+constexpr int kPackLdsMaxBytes = 72 * 1024;  // R = 16 up to m = kPackLdsMaxM
+
+__host__ __device__ constexpr int pack_lds_rows(int m) {
+  return (m + 4) * 64 * 4 <= kPackLdsBytes ? 64 : ((m + 4) * 32 * 4 <= kPackLdsBytes ? 32 : 16);
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_align_pack_lds(
+    const float* __restrict__ x, int m, const int* __restrict__ perm,
+    const long* __restrict__ offsets, int k, const int* __restrict__ shift,
+    const int* __restrict__ czero, uint16_t* __restrict__ yhi, uint16_t* __restrict__ ylo,
+    float* __restrict__ a0) {
+  extern __shared__ __align__(16) float rows_s[];  // R rows of stride ms
+  __shared__ double s_mean[64], s_isd[64];
+  __shared__ int s_row[64], s_sh[64];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int ms = m + 4;
+  const long p0 = (long)blockIdx.x * R;
+  int first = -1;  // lane q < R of wave 0: cluster whose lowest member is column q
+  if (wid == 0 && lane < R) {
+    const long p = p0 + lane;
+    const int row = perm[p];
+    int sh = 0;
+    if (row >= 0) {
+      const int cl = cluster_of(p, offsets, k);
+      sh = czero[cl] ? 0 : shift[row];
+      if (p == offsets[cl]) first = cl;
+    }
+    s_row[lane] = row;
+    s_sh[lane] = sh;
+  }
+  __syncthreads();
+  // stage the R source rows (unshifted; absent rows as zeros): wave w holds
+  // rows w, w+4, ...; the R/4 loads of a column step are issued together
+  // (holes read row 0 and are zeroed)
+  constexpr int RPW = R / 4;
+  if ((m & 3) == 0) {
+    const int nv = m >> 2;
+    for (int v0 = 0; v0 < nv; v0 += kWave) {
+      const int v = v0 + lane < nv ? v0 + lane : nv - 1;
+      float4 val[RPW];
+#pragma unroll
+      for (int r = 0; r < RPW; ++r)
+        val[r] = reinterpret_cast<const float4*>(x + (long)max(s_row[wid + 4 * r], 0) * m)[v];
+      if (v0 + lane < nv) {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+          const int q = wid + 4 * r;
+          *reinterpret_cast<float4*>(rows_s + q * ms + 4 * v) =
+              s_row[q] >= 0 ? val[r] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+      }
+    }
+  } else {
+    for (int t0 = 0; t0 < m; t0 += kWave) {
+      const int t = t0 + lane < m ? t0 + lane : m - 1;
+      float val[RPW];
+#pragma unroll
+      for (int r = 0; r < RPW; ++r) val[r] = x[(long)max(s_row[wid + 4 * r], 0) * m + t];
+      if (t0 + lane < m) {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+          const int q = wid + 4 * r;
+          rows_s[q * ms + t] = s_row[q] >= 0 ? val[r] : 0.f;
+        }
+      }
+    }
+  }
+  // two-pass statistics of the aligned rows, one wave per row
+  for (int q = wid; q < R; q += 4) {
+    const int sh = s_sh[q];
+    const float* r = rows_s + q * ms;
+    const int src0 = sh > 0 ? 0 : (-sh < m ? -sh : m);    // kept source range
+    const int src1 = sh > 0 ? (sh < m ? m - sh : 0) : m;
+    double a = 0.0;
+    for (int i = src0 + lane; i < src1; i += kWave) a += (double)r[i];
+    const double mean = wave_sum_f64(a) / (double)m;
+    double b = 0.0;
+    for (int i = src0 + lane; i < src1; i += kWave) {
+      const double d = (double)r[i] - mean;
+      b += d * d;
+    }
+    const double ss = wave_sum_f64(b) + (double)(m - (src1 - src0)) * mean * mean;
+    if (lane == 0) {
+      s_mean[q] = mean;
+      s_isd[q] = 1.0 / sqrt(ss / (double)(m - 1));
+    }
+  }
+  // the lowest-index member of a cluster writes its aligned row to a0
+  if (wid == 0) {
+    unsigned long long fm = __ballot(first >= 0);
+    while (fm) {
+      const int q = __builtin_ctzll(fm);
+      fm &= fm - 1;
+      const int cl = __shfl(first, q);
+      const int sh = s_sh[q];
+      const float* r = rows_s + q * ms;
+      float* o = a0 + (long)cl * m;
+      for (int t = lane; t < m; t += kWave) {
+        const int src = t - sh;
+        o[t] = (src >= 0 && src < m) ? r[src] : 0.f;
+      }
+    }
+  }
+  __syncthreads();
+  // panel chunks (R/16 column chunks x tm row chunks), 16 bytes per lane:
+  // lane ln packs columns qb..qb+7 (their parameters held in registers) at
+  // t = 32 * r32 + ln % 32 for the row chunks r32 = w, w+4, ...
+  const int tm = panel_blocks(m);
+#pragma unroll 1
+  for (int cc = 0; cc < R / 16; ++cc) {
+    const int qb = cc * 16 + 8 * (lane >> 5);
+    int c_row[8], c_sh[8];
+    double c_mean[8], c_isd[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      c_row[v] = s_row[qb + v];
+      c_sh[v] = s_sh[qb + v];
+      c_mean[v] = s_mean[qb + v];
+      c_isd[v] = s_isd[qb + v];
+    }
+    for (int r32 = wid; r32 < tm; r32 += 4) {
+      const int t = r32 * 32 + (lane & 31);
+      uint32_t h[4], l[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        uint16_t hh[2], ll[2];
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+          const int c = 2 * u + v;
+          float f = 0.f;
+          if (c_row[c] >= 0 && t < m) {
+            const int src = t - c_sh[c];
+            const double a = (src >= 0 && src < m) ? (double)rows_s[(qb + c) * ms + src] : 0.0;
+            const double y = (a - c_mean[c]) * c_isd[c];
+            f = isnan(y) ? 0.f
+                         : (isinf(y) ? (y > 0 ? 3.402823466e38f : -3.402823466e38f) : (float)y);
+          }
+          hh[v] = f2bf_rne(f);
+          ll[v] = f2bf_rne(f - bf2f(hh[v]));
+        }
+        h[u] = (uint32_t)hh[0] | ((uint32_t)hh[1] << 16);
+        l[u] = (uint32_t)ll[0] | ((uint32_t)ll[1] << 16);
+      }
+      const long off = (((p0 >> 4) + cc) * (long)tm + r32) * 512 + lane * 8;
+      *reinterpret_cast<uint4*>(yhi + off) = make_uint4(h[0], h[1], h[2], h[3]);
+      *reinterpret_cast<uint4*>(ylo + off) = make_uint4(l[0], l[1], l[2], l[3]);
+    }
+  }
+}
+
 // --------------------------------------------------------------- S = Y^T Y
 // One 256x256 tile of S_j over packed columns [p0, p1) per block: four waves
 // in a 2x2 arrangement, each a 128x128 quarter as 4x4 MFMA 32x32x16 BF16
@@ -1490,11 +1649,28 @@ hipError_t launch_row_stats(const float* x, long rows, int m, double2* stats, hi
   return hipGetLastError();
 }
 
+bool align_pack_uses_row_stats(int m) { return m > kPackLdsMaxM; }
+
 hipError_t launch_align_pack(const float* x, int m, const int* perm, long n_pad,
                              const long* offsets, int k, const int* shift, const int* czero,
                              const double2* stats, uint16_t* yhi, uint16_t* ylo, float* a0,
                              hipStream_t st) {
   if (n_pad <= 0) return hipSuccess;
+  if (m <= kPackLdsMaxM) {
+    const int R = pack_lds_rows(m);
+    const size_t lds = sizeof(float) * (size_t)R * (m + 4);
+    const dim3 grid((unsigned)(n_pad / R));
+    if (R == 64)
+      hipLaunchKernelGGL(k_align_pack_lds<64>, grid, dim3(256), lds, st, x, m, perm, offsets, k,
+                         shift, czero, yhi, ylo, a0);
+    else if (R == 32)
+      hipLaunchKernelGGL(k_align_pack_lds<32>, grid, dim3(256), lds, st, x, m, perm, offsets, k,
+                         shift, czero, yhi, ylo, a0);
+    else
+      hipLaunchKernelGGL(k_align_pack_lds<16>, grid, dim3(256), lds, st, x, m, perm, offsets, k,
+                         shift, czero, yhi, ylo, a0);
+    return hipGetLastError();
+  }
   long blocks = n_pad / 64;
   hipLaunchKernelGGL(k_align_pack, dim3((unsigned)blocks), dim3(256), 0, st, x, m, perm, n_pad,
                      offsets, k, shift, czero, stats, yhi, ylo, a0);
@@ -1597,6 +1773,15 @@ hipError_t configure_extract_kernels() {
     e = hipFuncSetAttribute((const void*)k_cluster_matvec,
                             hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)(sizeof(double) * 4096));
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_align_pack_lds<64>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kPackLdsMaxBytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_align_pack_lds<32>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kPackLdsMaxBytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_align_pack_lds<16>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kPackLdsMaxBytes);
   return e;
 }
 

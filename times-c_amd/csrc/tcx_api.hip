@@ -348,7 +348,7 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
            const int* d_shift, const int* d_czero, float* d_cout, int iter, uint64_t seed,
            int split, double tol, int cal_select, const double2* row_stats, RefineResult* res) {
   hipStream_t st = c->stream;
-  if (!row_stats) {
+  if (!row_stats && tcx::align_pack_uses_row_stats(m)) {
     double2* rs;
     TCX_TRY(ws(c, kSlotRowStats, (size_t)n, &rs));
     TCX_HIP(tcx::launch_row_stats(d_x, n, m, rs, st));
@@ -779,10 +779,13 @@ int tcx_fit_begin(tcx_ctx* c, const float* d_x, int n, int m, int k, const tcx_f
   }
   TCX_TRY(spectra(c, f.cent, k, m, &f.spec, &f.norm, &f.zero));
   if (o.init_mode == 0) TCX_HIP(tcx::launch_fill_int(f.zero, k, 1, st));  // centroids start at 0
-  double2* rs;
-  TCX_TRY(ws(c, kSlotFitRowStats, (size_t)n, &rs));
-  TCX_HIP(tcx::launch_row_stats(d_x, n, m, rs, st));
-  f.row_stats = rs;
+  f.row_stats = nullptr;
+  if (tcx::align_pack_uses_row_stats(m)) {
+    double2* rs;
+    TCX_TRY(ws(c, kSlotFitRowStats, (size_t)n, &rs));
+    TCX_HIP(tcx::launch_row_stats(d_x, n, m, rs, st));
+    f.row_stats = rs;
+  }
   // the host copy of the initial labels must outlive the async upload
   TCX_HIP(hipStreamSynchronize(st));
   f.o = o;

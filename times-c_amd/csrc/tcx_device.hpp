@@ -402,8 +402,9 @@ __device__ __forceinline__ void lag_argmax2(const float2* z, int s_lo, int s_hi,
                                             int& kb) {
   float bx = -INFINITY, by = -INFINITY;
   int sx = 0, sy = 0;
+  // s_lo < P/2 <= s_hi, so each half needs one of the two tests
   auto visit = [&](int s) {
-    const bool ok = (s <= s_lo) | (s >= s_hi);
+    const bool ok = s < P / 2 ? s <= s_lo : s >= s_hi;
     const float zx = ok ? z[s].x : -INFINITY;
     const float zy = ok ? z[s].y : -INFINITY;
     const bool tx = zx > bx, ty = zy > by;

@@ -109,6 +109,9 @@ hipError_t launch_group_place(const int* label, long n, int k, const int* blk_ba
                               const long* offsets, int* perm, hipStream_t st);
 // FP64 (sum, sum of squares) of every row.
 hipError_t launch_row_stats(const float* x, long rows, int m, double2* stats, hipStream_t st);
+// True when launch_align_pack reads per-row sums (launch_row_stats) for
+// this m; otherwise `stats` may be NULL.
+bool align_pack_uses_row_stats(int m);
 hipError_t launch_align_pack(const float* x, int m, const int* perm, long n_pad,
                              const long* offsets, int k, const int* shift, const int* czero,
                              const double2* stats, uint16_t* yhi, uint16_t* ylo, float* a0,
