@@ -171,8 +171,10 @@ int tcx_kshape_fit(tcx_ctx* ctx, const float* d_x, int n, int m, int k,
  * tcx_fit_begin sets up the initial state, each tcx_fit_step runs one
  * iteration (refine + assign, or assign + stop test + refine for
  * init_mode 1) and reports in *changed (may be NULL) whether any label
- * changed; the caller decides when to stop.  tcx_fit_end copies out the
- * result (h_labels, h_centroids and stats may each be NULL). */
+ * changed; the caller decides when to stop.  A step may return while its
+ * last refine still runs on the context's stream; the next call on the
+ * context orders after it.  tcx_fit_end copies out the result (h_labels,
+ * h_centroids and stats may each be NULL). */
 int tcx_fit_begin(tcx_ctx* ctx, const float* d_x, int n, int m, int k,
                   const tcx_fit_opts* opts);
 int tcx_fit_step(tcx_ctx* ctx, int* changed);

@@ -67,6 +67,7 @@ struct FitState {
   const double2* row_stats = nullptr;  // per-row sums of d_x (fixed during the fit)
   long long evals = 0, pairs = 0;
   float ms_ref = 0.f, ms_asg = 0.f, ms_asg_last = 0.f;
+  bool ref_timed = false;  // events 0..1 hold a refine not yet added to ms_ref
   int *label = nullptr, *shift = nullptr, *changed = nullptr, *zero = nullptr;
   float *dist = nullptr, *cent = nullptr, *norm = nullptr;
   float2* spec = nullptr;
@@ -90,6 +91,8 @@ struct tcx_ctx {
   std::vector<tcx::SyrkTile> tiles;
   int cus = 0;                 // compute units of the device
   long long wall_khz = 0;      // wall_clock64() rate
+  void* pinned = nullptr;      // page-locked host area for small readbacks
+  size_t pinned_bytes = 0;
 };
 
 namespace {
@@ -107,6 +110,23 @@ int fft_size(int m) {
 int set_dev(tcx_ctx* c) {
   if (!c) return fail(TCX_ERR_ARG, "null context");
   TCX_HIP(hipSetDevice(c->device));
+  return TCX_OK;
+}
+
+// Page-locked host memory of at least `bytes` for device-to-host readbacks
+// of a few hundred bytes (pageable targets make such copies synchronous
+// and staged).  The area is reused by every call on this context; the
+// caller has synchronized the stream before it reads or reuses it.
+int pinned_area(tcx_ctx* c, size_t bytes, void** out) {
+  if (bytes > c->pinned_bytes) {
+    if (c->pinned) TCX_HIP(hipHostFree(c->pinned));
+    c->pinned = nullptr;
+    c->pinned_bytes = 0;
+    const size_t want = std::max<size_t>(bytes, 64 * 1024);
+    TCX_HIP(hipHostMalloc(&c->pinned, want, hipHostMallocDefault));
+    c->pinned_bytes = want;
+  }
+  *out = c->pinned;
   return TCX_OK;
 }
 
@@ -366,14 +386,18 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
   TCX_HIP(tcx::launch_fill_int(bad, 1, 0, st));
   TCX_HIP(tcx::launch_group_count(d_label, n, k, blk_count, bad, st));
   TCX_HIP(tcx::launch_group_scan((int)nblk, k, blk_count, blk_base, counts, offsets, st));
-  std::vector<int> h_counts(k);
-  std::vector<long> h_off(k + 1);
-  int h_bad = 0;
-  TCX_HIP(hipMemcpyAsync(h_counts.data(), counts, sizeof(int) * k, hipMemcpyDeviceToHost, st));
-  TCX_HIP(hipMemcpyAsync(h_off.data(), offsets, sizeof(long) * (k + 1), hipMemcpyDeviceToHost, st));
-  TCX_HIP(hipMemcpyAsync(&h_bad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+  void* pin;
+  TCX_TRY(pinned_area(c, sizeof(long) * (k + 1) + sizeof(int) * (k + 2), &pin));
+  long* p_off = static_cast<long*>(pin);
+  int* p_counts = reinterpret_cast<int*>(p_off + k + 1);
+  int* p_bad = p_counts + k;
+  TCX_HIP(hipMemcpyAsync(p_counts, counts, sizeof(int) * k, hipMemcpyDeviceToHost, st));
+  TCX_HIP(hipMemcpyAsync(p_off, offsets, sizeof(long) * (k + 1), hipMemcpyDeviceToHost, st));
+  TCX_HIP(hipMemcpyAsync(p_bad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
   TCX_HIP(hipStreamSynchronize(st));
-  if (h_bad) return fail(TCX_ERR_ARG, "label outside [0, k)");
+  if (*p_bad) return fail(TCX_ERR_ARG, "label outside [0, k)");
+  const std::vector<int> h_counts(p_counts, p_counts + k);
+  const std::vector<long> h_off(p_off, p_off + k + 1);
   const long n_pad = h_off[k];
   TCX_TRY(ws(c, kSlotPerm, (size_t)(n_pad > 0 ? n_pad : 1), &perm));
   TCX_HIP(tcx::launch_fill_int(perm, n_pad, -1, st));
@@ -425,9 +449,11 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
                           &rsum, &steps_in, &conv_in));
   TCX_HIP(tcx::launch_power(S, m, k, counts, a0, seed, iter, max_steps, tol, vin, svin, rsum,
                             steps_in, conv_in, mode, nullptr, 1, d_cout, info, st));
-  std::vector<tcx::PowerInfo> h_info(k);
-  TCX_HIP(hipMemcpyAsync(h_info.data(), info, sizeof(tcx::PowerInfo) * k, hipMemcpyDeviceToHost, st));
+  TCX_TRY(pinned_area(c, sizeof(tcx::PowerInfo) * k, &pin));
+  tcx::PowerInfo* p_info = static_cast<tcx::PowerInfo*>(pin);
+  TCX_HIP(hipMemcpyAsync(p_info, info, sizeof(tcx::PowerInfo) * k, hipMemcpyDeviceToHost, st));
   TCX_HIP(hipStreamSynchronize(st));
+  std::vector<tcx::PowerInfo> h_info(p_info, p_info + k);
   if (mode == 0) {
     std::vector<int> redo(k, 0);
     bool any = false;
@@ -444,9 +470,10 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
                             &rsum, &steps_in, &conv_in));
       TCX_HIP(tcx::launch_power(S, m, k, counts, a0, seed, iter, max_steps, tol, vin, svin, rsum,
                                 steps_in, conv_in, 1, d_mask, 0, d_cout, info, st));
-      TCX_HIP(hipMemcpyAsync(h_info.data(), info, sizeof(tcx::PowerInfo) * k,
-                             hipMemcpyDeviceToHost, st));
+      TCX_HIP(hipMemcpyAsync(p_info, info, sizeof(tcx::PowerInfo) * k, hipMemcpyDeviceToHost,
+                             st));
       TCX_HIP(hipStreamSynchronize(st));
+      h_info.assign(p_info, p_info + k);
     }
   }
   if (res) {
@@ -550,6 +577,7 @@ int tcx_destroy(tcx_ctx* c) {
   for (auto& kv : c->user_allocs) keep(hipFree(kv.first));
   for (auto& ev : c->ev)
     if (ev) keep(hipEventDestroy(ev));
+  if (c->pinned) keep(hipHostFree(c->pinned));
   if (c->stream) keep(hipStreamDestroy(c->stream));
   delete c;
   if (e != hipSuccess) return fail(TCX_ERR_HIP, std::string("destroy: ") + hipGetErrorString(e));
@@ -814,10 +842,7 @@ int fit_refine(tcx_ctx* c, FitState& f) {
   TCX_HIP(hipMemcpyAsync(f.cent, cnew, sizeof(float) * f.k * f.m, hipMemcpyDeviceToDevice, st));
   TCX_TRY(spectra(c, f.cent, f.k, f.m, &f.spec, &f.norm, &f.zero));
   TCX_HIP(hipEventRecord(c->ev[1], st));
-  TCX_HIP(hipEventSynchronize(c->ev[1]));
-  float ms = 0.f;
-  TCX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-  f.ms_ref += ms;
+  f.ref_timed = true;  // read at the next synchronization (fit_assign)
   f.sum_class = rr.sum_class;
   f.n_pad = rr.n_pad;
   f.have_order = true;
@@ -837,14 +862,24 @@ int fit_assign(tcx_ctx* c, FitState& f, int* h_changed) {
   TCX_TRY(assign_core(c, f.d_x, f.n, f.m, f.k, f.spec, f.norm, f.o.convention, f.label, f.dist,
                       f.shift, nullptr, f.changed, order, f.n_pad / 2, evals));
   TCX_HIP(hipEventRecord(c->ev[3], st));
-  unsigned long long h_evals = 0;
-  TCX_HIP(hipMemcpyAsync(&h_evals, evals, sizeof(h_evals), hipMemcpyDeviceToHost, st));
-  TCX_HIP(hipMemcpyAsync(h_changed, f.changed, sizeof(int), hipMemcpyDeviceToHost, st));
+  void* pin;
+  TCX_TRY(pinned_area(c, 2 * sizeof(unsigned long long), &pin));
+  unsigned long long* p_evals = static_cast<unsigned long long*>(pin);
+  int* p_changed = reinterpret_cast<int*>(p_evals + 1);
+  TCX_HIP(hipMemcpyAsync(p_evals, evals, sizeof(*p_evals), hipMemcpyDeviceToHost, st));
+  TCX_HIP(hipMemcpyAsync(p_changed, f.changed, sizeof(int), hipMemcpyDeviceToHost, st));
   if (f.o.labels_per_iter && f.it < f.o.max_iter)
     TCX_HIP(hipMemcpyAsync(f.o.labels_per_iter + (size_t)f.it * f.n, f.label,
                            sizeof(int) * f.n, hipMemcpyDeviceToHost, st));
   TCX_HIP(hipStreamSynchronize(st));
+  *h_changed = *p_changed;
+  const unsigned long long h_evals = *p_evals;
   float ms = 0.f;
+  if (f.ref_timed) {
+    TCX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    f.ms_ref += ms;
+    f.ref_timed = false;
+  }
   TCX_HIP(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
   f.ms_asg += ms;
   f.ms_asg_last = ms;
@@ -878,10 +913,24 @@ int tcx_fit_step(tcx_ctx* c, int* changed) {
   return TCX_OK;
 }
 
+// Adds a finished refine's event time to ms_ref (waits for it).
+static int flush_refine_time(tcx_ctx* c) {
+  FitState& f = c->fit;
+  if (!f.ref_timed) return TCX_OK;
+  TCX_HIP(hipEventSynchronize(c->ev[1]));
+  float ms = 0.f;
+  TCX_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  f.ms_ref += ms;
+  f.ref_timed = false;
+  return TCX_OK;
+}
+
 int tcx_fit_query(tcx_ctx* c, tcx_fit_stats* stats) {
   if (!c || !stats) return fail(TCX_ERR_ARG, "fit_query: null argument");
   const FitState& f = c->fit;
   if (!f.active) return fail(TCX_ERR_ARG, "fit_query: no fit in progress");
+  TCX_TRY(set_dev(c));
+  TCX_TRY(flush_refine_time(c));
   stats->iterations = f.it;
   stats->converged = f.converged;
   stats->sum_class = f.sum_class;
@@ -907,6 +956,7 @@ int tcx_fit_end(tcx_ctx* c, int* h_labels, float* h_centroids, tcx_fit_stats* st
     TCX_HIP(hipMemcpyAsync(h_centroids, f.cent, sizeof(float) * f.k * f.m,
                            hipMemcpyDeviceToHost, st));
   TCX_HIP(hipStreamSynchronize(st));
+  TCX_TRY(flush_refine_time(c));
   if (stats) {
     stats->iterations = f.it;
     stats->converged = f.converged;
