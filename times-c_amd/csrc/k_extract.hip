@@ -149,6 +149,63 @@ __global__ __launch_bounds__(64) void k_group_place(const int* __restrict__ labe
   }
 }
 
+// The same placement with the kGroupChunk / 64 slices of a block handled by
+// one wave each at once: every wave counts its slice's labels (ballots over
+// the slice's distinct labels), the per-slice counts are scanned across
+// waves in LDS, then every wave places its rows.  LDS: 8 * k ints.
+constexpr int kPlaceWaves = kGroupChunk / kWave;
+constexpr int kPlaceParMaxK = 2048;  // 8 * k * 4 bytes <= 64 KB
+
+__global__ __launch_bounds__(kGroupChunk) void k_group_place_par(
+    const int* __restrict__ label, long n, int k, const int* __restrict__ blk_base,
+    const long* __restrict__ offsets, int* __restrict__ perm) {
+  extern __shared__ int cnt[];  // [kPlaceWaves][k]: counts, then bases
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
+  int* mine = cnt + w * k;
+  for (int j = lane; j < k; j += kWave) mine[j] = 0;
+  const long row = (long)blockIdx.x * kGroupChunk + threadIdx.x;
+  int lab = row < n ? label[row] : -1;
+  if (lab >= k) lab = -1;
+  const uint64_t below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  wave_lds_sync();
+  // 1. slice counts
+  bool todo = lab >= 0;
+  for (;;) {
+    const uint64_t pending = __ballot(todo);
+    if (pending == 0) break;
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const int j = __shfl(lab, leader);
+    const uint64_t mask = __ballot(lab == j);
+    if (lab == j) todo = false;
+    if (lane == leader) mine[j] = __popcll(mask);
+  }
+  __syncthreads();
+  // 2. exclusive scan over the slices, from the block's base
+  for (int j = threadIdx.x; j < k; j += kGroupChunk) {
+    int run = blk_base[(long)blockIdx.x * k + j];
+#pragma unroll
+    for (int v = 0; v < kPlaceWaves; ++v) {
+      const int c = cnt[v * k + j];
+      cnt[v * k + j] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  // 3. placement
+  todo = lab >= 0;
+  for (;;) {
+    const uint64_t pending = __ballot(todo);
+    if (pending == 0) break;
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const int j = __shfl(lab, leader);
+    const uint64_t mask = __ballot(lab == j);
+    if (lab == j) {
+      perm[offsets[j] + mine[j] + __popcll(mask & below)] = (int)row;
+      todo = false;
+    }
+  }
+}
+
 // ------------------------------------------------------------ align + pack
 __device__ __forceinline__ int cluster_of(long p, const long* offsets, int k) {
   int lo = 0, hi = k;  // offsets[lo] <= p < offsets[hi]
@@ -201,6 +258,7 @@ __global__ __launch_bounds__(256) void k_align_pack(
   __shared__ __align__(16) uint16_t t_hi[64][72], t_lo[64][72];  // 144-byte rows
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const long p0 = (long)blockIdx.x * 64;
+  if (p0 >= offsets[k]) return;  // past the padded end (grid sized for the maximum)
   // each column's row, shift and (for a cluster's first member) cluster,
   // resolved by the 64 lanes of wave 0 at once
   if (wid == 0) {
@@ -353,6 +411,7 @@ __global__ __launch_bounds__(256) void k_align_pack_lds(
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int ms = m + 4;
   const long p0 = (long)blockIdx.x * R;
+  if (p0 >= offsets[k]) return;  // past the padded end (grid sized for the maximum)
   int first = -1;  // lane q < R of wave 0: cluster whose lowest member is column q
   if (wid == 0 && lane < R) {
     const long p = p0 + lane;
@@ -1047,6 +1106,17 @@ __device__ void tridiag_power_wave(const double* alpha, const double* beta, int 
   const double bu0 = (e0 + 1 < len) ? beta[e0 + 1] * sc : 0.0;        // T[e0][e0+1]
   const double bu1 = (e1 + 1 < len) ? beta[e1 + 1] * sc : 0.0;        // T[e1][e1+1]
   double y0 = (e0 == 0) ? 1.0 : 0.0, y1 = 0.0;
+  if (len <= 64) {
+    // one entry per lane: neighbours by DPP lane shifts only
+    for (int s = 0; s < I; ++s) {
+      const double p0 = wave_shr1_f64(y0, 0.0), n0 = wave_shl1_f64(y0, 0.0);
+      y0 = bl0 * p0 + a0 * y0 + bu0 * n0;
+      if ((s & 31) == 31 || s == I - 1) {
+        const double mx = wave_max_f64(fabs(y0));
+        if (mx > 0.0) y0 *= 1.0 / mx;
+      }
+    }
+  } else {
   for (int s = 0; s < I; ++s) {
     // neighbours through DPP lane shifts: entries l-1, l+1 and l+63, l+65
     const double y63 = readlane_f64(y0, 63), y64 = readlane_f64(y1, 0);
@@ -1064,6 +1134,7 @@ __device__ void tridiag_power_wave(const double* alpha, const double* beta, int 
         y1 *= inv;
       }
     }
+  }
   }
   double nn = y0 * y0 + y1 * y1;
   nn = wave_sum_f64(nn);
@@ -1637,8 +1708,12 @@ hipError_t launch_group_scan(int nblk, int k, const int* blk_count, int* blk_bas
 hipError_t launch_group_place(const int* label, long n, int k, const int* blk_base,
                               const long* offsets, int* perm, hipStream_t st) {
   long nblk = (n + kGroupChunk - 1) / kGroupChunk;
-  hipLaunchKernelGGL(k_group_place, dim3((unsigned)nblk), dim3(64), sizeof(int) * k, st, label,
-                     n, k, blk_base, offsets, perm);
+  if (k <= kPlaceParMaxK)
+    hipLaunchKernelGGL(k_group_place_par, dim3((unsigned)nblk), dim3(kGroupChunk),
+                       sizeof(int) * kPlaceWaves * k, st, label, n, k, blk_base, offsets, perm);
+  else
+    hipLaunchKernelGGL(k_group_place, dim3((unsigned)nblk), dim3(64), sizeof(int) * k, st,
+                       label, n, k, blk_base, offsets, perm);
   return hipGetLastError();
 }
 

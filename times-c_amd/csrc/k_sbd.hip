@@ -51,7 +51,8 @@ __global__ __launch_bounds__(256) void k_znorm(float* __restrict__ x, long rows,
 // ------------------------------------------------------------- spectra
 // Forward FFT of each row (zero padded to F, imaginary part 0), stored
 // lane-strided, plus the row's L2 norm, an all-zero flag and (mag != NULL)
-// the spectrum's magnitudes in the same layout.
+// the spectrum's magnitudes lane-major: |X[lane + L*s]| at lane*P + s, so a
+// lane reads its values as 16-byte vectors.
 template <int F>
 __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_spectra(
     const float* __restrict__ y, int rows, int m, const float2* __restrict__ twg,
@@ -86,9 +87,9 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_spectra(
 #pragma unroll
     for (int s = 0; s < P; ++s) o[s * L + lane] = v[s];
     if (mag) {
-      float* mo = mag + (long)row * F;
+      float* mo = mag + (long)row * F + lane * P;
 #pragma unroll
-      for (int s = 0; s < P; ++s) mo[s * L + lane] = sqrtf(v[s].x * v[s].x + v[s].y * v[s].y);
+      for (int s = 0; s < P; ++s) mo[s] = sqrtf(v[s].x * v[s].x + v[s].y * v[s].y);
     }
     if (lane == 0) {
       norms[row] = sqrtf(ss);
@@ -210,6 +211,23 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_sbd(
   }
 }
 
+// The first N (<= P) magnitudes of a lane from its lane-major row segment
+// (16-byte aligned, P a multiple of 4): N/4 vector loads and a scalar tail.
+template <int P, int N>
+__device__ __forceinline__ void load_mag(const float* __restrict__ src, float (&out)[N]) {
+  static_assert(P % 4 == 0 && N <= P, "lane-major magnitudes");
+#pragma unroll
+  for (int i = 0; i + 4 <= N; i += 4) {
+    const float4 q = *reinterpret_cast<const float4*>(src + i);
+    out[i] = q.x;
+    out[i + 1] = q.y;
+    out[i + 2] = q.z;
+    out[i + 3] = q.w;
+  }
+#pragma unroll
+  for (int i = N / 4 * 4; i < N; ++i) out[i] = src[i];
+}
+
 // ------------------------------------------- pruned assignment (exact)
 // For every centroid j the lag maximum of the correlation is bounded by the
 // spectra:  max_l cc[l] = max_l (1/F) |sum_f X_f conj(C_f) e^{2 pi i f l/F}|
@@ -228,6 +246,22 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_sbd(
 constexpr float kUbRel = 1e-4f;   // rounding margin of the bound
 constexpr float kUbAbs = 1e-5f;
 
+// a' + b' after v_permlane16_swap (O = 16: odd rows of a with even rows of
+// b) or v_permlane32_swap (O = 32: upper half of a with lower half of b).
+template <int O>
+__device__ __forceinline__ float swap_sum(float a, float b) {
+  static_assert(O == 16 || O == 32, "row or half swap");
+  if constexpr (O == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false,
+                                                    false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  } else {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false,
+                                                    false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  }
+}
+
 // Reduce-scatter of the CNT values v[0..CNT) of every lane over a group of
 // L lanes (one wave or less): at lane-bit O a lane keeps the lower or upper
 // half of its values (by that bit) plus the partner's matching half; once a
@@ -238,19 +272,40 @@ template <int CNT, int O, int L, int N>
 __device__ __forceinline__ void reduce_scatter(float (&v)[N], int lane, int& base) {
   if constexpr (O >= 1) {
     const bool up = (lane & O) != 0;
-    if constexpr (CNT > 1) {
-      constexpr int HF = CNT / 2;
+    if constexpr (O >= 16) {
+      // rows of 16 (O = 16) or halves of 32 (O = 32) swapped between two
+      // registers a, b: afterwards a lane without bit O holds its own a and
+      // its partner's a, a lane with bit O its partner's b and its own b, so
+      // the sum of the two is each lane's kept half plus the partner's
+      if constexpr (CNT > 1) {
+        constexpr int HF = CNT / 2;
 #pragma unroll
-      for (int i = 0; i < HF; ++i) {
-        const float send = up ? v[i] : v[i + HF];
-        const float keep = up ? v[i + HF] : v[i];
-        v[i] = keep + __shfl_xor(send, O, L);
+        for (int i = 0; i < HF; ++i) v[i] = swap_sum<O>(v[i], v[i + HF]);
+        if (up) base += HF;
+        reduce_scatter<HF, O / 2, L>(v, lane, base);
+      } else {
+        v[0] = swap_sum<O>(v[0], v[0]);
+        reduce_scatter<1, O / 2, L>(v, lane, base);
       }
-      if (up) base += HF;
-      reduce_scatter<HF, O / 2, L>(v, lane, base);
     } else {
-      v[0] += __shfl_xor(v[0], O, L);
-      reduce_scatter<1, O / 2, L>(v, lane, base);
+      // partner by DPP: lane ^ 1, lane ^ 2, the 8-lane mirror (bit 2 flips),
+      // the 16-lane mirror (bit 3 flips); each pairs the two halves of the
+      // current subgroup one to one
+      constexpr int CTRL = O == 1 ? 0xB1 : O == 2 ? 0x4E : O == 4 ? 0x141 : 0x140;
+      if constexpr (CNT > 1) {
+        constexpr int HF = CNT / 2;
+#pragma unroll
+        for (int i = 0; i < HF; ++i) {
+          const float send = up ? v[i] : v[i + HF];
+          const float keep = up ? v[i + HF] : v[i];
+          v[i] = keep + dpp_f32<CTRL>(send);
+        }
+        if (up) base += HF;
+        reduce_scatter<HF, O / 2, L>(v, lane, base);
+      } else {
+        v[0] += dpp_f32<CTRL>(v[0]);
+        reduce_scatter<1, O / 2, L>(v, lane, base);
+      }
     }
   }
 }
@@ -261,10 +316,13 @@ struct PruneGeom {
   static constexpr int L = FftGeom<F>::L;
   static constexpr int kBlock = FftGeom<F>::kBlock;
   static constexpr int kGroups = kBlock / L;
+  // waves per SIMD the register allocation must allow: three 256-thread
+  // blocks fit a CU's LDS up to F = 1024 (168 VGPRs per lane)
+  static constexpr int kMinWaves = F <= 1024 ? 3 : 1;
 };
 
 template <int F>
-__global__ __launch_bounds__(PruneGeom<F>::kBlock) void k_assign_prune(
+__global__ __launch_bounds__(PruneGeom<F>::kBlock, PruneGeom<F>::kMinWaves) void k_assign_prune(
     const float* __restrict__ x, long n, long npairs, int m, int k,
     const float2* __restrict__ cspec, const float* __restrict__ cmag,
     const float* __restrict__ cnorm, const float2* __restrict__ twg, int rev,
@@ -359,19 +417,19 @@ __global__ __launch_bounds__(PruneGeom<F>::kBlock) void k_assign_prune(
       // KC centroids at a time; the 2*KC per-lane partial sums are reduced
       // by a reduce-scatter over the L lanes (each level halves the values
       // a lane keeps)
-      constexpr int KC = 8;
+      constexpr int KC = 4;
       for (int j0 = 0; j0 < k; j0 += KC) {
         float v[2 * KC];
 #pragma unroll
         for (int c = 0; c < KC; ++c) {
           float aa = 0.f, ab = 0.f;
           if (j0 + c < k) {
-            const float* cm = cmag + (long)(j0 + c) * F;
+            float cv[H + 1];
+            load_mag<P, H + 1>(cmag + (long)(j0 + c) * F + lane * P, cv);
 #pragma unroll
             for (int s = 0; s <= H; ++s) {
-              const float cv = cm[s * L + lane];
-              aa += ma[s] * cv;
-              ab += mb[s] * cv;
+              aa += ma[s] * cv[s];
+              ab += mb[s] * cv[s];
             }
           }
           v[c] = aa;
@@ -391,13 +449,13 @@ __global__ __launch_bounds__(PruneGeom<F>::kBlock) void k_assign_prune(
       }
     } else {
       for (int j = 0; j < k; ++j) {
-        const float* cm = cmag + (long)j * F;
+        float cv[H + 1];
+        load_mag<P, H + 1>(cmag + (long)j * F + lane * P, cv);
         float aa = 0.f, ab = 0.f;
 #pragma unroll
         for (int s = 0; s <= H; ++s) {
-          const float cv = cm[s * L + lane];
-          aa += ma[s] * cv;
-          ab += mb[s] * cv;
+          aa += ma[s] * cv[s];
+          ab += mb[s] * cv[s];
         }
         aa = group_sum<L>(aa, red);
         ab = group_sum<L>(ab, red);

@@ -386,6 +386,23 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
   TCX_HIP(tcx::launch_fill_int(bad, 1, 0, st));
   TCX_HIP(tcx::launch_group_count(d_label, n, k, blk_count, bad, st));
   TCX_HIP(tcx::launch_group_scan((int)nblk, k, blk_count, blk_base, counts, offsets, st));
+  // Placement and align + pack are launched before the host reads the
+  // cluster sizes, on buffers sized for the largest padded count
+  // (n + k * (kPackPad - 1)); their blocks past the padded end offsets[k]
+  // exit at once.  The host then plans S = Y^T Y while they run.
+  const long n_pad_max = std::max<long>(
+      (n + (long)k * (tcx::kPackPad - 1)) / tcx::kPackPad * tcx::kPackPad, tcx::kPackPad);
+  TCX_TRY(ws(c, kSlotPerm, (size_t)n_pad_max, &perm));
+  TCX_HIP(tcx::launch_fill_int(perm, n_pad_max, -1, st));
+  TCX_HIP(tcx::launch_group_place(d_label, n, k, blk_base, offsets, perm, st));
+  uint16_t *yhi, *ylo;
+  float* a0;
+  const size_t panel = (size_t)tcx::panel_blocks(m) * 32 * (size_t)n_pad_max;
+  TCX_TRY(ws(c, kSlotYhi, panel, &yhi));
+  TCX_TRY(ws(c, kSlotYlo, panel, &ylo));
+  TCX_TRY(ws(c, kSlotA0, (size_t)k * m, &a0));
+  TCX_HIP(tcx::launch_align_pack(d_x, m, perm, n_pad_max, offsets, k, d_shift, d_czero,
+                                 row_stats, yhi, ylo, a0, st));
   void* pin;
   TCX_TRY(pinned_area(c, sizeof(long) * (k + 1) + sizeof(int) * (k + 2), &pin));
   long* p_off = static_cast<long*>(pin);
@@ -395,22 +412,11 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
   TCX_HIP(hipMemcpyAsync(p_off, offsets, sizeof(long) * (k + 1), hipMemcpyDeviceToHost, st));
   TCX_HIP(hipMemcpyAsync(p_bad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
   TCX_HIP(hipStreamSynchronize(st));
+  // out-of-range labels were left out of the grouping above
   if (*p_bad) return fail(TCX_ERR_ARG, "label outside [0, k)");
   const std::vector<int> h_counts(p_counts, p_counts + k);
   const std::vector<long> h_off(p_off, p_off + k + 1);
   const long n_pad = h_off[k];
-  TCX_TRY(ws(c, kSlotPerm, (size_t)(n_pad > 0 ? n_pad : 1), &perm));
-  TCX_HIP(tcx::launch_fill_int(perm, n_pad, -1, st));
-  TCX_HIP(tcx::launch_group_place(d_label, n, k, blk_base, offsets, perm, st));
-  uint16_t *yhi, *ylo;
-  float* a0;
-  const size_t panel = (size_t)tcx::panel_blocks(m) * 32 * (n_pad > 0 ? n_pad : 1);
-  TCX_TRY(ws(c, kSlotYhi, panel, &yhi));
-  TCX_TRY(ws(c, kSlotYlo, panel, &ylo));
-  TCX_TRY(ws(c, kSlotA0, (size_t)k * m, &a0));
-  TCX_HIP(tcx::launch_align_pack(d_x, m, perm, n_pad, offsets, k, d_shift, d_czero, row_stats,
-                                 yhi, ylo,
-                                 a0, st));
   int nonempty = 0;
   for (int j = 0; j < k; ++j) nonempty += h_counts[j] > 0;
   std::vector<long> seg_len(k);

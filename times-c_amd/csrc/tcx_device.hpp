@@ -317,11 +317,24 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[F / L], float2* buf,
 #pragma unroll
     for (int r = 0; r < P; ++r) u[r] = v[r];
   } else {
+    // element lane + L*t + r*(F/R): with F/R a multiple of 32 the offset
+    // r*(F/R) keeps the low five bits of lane + L*t
+    if constexpr ((F / R) % 32 == 0) {
 #pragma unroll
-    for (int t = 0; t < NB; ++t)
+      for (int t = 0; t < NB; ++t) {
+        const int pb = lds_pad(lane + L * t);
+        const float2* src = buf + pb;
+        constexpr int kStep = (F / R) + (F / R) / 32;
 #pragma unroll
-      for (int r = 0; r < R; ++r)
-        u[t * R + r] = buf[lds_pad(lane + L * t + r * (F / R))];
+        for (int r = 0; r < R; ++r) u[t * R + r] = src[r * kStep];
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < NB; ++t)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          u[t * R + r] = buf[lds_pad(lane + L * t + r * (F / R))];
+    }
     if constexpr (L > kWave) __syncthreads();  // reads done before writes
   }
 #pragma unroll
@@ -339,9 +352,14 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[F / L], float2* buf,
     }
     dft<R, INV>(b);
     if constexpr (!kLast) {
+      // base + r*Ns: (j-kk)*R is a multiple of Ns*R and kk < Ns, so
+      // (base & 31) + ((r*Ns) & 31) < 32 for every r < R (Ns, R powers of
+      // two): one padded base plus immediate offsets
       const int base = (j - kk) * R + kk;
+      float2* dst = buf + lds_pad(base);
+      constexpr int kStep = Ns;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[lds_pad(base + r * Ns)] = b[r];
+      for (int r = 0; r < R; ++r) dst[r * kStep + ((r * kStep) >> 5)] = b[r];
     } else {
 #pragma unroll
       for (int r = 0; r < R; ++r) v[t + r * NB] = b[r];
