@@ -1077,10 +1077,10 @@ __global__ __launch_bounds__(kPowerThreads) void k_power(
 // Lanczos basis, T_j = Q_j^T M Q_j tridiagonal.  Lanczos with full
 // reorthogonalization in FP64 (Q in global memory, steps below);
 // y = T_j^I e1 by I steps of the tridiagonal recurrence in one wave (FP64,
-// T scaled by its Gershgorin bound, renormalized every 32 steps).  From 16
-// steps on, every 8 steps y is compared with the previous check; the result
-// is accepted when |y_j - y_prev| <= lz_tol or the Krylov space became
-// invariant (breakdown).
+// T scaled by its Gershgorin bound, renormalized every 32 steps).  At the
+// lengths lz_power_check selects y is compared with the previous check; the
+// result is accepted when |y_j - y_prev| <= lz_tol or the Krylov space
+// became invariant (breakdown).
 // Clusters that do not converge within kLzMax steps get conv_out = 0 and the
 // caller runs the plain power iteration for them.
 constexpr int kLzMax = kLanczosMax;
@@ -1374,7 +1374,7 @@ __global__ __launch_bounds__(kPowerThreads) void k_power_lanczos(
     if (threadIdx.x == 0) alpha[it] = qw;
     len = it + 1;
     const bool breakdown = !(b > 1e-12 * sqrt(nw2));
-    if (breakdown || len == jmax || (len >= 16 && (len & 7) == 0)) {
+    if (breakdown || len == jmax || lz_power_check(len)) {
       __syncthreads();
       if (wid == 0) {
         tridiag_power_wave(alpha, beta, len, I, y);
@@ -1606,7 +1606,7 @@ __global__ __launch_bounds__(kPowerThreads) void k_lz_step(
   const int jmax = m < kLzMax ? m : kLzMax;
   const bool breakdown = !(b > 1e-12 * sqrt(nw2));
   int finished = breakdown || len == jmax;
-  const bool check = mode == kLzPower ? (len >= 16 && (len & 7) == 0) : (len >= 4 && (len & 3) == 0);
+  const bool check = mode == kLzPower ? lz_power_check(len) : (len >= 4 && (len & 3) == 0);
   if (breakdown || len == jmax || check) {
     __syncthreads();  // alpha[it] visible to wave 0
     if (wid == 0) {

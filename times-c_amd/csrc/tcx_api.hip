@@ -322,8 +322,8 @@ int power_lanczos(tcx_ctx* c, const float* S, int m, int k, const int* counts, u
   TCX_TRY(ws(c, kSlotLzState, (size_t)k, &state));
   if (mode == 0 && !mask && m <= tcx::kFusedLanczosMaxM) {
     // S_j fits in L2: one block per cluster runs every step in one launch
-    TCX_HIP(tcx::launch_power_lanczos(S, m, k, counts, seed, iter, steps, 1e-9, Q, v, r, st_out,
-                                      cv_out, st));
+    TCX_HIP(tcx::launch_power_lanczos(S, m, k, counts, seed, iter, steps, tcx::kLzPowerTol, Q, v,
+                                      r, st_out, cv_out, st));
     TCX_HIP(tcx::launch_cluster_matvec(S, m, k, v, m, sv, nullptr, st));  // S v
     *vout = v;
     *svout = sv;
@@ -339,10 +339,11 @@ int power_lanczos(tcx_ctx* c, const float* S, int m, int k, const int* counts, u
   std::vector<int> done(k);
   for (int it = 0; it < jmax; ++it) {
     TCX_HIP(tcx::launch_cluster_matvec(S, m, k, Q + (long)it * m, qstride, w, state, st));
-    TCX_HIP(tcx::launch_lz_step(m, k, mode, steps, mode == 0 ? 1e-9 : 1e-12, r, w, Q, state,
+    TCX_HIP(tcx::launch_lz_step(m, k, mode, steps, mode == 0 ? tcx::kLzPowerTol : 1e-12, r, w, Q,
+                                state,
                                 st));
     const int len = it + 1;
-    const bool check = mode == 0 ? (len >= 16 && (len & 7) == 0) : (len >= 8 && (len & 7) == 0);
+    const bool check = mode == 0 ? tcx::lz_power_check(len) : (len >= 8 && (len & 7) == 0);
     if (len == jmax || check) {
       TCX_HIP(hipMemcpy2DAsync(done.data(), sizeof(int), &state[0].done, sizeof(tcx::LzState),
                                sizeof(int), k, hipMemcpyDeviceToHost, st));

@@ -142,6 +142,19 @@ struct LzState {
 };
 // Fused single-launch form (one block per cluster, for small m).
 constexpr int kFusedLanczosMaxM = 768;
+// Acceptance of the Lanczos evaluation of M^I v0: y = T_j^I e1 is formed at
+// j = kLzFirstCheck and every kLzCheckEvery steps after, and accepted when
+// it moved by <= kLzPowerTol since the previous check (for a geometric
+// convergence rate rho per step the accepted y is then within about
+// rho^4 / (1 - rho^4) * kLzPowerTol of the limit, far below the ~1e-7
+// relative error that the FP32 S itself leaves in the centroid).
+constexpr int kLzFirstCheck = 12;
+constexpr int kLzCheckEvery = 4;
+constexpr double kLzPowerTol = 1e-9;
+__host__ __device__ constexpr bool lz_power_check(int len) {
+  return len >= kLzFirstCheck && len % kLzCheckEvery == 0;
+}
+
 hipError_t launch_power_lanczos(const float* S, int m, int k, const int* counts, uint64_t seed,
                                 int iter, int I, double lz_tol, double* Q, double* vout,
                                 double* rout, int* steps_out, int* conv_out, hipStream_t st);
