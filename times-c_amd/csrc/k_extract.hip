@@ -33,30 +33,62 @@ __global__ void k_fill_int(int* p, long count, int value) {
 }
 
 // -------------------------------------------------------------- grouping
-// Per block of kGroupChunk rows: label histogram.
+// Rows are grouped by the key label * B + bin, bin = the row's previous
+// SBD distance (in [0, 2]) in B equal bins (B = 1 without distances).
+// Clusters stay contiguous and ordered; inside a cluster rows run by bin,
+// then by index, so the assignment pairs series of similar distance.  The
+// grouping is stable and deterministic; the cluster's lowest row index
+// (the reference's first member, for the sign fix) is tracked separately.
+__device__ __forceinline__ int group_key(int l, long row, const float* dist, int B) {
+  if (B == 1) return l;
+  const float d = dist[row];
+  int b = 0;
+  if (d == d) {  // not NaN
+    const float f = d * (0.5f * (float)B);
+    b = f >= (float)(B - 1) ? B - 1 : (f > 0.f ? (int)f : 0);
+  }
+  return l * B + b;
+}
+
+// Per block of kGroupChunk rows: histogram of the K = k * B keys and the
+// lowest row of each cluster (atomicMin into minrow, INT_MAX when empty).
 __global__ __launch_bounds__(256) void k_group_count(const int* __restrict__ label, long n,
-                                                     int k, int* __restrict__ blk_count,
+                                                     int k, const float* __restrict__ dist, int B,
+                                                     int* __restrict__ blk_count,
+                                                     int* __restrict__ minrow, int lds_min,
                                                      int* __restrict__ bad) {
-  extern __shared__ int cnt[];
-  for (int j = threadIdx.x; j < k; j += blockDim.x) cnt[j] = 0;
+  extern __shared__ int cnt[];  // K counts, then (lds_min) k minima
+  const int K = k * B;
+  int* mn = cnt + K;
+  for (int j = threadIdx.x; j < K; j += blockDim.x) cnt[j] = 0;
+  if (lds_min)
+    for (int j = threadIdx.x; j < k; j += blockDim.x) mn[j] = 0x7fffffff;
   __syncthreads();
   const long b0 = (long)blockIdx.x * kGroupChunk;
   const long b1 = b0 + kGroupChunk < n ? b0 + kGroupChunk : n;
   for (long i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
     const int l = label[i];
-    if (l >= 0 && l < k)
-      atomicAdd(&cnt[l], 1);
-    else
+    if (l >= 0 && l < k) {
+      atomicAdd(&cnt[group_key(l, i, dist, B)], 1);
+      if (lds_min)
+        atomicMin(&mn[l], (int)i);
+      else
+        atomicMin(&minrow[l], (int)i);
+    } else {
       *bad = 1;  // out-of-range label: the row is left out, the host reports it
+    }
   }
   __syncthreads();
-  for (int j = threadIdx.x; j < k; j += blockDim.x) blk_count[(long)blockIdx.x * k + j] = cnt[j];
+  for (int j = threadIdx.x; j < K; j += blockDim.x) blk_count[(long)blockIdx.x * K + j] = cnt[j];
+  if (lds_min)
+    for (int j = threadIdx.x; j < k; j += blockDim.x)
+      if (mn[j] != 0x7fffffff) atomicMin(&minrow[j], mn[j]);
 }
 
-// Exclusive scan over blocks for one cluster per block (blockIdx.x = j):
-// blk_base[b][j] = sum of blk_count[b'][j] for b' < b; counts[j] = total.
-// Each of the 256 threads scans a contiguous run of blocks, then the run
-// totals are scanned in LDS.
+// Exclusive scan over blocks for one key per block (blockIdx.x = key):
+// blk_base[b][key] = sum of blk_count[b'][key] for b' < b; counts[key] =
+// total.  Each of the 256 threads scans a contiguous run of blocks, then the
+// run totals are scanned in LDS.
 __global__ __launch_bounds__(256) void k_group_scan(int nblk, int k,
                                                     const int* __restrict__ blk_count,
                                                     int* __restrict__ blk_base,
@@ -83,13 +115,31 @@ __global__ __launch_bounds__(256) void k_group_scan(int nblk, int k,
   if (t == 255) counts[j] = tot[255];
 }
 
-// Padded segment offsets: each cluster's columns start at a multiple of
-// kPackPad.  One block; the k cluster sizes are scanned 1024 at a time.
-__global__ __launch_bounds__(1024) void k_group_offsets(int k, const int* __restrict__ counts,
-                                                        long* __restrict__ offsets) {
+// One block of 1024 threads: cluster sizes from the key counts, padded
+// cluster offsets (each cluster's columns start at a multiple of kPackPad;
+// the k sizes are scanned 1024 at a time) and the offset of every key
+// inside its cluster.  Phase 1 runs one wave per cluster (lane b holds bin
+// b, B <= 64) and keeps the in-cluster exclusive prefix in koff; phase 3
+// adds the cluster offsets.
+__global__ __launch_bounds__(1024) void k_group_offsets(int k, int B,
+                                                        const int* __restrict__ kcounts,
+                                                        int* __restrict__ counts,
+                                                        long* __restrict__ offsets,
+                                                        long* __restrict__ koff) {
   __shared__ long tot[1024];
   __shared__ long carry;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, lane = t & (kWave - 1), w = t >> 6;
+  for (int j = w; j < k; j += 1024 / kWave) {
+    const int c = lane < B ? kcounts[j * B + lane] : 0;
+    int inc = c;  // inclusive scan over the lanes
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const int u = __shfl_up(inc, o);
+      if (lane >= o) inc += u;
+    }
+    if (lane < B) koff[j * B + lane] = inc - c;
+    if (lane == kWave - 1) counts[j] = inc;
+  }
   if (t == 0) {
     carry = 0;
     offsets[0] = 0;
@@ -97,7 +147,8 @@ __global__ __launch_bounds__(1024) void k_group_offsets(int k, const int* __rest
   __syncthreads();
   for (int j0 = 0; j0 < k; j0 += 1024) {
     const int j = j0 + t;
-    tot[t] = j < k ? ((long)counts[j] + kPackPad - 1) / kPackPad * kPackPad : 0;
+    const long c = j < k ? counts[j] : 0;
+    tot[t] = (c + kPackPad - 1) / kPackPad * kPackPad;
     __syncthreads();
     for (int o = 1; o < 1024; o <<= 1) {
       const long v = t >= o ? tot[t - o] : 0;
@@ -110,14 +161,16 @@ __global__ __launch_bounds__(1024) void k_group_offsets(int k, const int* __rest
     if (t == 0) carry += tot[1023];
     __syncthreads();
   }
+  for (int e = t; e < k * B; e += 1024) koff[e] += offsets[e / B];
 }
 
-// Stable placement: perm[offsets[l] + rank] = row, rank = number of earlier
-// rows with the same label.  One wave per block of rows; ranks inside a
-// 64-row slice come from ballots over the distinct labels of the slice.
+// Stable placement: perm[koff[key] + rank] = row, rank = number of earlier
+// rows with the same key.  One wave per block of rows; ranks inside a
+// 64-row slice come from ballots over the distinct keys of the slice.
+// (The fallback for K > kPlaceParMaxK: B is 1 there, the key the label.)
 __global__ __launch_bounds__(64) void k_group_place(const int* __restrict__ label, long n,
                                                     int k, const int* __restrict__ blk_base,
-                                                    const long* __restrict__ offsets,
+                                                    const long* __restrict__ koff,
                                                     int* __restrict__ perm) {
   extern __shared__ int run[];
   const int lane = threadIdx.x;
@@ -139,7 +192,7 @@ __global__ __launch_bounds__(64) void k_group_place(const int* __restrict__ labe
       const uint64_t mask = __ballot(lab == j);
       if (lab == j) {
         const int rank = run[j] + __popcll(mask & below);
-        perm[offsets[j] + rank] = (int)row;
+        perm[koff[j] + rank] = (int)row;
         todo = false;
       }
       wave_lds_sync();
@@ -149,61 +202,54 @@ __global__ __launch_bounds__(64) void k_group_place(const int* __restrict__ labe
   }
 }
 
-// The same placement with the kGroupChunk / 64 slices of a block handled by
-// one wave each at once: every wave counts its slice's labels (ballots over
-// the slice's distinct labels), the per-slice counts are scanned across
-// waves in LDS, then every wave places its rows.  LDS: 8 * k ints.
+// The same placement for keys label * B + bin, with the kGroupChunk / 64
+// slices of a block handled by one wave each at once.  A lane's peers (the
+// lanes of its slice with the same key) come from one ballot per key bit;
+// the lowest peer writes the slice's count of that key, the per-slice
+// counts are scanned across waves in LDS, then every lane places its row
+// at its key's base plus the number of lower peers.  LDS: 8 * K ints.
 constexpr int kPlaceWaves = kGroupChunk / kWave;
-constexpr int kPlaceParMaxK = 2048;  // 8 * k * 4 bytes <= 64 KB
+constexpr int kPlaceParMaxK = 2048;  // 8 * K * 4 bytes <= 64 KB
 
 __global__ __launch_bounds__(kGroupChunk) void k_group_place_par(
-    const int* __restrict__ label, long n, int k, const int* __restrict__ blk_base,
-    const long* __restrict__ offsets, int* __restrict__ perm) {
-  extern __shared__ int cnt[];  // [kPlaceWaves][k]: counts, then bases
+    const int* __restrict__ label, long n, int k, const float* __restrict__ dist, int B,
+    const int* __restrict__ blk_base, const long* __restrict__ koff, int* __restrict__ perm) {
+  extern __shared__ int cnt[];  // [kPlaceWaves][K]: counts, then bases
+  const int K = k * B;
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
-  int* mine = cnt + w * k;
-  for (int j = lane; j < k; j += kWave) mine[j] = 0;
+  int* mine = cnt + w * K;
+  for (int j = lane; j < K; j += kWave) mine[j] = 0;
   const long row = (long)blockIdx.x * kGroupChunk + threadIdx.x;
-  int lab = row < n ? label[row] : -1;
-  if (lab >= k) lab = -1;
-  const uint64_t below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  wave_lds_sync();
-  // 1. slice counts
-  bool todo = lab >= 0;
-  for (;;) {
-    const uint64_t pending = __ballot(todo);
-    if (pending == 0) break;
-    const int leader = __ffsll((unsigned long long)pending) - 1;
-    const int j = __shfl(lab, leader);
-    const uint64_t mask = __ballot(lab == j);
-    if (lab == j) todo = false;
-    if (lane == leader) mine[j] = __popcll(mask);
+  int key = -1;
+  if (row < n) {
+    const int l = label[row];
+    if (l >= 0 && l < k) key = group_key(l, row, dist, B);
   }
+  const uint64_t below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  uint64_t peers = __ballot(key >= 0);
+  const int nbits = 32 - __clz(K - 1 > 0 ? K - 1 : 1);
+  for (int b = 0; b < nbits; ++b) {
+    const bool bit = (key >> b) & 1;
+    const uint64_t set = __ballot(bit);
+    peers &= bit ? set : ~set;
+  }
+  wave_lds_sync();
+  // 1. slice counts, written by the lowest lane of each key
+  if (key >= 0 && (peers & below) == 0) mine[key] = __popcll(peers);
   __syncthreads();
   // 2. exclusive scan over the slices, from the block's base
-  for (int j = threadIdx.x; j < k; j += kGroupChunk) {
-    int run = blk_base[(long)blockIdx.x * k + j];
+  for (int j = threadIdx.x; j < K; j += kGroupChunk) {
+    int run = blk_base[(long)blockIdx.x * K + j];
 #pragma unroll
     for (int v = 0; v < kPlaceWaves; ++v) {
-      const int c = cnt[v * k + j];
-      cnt[v * k + j] = run;
+      const int c = cnt[v * K + j];
+      cnt[v * K + j] = run;
       run += c;
     }
   }
   __syncthreads();
   // 3. placement
-  todo = lab >= 0;
-  for (;;) {
-    const uint64_t pending = __ballot(todo);
-    if (pending == 0) break;
-    const int leader = __ffsll((unsigned long long)pending) - 1;
-    const int j = __shfl(lab, leader);
-    const uint64_t mask = __ballot(lab == j);
-    if (lab == j) {
-      perm[offsets[j] + mine[j] + __popcll(mask & below)] = (int)row;
-      todo = false;
-    }
-  }
+  if (key >= 0) perm[koff[key] + mine[key] + __popcll(peers & below)] = (int)row;
 }
 
 // ------------------------------------------------------------ align + pack
@@ -250,7 +296,8 @@ __global__ __launch_bounds__(256) void k_row_stats(const float* __restrict__ x, 
 // once, to pack.
 __global__ __launch_bounds__(256) void k_align_pack(
     const float* __restrict__ x, int m, const int* __restrict__ perm, long n_pad,
-    const long* __restrict__ offsets, int k, const int* __restrict__ shift,
+    const long* __restrict__ offsets, const int* __restrict__ minrow, int k,
+    const int* __restrict__ shift,
     const int* __restrict__ czero, const double2* __restrict__ stats,
     uint16_t* __restrict__ yhi, uint16_t* __restrict__ ylo, float* __restrict__ a0) {
   __shared__ double s_mean[64], s_isd[64];
@@ -268,7 +315,7 @@ __global__ __launch_bounds__(256) void k_align_pack(
     if (row >= 0) {
       const int cl = cluster_of(p, offsets, k);
       sh = czero[cl] ? 0 : shift[row];
-      if (p == offsets[cl]) first = cl;
+      if (row == minrow[cl]) first = cl;
     }
     s_row[lane] = row;
     s_sh[lane] = sh;
@@ -402,7 +449,8 @@ __host__ __device__ constexpr int pack_lds_rows(int m) {
 template <int R>
 __global__ __launch_bounds__(256) void k_align_pack_lds(
     const float* __restrict__ x, int m, const int* __restrict__ perm,
-    const long* __restrict__ offsets, int k, const int* __restrict__ shift,
+    const long* __restrict__ offsets, const int* __restrict__ minrow, int k,
+    const int* __restrict__ shift,
     const int* __restrict__ czero, uint16_t* __restrict__ yhi, uint16_t* __restrict__ ylo,
     float* __restrict__ a0) {
   extern __shared__ __align__(16) float rows_s[];  // R rows of stride ms
@@ -420,7 +468,7 @@ __global__ __launch_bounds__(256) void k_align_pack_lds(
     if (row >= 0) {
       const int cl = cluster_of(p, offsets, k);
       sh = czero[cl] ? 0 : shift[row];
-      if (p == offsets[cl]) first = cl;
+      if (row == minrow[cl]) first = cl;
     }
     s_row[lane] = row;
     s_sh[lane] = sh;
@@ -1687,33 +1735,50 @@ hipError_t launch_fill_int(int* p, long count, int value, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_group_count(const int* label, long n, int k, int* blk_count, int* bad,
-                              hipStream_t st) {
+int group_bins(long n, int k, bool have_dist) {
+  if (!have_dist || k > kPlaceParMaxK) return 1;
+  const long nblk = (n + kGroupChunk - 1) / kGroupChunk;
+  int B = kGroupBins;
+  while (B > 1 && ((long)k * B > kPlaceParMaxK || nblk * k * B > kGroupMaxBlkKeys)) B >>= 1;
+  return B;
+}
+
+hipError_t launch_group_count(const int* label, long n, int k, const float* dist, int B,
+                              int* blk_count, int* minrow, int* bad, hipStream_t st) {
   long nblk = (n + kGroupChunk - 1) / kGroupChunk;
-  hipLaunchKernelGGL(k_group_count, dim3((unsigned)nblk), dim3(256), sizeof(int) * k, st,
-                     label, n, k, blk_count, bad);
+  // per-block cluster minima in LDS while counts and minima fit 64 KB
+  const size_t lds = sizeof(int) * ((size_t)k * B + k);
+  const int lds_min = lds <= 65536;
+  hipLaunchKernelGGL(k_group_count, dim3((unsigned)nblk), dim3(256),
+                     lds_min ? lds : sizeof(int) * (size_t)k * B, st, label, n, k, dist, B,
+                     blk_count, minrow, lds_min, bad);
   return hipGetLastError();
 }
 
-hipError_t launch_group_scan(int nblk, int k, const int* blk_count, int* blk_base, int* counts,
-                             long* offsets, hipStream_t st) {
-  hipLaunchKernelGGL(k_group_scan, dim3(k), dim3(256), 0, st, nblk, k, blk_count, blk_base,
-                     counts);
+hipError_t launch_group_scan(int nblk, int k, int B, const int* blk_count, int* blk_base,
+                             int* kcounts, int* counts, long* offsets, long* koff,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_group_scan, dim3(k * B), dim3(256), 0, st, nblk, k * B, blk_count,
+                     blk_base, kcounts);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_group_offsets, dim3(1), dim3(1024), 0, st, k, counts, offsets);
+  hipLaunchKernelGGL(k_group_offsets, dim3(1), dim3(1024), 0, st, k, B, kcounts, counts, offsets,
+                     koff);
   return hipGetLastError();
 }
 
-hipError_t launch_group_place(const int* label, long n, int k, const int* blk_base,
-                              const long* offsets, int* perm, hipStream_t st) {
+hipError_t launch_group_place(const int* label, long n, int k, const float* dist, int B,
+                              const int* blk_base, const long* koff, int* perm, hipStream_t st) {
   long nblk = (n + kGroupChunk - 1) / kGroupChunk;
-  if (k <= kPlaceParMaxK)
+  if ((long)k * B <= kPlaceParMaxK)
     hipLaunchKernelGGL(k_group_place_par, dim3((unsigned)nblk), dim3(kGroupChunk),
-                       sizeof(int) * kPlaceWaves * k, st, label, n, k, blk_base, offsets, perm);
-  else
+                       sizeof(int) * kPlaceWaves * k * B, st, label, n, k, dist, B, blk_base, koff,
+                       perm);
+  else if (B == 1)
     hipLaunchKernelGGL(k_group_place, dim3((unsigned)nblk), dim3(64), sizeof(int) * k, st,
-                       label, n, k, blk_base, offsets, perm);
+                       label, n, k, blk_base, koff, perm);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
@@ -1727,28 +1792,28 @@ hipError_t launch_row_stats(const float* x, long rows, int m, double2* stats, hi
 bool align_pack_uses_row_stats(int m) { return m > kPackLdsMaxM; }
 
 hipError_t launch_align_pack(const float* x, int m, const int* perm, long n_pad,
-                             const long* offsets, int k, const int* shift, const int* czero,
-                             const double2* stats, uint16_t* yhi, uint16_t* ylo, float* a0,
-                             hipStream_t st) {
+                             const long* offsets, const int* minrow, int k, const int* shift,
+                             const int* czero, const double2* stats, uint16_t* yhi,
+                             uint16_t* ylo, float* a0, hipStream_t st) {
   if (n_pad <= 0) return hipSuccess;
   if (m <= kPackLdsMaxM) {
     const int R = pack_lds_rows(m);
     const size_t lds = sizeof(float) * (size_t)R * (m + 4);
     const dim3 grid((unsigned)(n_pad / R));
     if (R == 64)
-      hipLaunchKernelGGL(k_align_pack_lds<64>, grid, dim3(256), lds, st, x, m, perm, offsets, k,
-                         shift, czero, yhi, ylo, a0);
+      hipLaunchKernelGGL(k_align_pack_lds<64>, grid, dim3(256), lds, st, x, m, perm, offsets,
+                         minrow, k, shift, czero, yhi, ylo, a0);
     else if (R == 32)
-      hipLaunchKernelGGL(k_align_pack_lds<32>, grid, dim3(256), lds, st, x, m, perm, offsets, k,
-                         shift, czero, yhi, ylo, a0);
+      hipLaunchKernelGGL(k_align_pack_lds<32>, grid, dim3(256), lds, st, x, m, perm, offsets,
+                         minrow, k, shift, czero, yhi, ylo, a0);
     else
-      hipLaunchKernelGGL(k_align_pack_lds<16>, grid, dim3(256), lds, st, x, m, perm, offsets, k,
-                         shift, czero, yhi, ylo, a0);
+      hipLaunchKernelGGL(k_align_pack_lds<16>, grid, dim3(256), lds, st, x, m, perm, offsets,
+                         minrow, k, shift, czero, yhi, ylo, a0);
     return hipGetLastError();
   }
   long blocks = n_pad / 64;
   hipLaunchKernelGGL(k_align_pack, dim3((unsigned)blocks), dim3(256), 0, st, x, m, perm, n_pad,
-                     offsets, k, shift, czero, stats, yhi, ylo, a0);
+                     offsets, minrow, k, shift, czero, stats, yhi, ylo, a0);
   return hipGetLastError();
 }
 

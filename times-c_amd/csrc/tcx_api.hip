@@ -51,7 +51,7 @@ enum Slot {
   kSlotPerm, kSlotBad, kSlotYhi, kSlotYlo, kSlotA0, kSlotItems, kSlotTiles, kSlotPart,
   kSlotS, kSlotInfo, kSlotDummy, kSlotWbuf, kSlotRbuf, kSlotBar, kSlotVout, kSlotSteps,
   kSlotConv, kSlotErr, kSlotCmag, kSlotEvals, kSlotSvout, kSlotLzW, kSlotLzState, kSlotMask,
-  kSlotRowStats, kSlotFitRowStats, kNumSlots
+  kSlotRowStats, kSlotFitRowStats, kSlotKeyCounts, kSlotKeyOff, kSlotMinRow, kNumSlots
 };
 
 struct FitState {
@@ -68,6 +68,7 @@ struct FitState {
   long long evals = 0, pairs = 0;
   float ms_ref = 0.f, ms_asg = 0.f, ms_asg_last = 0.f;
   bool ref_timed = false;  // events 0..1 hold a refine not yet added to ms_ref
+  bool have_dist = false;  // dist holds an assignment's distances (orders the next grouping)
   int *label = nullptr, *shift = nullptr, *changed = nullptr, *zero = nullptr;
   float *dist = nullptr, *cent = nullptr, *norm = nullptr;
   float2* spec = nullptr;
@@ -367,7 +368,8 @@ int power_lanczos(tcx_ctx* c, const float* S, int m, int k, const int* counts, u
 // labels, shifts and old-centroid zero flags; writes k*m new centroids.
 int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_label,
            const int* d_shift, const int* d_czero, float* d_cout, int iter, uint64_t seed,
-           int split, double tol, int cal_select, const double2* row_stats, RefineResult* res) {
+           int split, double tol, int cal_select, const double2* row_stats, RefineResult* res,
+           const float* d_dist = nullptr) {
   hipStream_t st = c->stream;
   if (!row_stats && tcx::align_pack_uses_row_stats(m)) {
     double2* rs;
@@ -376,17 +378,28 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
     row_stats = rs;
   }
   c->fit.have_order = false;  // the row order is rewritten below
+  // rows grouped by (label, bin of the previous distance): clusters stay
+  // contiguous, and inside a cluster series of similar distance are
+  // adjacent, so the next assignment pairs them (fewer evaluated centroids
+  // per pair than pairs by row index)
+  const int B = tcx::group_bins(n, k, d_dist != nullptr);
+  const long K = (long)k * B;
   long nblk = (n + tcx::kGroupChunk - 1) / tcx::kGroupChunk;
-  int *blk_count, *blk_base, *counts, *perm, *bad;
-  long* offsets;
-  TCX_TRY(ws(c, kSlotBlkCount, (size_t)nblk * k, &blk_count));
-  TCX_TRY(ws(c, kSlotBlkBase, (size_t)nblk * k, &blk_base));
+  int *blk_count, *blk_base, *counts, *kcounts, *minrow, *perm, *bad;
+  long *offsets, *koff;
+  TCX_TRY(ws(c, kSlotBlkCount, (size_t)(nblk * K), &blk_count));
+  TCX_TRY(ws(c, kSlotBlkBase, (size_t)(nblk * K), &blk_base));
   TCX_TRY(ws(c, kSlotCounts, (size_t)k, &counts));
+  TCX_TRY(ws(c, kSlotKeyCounts, (size_t)K, &kcounts));
   TCX_TRY(ws(c, kSlotOffsets, (size_t)k + 1, &offsets));
+  TCX_TRY(ws(c, kSlotKeyOff, (size_t)K, &koff));
+  TCX_TRY(ws(c, kSlotMinRow, (size_t)k, &minrow));
   TCX_TRY(ws(c, kSlotBad, 1, &bad));
   TCX_HIP(tcx::launch_fill_int(bad, 1, 0, st));
-  TCX_HIP(tcx::launch_group_count(d_label, n, k, blk_count, bad, st));
-  TCX_HIP(tcx::launch_group_scan((int)nblk, k, blk_count, blk_base, counts, offsets, st));
+  TCX_HIP(tcx::launch_fill_int(minrow, k, 0x7fffffff, st));
+  TCX_HIP(tcx::launch_group_count(d_label, n, k, d_dist, B, blk_count, minrow, bad, st));
+  TCX_HIP(tcx::launch_group_scan((int)nblk, k, B, blk_count, blk_base, kcounts, counts, offsets,
+                                 koff, st));
   // Placement and align + pack are launched before the host reads the
   // cluster sizes, on buffers sized for the largest padded count
   // (n + k * (kPackPad - 1)); their blocks past the padded end offsets[k]
@@ -395,14 +408,14 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
       (n + (long)k * (tcx::kPackPad - 1)) / tcx::kPackPad * tcx::kPackPad, tcx::kPackPad);
   TCX_TRY(ws(c, kSlotPerm, (size_t)n_pad_max, &perm));
   TCX_HIP(tcx::launch_fill_int(perm, n_pad_max, -1, st));
-  TCX_HIP(tcx::launch_group_place(d_label, n, k, blk_base, offsets, perm, st));
+  TCX_HIP(tcx::launch_group_place(d_label, n, k, d_dist, B, blk_base, koff, perm, st));
   uint16_t *yhi, *ylo;
   float* a0;
   const size_t panel = (size_t)tcx::panel_blocks(m) * 32 * (size_t)n_pad_max;
   TCX_TRY(ws(c, kSlotYhi, panel, &yhi));
   TCX_TRY(ws(c, kSlotYlo, panel, &ylo));
   TCX_TRY(ws(c, kSlotA0, (size_t)k * m, &a0));
-  TCX_HIP(tcx::launch_align_pack(d_x, m, perm, n_pad_max, offsets, k, d_shift, d_czero,
+  TCX_HIP(tcx::launch_align_pack(d_x, m, perm, n_pad_max, offsets, minrow, k, d_shift, d_czero,
                                  row_stats, yhi, ylo, a0, st));
   void* pin;
   TCX_TRY(pinned_area(c, sizeof(long) * (k + 1) + sizeof(int) * (k + 2), &pin));
@@ -779,6 +792,7 @@ int tcx_fit_begin(tcx_ctx* c, const float* d_x, int n, int m, int k, const tcx_f
   TCX_TRY(ws(c, kSlotLabel, (size_t)n, &f.label));
   TCX_TRY(ws(c, kSlotShift, (size_t)n, &f.shift));
   TCX_TRY(ws(c, kSlotDist, (size_t)n, &f.dist));
+  f.have_dist = false;
   TCX_TRY(ws(c, kSlotCent, (size_t)k * m, &f.cent));
   TCX_TRY(ws(c, kSlotChanged, 1, &f.changed));
   std::vector<int> init;
@@ -845,7 +859,8 @@ int fit_refine(tcx_ctx* c, FitState& f) {
   TCX_HIP(hipEventRecord(c->ev[0], st));
   RefineResult rr;
   TCX_TRY(refine(c, f.d_x, f.n, f.m, f.k, f.label, f.shift, f.zero, cnew, f.it, f.o.seed,
-                 f.split, f.tol, f.o.cal_select, f.row_stats, &rr));
+                 f.split, f.tol, f.o.cal_select, f.row_stats, &rr,
+                 f.have_dist ? f.dist : nullptr));
   TCX_HIP(hipMemcpyAsync(f.cent, cnew, sizeof(float) * f.k * f.m, hipMemcpyDeviceToDevice, st));
   TCX_TRY(spectra(c, f.cent, f.k, f.m, &f.spec, &f.norm, &f.zero));
   TCX_HIP(hipEventRecord(c->ev[1], st));
@@ -868,6 +883,7 @@ int fit_assign(tcx_ctx* c, FitState& f, int* h_changed) {
   const int* order = f.have_order ? reinterpret_cast<const int*>(c->slot_ptr[kSlotPerm]) : nullptr;
   TCX_TRY(assign_core(c, f.d_x, f.n, f.m, f.k, f.spec, f.norm, f.o.convention, f.label, f.dist,
                       f.shift, nullptr, f.changed, order, f.n_pad / 2, evals));
+  f.have_dist = true;
   TCX_HIP(hipEventRecord(c->ev[3], st));
   void* pin;
   TCX_TRY(pinned_area(c, 2 * sizeof(unsigned long long), &pin));

@@ -101,21 +101,27 @@ struct PowerInfo {
 };
 
 hipError_t launch_fill_int(int* p, long count, int value, hipStream_t st);
-hipError_t launch_group_count(const int* label, long n, int k, int* blk_count, int* bad,
-                              hipStream_t st);
-hipError_t launch_group_scan(int nblk, int k, const int* blk_count, int* blk_base,
-                             int* counts, long* offsets, hipStream_t st);
-hipError_t launch_group_place(const int* label, long n, int k, const int* blk_base,
-                              const long* offsets, int* perm, hipStream_t st);
+// Grouping by the key label * B + bin(dist) (see k_group_count): bins per
+// cluster for n rows and k clusters, 1 without distances or for large k.
+constexpr int kGroupBins = 64;
+constexpr long kGroupMaxBlkKeys = 32L << 20;  // per-block key counts kept (ints)
+int group_bins(long n, int k, bool have_dist);
+hipError_t launch_group_count(const int* label, long n, int k, const float* dist, int B,
+                              int* blk_count, int* minrow, int* bad, hipStream_t st);
+hipError_t launch_group_scan(int nblk, int k, int B, const int* blk_count, int* blk_base,
+                             int* kcounts, int* counts, long* offsets, long* koff,
+                             hipStream_t st);
+hipError_t launch_group_place(const int* label, long n, int k, const float* dist, int B,
+                              const int* blk_base, const long* koff, int* perm, hipStream_t st);
 // FP64 (sum, sum of squares) of every row.
 hipError_t launch_row_stats(const float* x, long rows, int m, double2* stats, hipStream_t st);
 // True when launch_align_pack reads per-row sums (launch_row_stats) for
 // this m; otherwise `stats` may be NULL.
 bool align_pack_uses_row_stats(int m);
 hipError_t launch_align_pack(const float* x, int m, const int* perm, long n_pad,
-                             const long* offsets, int k, const int* shift, const int* czero,
-                             const double2* stats, uint16_t* yhi, uint16_t* ylo, float* a0,
-                             hipStream_t st);
+                             const long* offsets, const int* minrow, int k, const int* shift,
+                             const int* czero, const double2* stats, uint16_t* yhi,
+                             uint16_t* ylo, float* a0, hipStream_t st);
 hipError_t launch_syrk(const SyrkItem* items, int nitems, const uint16_t* yhi,
                        const uint16_t* ylo, int m, int split, float* part, hipStream_t st);
 hipError_t launch_syrk_reduce(const SyrkTile* tiles, int ntiles, const float* part, int m,
