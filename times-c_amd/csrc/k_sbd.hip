@@ -310,6 +310,99 @@ __device__ __forceinline__ void reduce_scatter(float (&v)[N], int lane, int& bas
   }
 }
 
+// Partner values of the packed spectrum Z = FFT(x_a + i x_b) held
+// lane-strided (Z[lane + L*s] in zx[s]): for s <= P/2, (px[s], py[s]) =
+// Z[F - f] with f = lane + L*s, from lane' = (L - lane) % L, s' = P-1-s
+// (lane 0: s' = (P - s) % P).  Then X_a[f] = (Z[f] + conj(Z[F-f])) / 2 and
+// X_b[f] = (Z[f] - conj(Z[F-f])) / (2i).  Groups wider than a wave exchange
+// through `buf` (LDS) between barriers.
+template <int F, int L, int P>
+__device__ __forceinline__ void partner_half(const float2 (&zx)[P], float2* buf, int lane,
+                                             float (&px)[P / 2 + 1], float (&py)[P / 2 + 1]) {
+  constexpr int H = P / 2;
+  const int pl = (L - lane) & (L - 1);
+  if constexpr (L > kWave) {
+#pragma unroll
+    for (int s = 0; s < P; ++s) buf[lds_pad(lane + L * s)] = zx[s];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int s = 0; s <= H; ++s) {
+    if constexpr (L > kWave) {
+      const float2 q = buf[lds_pad(pl + L * (P - 1 - s))];
+      px[s] = q.x;
+      py[s] = q.y;
+    } else {
+      px[s] = __shfl(zx[P - 1 - s].x, pl, L);
+      py[s] = __shfl(zx[P - 1 - s].y, pl, L);
+    }
+    if (lane == 0) {
+      px[s] = zx[(P - s) & (P - 1)].x;
+      py[s] = zx[(P - s) & (P - 1)].y;
+    }
+  }
+  if constexpr (L > kWave) __syncthreads();  // reads done before buf is reused
+}
+
+// Half spectra of every series, once per fit (the series do not change
+// between iterations): row r holds X_r[f] for f = 0 .. F/2 (F/2 + 1 values,
+// X_r[F-f] = conj(X_r[f])), and xn[r] = |x_r|.  Rows are transformed in
+// natural pairs packed as x_a + i x_b and split with partner_half.
+template <int F>
+__global__ __launch_bounds__(FftGeom<F>::kBlock) void k_series_spec(
+    const float* __restrict__ x, long n, int m, const float2* __restrict__ twg,
+    float2* __restrict__ xs, float* __restrict__ xn) {
+  using G = FftGeom<F>;
+  constexpr int L = G::L, P = G::P, H = P / 2;
+  constexpr long HS = F / 2 + 1;
+  extern __shared__ float2 smem[];
+  float2* tw = smem;
+  float* red = reinterpret_cast<float*>(smem + F + G::kGroups * fft_buf_len<F>());
+  for (int i = threadIdx.x; i < F; i += blockDim.x) tw[i] = twg[i];
+  __syncthreads();
+  const int g = threadIdx.x / L, lane = threadIdx.x % L;
+  float2* buf = smem + F + g * fft_buf_len<F>();
+  const long npairs = (n + 1) / 2;
+  for (long pr = (long)blockIdx.x * G::kGroups + g; pr < npairs;
+       pr += (long)gridDim.x * G::kGroups) {
+    const long ia = 2 * pr, ib = 2 * pr + 1;
+    const bool hasb = ib < n;
+    const float* xa = x + ia * (long)m;
+    const float* xb = x + (hasb ? ib : ia) * (long)m;
+    float2 zx[P];
+    float sa = 0.f, sb = 0.f;
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+      const int t = lane + L * s;
+      const float a = t < m ? xa[t] : 0.f;
+      const float b = (hasb && t < m) ? xb[t] : 0.f;
+      zx[s] = make_float2(a, b);
+      sa += a * a;
+      sb += b * b;
+    }
+    sa = group_sum<L>(sa, red);
+    sb = group_sum<L>(sb, red);
+    if (lane == 0) {
+      xn[ia] = sqrtf(sa);
+      if (hasb) xn[ib] = sqrtf(sb);
+    }
+    fft<F, L, false>(zx, buf, tw, lane);
+    float px[H + 1], py[H + 1];
+    partner_half<F, L, P>(zx, buf, lane, px, py);
+    float2* oa = xs + ia * HS;
+    float2* ob = xs + ib * HS;
+#pragma unroll
+    for (int s = 0; s <= H; ++s) {
+      const int f = lane + L * s;
+      if (s < H || lane == 0) {  // f < F/2, or f = F/2
+        const int fo = s < H ? f : F / 2;
+        oa[fo] = make_float2(0.5f * (zx[s].x + px[s]), 0.5f * (zx[s].y - py[s]));
+        if (hasb) ob[fo] = make_float2(0.5f * (zx[s].y + py[s]), 0.5f * (px[s] - zx[s].x));
+      }
+    }
+  }
+}
+
 // Launch geometry of the pruned assignment (that of the dense kernel).
 template <int F>
 struct PruneGeom {
@@ -321,9 +414,10 @@ struct PruneGeom {
   static constexpr int kMinWaves = F <= 1024 ? 3 : 1;
 };
 
-template <int F>
+template <int F, bool kPre>
 __global__ __launch_bounds__(PruneGeom<F>::kBlock, PruneGeom<F>::kMinWaves) void k_assign_prune(
-    const float* __restrict__ x, long n, long npairs, int m, int k,
+    const float* __restrict__ x, const float2* __restrict__ xs, const float* __restrict__ xn,
+    long n, long npairs, int m, int k,
     const float2* __restrict__ cspec, const float* __restrict__ cmag,
     const float* __restrict__ cnorm, const float2* __restrict__ twg, int rev,
     const int* __restrict__ order, int* __restrict__ label, float* __restrict__ dist,
@@ -354,64 +448,79 @@ __global__ __launch_bounds__(PruneGeom<F>::kBlock, PruneGeom<F>::kMinWaves) void
     }
     if (ia < 0) continue;  // hole pair (uniform for the group)
     const bool hasb = ib >= 0;
-    const float* xa = x + ia * (long)m;
-    const float* xb = x + (hasb ? ib : ia) * (long)m;
-    float2 zx[P];
-    float sa = 0.f, sb = 0.f;
-#pragma unroll
-    for (int s = 0; s < P; ++s) {
-      const int t = lane + L * s;
-      const float a = t < m ? xa[t] : 0.f;
-      const float b = (hasb && t < m) ? xb[t] : 0.f;
-      zx[s] = make_float2(a, b);
-      sa += a * a;
-      sb += b * b;
-    }
-    sa = group_sum<L>(sa, red);
-    sb = group_sum<L>(sb, red);
-    const float na = sqrtf(sa), nb = sqrtf(sb);
-    fft<F, L, false>(zx, buf, tw, lane);
-    // magnitudes |X_a[f]|, |X_b[f]| of this lane's frequencies f = lane + L*s;
-    // partner F - f is (lane' = (L - lane) % L, s' = P-1-s) for lane != 0 and
-    // (lane 0, s' = (P - s) % P) for lane 0
-    // weighted magnitudes of the half spectrum f <= F/2 held by this lane,
-    // f = lane + L*s for s < P/2 and f = F/2 (lane 0, s = P/2):
-    // |X_a[f]| = |Z[f] + conj(Z[F-f])| / 2, |X_b[f]| = |Z[f] - conj(Z[F-f])| / 2,
-    // weight 2 for 0 < f < F/2 (f and F-f have equal magnitudes), 1 for f = 0
-    // and f = F/2.  The partner F-f sits at (lane' = (L - lane) % L,
-    // s' = P-1-s) for lane != 0 and at (lane 0, s' = (P - s) % P) for lane 0.
+    // packed spectrum zx[s] = Z[lane + L*s], Z = X_a + i X_b, and the weighted
+    // half-spectrum magnitudes ma, mb (f = lane + L*s for s < P/2, f = F/2
+    // at lane 0, s = P/2; weight 2 inside, 1 at f = 0 and f = F/2)
     constexpr int H = P / 2;
+    float2 zx[P];
     float ma[H + 1], mb[H + 1];
-    const int pl = (L - lane) & (L - 1);
-    if constexpr (L > kWave) {
-      // partner lanes sit in other waves: exchange through LDS (the
-      // transform's last pass ends with a barrier after its LDS reads)
+    float na, nb;
+    if constexpr (kPre) {
+      // from the per-fit half spectra: X[f] for f <= F/2, conj(X[F-f]) above
+      constexpr long HS = F / 2 + 1;
+      // lane bases: f = lane + L*s below F/2 and F - f above, so that every
+      // load is a base plus an immediate offset
+      const float2* pa = xs + ia * HS + lane;
+      const float2* pb = xs + (hasb ? ib : ia) * HS + lane;
+      const float2* qa = xs + ia * HS + (F - L * H) - lane;
+      const float2* qb = xs + (hasb ? ib : ia) * HS + (F - L * H) - lane;
+      na = xn[ia];
+      nb = hasb ? xn[ib] : 0.f;
+      // two halves, so that only half of the loads are in flight at once
+      // (register pressure)
+      auto assemble = [&](int s) {
+        float2 A, B;
+        if (s < H) {
+          A = pa[L * s];
+          B = hasb ? pb[L * s] : make_float2(0.f, 0.f);
+        } else {
+          const float2 a2 = qa[-L * (s - H)];
+          const float2 b2 = hasb ? qb[-L * (s - H)] : make_float2(0.f, 0.f);
+          A = make_float2(a2.x, -a2.y);
+          B = make_float2(b2.x, -b2.y);
+        }
+        zx[s] = make_float2(A.x - B.y, A.y + B.x);
+        if (s <= H) {
+          const float w = s == H ? (lane == 0 ? 1.f : 0.f) : ((s == 0 && lane == 0) ? 1.f : 2.f);
+          ma[s] = w * sqrtf(A.x * A.x + A.y * A.y);
+          mb[s] = w * sqrtf(B.x * B.x + B.y * B.y);
+        }
+      };
 #pragma unroll
-      for (int s = 0; s < P; ++s) buf[lds_pad(lane + L * s)] = zx[s];
-      __syncthreads();
-    }
+      for (int s = 0; s < H; ++s) assemble(s);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int s = 0; s <= H; ++s) {
-      float px, py;
-      if constexpr (L > kWave) {
-        const float2 q = buf[lds_pad(pl + L * (P - 1 - s))];
-        px = q.x;
-        py = q.y;
-      } else {
-        px = __shfl(zx[P - 1 - s].x, pl, L);
-        py = __shfl(zx[P - 1 - s].y, pl, L);
+      for (int s = H; s < P; ++s) assemble(s);
+    } else {
+      const float* xa = x + ia * (long)m;
+      const float* xb = x + (hasb ? ib : ia) * (long)m;
+      float sa = 0.f, sb = 0.f;
+#pragma unroll
+      for (int s = 0; s < P; ++s) {
+        const int t = lane + L * s;
+        const float a = t < m ? xa[t] : 0.f;
+        const float b = (hasb && t < m) ? xb[t] : 0.f;
+        zx[s] = make_float2(a, b);
+        sa += a * a;
+        sb += b * b;
       }
-      if (lane == 0) {
-        px = zx[(P - s) & (P - 1)].x;
-        py = zx[(P - s) & (P - 1)].y;
+      sa = group_sum<L>(sa, red);
+      sb = group_sum<L>(sb, red);
+      na = sqrtf(sa);
+      nb = sqrtf(sb);
+      fft<F, L, false>(zx, buf, tw, lane);
+      // |X_a[f]| = |Z[f] + conj(Z[F-f])| / 2, |X_b[f]| = |Z[f] - conj(Z[F-f])| / 2
+      float px[H + 1], py[H + 1];
+      partner_half<F, L, P>(zx, buf, lane, px, py);
+#pragma unroll
+      for (int s = 0; s <= H; ++s) {
+        const float ar = zx[s].x + px[s], ai = zx[s].y - py[s];
+        const float br = zx[s].x - px[s], bi = zx[s].y + py[s];
+        const float wgt = s == H ? (lane == 0 ? 0.5f : 0.f) : ((s == 0 && lane == 0) ? 0.5f : 1.f);
+        ma[s] = wgt * sqrtf(ar * ar + ai * ai);
+        mb[s] = wgt * sqrtf(br * br + bi * bi);
       }
-      const float ar = zx[s].x + px, ai = zx[s].y - py;
-      const float br = zx[s].x - px, bi = zx[s].y + py;
-      const float wgt = s == H ? (lane == 0 ? 0.5f : 0.f) : ((s == 0 && lane == 0) ? 0.5f : 1.f);
-      ma[s] = wgt * sqrtf(ar * ar + ai * ai);
-      mb[s] = wgt * sqrtf(br * br + bi * bi);
     }
-    if constexpr (L > kWave) __syncthreads();  // exchange reads done before buf is reused
     // bounds of every centroid: ub = sum_f |X_f| |C_f| / (F |x| |c|)
     if constexpr (L <= kWave) {
       // KC centroids at a time; the 2*KC per-lane partial sums are reduced
@@ -607,9 +716,28 @@ static hipError_t launch_prune_F(const PruneArgs& a, hipStream_t st) {
   long blocks = (a.npairs + G::kGroups - 1) / G::kGroups;
   if (blocks < 1) blocks = 1;
   if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL((k_assign_prune<F>), dim3((unsigned)blocks), dim3(G::kBlock),
-                     prune_smem<F>(a.k), st, a.x, a.n, a.npairs, a.m, a.k, a.cspec, a.cmag,
-                     a.cnorm, a.tw, a.rev, a.order, a.label, a.dist, a.shift, a.changed, a.evals);
+  if (a.xs)
+    hipLaunchKernelGGL((k_assign_prune<F, true>), dim3((unsigned)blocks), dim3(G::kBlock),
+                       prune_smem<F>(a.k), st, a.x, a.xs, a.xn, a.n, a.npairs, a.m, a.k, a.cspec,
+                       a.cmag, a.cnorm, a.tw, a.rev, a.order, a.label, a.dist, a.shift, a.changed,
+                       a.evals);
+  else
+    hipLaunchKernelGGL((k_assign_prune<F, false>), dim3((unsigned)blocks), dim3(G::kBlock),
+                       prune_smem<F>(a.k), st, a.x, a.xs, a.xn, a.n, a.npairs, a.m, a.k, a.cspec,
+                       a.cmag, a.cnorm, a.tw, a.rev, a.order, a.label, a.dist, a.shift, a.changed,
+                       a.evals);
+  return hipGetLastError();
+}
+
+template <int F>
+static hipError_t launch_series_spec_F(const float* x, long n, int m, const float2* tw,
+                                       float2* xs, float* xn, hipStream_t st) {
+  using G = FftGeom<F>;
+  long blocks = ((n + 1) / 2 + G::kGroups - 1) / G::kGroups;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL((k_series_spec<F>), dim3((unsigned)blocks), dim3(G::kBlock), sbd_smem<F>(),
+                     st, x, n, m, tw, xs, xn);
   return hipGetLastError();
 }
 
@@ -660,6 +788,20 @@ int prune_max_k(int F) {
   }
 }
 
+hipError_t launch_series_spec(int F, const float* x, long n, int m, const float2* tw, float2* xs,
+                              float* xn, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  switch (F) {
+#define X(FF) \
+  case FF:    \
+    return launch_series_spec_F<FF>(x, n, m, tw, xs, xn, st);
+    TCX_FOR_EACH_F(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_assign_prune(int F, const PruneArgs& a, hipStream_t st) {
   if (a.k > prune_max_k(F)) return hipErrorInvalidValue;
   switch (F) {
@@ -700,9 +842,16 @@ hipError_t configure_sbd_kernels() {
     e = hipFuncSetAttribute((const void*)k_spectra<FF>,                                       \
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sbd_smem<FF>()); \
   if (e == hipSuccess)                                                                        \
-    e = hipFuncSetAttribute((const void*)k_assign_prune<FF>,                                  \
+    e = hipFuncSetAttribute((const void*)k_assign_prune<FF, false>,                           \
                             hipFuncAttributeMaxDynamicSharedMemorySize,                       \
-                            (int)prune_smem<FF>(prune_max_k_F<FF>()));
+                            (int)prune_smem<FF>(prune_max_k_F<FF>()));                        \
+  if (e == hipSuccess)                                                                        \
+    e = hipFuncSetAttribute((const void*)k_assign_prune<FF, true>,                            \
+                            hipFuncAttributeMaxDynamicSharedMemorySize,                       \
+                            (int)prune_smem<FF>(prune_max_k_F<FF>()));                        \
+  if (e == hipSuccess)                                                                        \
+    e = hipFuncSetAttribute((const void*)k_series_spec<FF>,                                   \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sbd_smem<FF>());
   TCX_FOR_EACH_F(X)
 #undef X
   return e;

@@ -51,7 +51,8 @@ enum Slot {
   kSlotPerm, kSlotBad, kSlotYhi, kSlotYlo, kSlotA0, kSlotItems, kSlotTiles, kSlotPart,
   kSlotS, kSlotInfo, kSlotDummy, kSlotWbuf, kSlotRbuf, kSlotBar, kSlotVout, kSlotSteps,
   kSlotConv, kSlotErr, kSlotCmag, kSlotEvals, kSlotSvout, kSlotLzW, kSlotLzState, kSlotMask,
-  kSlotRowStats, kSlotFitRowStats, kSlotKeyCounts, kSlotKeyOff, kSlotMinRow, kNumSlots
+  kSlotRowStats, kSlotFitRowStats, kSlotKeyCounts, kSlotKeyOff, kSlotMinRow, kSlotXs, kSlotXn,
+  kNumSlots
 };
 
 struct FitState {
@@ -69,6 +70,8 @@ struct FitState {
   float ms_ref = 0.f, ms_asg = 0.f, ms_asg_last = 0.f;
   bool ref_timed = false;  // events 0..1 hold a refine not yet added to ms_ref
   bool have_dist = false;  // dist holds an assignment's distances (orders the next grouping)
+  float2* xs = nullptr;    // half spectra of the series (per fit; NULL: transformed per call)
+  float* xn = nullptr;     // their norms
   int *label = nullptr, *shift = nullptr, *changed = nullptr, *zero = nullptr;
   float *dist = nullptr, *cent = nullptr, *norm = nullptr;
   float2* spec = nullptr;
@@ -209,7 +212,8 @@ int spectra(tcx_ctx* c, const float* d_rows, int rows, int m, float2** spec, flo
 int assign_core(tcx_ctx* c, const float* d_x, long n, int m, int k, const float2* cspec,
                 const float* cnorm, int rev, int* label, float* dist, int* shift, float* dmat,
                 int* changed, const int* order = nullptr, long npairs = 0,
-                unsigned long long* evals = nullptr, const float* cmag = nullptr) {
+                unsigned long long* evals = nullptr, const float* cmag = nullptr,
+                const float2* xs = nullptr, const float* xn = nullptr) {
   int F = fft_size(m);
   const float2* tw;
   TCX_TRY(twiddles(c, F, &tw));
@@ -221,6 +225,8 @@ int assign_core(tcx_ctx* c, const float* d_x, long n, int m, int k, const float2
     p.order = order;
     p.npairs = order ? npairs : (n + 1) / 2;
     p.evals = evals;
+    p.xs = xs;
+    p.xn = xn;
     TCX_HIP(tcx::launch_assign_prune(F, p, c->stream));
     return TCX_OK;
   }
@@ -236,8 +242,9 @@ int assign_core(tcx_ctx* c, const float* d_x, long n, int m, int k, const float2
 // packed columns starting at off[j], the upper triangle of
 // ceil(m/256)^2 tiles, each split along K into chunks of one length.  The
 // tiles of one (cluster, chunk) read the same panel rows, so they are
-// placed on one XCD (block b runs on XCD b % 8) at adjacent dispatch
-// slots; free slots are holes.  The chunk is the shortest for which every
+// placed on one XCD (consecutive blocks run on consecutive XCDs, so blocks
+// with equal b % 8 share one) at adjacent dispatch slots; free slots are
+// holes.  The chunk is the shortest for which every
 // XCD holds at most one block per CU (one 4-wave block fills a CU's
 // register file), so all blocks run in one round.
 void plan_syrk(int m, int k, const long* off, const long* seg_len, int cus,
@@ -793,6 +800,22 @@ int tcx_fit_begin(tcx_ctx* c, const float* d_x, int n, int m, int k, const tcx_f
   TCX_TRY(ws(c, kSlotShift, (size_t)n, &f.shift));
   TCX_TRY(ws(c, kSlotDist, (size_t)n, &f.dist));
   f.have_dist = false;
+  // The series do not change during the fit: transform them once for the
+  // pruned assignment (n * (F/2 + 1) complex values) when the budget allows.
+  f.xs = nullptr;
+  f.xn = nullptr;
+  {
+    const int F = fft_size(m);
+    const size_t need = sizeof(float2) * (size_t)n * (F / 2 + 1) + sizeof(float) * (size_t)n;
+    const bool fits = !c->budget || c->ws_bytes + c->user_bytes + need <= c->budget;
+    if (k <= tcx::prune_max_k(F) && fits) {
+      const float2* tw;
+      TCX_TRY(twiddles(c, F, &tw));
+      TCX_TRY(ws(c, kSlotXs, (size_t)n * (F / 2 + 1), &f.xs));
+      TCX_TRY(ws(c, kSlotXn, (size_t)n, &f.xn));
+      TCX_HIP(tcx::launch_series_spec(F, d_x, n, m, tw, f.xs, f.xn, st));
+    }
+  }
   TCX_TRY(ws(c, kSlotCent, (size_t)k * m, &f.cent));
   TCX_TRY(ws(c, kSlotChanged, 1, &f.changed));
   std::vector<int> init;
@@ -882,7 +905,8 @@ int fit_assign(tcx_ctx* c, FitState& f, int* h_changed) {
   TCX_HIP(hipEventRecord(c->ev[2], st));
   const int* order = f.have_order ? reinterpret_cast<const int*>(c->slot_ptr[kSlotPerm]) : nullptr;
   TCX_TRY(assign_core(c, f.d_x, f.n, f.m, f.k, f.spec, f.norm, f.o.convention, f.label, f.dist,
-                      f.shift, nullptr, f.changed, order, f.n_pad / 2, evals));
+                      f.shift, nullptr, f.changed, order, f.n_pad / 2, evals, nullptr, f.xs,
+                      f.xn));
   f.have_dist = true;
   TCX_HIP(hipEventRecord(c->ev[3], st));
   void* pin;

@@ -42,6 +42,8 @@ constexpr int kLdsBytes = 160 * 1024;  // LDS of one workgroup on gfx950
 int prune_max_k(int F);
 struct PruneArgs {
   const float* x = nullptr;
+  const float2* xs = nullptr;    // per-series half spectra [n][F/2+1] (may be NULL: from x)
+  const float* xn = nullptr;     // per-series norms (with xs)
   long n = 0;
   long npairs = 0;               // pairs of order[] (or ceil(n/2) without order)
   int m = 0, k = 0;
@@ -58,6 +60,9 @@ struct PruneArgs {
   unsigned long long* evals = nullptr;  // inverse transforms run (may be NULL)
 };
 hipError_t launch_assign_prune(int F, const PruneArgs& a, hipStream_t st);
+// Half spectra X_r[0 .. F/2] of every row and the row norms (k_series_spec).
+hipError_t launch_series_spec(int F, const float* x, long n, int m, const float2* tw, float2* xs,
+                              float* xn, hipStream_t st);
 hipError_t launch_sbd(int F, int mode, const SbdArgs& a, hipStream_t st);
 hipError_t launch_apply_shift(float* x, long rows, int m, const int* shift, const int* skip,
                               hipStream_t st);
