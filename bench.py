@@ -40,17 +40,19 @@ def sbd_assign_flops(n, m, k, evals=None):
 
     Full search: per series a half forward transform and its norm, per
     (series, centroid) a half inverse transform, the conj-multiply of F/2+1
-    bins and the 2m-1 lag max.  Pruned search (`evals` inverse transforms of
-    series pairs were run): per pair the forward transform, the magnitude
-    split of the half spectrum (~6 F) and the bound of every centroid (2 F:
-    F/2 + 1 bins, two series, one multiply-add each), per evaluation the
-    conj-multiply (6 F), the inverse transform and both lag maxima."""
+    bins and the 2m-1 lag max.  Pruned search inside a fit (`evals` inverse
+    transforms of series pairs were run; the series' half spectra and norms
+    are transformed once per fit, outside the iteration): per pair the
+    packed spectrum from two half spectra and the weighted magnitudes (~6 F)
+    and the bound of every centroid (2 F: F/2 + 1 bins, two series, one
+    multiply-add each), per evaluation the conj-multiply (6 F), the inverse
+    transform and both lag maxima."""
     F = fft_size(m)
     t = 5.0 * F * math.log2(F)
     if evals is None:
         return n * (t / 2 + 2 * m) + n * k * (t / 2 + 6 * (F // 2 + 1) + 2 * m - 1)
     pairs = (n + 1) // 2
-    return pairs * (t + 6 * F + 4 * m) + pairs * k * 2 * F + evals * (t + 6 * F + 4 * m)
+    return pairs * 6 * F + pairs * k * 2 * F + evals * (t + 6 * F + 4 * m)
 
 
 def iteration_bytes(n, m, k):

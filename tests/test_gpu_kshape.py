@@ -17,7 +17,8 @@ def _znormed(ctx, seed, n, m, K, sigma=0.3, frac=None):
     return d, d.numpy(), y
 
 
-@pytest.mark.parametrize("m,n,k", [(128, 512, 4), (48, 300, 6), (512, 2000, 16)])
+@pytest.mark.parametrize("m,n,k", [(128, 512, 4), (48, 300, 6), (512, 2000, 16), (300, 700, 5),
+                                   (1000, 600, 3)])
 @pytest.mark.parametrize("aligned", [False, True])
 def test_extract_shape_matches_oracle(ctx, m, n, k, aligned):
     d, Z, y = _znormed(ctx, m + k, n, m, k)
@@ -41,9 +42,10 @@ def test_extract_shape_matches_oracle(ctx, m, n, k, aligned):
     assert max(errs) <= CENT_RTOL, errs
 
 
-def test_extract_shape_edge_clusters(ctx):
+@pytest.mark.parametrize("m", [64, 300])
+def test_extract_shape_edge_clusters(ctx, m):
     """single-member cluster -> zscore_ddof1(member); constant members -> 0."""
-    m, n = 64, 10
+    n = 10
     rng = np.random.default_rng(3)
     X = rng.standard_normal((n, m)).astype(np.float32)
     X[7:] = 1.5  # cluster 2: constant rows, z-normalized to zeros

@@ -1155,14 +1155,33 @@ __device__ void tridiag_power_wave(const double* alpha, const double* beta, int 
   const double bu1 = (e1 + 1 < len) ? beta[e1 + 1] * sc : 0.0;        // T[e1][e1+1]
   double y0 = (e0 == 0) ? 1.0 : 0.0, y1 = 0.0;
   if (len <= 64) {
-    // one entry per lane: neighbours by DPP lane shifts only
-    for (int s = 0; s < I; ++s) {
-      const double p0 = wave_shr1_f64(y0, 0.0), n0 = wave_shl1_f64(y0, 0.0);
-      y0 = bl0 * p0 + a0 * y0 + bu0 * n0;
-      if ((s & 31) == 31 || s == I - 1) {
+    // one entry per lane, two steps at a time with the pentadiagonal T^2
+    // (neighbours by DPP lane shifts only): row e of T^2 is
+    // [T_e,e-1 T_e-1,e-2,  T_e,e-1 (T_e-1,e-1 + T_e,e),
+    //  T_e,e-1 T_e-1,e + T_e,e^2 + T_e,e+1 T_e+1,e,  T_e,e+1 (T_e,e + T_e+1,e+1),
+    //  T_e,e+1 T_e+1,e+2]
+    const double blm = wave_shr1_f64(bl0, 0.0), am = wave_shr1_f64(a0, 0.0);
+    const double bum = wave_shr1_f64(bu0, 0.0);
+    const double ap = wave_shl1_f64(a0, 0.0), blp = wave_shl1_f64(bl0, 0.0);
+    const double bup = wave_shl1_f64(bu0, 0.0);
+    const double c2m = bl0 * blm, c1m = bl0 * (am + a0);
+    const double c0 = bl0 * bum + a0 * a0 + bu0 * blp;
+    const double c1p = bu0 * (a0 + ap), c2p = bu0 * bup;
+    int s = 0;
+    for (; s + 1 < I; s += 2) {
+      const double p1 = wave_shr1_f64(y0, 0.0), n1 = wave_shl1_f64(y0, 0.0);
+      const double p2 = wave_shr1_f64(p1, 0.0), n2 = wave_shl1_f64(n1, 0.0);
+      y0 = (c2m * p2 + c1m * p1) + c0 * y0 + (c1p * n1 + c2p * n2);
+      if ((s & 31) == 30 || s + 2 >= I) {
         const double mx = wave_max_f64(fabs(y0));
         if (mx > 0.0) y0 *= 1.0 / mx;
       }
+    }
+    if (s < I) {  // odd I: one more step with T
+      const double p0 = wave_shr1_f64(y0, 0.0), n0 = wave_shl1_f64(y0, 0.0);
+      y0 = bl0 * p0 + a0 * y0 + bu0 * n0;
+      const double mx = wave_max_f64(fabs(y0));
+      if (mx > 0.0) y0 *= 1.0 / mx;
     }
   } else {
   for (int s = 0; s < I; ++s) {
@@ -1726,6 +1745,562 @@ __global__ __launch_bounds__(kPowerThreads) void k_lz_finish(
   }
 }
 
+// ------------------------------------ Lanczos, several blocks per cluster
+// One launch in which the G member blocks of a cluster share its Lanczos
+// process: member b owns rows [b R, b R + R) of S_j (in LDS when they fit)
+// and the same rows of the basis Q (LDS, one row of kLzCoopLQ columns per
+// matrix row), so a step reads S_j once over G CUs instead of once on one
+// CU.  Step it (q = q_it held whole by every member):
+//   A  w = M q on the member's rows (M q = S q - (r 1^T q + 1 r^T q)/m +
+//      sigma (1^T q)/m^2 1); partial dots Q^T w and |w|^2 published
+//   B  c = Q^T w (partials summed in member order); w -= Q c; partial dots
+//      of the new w published
+//   C  d = Q^T w; w -= Q d (classical Gram-Schmidt twice against the whole
+//      basis); the member's rows of w and its |w|^2 published
+//   D  beta = |w|, q_{it+1} = w / beta assembled from every member's rows.
+// alpha_it = c_it + d_it.  Every member sums the same values in the same
+// order, so alpha, beta and the acceptance test of y = T^I e1 (run by every
+// member) agree bit for bit and all members leave the loop at the same step.
+//
+// Exchange without flags or fences: a published FP64 value is two 64-bit
+// words, each holding a 32-bit tag and one 32-bit half of the value, stored
+// and loaded with device-scope (relaxed atomic) accesses; a reader polls the
+// words until both carry the tag it expects.  tag = launch generation << 10
+// | exchange epoch, so words left by an earlier epoch or launch never match
+// and the buffer needs no reset between launches (zero at allocation; the
+// generation starts at 1).  A slot is rewritten three epochs later at the
+// earliest, and only after every member has read it (each member writes
+// the next epoch's values only after reading every member's values of the
+// epoch before).  Readers wait with a wall-clock bound; a member that times
+// out marks the cluster failed (fail word = generation) and leaves, and the
+// cluster then reports "not accepted" (its caller runs the plain power
+// iteration).
+constexpr int kLzCoopThreads = 256;
+constexpr int kLzCoopWaves = kLzCoopThreads / kWave;
+constexpr int kLzCoopLQ = kLzMax + 1;    // basis columns per row (odd: spreads LDS banks)
+constexpr int kLzCoopPart = kLzMax + 2;  // partials per member: dots l <= it, then |w|^2
+constexpr int kLzCoopEpochBits = 10;
+
+__device__ __forceinline__ double coop_sum(double v, double* red) {
+  v = wave_sum_f64(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// Two block sums with one pair of barriers.
+__device__ __forceinline__ void coop_sum2(double& a, double& b, double* red) {
+  a = wave_sum_f64(a);
+  b = wave_sum_f64(b);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    red[threadIdx.x >> 6] = a;
+    red[4 + (threadIdx.x >> 6)] = b;
+  }
+  __syncthreads();
+  a = (red[0] + red[1]) + (red[2] + red[3]);
+  b = (red[4] + red[5]) + (red[6] + red[7]);
+}
+
+// Block max with one pair of barriers.
+__device__ __forceinline__ double coop_max(double v, double* red4) {
+  v = wave_max_f64(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red4[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return fmax(fmax(red4[0], red4[1]), fmax(red4[2], red4[3]));
+}
+
+// y = T^I e1 / |T^I e1| for T = tridiag(beta[1..len-1], alpha[0..len-1],
+// beta[1..len-1]), len <= kLzSqMax: binary powering with dense symmetric
+// matrices in LDS (wk: kLzSqDoubles), all threads of the block, instead of
+// I dependent steps of one wave.  T is scaled by its Gershgorin bound;
+// every later power is the square of a symmetric matrix, hence positive
+// semidefinite with its largest entry on the diagonal, and is scaled by the
+// reciprocal of its trace as it is used (entries in [1/len, 1] in size):
+// nothing overflows or underflows.  Every thread computes the scale factors
+// and the final norm itself (same LDS values, same order), so each product
+// costs one barrier.  Ends with a barrier.
+constexpr int kLzSqMax = 48;
+constexpr int kLzSqLd = kLzSqMax + 1;  // padded row stride
+constexpr int kLzSqDoubles = 2 * kLzSqMax * kLzSqLd + 2 * kLzSqMax;
+__device__ void tridiag_power_block(const double* alpha, const double* beta, int len, int I,
+                                    double* wk, double* y) {
+  const int tid = threadIdx.x, n = len;
+  double* A = wk;
+  double* B = wk + kLzSqMax * kLzSqLd;
+  double* yv = wk + 2 * kLzSqMax * kLzSqLd;  // kLzSqMax
+  double* yt = yv + kLzSqMax;                // kLzSqMax
+  double g = 0.0;
+#pragma unroll 8
+  for (int e = 0; e < n; ++e)
+    g = fmax(g, fabs(alpha[e]) + (e > 0 ? fabs(beta[e]) : 0.0) +
+                    (e + 1 < n ? fabs(beta[e + 1]) : 0.0));
+  const double sc = g > 0.0 ? 1.0 / g : 0.0;
+  for (int idx = tid; idx < n * n; idx += kLzCoopThreads) {
+    const int i = idx / n, jj = idx - i * n;
+    const double v = i == jj ? alpha[i] : (jj == i + 1 ? beta[jj] : (i == jj + 1 ? beta[i] : 0.0));
+    A[i * kLzSqLd + jj] = v * sc;
+  }
+  for (int i = tid; i < n; i += kLzCoopThreads) yv[i] = i == 0 ? 1.0 : 0.0;
+  __syncthreads();
+  double f = 1.0;  // the current power is f * A
+  for (int e = I;;) {
+    if (e & 1) {  // yt = f A yv, then swap
+      for (int i = tid; i < n; i += kLzCoopThreads) {
+        const double* ai = A + i * kLzSqLd;
+        double s0 = 0.0, s1 = 0.0;
+        int c = 0;
+#pragma unroll 4
+        for (; c + 1 < n; c += 2) {
+          s0 += ai[c] * yv[c];
+          s1 += ai[c + 1] * yv[c + 1];
+        }
+        if (c < n) s0 += ai[c] * yv[c];
+        yt[i] = (s0 + s1) * f;
+      }
+      __syncthreads();
+      double* t = yv;
+      yv = yt;
+      yt = t;
+    }
+    e >>= 1;
+    if (e == 0) break;
+    // B = (f A)^2; A symmetric, so B[i][j] = f^2 sum_c A[i][c] A[j][c]
+    const double f2 = f * f;
+    for (int idx = tid; idx < n * n; idx += kLzCoopThreads) {
+      const int i = idx / n, jj = idx - i * n;
+      const double *ai = A + i * kLzSqLd, *aj = A + jj * kLzSqLd;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      int c = 0;
+#pragma unroll 2
+      for (; c + 3 < n; c += 4) {
+        s0 += ai[c] * aj[c];
+        s1 += ai[c + 1] * aj[c + 1];
+        s2 += ai[c + 2] * aj[c + 2];
+        s3 += ai[c + 3] * aj[c + 3];
+      }
+      for (; c < n; ++c) s0 += ai[c] * aj[c];
+      B[i * kLzSqLd + jj] = ((s0 + s1) + (s2 + s3)) * f2;
+    }
+    __syncthreads();
+    double tr = 0.0;
+#pragma unroll 8
+    for (int i = 0; i < n; ++i) tr += B[i * kLzSqLd + i];
+    f = tr > 0.0 ? 1.0 / tr : 0.0;
+    double* t = A;
+    A = B;
+    B = t;
+  }
+  double nn = 0.0;
+#pragma unroll 8
+  for (int i = 0; i < n; ++i) nn += yv[i] * yv[i];
+  const double inv = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
+  for (int i = tid; i < n; i += kLzCoopThreads) y[i] = yv[i] * inv;
+  __syncthreads();
+}
+
+// Publishes v in the slot pair at p (tag in the upper halves).
+__device__ __forceinline__ void put_tagged(uint64_t* p, double v, uint32_t tag) {
+  const uint64_t t = (uint64_t)tag << 32;
+  st_dev_u64(p, t | (uint32_t)__double2hiint(v));
+  st_dev_u64(p + 1, t | (uint32_t)__double2loint(v));
+}
+
+// Reads the n <= 16 values published at the slot pairs p + 2 u stride
+// (u < n) into v[u], polling until every word carries `tag`.  All loads of a
+// round are in flight together.  False on timeout or when the cluster was
+// marked failed.
+__device__ __forceinline__ bool poll_tagged(const uint64_t* p, long stride, int n, uint32_t tag,
+                                            double* v, const uint32_t* fail, uint32_t gen,
+                                            long long timeout) {
+  if (n <= 0) return true;
+  const long long t0 = wall_clock64();
+  for (int round = 1;; ++round) {
+    uint64_t h[16], l[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const long o = 2 * (long)(u < n ? u : n - 1) * stride;
+      h[u] = ld_dev_u64(p + o);
+      l[u] = ld_dev_u64(p + o + 1);
+    }
+    bool all = true;
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      all = all && (uint32_t)(h[u] >> 32) == tag && (uint32_t)(l[u] >> 32) == tag;
+    if (all) {
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (u < n) v[u] = __hiloint2double((int)(uint32_t)h[u], (int)(uint32_t)l[u]);
+      return true;
+    }
+    // the failure word and the clock every 8 rounds (a device-scope load
+    // per round would double the round time)
+    if ((round & 7) == 0 &&
+        (__hip_atomic_load(const_cast<uint32_t*>(fail), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT) == gen ||
+         wall_clock64() - t0 > timeout))
+      return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// Sum in member order of the values published by the G members at
+// p + 2 g stride (g < G), polled 16 at a time.
+__device__ __forceinline__ bool poll_sum(const uint64_t* p, long stride, int G, uint32_t tag,
+                                         const uint32_t* fail, uint32_t gen, long long timeout,
+                                         double& sum) {
+  double s = 0.0;
+  for (int g0 = 0; g0 < G; g0 += 16) {
+    double v[16];
+    const int n = G - g0 < 16 ? G - g0 : 16;
+    if (!poll_tagged(p + 2 * (long)g0 * stride, stride, n, tag, v, fail, gen, timeout)) return false;
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (u < n) s += v[u];
+  }
+  sum = s;
+  return true;
+}
+
+// Row stride of the member's S rows in LDS: odd, so that the lanes of a
+// wave, which read one column of consecutive rows, hit distinct banks.
+__host__ __device__ constexpr int lz_coop_sl_stride(int m) { return m | 1; }
+
+// out[e] = sum_c S[e][c] x[c] for the member's nr <= 64 rows held in LDS
+// (row stride st), FP64 accumulation.  Thread (s, e): row e, column range s
+// of ns; the ns partials are summed in LDS (tmp) in part order.
+__device__ __forceinline__ void coop_rows_matvec_lds(const float* Sl, int st, int m, int nr,
+                                                     const double* x, double* tmp, double* out) {
+  const int Rp = nr > 32 ? 64 : 32;
+  const int ns = kLzCoopThreads / Rp;
+  const int e = threadIdx.x % Rp, s = threadIdx.x / Rp;
+  const int seg = (m + ns - 1) / ns;
+  const int c0 = s * seg, c1 = c0 + seg < m ? c0 + seg : m;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  if (e < nr) {
+    const float* row = Sl + (long)e * st;
+    int c = c0;
+    for (; c + 3 < c1; c += 4) {
+      a0 += (double)row[c] * x[c];
+      a1 += (double)row[c + 1] * x[c + 1];
+      a2 += (double)row[c + 2] * x[c + 2];
+      a3 += (double)row[c + 3] * x[c + 3];
+    }
+    for (; c < c1; ++c) a0 += (double)row[c] * x[c];
+  }
+  tmp[s * Rp + e] = (a0 + a1) + (a2 + a3);
+  __syncthreads();
+  for (int i = threadIdx.x; i < nr; i += kLzCoopThreads) {
+    double t = 0.0;
+    for (int ss = 0; ss < ns; ++ss) t += tmp[ss * Rp + i];
+    out[i] = t;
+  }
+}
+
+// out[e] = sum_c S[e][c] x[c] for the member's nr rows in global memory (row
+// stride m), FP64 accumulation; a wave takes 4 rows at a time.
+__device__ __forceinline__ void coop_rows_matvec(const float* __restrict__ Sr, int m, int nr,
+                                                 const double* x, double* out) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int e0 = wid * 4; e0 < nr; e0 += kLzCoopWaves * 4) {
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    const float* row[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) row[u] = Sr + (long)(e0 + u < nr ? e0 + u : nr - 1) * m;
+    for (int c = lane; c < m; c += kWave) {
+      const double xc = x[c];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a[u] += (double)row[u][c] * xc;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double s = wave_sum_f64(a[u]);
+      if (lane == 0 && e0 + u < nr) out[e0 + u] = s;
+    }
+  }
+}
+
+// Publishes dst[l] = sum_{e < nr} Qr[e][l] w[e] for l <= it and dst[it + 1]
+// = |w|^2 over the member's rows.  Thread (g, l): column l, rows g, g + ng,
+// ...; the ng row-group partials are then summed in LDS (tmp:
+// kLzCoopThreads doubles) in group order.
+__device__ __forceinline__ void coop_partials(const double* Qr, const double* w, int nr, int it,
+                                              double* tmp, uint64_t* dst, uint32_t tag) {
+  const int nc = it + 2;
+  int ncp = 16;
+  while (ncp < nc) ncp <<= 1;
+  const int ng = kLzCoopThreads / ncp;
+  const int l = threadIdx.x % ncp, g = threadIdx.x / ncp;
+  double a0 = 0.0, a1 = 0.0;
+  if (l <= it) {
+    int e = g;
+    for (; e + ng < nr; e += 2 * ng) {
+      a0 += Qr[(long)e * kLzCoopLQ + l] * w[e];
+      a1 += Qr[(long)(e + ng) * kLzCoopLQ + l] * w[e + ng];
+    }
+    if (e < nr) a0 += Qr[(long)e * kLzCoopLQ + l] * w[e];
+  } else if (l == it + 1) {
+    for (int e = g; e < nr; e += ng) a0 += w[e] * w[e];
+  }
+  tmp[g * ncp + l] = a0 + a1;
+  __syncthreads();
+  for (int c = threadIdx.x; c < nc; c += kLzCoopThreads) {
+    double t = 0.0;
+    for (int gg = 0; gg < ng; ++gg) t += tmp[gg * ncp + c];
+    put_tagged(dst + 2 * c, t, tag);
+  }
+}
+
+// w[e] -= sum_{l <= it} cf[l] Qr[e][l] on the member's rows (R <= 64).
+// Thread (s, e): row e, columns s, s + ns, ...; the ns partials are summed
+// in LDS (tmp) in part order.  Ends with a barrier.
+__device__ __forceinline__ void coop_project(const double* Qr, const double* cf, int nr, int it,
+                                             double* tmp, double* w) {
+  const int Rp = nr > 32 ? 64 : 32;
+  const int ns = kLzCoopThreads / Rp;
+  const int e = threadIdx.x % Rp, s = threadIdx.x / Rp;
+  double a0 = 0.0, a1 = 0.0;
+  if (e < nr) {
+    const double* qe = Qr + (long)e * kLzCoopLQ;
+    int l = s;
+    for (; l + ns <= it; l += 2 * ns) {
+      a0 += cf[l] * qe[l];
+      a1 += cf[l + ns] * qe[l + ns];
+    }
+    if (l <= it) a0 += cf[l] * qe[l];
+  }
+  tmp[s * Rp + e] = a0 + a1;
+  __syncthreads();
+  for (int i = threadIdx.x; i < nr; i += kLzCoopThreads) {
+    double t = 0.0;
+    for (int ss = 0; ss < ns; ++ss) t += tmp[ss * Rp + i];
+    w[i] -= t;
+  }
+  __syncthreads();
+}
+
+// Exchange words per cluster: partials P1, P2 (G x kLzCoopPart pairs each),
+// rows of r / w (m pairs), end marks (G words).
+__host__ __device__ constexpr long lz_coop_xch_len(int m, int G) {
+  return 2L * (2L * G * kLzCoopPart + m) + G;
+}
+__host__ __device__ constexpr long lz_coop_lds_doubles(int m, int R) {
+  return 2L * m + (long)R * kLzCoopLQ + R + kLzCoopPart + 2L * kLzCoopLQ + 2L * kLzMax + 16 +
+         kLzCoopThreads + kLzSqDoubles;
+}
+
+template <bool kSLds>
+__global__ __launch_bounds__(kLzCoopThreads) void k_lz_coop(
+    const float* __restrict__ S, int m, int k, int G, int R, const int* __restrict__ counts,
+    uint64_t seed, int iter, int I, double lz_tol, uint64_t* __restrict__ xch,
+    uint32_t* __restrict__ fails, uint32_t gen, long long timeout, double* __restrict__ vout,
+    double* __restrict__ rout, int* __restrict__ steps_out, int* __restrict__ conv_out) {
+  extern __shared__ double cz[];
+  double* q = cz;                                // m: q_it, every row
+  double* r = q + m;                             // m: r = S 1, every row
+  double* Qr = r + m;                            // R x kLzCoopLQ: the member's basis rows
+  double* w = Qr + (long)R * kLzCoopLQ;          // R
+  double* cf = w + R;                            // kLzCoopPart
+  double* alpha = cf + kLzCoopPart;              // kLzCoopLQ
+  double* beta = alpha + kLzCoopLQ;              // kLzCoopLQ
+  double* y = beta + kLzCoopLQ;                  // kLzMax
+  double* yp = y + kLzMax;                       // kLzMax
+  double* red = yp + kLzMax;                     // 8 (block sums)
+  int* ok = reinterpret_cast<int*>(red + 8);     // 2 ints
+  double* tmp = red + 16;                        // kLzCoopThreads: partial sums
+  double* sq = tmp + kLzCoopThreads;             // kLzSqDoubles: acceptance test
+  float* Sl = reinterpret_cast<float*>(sq + kLzSqDoubles);  // R x lz_coop_sl_stride(m)
+  // member -> (cluster, index): consecutive blocks go to consecutive XCDs,
+  // so numbering XCD by XCD puts the members of a cluster on one XCD
+  const int per = gridDim.x / 8;
+  const int p = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  const int j = p / G, b = p % G;
+  if (j >= k || counts[j] == 0) return;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int e0 = b * R;
+  const int nr = m - e0 < R ? m - e0 : R;
+  const float* Sg = S + (long)j * m * m + (long)e0 * m;  // the member's rows
+  uint64_t* X = xch + (long)j * lz_coop_xch_len(m, G);
+  uint64_t* P1 = X;                               // [G][kLzCoopPart] pairs
+  uint64_t* P2 = P1 + 2L * G * kLzCoopPart;
+  uint64_t* Wx = P2 + 2L * G * kLzCoopPart;       // m pairs: rows of r, then of w
+  uint64_t* Done = Wx + 2L * m;                   // G words
+  uint32_t* fail = fails + j;
+  const uint32_t tag0 = gen << kLzCoopEpochBits;
+  const int sst = lz_coop_sl_stride(m);
+  auto matvec = [&](const double* x, double* out) {
+    if constexpr (kSLds)
+      coop_rows_matvec_lds(Sl, sst, m, nr, x, tmp, out);
+    else
+      coop_rows_matvec(Sg, m, nr, x, out);
+  };
+  int len = 0, conv = 0;
+  auto give_up = [&]() {
+    if (tid == 0) __hip_atomic_store(fail, gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (b == 0 && tid == 0) {
+      steps_out[j] = len;
+      conv_out[j] = 0;
+    }
+  };
+  // every thread's polls must have succeeded (the reduction is a barrier)
+  auto all_ok = [&](bool good) { return __syncthreads_or(!good) == 0; };
+  if constexpr (kSLds)
+    for (long i = tid; i < (long)nr * m; i += kLzCoopThreads) {
+      const long e = i / m, c = i - e * m;
+      Sl[e * sst + c] = Sg[i];
+    }
+  // r = S 1 on the member's rows, exchanged; q_0 = power_v0 / |power_v0|
+  for (int i = tid; i < m; i += kLzCoopThreads) q[i] = 1.0;
+  __syncthreads();
+  matvec(q, w);
+  __syncthreads();
+  uint32_t tag = tag0 | 1;
+  for (int e = tid; e < nr; e += kLzCoopThreads) {
+    put_tagged(Wx + 2L * (e0 + e), w[e], tag);
+    rout[(long)j * m + e0 + e] = w[e];
+  }
+  double n0 = 0.0;
+  for (int i = tid; i < m; i += kLzCoopThreads) {
+    const double v = power_v0(seed, iter, j, i);
+    n0 += v * v;
+  }
+  n0 = sqrt(coop_sum(n0, red));
+  for (int i = tid; i < m; i += kLzCoopThreads) q[i] = power_v0(seed, iter, j, i) / n0;
+  {
+    double v[16];
+    const int n = tid < m ? (m - 1 - tid) / kLzCoopThreads + 1 : 0;  // rows tid + 256 u
+    const bool good = poll_tagged(Wx + 2L * tid, kLzCoopThreads, n, tag, v, fail, gen, timeout);
+    if (!all_ok(good)) return give_up();
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (u < n) r[tid + u * kLzCoopThreads] = v[u];
+  }
+  double sg = 0.0;
+  for (int i = tid; i < m; i += kLzCoopThreads) sg += r[i];
+  const double sigma = coop_sum(sg, red);
+  for (int e = tid; e < nr; e += kLzCoopThreads) Qr[(long)e * kLzCoopLQ] = q[e0 + e];
+  if (tid == 0) beta[0] = 0.0;
+  __syncthreads();
+  const int jmax = m < kLzMax ? m : kLzMax;
+  const double inv_m = 1.0 / (double)m;
+  int prev_len = 0;
+  for (int it = 0; it < jmax; ++it) {
+    // A
+    double sv = 0.0, rv = 0.0;
+    for (int i = tid; i < m; i += kLzCoopThreads) {
+      sv += q[i];
+      rv += r[i] * q[i];
+    }
+    coop_sum2(sv, rv, red);
+    matvec(q, w);
+    __syncthreads();
+    for (int e = tid; e < nr; e += kLzCoopThreads)
+      w[e] = w[e] - (r[e0 + e] * sv + rv) * inv_m + sigma * sv * inv_m * inv_m;
+    __syncthreads();
+    tag = tag0 | (uint32_t)(2 + 3 * it);
+    coop_partials(Qr, w, nr, it, tmp, P1 + 2L * b * kLzCoopPart, tag);
+    // B
+    bool good = true;
+    for (int l = tid; l <= it + 1; l += kLzCoopThreads)
+      good = good && poll_sum(P1 + 2L * l, kLzCoopPart, G, tag, fail, gen, timeout, cf[l]);
+    if (!all_ok(good)) return give_up();
+    const double nw2 = cf[it + 1], a1 = cf[it];
+    coop_project(Qr, cf, nr, it, tmp, w);
+    tag = tag0 | (uint32_t)(3 + 3 * it);
+    coop_partials(Qr, w, nr, it, tmp, P2 + 2L * b * kLzCoopPart, tag);
+    // C
+    for (int l = tid; l <= it + 1; l += kLzCoopThreads)
+      good = good && poll_sum(P2 + 2L * l, kLzCoopPart, G, tag, fail, gen, timeout, cf[l]);
+    if (!all_ok(good)) return give_up();
+    const double a = a1 + cf[it];
+    coop_project(Qr, cf, nr, it, tmp, w);
+    tag = tag0 | (uint32_t)(4 + 3 * it);
+    for (int e = tid; e < nr; e += kLzCoopThreads) put_tagged(Wx + 2L * (e0 + e), w[e], tag);
+    // D: every member's rows of w; |w|^2 from them (the same block sum in
+    // every member)
+    double wv[16];
+    const int nw = tid < m ? (m - 1 - tid) / kLzCoopThreads + 1 : 0;
+    good = poll_tagged(Wx + 2L * tid, kLzCoopThreads, nw, tag, wv, fail, gen, timeout);
+    if (!all_ok(good)) return give_up();
+    double bb = 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (u < nw) bb += wv[u] * wv[u];
+    const double bnorm = sqrt(coop_sum(bb, red));
+    if (tid == 0) alpha[it] = a;
+    len = it + 1;
+    const bool breakdown = !(bnorm > 1e-12 * sqrt(nw2));
+    if (breakdown || len == jmax || lz_power_check(len)) {
+      __syncthreads();  // alpha[it] visible
+      if (len <= kLzSqMax) {
+        tridiag_power_block(alpha, beta, len, I, sq, y);
+      } else {
+        if (wid == 0) tridiag_power_wave(alpha, beta, len, I, y);
+        __syncthreads();
+      }
+      if (wid == 0) {
+        double d = 0.0;
+        for (int e = lane; e < len; e += 64) {
+          const double pv = e < prev_len ? yp[e] : 0.0;  // previous y, zero padded
+          const double dd = y[e] - pv;
+          d += dd * dd;
+        }
+        d = wave_sum_f64(d);
+        if (lane == 0) ok[1] = breakdown || (prev_len > 0 && sqrt(d) <= lz_tol);
+        for (int e = lane; e < len; e += 64) yp[e] = y[e];
+      }
+      prev_len = len;
+      __syncthreads();
+      conv = ok[1];
+      if (conv || breakdown) break;
+    }
+    if (len == jmax) break;
+    if (tid == 0) beta[it + 1] = bnorm;
+    const double inv_b = 1.0 / bnorm;
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (u < nw) q[tid + u * kLzCoopThreads] = wv[u] * inv_b;
+    __syncthreads();
+    for (int e = tid; e < nr; e += kLzCoopThreads)
+      Qr[(long)e * kLzCoopLQ + it + 1] = q[e0 + e];
+    __syncthreads();
+  }
+  // v = Q y on the member's rows
+  for (int e = tid; e < nr; e += kLzCoopThreads) {
+    double acc = 0.0;
+    if (conv)
+      for (int l = 0; l < len; ++l) acc += y[l] * Qr[(long)e * kLzCoopLQ + l];
+    vout[(long)j * m + e0 + e] = acc;
+  }
+  // member 0 reports the cluster once every member has written its rows
+  tag = tag0 | (uint32_t)(5 + 3 * (len - 1));
+  __syncthreads();
+  if (tid == 0) st_dev_u64(Done + b, (uint64_t)tag << 32);
+  if (b == 0) {
+    bool good = true;
+    if (tid < kWave) {
+      const long long t0 = wall_clock64();
+      for (;;) {
+        bool mine = true;
+        for (int g = tid; g < G; g += kWave) mine = mine && (uint32_t)(ld_dev_u64(Done + g) >> 32) == tag;
+        if (__all(mine)) break;
+        if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen ||
+            wall_clock64() - t0 > timeout) {
+          good = false;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    if (tid == 0) {
+      steps_out[j] = len;
+      conv_out[j] = good ? conv : 0;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- launchers
 hipError_t launch_fill_int(int* p, long count, int value, hipStream_t st) {
   if (count <= 0) return hipSuccess;
@@ -1863,6 +2438,55 @@ hipError_t launch_power_lanczos(const float* S, int m, int k, const int* counts,
   return hipGetLastError();
 }
 
+bool plan_lz_coop(int m, int k, int cus, LzCoopPlan* p) {
+  if (m < kLzCoopMinM || k <= 0 || cus < 8) return false;
+  int G = 1;
+  while (G < 16 && 2L * G * k <= cus) G *= 2;
+  int R = (m + G - 1) / G;
+  if (R > kLzCoopMaxR) R = kLzCoopMaxR;
+  G = (m + R - 1) / R;
+  R = (m + G - 1) / G;  // rows balanced over the members
+  if (G > kLzCoopMaxG || (long)(G - 1) * R >= m) return false;  // every member owns rows
+  const size_t base = sizeof(double) * (size_t)lz_coop_lds_doubles(m, R);
+  const size_t sbytes = sizeof(float) * (size_t)R * lz_coop_sl_stride(m);
+  const bool s_lds = sbytes <= kLzCoopSLdsBytes && base + sbytes <= kLzCoopLdsMax;
+  const size_t lds = base + (s_lds ? sbytes : 0);
+  if (lds > kLzCoopLdsMax) return false;
+  const long nb = ((long)k * G + 7) / 8 * 8;
+  // every member must be resident at once
+  int per_cu = 0;
+  const void* fn = s_lds ? (const void*)k_lz_coop<true> : (const void*)k_lz_coop<false>;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kLzCoopThreads, lds) !=
+      hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  if ((long)per_cu * cus < nb) return false;
+  p->G = G;
+  p->R = R;
+  p->nb = (int)nb;
+  p->s_lds = s_lds;
+  p->lds = lds;
+  p->xch_len = lz_coop_xch_len(m, G);
+  return true;
+}
+
+hipError_t launch_lz_coop(const LzCoopPlan& p, const float* S, int m, int k, const int* counts,
+                          uint64_t seed, int iter, int I, double lz_tol, uint64_t* xch,
+                          uint32_t* fails, uint32_t gen, long long timeout, double* vout,
+                          double* rout, int* steps_out, int* conv_out, hipStream_t st) {
+  if (gen == 0 || gen >= (1u << (32 - kLzCoopEpochBits))) return hipErrorInvalidValue;
+  if (p.s_lds)
+    hipLaunchKernelGGL(k_lz_coop<true>, dim3(p.nb), dim3(kLzCoopThreads), p.lds, st, S, m, k, p.G,
+                       p.R, counts, seed, iter, I, lz_tol, xch, fails, gen, timeout, vout, rout,
+                       steps_out, conv_out);
+  else
+    hipLaunchKernelGGL(k_lz_coop<false>, dim3(p.nb), dim3(kLzCoopThreads), p.lds, st, S, m, k,
+                       p.G, p.R, counts, seed, iter, I, lz_tol, xch, fails, gen, timeout, vout,
+                       rout, steps_out, conv_out);
+  return hipGetLastError();
+}
+
 static size_t lz_step_smem(int m) {
   return sizeof(double) * (4 * (size_t)m + 8 * kPowerWaves + (kLzMax + 1) + sym_part_len(m));
 }
@@ -1909,6 +2533,12 @@ hipError_t configure_extract_kernels() {
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void*)k_lz_step, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lz_step_smem(4096));
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_lz_coop<true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLzCoopLdsMax);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_lz_coop<false>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLzCoopLdsMax);
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void*)k_cluster_matvec,
                             hipFuncAttributeMaxDynamicSharedMemorySize,

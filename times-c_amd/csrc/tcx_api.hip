@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -52,7 +53,7 @@ enum Slot {
   kSlotS, kSlotInfo, kSlotDummy, kSlotWbuf, kSlotRbuf, kSlotBar, kSlotVout, kSlotSteps,
   kSlotConv, kSlotErr, kSlotCmag, kSlotEvals, kSlotSvout, kSlotLzW, kSlotLzState, kSlotMask,
   kSlotRowStats, kSlotFitRowStats, kSlotKeyCounts, kSlotKeyOff, kSlotMinRow, kSlotXs, kSlotXn,
-  kNumSlots
+  kSlotLzXch, kNumSlots
 };
 
 struct FitState {
@@ -90,6 +91,10 @@ struct tcx_ctx {
   size_t user_bytes = 0;
   std::map<void*, size_t> user_allocs;
   hipEvent_t ev[4] = {};
+  hipEvent_t coop_ev = nullptr;  // after this context's last multi-block Lanczos launch
+  void* lz_xch = nullptr;        // exchange buffer (and its size) the generation below refers to
+  size_t lz_xch_bytes = 0;
+  uint32_t lz_gen = 0;           // generation of the last multi-block Lanczos launch
   // host copies kept alive until the stream has consumed them
   std::vector<tcx::SyrkItem> items;
   std::vector<tcx::SyrkTile> tiles;
@@ -305,11 +310,20 @@ struct RefineResult {
   int fallbacks = 0;
 };
 
-// M^I v0 of every non-empty cluster by Lanczos.  For m <= 768 one fused
-// launch (one block per cluster, S_j read from L2 every step); for larger m
-// per step one grid-wide matvec launch and one per-cluster update launch,
-// and every 8 steps from 16 on the host reads the clusters' done flags and
-// stops once all are done.
+// Multi-block Lanczos launches wait for each other: the members of one
+// launch wait for each other on the device, so two such launches that were
+// each partly resident could wait on each other for good.  Each launch is
+// ordered after the previous one on the same device (any context, any
+// stream) by a stream wait on that launch's event.
+std::mutex g_coop_mu;
+std::map<int, hipEvent_t> g_coop_last;
+
+// M^I v0 of every non-empty cluster by Lanczos.  From m = kLzCoopMinM, when
+// all k*G blocks fit on the device at once, one launch with G blocks per
+// cluster; else for m <= 768 one fused launch (one block per cluster, S_j
+// read from L2 every step); else per step one grid-wide matvec launch and
+// one per-cluster update launch, and every 8 steps from 16 on the host
+// reads the clusters' done flags and stops once all are done.
 // Returns on device the vectors v, S v, the row sums r = S 1, the Lanczos
 // step counts and the per-cluster acceptance flags.
 int power_lanczos(tcx_ctx* c, const float* S, int m, int k, const int* counts, uint64_t seed,
@@ -328,6 +342,44 @@ int power_lanczos(tcx_ctx* c, const float* S, int m, int k, const int* counts, u
   TCX_TRY(ws(c, kSlotSteps, (size_t)k, &st_out));
   TCX_TRY(ws(c, kSlotConv, (size_t)k, &cv_out));
   TCX_TRY(ws(c, kSlotLzState, (size_t)k, &state));
+  tcx::LzCoopPlan cp;
+  if (mode == 0 && !mask && tcx::plan_lz_coop(m, k, c->cus, &cp)) {
+    // exchange words and per-cluster failure words; their contents only
+    // ever match the current launch generation, so they are zeroed only
+    // when (re)allocated or when the generation wraps
+    uint64_t* xch;
+    const size_t words = (size_t)k * cp.xch_len + ((size_t)k + 1) / 2;
+    TCX_TRY(ws(c, kSlotLzXch, words, &xch));
+    uint32_t* fails = reinterpret_cast<uint32_t*>(xch + (size_t)k * cp.xch_len);
+    if (xch != c->lz_xch || c->slot_bytes[kSlotLzXch] != c->lz_xch_bytes ||
+        c->lz_gen + 1 >= tcx::kLzCoopMaxGen) {
+      TCX_HIP(tcx::launch_fill_int(reinterpret_cast<int*>(xch),
+                                   (long)(c->slot_bytes[kSlotLzXch] / sizeof(int)), 0, st));
+      c->lz_xch = xch;
+      c->lz_xch_bytes = c->slot_bytes[kSlotLzXch];
+      c->lz_gen = 0;
+    }
+    const uint32_t gen = ++c->lz_gen;
+    if (!c->coop_ev) TCX_HIP(hipEventCreateWithFlags(&c->coop_ev, hipEventDisableTiming));
+    {
+      std::lock_guard<std::mutex> lk(g_coop_mu);
+      auto prev = g_coop_last.find(c->device);
+      if (prev != g_coop_last.end() && prev->second != c->coop_ev)
+        TCX_HIP(hipStreamWaitEvent(st, prev->second, 0));
+      const long long timeout = c->wall_khz * 1000LL * 2;  // 2 s
+      TCX_HIP(tcx::launch_lz_coop(cp, S, m, k, counts, seed, iter, steps, tcx::kLzPowerTol, xch,
+                                  fails, gen, timeout, v, r, st_out, cv_out, st));
+      TCX_HIP(hipEventRecord(c->coop_ev, st));
+      g_coop_last[c->device] = c->coop_ev;
+    }
+    TCX_HIP(tcx::launch_cluster_matvec(S, m, k, v, m, sv, nullptr, st));  // S v
+    *vout = v;
+    *svout = sv;
+    *rout = r;
+    *steps_out = st_out;
+    *conv_out = cv_out;
+    return TCX_OK;
+  }
   if (mode == 0 && !mask && m <= tcx::kFusedLanczosMaxM) {
     // S_j fits in L2: one block per cluster runs every step in one launch
     TCX_HIP(tcx::launch_power_lanczos(S, m, k, counts, seed, iter, steps, tcx::kLzPowerTol, Q, v,
@@ -604,6 +656,14 @@ int tcx_destroy(tcx_ctx* c) {
   for (auto& kv : c->user_allocs) keep(hipFree(kv.first));
   for (auto& ev : c->ev)
     if (ev) keep(hipEventDestroy(ev));
+  if (c->coop_ev) {
+    {
+      std::lock_guard<std::mutex> lk(g_coop_mu);
+      auto it = g_coop_last.find(c->device);
+      if (it != g_coop_last.end() && it->second == c->coop_ev) g_coop_last.erase(it);
+    }
+    keep(hipEventDestroy(c->coop_ev));
+  }
   if (c->pinned) keep(hipHostFree(c->pinned));
   if (c->stream) keep(hipStreamDestroy(c->stream));
   delete c;

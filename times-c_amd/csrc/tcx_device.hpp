@@ -33,6 +33,25 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Device-scope 64-bit stores and loads (relaxed atomics at agent scope:
+// coherent across the XCDs' L2 caches without a cache-wide fence), used to
+// exchange values between the blocks of one launch.
+__device__ __forceinline__ void st_dev_u64(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_dev_u64(const uint64_t* p) {
+  return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev_f64(uint64_t* p, double v) {
+  st_dev_u64(p, (uint64_t)__double_as_longlong(v));
+}
+__device__ __forceinline__ double ld_dev_f64(const uint64_t* p) {
+  return __longlong_as_double((long long)ld_dev_u64(p));
+}
+// Waits until this wave's vector-memory operations have completed; a
+// device-scope store that completed is visible to every block.
+__device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 // Sync of an FFT lane group of L lanes: wave-local when the group lives in
 // one wave, a block barrier otherwise (then the group is the whole block).
 template <int L>

@@ -166,6 +166,32 @@ __host__ __device__ constexpr bool lz_power_check(int len) {
   return len >= kLzFirstCheck && len % kLzCheckEvery == 0;
 }
 
+// Lanczos with G blocks per cluster in one launch (members exchange through
+// device-scope memory and wait for each other, so all k*G blocks must be
+// resident at once).  Used from m = kLzCoopMinM on, when the plan fits.
+constexpr int kLzCoopMinM = 256;
+constexpr int kLzCoopMaxR = 64;   // rows of S per member
+constexpr int kLzCoopMaxG = 64;   // members per cluster
+constexpr int kLzCoopMaxM = kLzCoopMaxR * kLzCoopMaxG;
+constexpr size_t kLzCoopSLdsBytes = 96 * 1024;  // S rows kept in LDS up to this size
+constexpr size_t kLzCoopLdsMax = 160 * 1024;    // LDS of one CU
+struct LzCoopPlan {
+  int G = 0, R = 0, nb = 0;  // members per cluster, rows per member, blocks
+  bool s_lds = false;
+  size_t lds = 0;            // dynamic LDS bytes per block
+  long xch_len = 0;          // exchange words per cluster
+};
+// False when m, k or the device do not allow every member to be resident.
+bool plan_lz_coop(int m, int k, int cus, LzCoopPlan* p);
+// xch: k * p.xch_len words and fails: k words, zero when first used; gen:
+// launch generation, 1 <= gen < kLzCoopMaxGen, different from every earlier
+// launch on the same buffers since they were zeroed.  timeout: wall-clock
+// ticks a member waits for the others.
+constexpr uint32_t kLzCoopMaxGen = 1u << 22;
+hipError_t launch_lz_coop(const LzCoopPlan& p, const float* S, int m, int k, const int* counts,
+                          uint64_t seed, int iter, int I, double lz_tol, uint64_t* xch,
+                          uint32_t* fails, uint32_t gen, long long timeout, double* vout,
+                          double* rout, int* steps_out, int* conv_out, hipStream_t st);
 hipError_t launch_power_lanczos(const float* S, int m, int k, const int* counts, uint64_t seed,
                                 int iter, int I, double lz_tol, double* Q, double* vout,
                                 double* rout, int* steps_out, int* conv_out, hipStream_t st);
