@@ -1210,6 +1210,96 @@ __device__ void tridiag_power_wave(const double* alpha, const double* beta, int 
   if (e1 < len) y[e1] = y1 * inv;
 }
 
+// y = T^I e1 / |T^I e1| for T = tridiag(beta[1..len-1], alpha[0..len-1],
+// beta[1..len-1]), len <= kLzSqMax: binary powering with dense symmetric
+// matrices in LDS (wk: kLzSqDoubles), all NT threads of the block, instead of
+// I dependent steps of one wave.  T is scaled by its Gershgorin bound;
+// every later power is the square of a symmetric matrix, hence positive
+// semidefinite with its largest entry on the diagonal, and is scaled by the
+// reciprocal of its trace as it is used (entries in [1/len, 1] in size):
+// nothing overflows or underflows.  Every thread computes the scale factors
+// and the final norm itself (same LDS values, same order), so each product
+// costs one barrier.  Ends with a barrier.
+constexpr int kLzSqMax = 48;
+constexpr int kLzSqLd = kLzSqMax + 1;  // padded row stride
+constexpr int kLzSqDoubles = 2 * kLzSqMax * kLzSqLd + 2 * kLzSqMax;
+template <int NT>
+__device__ void tridiag_power_block(const double* alpha, const double* beta, int len, int I,
+                                    double* wk, double* y) {
+  const int tid = threadIdx.x, n = len;
+  double* A = wk;
+  double* B = wk + kLzSqMax * kLzSqLd;
+  double* yv = wk + 2 * kLzSqMax * kLzSqLd;  // kLzSqMax
+  double* yt = yv + kLzSqMax;                // kLzSqMax
+  double g = 0.0;
+#pragma unroll 8
+  for (int e = 0; e < n; ++e)
+    g = fmax(g, fabs(alpha[e]) + (e > 0 ? fabs(beta[e]) : 0.0) +
+                    (e + 1 < n ? fabs(beta[e + 1]) : 0.0));
+  const double sc = g > 0.0 ? 1.0 / g : 0.0;
+  for (int idx = tid; idx < n * n; idx += NT) {
+    const int i = idx / n, jj = idx - i * n;
+    const double v = i == jj ? alpha[i] : (jj == i + 1 ? beta[jj] : (i == jj + 1 ? beta[i] : 0.0));
+    A[i * kLzSqLd + jj] = v * sc;
+  }
+  for (int i = tid; i < n; i += NT) yv[i] = i == 0 ? 1.0 : 0.0;
+  __syncthreads();
+  double f = 1.0;  // the current power is f * A
+  for (int e = I;;) {
+    if (e & 1) {  // yt = f A yv, then swap
+      for (int i = tid; i < n; i += NT) {
+        const double* ai = A + i * kLzSqLd;
+        double s0 = 0.0, s1 = 0.0;
+        int c = 0;
+#pragma unroll 4
+        for (; c + 1 < n; c += 2) {
+          s0 += ai[c] * yv[c];
+          s1 += ai[c + 1] * yv[c + 1];
+        }
+        if (c < n) s0 += ai[c] * yv[c];
+        yt[i] = (s0 + s1) * f;
+      }
+      __syncthreads();
+      double* t = yv;
+      yv = yt;
+      yt = t;
+    }
+    e >>= 1;
+    if (e == 0) break;
+    // B = (f A)^2; A symmetric, so B[i][j] = f^2 sum_c A[i][c] A[j][c]
+    const double f2 = f * f;
+    for (int idx = tid; idx < n * n; idx += NT) {
+      const int i = idx / n, jj = idx - i * n;
+      const double *ai = A + i * kLzSqLd, *aj = A + jj * kLzSqLd;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      int c = 0;
+#pragma unroll 2
+      for (; c + 3 < n; c += 4) {
+        s0 += ai[c] * aj[c];
+        s1 += ai[c + 1] * aj[c + 1];
+        s2 += ai[c + 2] * aj[c + 2];
+        s3 += ai[c + 3] * aj[c + 3];
+      }
+      for (; c < n; ++c) s0 += ai[c] * aj[c];
+      B[i * kLzSqLd + jj] = ((s0 + s1) + (s2 + s3)) * f2;
+    }
+    __syncthreads();
+    double tr = 0.0;
+#pragma unroll 8
+    for (int i = 0; i < n; ++i) tr += B[i * kLzSqLd + i];
+    f = tr > 0.0 ? 1.0 / tr : 0.0;
+    double* t = A;
+    A = B;
+    B = t;
+  }
+  double nn = 0.0;
+#pragma unroll 8
+  for (int i = 0; i < n; ++i) nn += yv[i] * yv[i];
+  const double inv = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
+  for (int i = tid; i < n; i += NT) y[i] = yv[i] * inv;
+  __syncthreads();
+}
+
 // Number of eigenvalues of T = tridiag(beta, alpha, beta) (len n) below x:
 // sign count of the LDL^T pivots (Sturm sequence), pivots guarded away
 // from zero by pivmin.
@@ -1353,6 +1443,7 @@ __global__ __launch_bounds__(kPowerThreads) void k_power_lanczos(
   double* yp = y + kLzMax;            // kLzMax
   int* flag = reinterpret_cast<int*>(yp + kLzMax);  // 2 ints (padded to 1 double)
   double* part = yp + kLzMax + 1;     // sym_part_len(m)
+  double* sq = part + sym_part_len(m);  // kLzSqDoubles: acceptance test
   const int j = blockIdx.x;
   if (counts[j] == 0) return;
   const float* Sj = S + (long)j * m * m;
@@ -1443,8 +1534,13 @@ __global__ __launch_bounds__(kPowerThreads) void k_power_lanczos(
     const bool breakdown = !(b > 1e-12 * sqrt(nw2));
     if (breakdown || len == jmax || lz_power_check(len)) {
       __syncthreads();
+      if (len <= kLzSqMax) {
+        tridiag_power_block<kPowerThreads>(alpha, beta, len, I, sq, y);
+      } else {
+        if (wid == 0) tridiag_power_wave(alpha, beta, len, I, y);
+        __syncthreads();
+      }
       if (wid == 0) {
-        tridiag_power_wave(alpha, beta, len, I, y);
         double d = 0.0;
         for (int e = lane; e < len; e += 64) {
           const double pv = e < prev_len ? yp[e] : 0.0;  // previous y, zero padded
@@ -1812,95 +1908,6 @@ __device__ __forceinline__ double coop_max(double v, double* red4) {
   return fmax(fmax(red4[0], red4[1]), fmax(red4[2], red4[3]));
 }
 
-// y = T^I e1 / |T^I e1| for T = tridiag(beta[1..len-1], alpha[0..len-1],
-// beta[1..len-1]), len <= kLzSqMax: binary powering with dense symmetric
-// matrices in LDS (wk: kLzSqDoubles), all threads of the block, instead of
-// I dependent steps of one wave.  T is scaled by its Gershgorin bound;
-// every later power is the square of a symmetric matrix, hence positive
-// semidefinite with its largest entry on the diagonal, and is scaled by the
-// reciprocal of its trace as it is used (entries in [1/len, 1] in size):
-// nothing overflows or underflows.  Every thread computes the scale factors
-// and the final norm itself (same LDS values, same order), so each product
-// costs one barrier.  Ends with a barrier.
-constexpr int kLzSqMax = 48;
-constexpr int kLzSqLd = kLzSqMax + 1;  // padded row stride
-constexpr int kLzSqDoubles = 2 * kLzSqMax * kLzSqLd + 2 * kLzSqMax;
-__device__ void tridiag_power_block(const double* alpha, const double* beta, int len, int I,
-                                    double* wk, double* y) {
-  const int tid = threadIdx.x, n = len;
-  double* A = wk;
-  double* B = wk + kLzSqMax * kLzSqLd;
-  double* yv = wk + 2 * kLzSqMax * kLzSqLd;  // kLzSqMax
-  double* yt = yv + kLzSqMax;                // kLzSqMax
-  double g = 0.0;
-#pragma unroll 8
-  for (int e = 0; e < n; ++e)
-    g = fmax(g, fabs(alpha[e]) + (e > 0 ? fabs(beta[e]) : 0.0) +
-                    (e + 1 < n ? fabs(beta[e + 1]) : 0.0));
-  const double sc = g > 0.0 ? 1.0 / g : 0.0;
-  for (int idx = tid; idx < n * n; idx += kLzCoopThreads) {
-    const int i = idx / n, jj = idx - i * n;
-    const double v = i == jj ? alpha[i] : (jj == i + 1 ? beta[jj] : (i == jj + 1 ? beta[i] : 0.0));
-    A[i * kLzSqLd + jj] = v * sc;
-  }
-  for (int i = tid; i < n; i += kLzCoopThreads) yv[i] = i == 0 ? 1.0 : 0.0;
-  __syncthreads();
-  double f = 1.0;  // the current power is f * A
-  for (int e = I;;) {
-    if (e & 1) {  // yt = f A yv, then swap
-      for (int i = tid; i < n; i += kLzCoopThreads) {
-        const double* ai = A + i * kLzSqLd;
-        double s0 = 0.0, s1 = 0.0;
-        int c = 0;
-#pragma unroll 4
-        for (; c + 1 < n; c += 2) {
-          s0 += ai[c] * yv[c];
-          s1 += ai[c + 1] * yv[c + 1];
-        }
-        if (c < n) s0 += ai[c] * yv[c];
-        yt[i] = (s0 + s1) * f;
-      }
-      __syncthreads();
-      double* t = yv;
-      yv = yt;
-      yt = t;
-    }
-    e >>= 1;
-    if (e == 0) break;
-    // B = (f A)^2; A symmetric, so B[i][j] = f^2 sum_c A[i][c] A[j][c]
-    const double f2 = f * f;
-    for (int idx = tid; idx < n * n; idx += kLzCoopThreads) {
-      const int i = idx / n, jj = idx - i * n;
-      const double *ai = A + i * kLzSqLd, *aj = A + jj * kLzSqLd;
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-      int c = 0;
-#pragma unroll 2
-      for (; c + 3 < n; c += 4) {
-        s0 += ai[c] * aj[c];
-        s1 += ai[c + 1] * aj[c + 1];
-        s2 += ai[c + 2] * aj[c + 2];
-        s3 += ai[c + 3] * aj[c + 3];
-      }
-      for (; c < n; ++c) s0 += ai[c] * aj[c];
-      B[i * kLzSqLd + jj] = ((s0 + s1) + (s2 + s3)) * f2;
-    }
-    __syncthreads();
-    double tr = 0.0;
-#pragma unroll 8
-    for (int i = 0; i < n; ++i) tr += B[i * kLzSqLd + i];
-    f = tr > 0.0 ? 1.0 / tr : 0.0;
-    double* t = A;
-    A = B;
-    B = t;
-  }
-  double nn = 0.0;
-#pragma unroll 8
-  for (int i = 0; i < n; ++i) nn += yv[i] * yv[i];
-  const double inv = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
-  for (int i = tid; i < n; i += kLzCoopThreads) y[i] = yv[i] * inv;
-  __syncthreads();
-}
-
 // Publishes v in the slot pair at p (tag in the upper halves).
 __device__ __forceinline__ void put_tagged(uint64_t* p, double v, uint32_t tag) {
   const uint64_t t = (uint64_t)tag << 32;
@@ -2235,7 +2242,7 @@ __global__ __launch_bounds__(kLzCoopThreads) void k_lz_coop(
     if (breakdown || len == jmax || lz_power_check(len)) {
       __syncthreads();  // alpha[it] visible
       if (len <= kLzSqMax) {
-        tridiag_power_block(alpha, beta, len, I, sq, y);
+        tridiag_power_block<kLzCoopThreads>(alpha, beta, len, I, sq, y);
       } else {
         if (wid == 0) tridiag_power_wave(alpha, beta, len, I, y);
         __syncthreads();
@@ -2427,7 +2434,8 @@ hipError_t launch_power(const float* S, int m, int k, const int* counts, const f
 }
 
 static size_t lanczos_smem(int m) {
-  return sizeof(double) * (4 * (size_t)m + 8 * kPowerWaves + 5 * (kLzMax + 1) + sym_part_len(m));
+  return sizeof(double) *
+         (4 * (size_t)m + 8 * kPowerWaves + 5 * (kLzMax + 1) + sym_part_len(m) + kLzSqDoubles);
 }
 
 hipError_t launch_power_lanczos(const float* S, int m, int k, const int* counts, uint64_t seed,
@@ -2529,7 +2537,8 @@ hipError_t configure_extract_kernels() {
                                      (int)power_smem(4096));
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void*)k_power_lanczos,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lanczos_smem(4096));
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lanczos_smem(kFusedLanczosMaxM));
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void*)k_lz_step, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lz_step_smem(4096));
