@@ -27,7 +27,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per k_assign_prune launch at the default workload (n=100000,
 # m=512, k=16), from rocprofv3 FETCH_SIZE (x2) and WRITE_SIZE (x1) counter
 # runs calibrated on a plain 256 MiB copy: profiles/kshape_iteration_n100k_m512_k16.md
-ASSIGN_TRAFFIC_BYTES = {(100000, 512, 16): (208832 + 8395) * 1024}
+ASSIGN_TRAFFIC_BYTES = {(100000, 512, 16): (440616 + 25480) * 1024}
 
 
 def fft_size(m):
@@ -53,6 +53,14 @@ def sbd_assign_flops(n, m, k, evals=None):
         return n * (t / 2 + 2 * m) + n * k * (t / 2 + 6 * (F // 2 + 1) + 2 * m - 1)
     pairs = (n + 1) // 2
     return pairs * 6 * F + pairs * k * 2 * F + evals * (t + 6 * F + 4 * m)
+
+
+def assign_bytes(n, m):
+    """Algorithmic HBM bytes of one pruned assignment inside a fit: the
+    series' half spectra (F/2+1 complex FP32 each) and norms, the old labels
+    read and labels, distances and shifts written."""
+    F = fft_size(m)
+    return n * (F // 2 + 1) * 8 + n * 4 + 4 * n * 4
 
 
 def iteration_bytes(n, m, k):
@@ -313,7 +321,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": ASSIGN_TRAFFIC_BYTES.get((n, m, k)),
-            "algorithmic_bytes": n * m * 4 + 5 * n * 4,
+            "algorithmic_bytes": assign_bytes(n, m),
             "note": "compute-bound FP32 VALU (FFT butterflies, no matrix product); the FP32 "
                     "peak counts packed FP32, which this build disables (78.6 TFLOP/s unpacked)",
         },
