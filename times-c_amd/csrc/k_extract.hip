@@ -1915,30 +1915,32 @@ __device__ __forceinline__ void put_tagged(uint64_t* p, double v, uint32_t tag) 
   st_dev_u64(p + 1, t | (uint32_t)__double2loint(v));
 }
 
-// Reads the n <= 16 values published at the slot pairs p + 2 u stride
+// Reads the n <= N values published at the slot pairs p + 2 u stride
 // (u < n) into v[u], polling until every word carries `tag`.  All loads of a
-// round are in flight together.  False on timeout or when the cluster was
-// marked failed.
+// round are in flight together (N slots, the ones past n repeat the last
+// value, so N should be the smallest that covers n).  False on timeout or
+// when the cluster was marked failed.
+template <int N>
 __device__ __forceinline__ bool poll_tagged(const uint64_t* p, long stride, int n, uint32_t tag,
                                             double* v, const uint32_t* fail, uint32_t gen,
                                             long long timeout) {
   if (n <= 0) return true;
   const long long t0 = wall_clock64();
   for (int round = 1;; ++round) {
-    uint64_t h[16], l[16];
+    uint64_t h[N], l[N];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
+    for (int u = 0; u < N; ++u) {
       const long o = 2 * (long)(u < n ? u : n - 1) * stride;
       h[u] = ld_dev_u64(p + o);
       l[u] = ld_dev_u64(p + o + 1);
     }
     bool all = true;
 #pragma unroll
-    for (int u = 0; u < 16; ++u)
+    for (int u = 0; u < N; ++u)
       all = all && (uint32_t)(h[u] >> 32) == tag && (uint32_t)(l[u] >> 32) == tag;
     if (all) {
 #pragma unroll
-      for (int u = 0; u < 16; ++u)
+      for (int u = 0; u < N; ++u)
         if (u < n) v[u] = __hiloint2double((int)(uint32_t)h[u], (int)(uint32_t)l[u]);
       return true;
     }
@@ -1954,7 +1956,7 @@ __device__ __forceinline__ bool poll_tagged(const uint64_t* p, long stride, int 
 }
 
 // Sum in member order of the values published by the G members at
-// p + 2 g stride (g < G), polled 16 at a time.
+// p + 2 g stride (g < G), polled up to 16 at a time.
 __device__ __forceinline__ bool poll_sum(const uint64_t* p, long stride, int G, uint32_t tag,
                                          const uint32_t* fail, uint32_t gen, long long timeout,
                                          double& sum) {
@@ -1962,13 +1964,29 @@ __device__ __forceinline__ bool poll_sum(const uint64_t* p, long stride, int G, 
   for (int g0 = 0; g0 < G; g0 += 16) {
     double v[16];
     const int n = G - g0 < 16 ? G - g0 : 16;
-    if (!poll_tagged(p + 2 * (long)g0 * stride, stride, n, tag, v, fail, gen, timeout)) return false;
+    const uint64_t* q = p + 2 * (long)g0 * stride;
+    const bool ok = n <= 4    ? poll_tagged<4>(q, stride, n, tag, v, fail, gen, timeout)
+                    : n <= 8  ? poll_tagged<8>(q, stride, n, tag, v, fail, gen, timeout)
+                              : poll_tagged<16>(q, stride, n, tag, v, fail, gen, timeout);
+    if (!ok) return false;
 #pragma unroll
     for (int u = 0; u < 16; ++u)
       if (u < n) s += v[u];
   }
   sum = s;
   return true;
+}
+
+// Reads this thread's rows tid + 256 u (u < n <= 16) of a published vector
+// (slot pairs from p), with the smallest polling batch that covers n.
+__device__ __forceinline__ bool poll_rows(const uint64_t* p, int n, uint32_t tag, double* v,
+                                          const uint32_t* fail, uint32_t gen, long long timeout) {
+  const uint64_t* q = p + 2L * threadIdx.x;
+  if (n <= 1) return poll_tagged<1>(q, kLzCoopThreads, n, tag, v, fail, gen, timeout);
+  if (n <= 2) return poll_tagged<2>(q, kLzCoopThreads, n, tag, v, fail, gen, timeout);
+  if (n <= 4) return poll_tagged<4>(q, kLzCoopThreads, n, tag, v, fail, gen, timeout);
+  if (n <= 8) return poll_tagged<8>(q, kLzCoopThreads, n, tag, v, fail, gen, timeout);
+  return poll_tagged<16>(q, kLzCoopThreads, n, tag, v, fail, gen, timeout);
 }
 
 // Row stride of the member's S rows in LDS: odd, so that the lanes of a
@@ -2178,7 +2196,7 @@ __global__ __launch_bounds__(kLzCoopThreads) void k_lz_coop(
   {
     double v[16];
     const int n = tid < m ? (m - 1 - tid) / kLzCoopThreads + 1 : 0;  // rows tid + 256 u
-    const bool good = poll_tagged(Wx + 2L * tid, kLzCoopThreads, n, tag, v, fail, gen, timeout);
+    const bool good = poll_rows(Wx, n, tag, v, fail, gen, timeout);
     if (!all_ok(good)) return give_up();
 #pragma unroll
     for (int u = 0; u < 16; ++u)
@@ -2229,7 +2247,7 @@ __global__ __launch_bounds__(kLzCoopThreads) void k_lz_coop(
     // every member)
     double wv[16];
     const int nw = tid < m ? (m - 1 - tid) / kLzCoopThreads + 1 : 0;
-    good = poll_tagged(Wx + 2L * tid, kLzCoopThreads, nw, tag, wv, fail, gen, timeout);
+    good = poll_rows(Wx, nw, tag, wv, fail, gen, timeout);
     if (!all_ok(good)) return give_up();
     double bb = 0.0;
 #pragma unroll
