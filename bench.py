@@ -42,25 +42,24 @@ def sbd_assign_flops(n, m, k, evals=None):
     (series, centroid) a half inverse transform, the conj-multiply of F/2+1
     bins and the 2m-1 lag max.  Pruned search inside a fit (`evals` inverse
     transforms of series pairs were run; the series' half spectra and norms
-    are transformed once per fit, outside the iteration): per pair the
-    packed spectrum from two half spectra and the weighted magnitudes (~6 F)
-    and the bound of every centroid (2 F: F/2 + 1 bins, two series, one
-    multiply-add each), per evaluation the conj-multiply (6 F), the inverse
-    transform and both lag maxima."""
+    are transformed once per fit, and the bounds of every (series, centroid)
+    come from a separate product kernel before the assignment): per pair the
+    packed spectrum from two half spectra (2 F), per evaluation the
+    conj-multiply (6 F), the inverse transform and both lag maxima."""
     F = fft_size(m)
     t = 5.0 * F * math.log2(F)
     if evals is None:
         return n * (t / 2 + 2 * m) + n * k * (t / 2 + 6 * (F // 2 + 1) + 2 * m - 1)
     pairs = (n + 1) // 2
-    return pairs * 6 * F + pairs * k * 2 * F + evals * (t + 6 * F + 4 * m)
+    return pairs * 2 * F + evals * (t + 6 * F + 4 * m)
 
 
-def assign_bytes(n, m):
+def assign_bytes(n, m, k):
     """Algorithmic HBM bytes of one pruned assignment inside a fit: the
-    series' half spectra (F/2+1 complex FP32 each) and norms, the old labels
-    read and labels, distances and shifts written."""
+    series' half spectra (F/2+1 complex FP32 each), norms and k bounds, the
+    old labels read and labels, distances and shifts written."""
     F = fft_size(m)
-    return n * (F // 2 + 1) * 8 + n * 4 + 4 * n * 4
+    return n * (F // 2 + 1) * 8 + n * 4 + n * k * 4 + 4 * n * 4
 
 
 def iteration_bytes(n, m, k):
@@ -321,7 +320,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": ASSIGN_TRAFFIC_BYTES.get((n, m, k)),
-            "algorithmic_bytes": assign_bytes(n, m),
+            "algorithmic_bytes": assign_bytes(n, m, k),
             "note": "compute-bound FP32 VALU (FFT butterflies, no matrix product); the FP32 "
                     "peak counts packed FP32, which this build disables (78.6 TFLOP/s unpacked)",
         },

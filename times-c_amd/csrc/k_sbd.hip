@@ -57,7 +57,7 @@ template <int F>
 __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_spectra(
     const float* __restrict__ y, int rows, int m, const float2* __restrict__ twg,
     float2* __restrict__ spec, float* __restrict__ norms, int* __restrict__ zero,
-    float* __restrict__ mag) {
+    float* __restrict__ mag, float* __restrict__ bmag) {
   using G = FftGeom<F>;
   constexpr int L = G::L, P = G::P;
   extern __shared__ float2 smem[];
@@ -90,6 +90,16 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_spectra(
       float* mo = mag + (long)row * F + lane * P;
 #pragma unroll
       for (int s = 0; s < P; ++s) mo[s] = sqrtf(v[s].x * v[s].x + v[s].y * v[s].y);
+    }
+    if (bmag) {  // bound operand row: |C[f]| / |c| for f <= F/2, zero padded
+      constexpr int KP = bound_kpad(F);
+      const float rn = ss > 0.f ? 1.f / sqrtf(ss) : 0.f;
+#pragma unroll
+      for (int s = 0; s < P; ++s) {
+        const int f = lane + L * s;
+        if (f <= F / 2) bmag[bound_frag(row, f, KP)] = sqrtf(v[s].x * v[s].x + v[s].y * v[s].y) * rn;
+      }
+      for (int f = F / 2 + 1 + lane; f < KP; f += L) bmag[bound_frag(row, f, KP)] = 0.f;
     }
     if (lane == 0) {
       norms[row] = sqrtf(ss);
@@ -351,7 +361,7 @@ __device__ __forceinline__ void partner_half(const float2 (&zx)[P], float2* buf,
 template <int F>
 __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_series_spec(
     const float* __restrict__ x, long n, int m, const float2* __restrict__ twg,
-    float2* __restrict__ xs, float* __restrict__ xn) {
+    float2* __restrict__ xs, float* __restrict__ xn, float* __restrict__ xm) {
   using G = FftGeom<F>;
   constexpr int L = G::L, P = G::P, H = P / 2;
   constexpr long HS = F / 2 + 1;
@@ -391,13 +401,163 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_series_spec(
     partner_half<F, L, P>(zx, buf, lane, px, py);
     float2* oa = xs + ia * HS;
     float2* ob = xs + ib * HS;
+    // bound rows: weight * |X_f| / (F |x|), weight 2 inside, 1 at f = 0, F/2
+    constexpr int KP = bound_kpad(F);
+    const float ra = sa > 0.f ? 1.f / ((float)F * sqrtf(sa)) : 0.f;
+    const float rb = sb > 0.f ? 1.f / ((float)F * sqrtf(sb)) : 0.f;
 #pragma unroll
     for (int s = 0; s <= H; ++s) {
       const int f = lane + L * s;
       if (s < H || lane == 0) {  // f < F/2, or f = F/2
         const int fo = s < H ? f : F / 2;
-        oa[fo] = make_float2(0.5f * (zx[s].x + px[s]), 0.5f * (zx[s].y - py[s]));
-        if (hasb) ob[fo] = make_float2(0.5f * (zx[s].y + py[s]), 0.5f * (px[s] - zx[s].x));
+        const float2 A = make_float2(0.5f * (zx[s].x + px[s]), 0.5f * (zx[s].y - py[s]));
+        const float2 B = make_float2(0.5f * (zx[s].y + py[s]), 0.5f * (px[s] - zx[s].x));
+        oa[fo] = A;
+        if (hasb) ob[fo] = B;
+        if (xm) {
+          const float w = (fo == 0 || fo == F / 2) ? 1.f : 2.f;
+          xm[bound_frag(ia, fo, KP)] = w * sqrtf(A.x * A.x + A.y * A.y) * ra;
+          if (hasb) xm[bound_frag(ib, fo, KP)] = w * sqrtf(B.x * B.x + B.y * B.y) * rb;
+        }
+      }
+    }
+    if (xm)
+      for (int f = F / 2 + 1 + lane; f < KP; f += L) {  // padding of the K dimension
+        xm[bound_frag(ia, f, KP)] = 0.f;
+        if (hasb) xm[bound_frag(ib, f, KP)] = 0.f;
+      }
+  }
+}
+
+// ------------------------------------------ bounds of the pruned assignment
+// ub[i][j] = sum_f xm[i][f] cm[j][f] for every series i and centroid j:
+// the bounds k_assign_prune otherwise forms per pair, as one FP32 product
+// of the per-fit magnitude rows xm (k_series_spec) and the centroids' rows
+// cm = |C_j[f]| / |c_j| (k_spectra), both in fragment order (bound_frag),
+// on v_mfma_f32_16x16x4f32.  A wave computes 16 series x 16 centroids per
+// tile and NT tiles per pass over its rows of xm.  For the 16 columns of a
+// step lane l reads the l-th 16-byte vector of the fragment block of each
+// operand (one contiguous 1 KB per wave): row l % 16 of A and column l % 16
+// of B at columns 4 (l / 16) + c, c < 4, and MFMA number c takes component
+// c.  Result element (i, j) is in lane j + 16 (i / 4), register i % 4.
+// Rows past n (or centroids past k) in the last block are computed from
+// whatever the padding holds and never stored: a result element depends on
+// its own row and column only.
+typedef float bound_f4 __attribute__((ext_vector_type(4)));
+template <int NT, int NS>
+__global__ __launch_bounds__(256) void k_bounds(const float* __restrict__ xm, long n, int kpad,
+                                                const float* __restrict__ cm, int k, int ld,
+                                                float* __restrict__ ub) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const long blk = (long)blockIdx.x * 4 + wid;
+  const long i0 = blk * 16;
+  if (i0 >= n) return;
+  const int r = lane & 15, kq = lane >> 4;
+  const int nsteps = kpad >> 4;
+  const float4* xa = reinterpret_cast<const float4*>(xm) + blk * nsteps * 64 + lane;
+  for (int t0 = 0; t0 * 16 < k; t0 += NT) {
+    bound_f4 acc[NT];
+    const float4* cb[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int jt = (t0 + t) * 16 < k ? t0 + t : t0;  // tiles past k: computed, not stored
+      cb[t] = reinterpret_cast<const float4*>(cm) + (long)jt * nsteps * 64 + lane;
+      acc[t] = bound_f4{0.f, 0.f, 0.f, 0.f};
+    }
+    for (int s0 = 0; s0 < nsteps; s0 += NS) {
+      float4 a[NS], bv[NT][NS];
+#pragma unroll
+      for (int u = 0; u < NS; ++u) {
+        const int s = s0 + u < nsteps ? s0 + u : nsteps - 1;  // clamped, dropped below
+        a[u] = xa[(long)s * 64];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) bv[t][u] = cb[t][(long)s * 64];
+      }
+#pragma unroll
+      for (int u = 0; u < NS; ++u) {
+        if (s0 + u >= nsteps) break;  // uniform
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, bv[t][u].x, acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, bv[t][u].y, acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].z, bv[t][u].z, acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].w, bv[t][u].w, acc[t], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int j = (t0 + t) * 16 + r;
+      if (j >= k) continue;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const long i = i0 + 4 * kq + q;
+        if (i < n) ub[i * ld + j] = acc[t][q];
+      }
+    }
+  }
+}
+
+// Bounds of every centroid for one series pair (group of L lanes) from the
+// weighted half-spectrum magnitudes ma, mb: ub = sum_f |X_f| |C_f| /
+// (F |x| |c|), written to ubs[0, k) (series a) and ubs[k, 2k) (series b).
+template <int F, int L, int P>
+__device__ __forceinline__ void prune_bounds(const float* ma, const float* mb, float na, float nb,
+                                             const float* __restrict__ cmag,
+                                             const float* __restrict__ cnorm, int k, int lane,
+                                             float* red, float* ubs) {
+  constexpr int H = P / 2;
+  if constexpr (L <= kWave) {
+    // KC centroids at a time; the 2*KC per-lane partial sums are reduced
+    // by a reduce-scatter over the L lanes (each level halves the values
+    // a lane keeps)
+    constexpr int KC = 4;
+    for (int j0 = 0; j0 < k; j0 += KC) {
+      float v[2 * KC];
+#pragma unroll
+      for (int c = 0; c < KC; ++c) {
+        float aa = 0.f, ab = 0.f;
+        if (j0 + c < k) {
+          float cv[H + 1];
+          load_mag<P, H + 1>(cmag + (long)(j0 + c) * F + lane * P, cv);
+#pragma unroll
+          for (int s = 0; s <= H; ++s) {
+            aa += ma[s] * cv[s];
+            ab += mb[s] * cv[s];
+          }
+        }
+        v[c] = aa;
+        v[KC + c] = ab;
+      }
+      int base = 0;
+      reduce_scatter<2 * KC, L / 2, L>(v, lane, base);
+      constexpr int kHeld = 2 * KC >= L ? 2 * KC / L : 1;
+#pragma unroll
+      for (int i = 0; i < kHeld; ++i) {
+        const int idx = base + i, ser = idx / KC, jj = j0 + idx % KC;
+        if (jj < k) {
+          const float den = (ser ? nb : na) * cnorm[jj];
+          ubs[ser * k + jj] = den != 0.f ? (v[i] / (float)F) / den : 0.f;
+        }
+      }
+    }
+  } else {
+    for (int j = 0; j < k; ++j) {
+      float cv[H + 1];
+      load_mag<P, H + 1>(cmag + (long)j * F + lane * P, cv);
+      float aa = 0.f, ab = 0.f;
+#pragma unroll
+      for (int s = 0; s <= H; ++s) {
+        aa += ma[s] * cv[s];
+        ab += mb[s] * cv[s];
+      }
+      aa = group_sum<L>(aa, red);
+      ab = group_sum<L>(ab, red);
+      if (lane == 0) {
+        const float cn = cnorm[j];
+        const float dena = na * cn, denb = nb * cn;
+        ubs[j] = dena != 0.f ? (aa / (float)F) / dena : 0.f;
+        ubs[k + j] = denb != 0.f ? (ab / (float)F) / denb : 0.f;
       }
     }
   }
@@ -417,7 +577,7 @@ struct PruneGeom {
 template <int F, bool kPre>
 __global__ __launch_bounds__(PruneGeom<F>::kBlock, PruneGeom<F>::kMinWaves) void k_assign_prune(
     const float* __restrict__ x, const float2* __restrict__ xs, const float* __restrict__ xn,
-    long n, long npairs, int m, int k,
+    const float* __restrict__ ubg, int ub_ld, long n, long npairs, int m, int k,
     const float2* __restrict__ cspec, const float* __restrict__ cmag,
     const float* __restrict__ cnorm, const float2* __restrict__ twg, int rev,
     const int* __restrict__ order, int* __restrict__ label, float* __restrict__ dist,
@@ -448,12 +608,12 @@ __global__ __launch_bounds__(PruneGeom<F>::kBlock, PruneGeom<F>::kMinWaves) void
     }
     if (ia < 0) continue;  // hole pair (uniform for the group)
     const bool hasb = ib >= 0;
-    // packed spectrum zx[s] = Z[lane + L*s], Z = X_a + i X_b, and the weighted
-    // half-spectrum magnitudes ma, mb (f = lane + L*s for s < P/2, f = F/2
-    // at lane 0, s = P/2; weight 2 inside, 1 at f = 0 and f = F/2)
+    // packed spectrum zx[s] = Z[lane + L*s], Z = X_a + i X_b; without the
+    // per-fit data also the weighted half-spectrum magnitudes ma, mb
+    // (f = lane + L*s for s < P/2, f = F/2 at lane 0, s = P/2; weight 2
+    // inside, 1 at f = 0 and f = F/2) for the bounds
     constexpr int H = P / 2;
     float2 zx[P];
-    float ma[H + 1], mb[H + 1];
     float na, nb;
     if constexpr (kPre) {
       // from the per-fit half spectra: X[f] for f <= F/2, conj(X[F-f]) above
@@ -480,17 +640,17 @@ __global__ __launch_bounds__(PruneGeom<F>::kBlock, PruneGeom<F>::kMinWaves) void
           B = make_float2(b2.x, -b2.y);
         }
         zx[s] = make_float2(A.x - B.y, A.y + B.x);
-        if (s <= H) {
-          const float w = s == H ? (lane == 0 ? 1.f : 0.f) : ((s == 0 && lane == 0) ? 1.f : 2.f);
-          ma[s] = w * sqrtf(A.x * A.x + A.y * A.y);
-          mb[s] = w * sqrtf(B.x * B.x + B.y * B.y);
-        }
       };
 #pragma unroll
       for (int s = 0; s < H; ++s) assemble(s);
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int s = H; s < P; ++s) assemble(s);
+      // the bounds, computed for every series by k_bounds
+      for (int j = lane; j < k; j += L) {
+        ubs[j] = ubg[ia * ub_ld + j];
+        ubs[k + j] = hasb ? ubg[ib * ub_ld + j] : 0.f;
+      }
     } else {
       const float* xa = x + ia * (long)m;
       const float* xb = x + (hasb ? ib : ia) * (long)m;
@@ -511,6 +671,7 @@ __global__ __launch_bounds__(PruneGeom<F>::kBlock, PruneGeom<F>::kMinWaves) void
       fft<F, L, false>(zx, buf, tw, lane);
       // |X_a[f]| = |Z[f] + conj(Z[F-f])| / 2, |X_b[f]| = |Z[f] - conj(Z[F-f])| / 2
       float px[H + 1], py[H + 1];
+      float ma[H + 1], mb[H + 1];
       partner_half<F, L, P>(zx, buf, lane, px, py);
 #pragma unroll
       for (int s = 0; s <= H; ++s) {
@@ -520,61 +681,7 @@ __global__ __launch_bounds__(PruneGeom<F>::kBlock, PruneGeom<F>::kMinWaves) void
         ma[s] = wgt * sqrtf(ar * ar + ai * ai);
         mb[s] = wgt * sqrtf(br * br + bi * bi);
       }
-    }
-    // bounds of every centroid: ub = sum_f |X_f| |C_f| / (F |x| |c|)
-    if constexpr (L <= kWave) {
-      // KC centroids at a time; the 2*KC per-lane partial sums are reduced
-      // by a reduce-scatter over the L lanes (each level halves the values
-      // a lane keeps)
-      constexpr int KC = 4;
-      for (int j0 = 0; j0 < k; j0 += KC) {
-        float v[2 * KC];
-#pragma unroll
-        for (int c = 0; c < KC; ++c) {
-          float aa = 0.f, ab = 0.f;
-          if (j0 + c < k) {
-            float cv[H + 1];
-            load_mag<P, H + 1>(cmag + (long)(j0 + c) * F + lane * P, cv);
-#pragma unroll
-            for (int s = 0; s <= H; ++s) {
-              aa += ma[s] * cv[s];
-              ab += mb[s] * cv[s];
-            }
-          }
-          v[c] = aa;
-          v[KC + c] = ab;
-        }
-        int base = 0;
-        reduce_scatter<2 * KC, L / 2, L>(v, lane, base);
-        constexpr int kHeld = 2 * KC >= L ? 2 * KC / L : 1;
-#pragma unroll
-        for (int i = 0; i < kHeld; ++i) {
-          const int idx = base + i, ser = idx / KC, jj = j0 + idx % KC;
-          if (jj < k) {
-            const float den = (ser ? nb : na) * cnorm[jj];
-            ubs[ser * k + jj] = den != 0.f ? (v[i] / (float)F) / den : 0.f;
-          }
-        }
-      }
-    } else {
-      for (int j = 0; j < k; ++j) {
-        float cv[H + 1];
-        load_mag<P, H + 1>(cmag + (long)j * F + lane * P, cv);
-        float aa = 0.f, ab = 0.f;
-#pragma unroll
-        for (int s = 0; s <= H; ++s) {
-          aa += ma[s] * cv[s];
-          ab += mb[s] * cv[s];
-        }
-        aa = group_sum<L>(aa, red);
-        ab = group_sum<L>(ab, red);
-        if (lane == 0) {
-          const float cn = cnorm[j];
-          const float dena = na * cn, denb = nb * cn;
-          ubs[j] = dena != 0.f ? (aa / (float)F) / dena : 0.f;
-          ubs[k + j] = denb != 0.f ? (ab / (float)F) / denb : 0.f;
-        }
-      }
+      prune_bounds<F, L, P>(ma, mb, na, nb, cmag, cnorm, k, lane, red, ubs);
     }
     group_sync<L>();
     float bestA = INFINITY, bestB = hasb ? INFINITY : -INFINITY;
@@ -671,13 +778,13 @@ static size_t sbd_smem() {
 template <int F>
 static hipError_t launch_spectra_F(const float* y, int rows, int m, const float2* tw,
                                    float2* spec, float* norms, int* zero, float* mag,
-                                   hipStream_t st) {
+                                   float* bmag, hipStream_t st) {
   using G = FftGeom<F>;
   int blocks = (rows + G::kGroups - 1) / G::kGroups;
   if (blocks < 1) blocks = 1;
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL((k_spectra<F>), dim3(blocks), dim3(G::kBlock), sbd_smem<F>(), st, y,
-                     rows, m, tw, spec, norms, zero, mag);
+                     rows, m, tw, spec, norms, zero, mag, bmag);
   return hipGetLastError();
 }
 
@@ -716,28 +823,28 @@ static hipError_t launch_prune_F(const PruneArgs& a, hipStream_t st) {
   long blocks = (a.npairs + G::kGroups - 1) / G::kGroups;
   if (blocks < 1) blocks = 1;
   if (blocks > 8192) blocks = 8192;
-  if (a.xs)
+  if (a.xs && a.ub)
     hipLaunchKernelGGL((k_assign_prune<F, true>), dim3((unsigned)blocks), dim3(G::kBlock),
-                       prune_smem<F>(a.k), st, a.x, a.xs, a.xn, a.n, a.npairs, a.m, a.k, a.cspec,
-                       a.cmag, a.cnorm, a.tw, a.rev, a.order, a.label, a.dist, a.shift, a.changed,
-                       a.evals);
+                       prune_smem<F>(a.k), st, a.x, a.xs, a.xn, a.ub, a.ub_ld, a.n, a.npairs, a.m,
+                       a.k, a.cspec, a.cmag, a.cnorm, a.tw, a.rev, a.order, a.label, a.dist,
+                       a.shift, a.changed, a.evals);
   else
     hipLaunchKernelGGL((k_assign_prune<F, false>), dim3((unsigned)blocks), dim3(G::kBlock),
-                       prune_smem<F>(a.k), st, a.x, a.xs, a.xn, a.n, a.npairs, a.m, a.k, a.cspec,
-                       a.cmag, a.cnorm, a.tw, a.rev, a.order, a.label, a.dist, a.shift, a.changed,
-                       a.evals);
+                       prune_smem<F>(a.k), st, a.x, nullptr, nullptr, nullptr, 0, a.n, a.npairs,
+                       a.m, a.k, a.cspec, a.cmag, a.cnorm, a.tw, a.rev, a.order, a.label, a.dist,
+                       a.shift, a.changed, a.evals);
   return hipGetLastError();
 }
 
 template <int F>
 static hipError_t launch_series_spec_F(const float* x, long n, int m, const float2* tw,
-                                       float2* xs, float* xn, hipStream_t st) {
+                                       float2* xs, float* xn, float* xm, hipStream_t st) {
   using G = FftGeom<F>;
   long blocks = ((n + 1) / 2 + G::kGroups - 1) / G::kGroups;
   if (blocks < 1) blocks = 1;
   if (blocks > 8192) blocks = 8192;
   hipLaunchKernelGGL((k_series_spec<F>), dim3((unsigned)blocks), dim3(G::kBlock), sbd_smem<F>(),
-                     st, x, n, m, tw, xs, xn);
+                     st, x, n, m, tw, xs, xn, xm);
   return hipGetLastError();
 }
 
@@ -764,11 +871,12 @@ void fft_twiddle_table(int F, float2* out) {
   X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
 
 hipError_t launch_spectra(int F, const float* y, int rows, int m, const float2* tw,
-                          float2* spec, float* norms, int* zero, float* mag, hipStream_t st) {
+                          float2* spec, float* norms, int* zero, float* mag, float* bmag,
+                          hipStream_t st) {
   switch (F) {
 #define X(FF) \
   case FF:    \
-    return launch_spectra_F<FF>(y, rows, m, tw, spec, norms, zero, mag, st);
+    return launch_spectra_F<FF>(y, rows, m, tw, spec, norms, zero, mag, bmag, st);
     TCX_FOR_EACH_F(X)
 #undef X
     default:
@@ -789,17 +897,35 @@ int prune_max_k(int F) {
 }
 
 hipError_t launch_series_spec(int F, const float* x, long n, int m, const float2* tw, float2* xs,
-                              float* xn, hipStream_t st) {
+                              float* xn, float* xm, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   switch (F) {
 #define X(FF) \
   case FF:    \
-    return launch_series_spec_F<FF>(x, n, m, tw, xs, xn, st);
+    return launch_series_spec_F<FF>(x, n, m, tw, xs, xn, xm, st);
     TCX_FOR_EACH_F(X)
 #undef X
     default:
       return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_bounds(int F, const float* xm, long n, const float* cm, int k, float* ub,
+                         int ld, hipStream_t st) {
+  if (n <= 0 || k <= 0) return hipSuccess;
+  if (ld < k || F < 2) return hipErrorInvalidValue;
+  const long blocks = (n + 63) / 64;
+  const int kpad = bound_kpad(F);
+  if (k <= 16)
+    hipLaunchKernelGGL((k_bounds<1, 8>), dim3((unsigned)blocks), dim3(256), 0, st, xm, n, kpad, cm,
+                       k, ld, ub);
+  else if (k <= 32)
+    hipLaunchKernelGGL((k_bounds<2, 4>), dim3((unsigned)blocks), dim3(256), 0, st, xm, n, kpad, cm,
+                       k, ld, ub);
+  else
+    hipLaunchKernelGGL((k_bounds<4, 2>), dim3((unsigned)blocks), dim3(256), 0, st, xm, n, kpad, cm,
+                       k, ld, ub);
+  return hipGetLastError();
 }
 
 hipError_t launch_assign_prune(int F, const PruneArgs& a, hipStream_t st) {

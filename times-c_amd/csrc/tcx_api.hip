@@ -53,7 +53,7 @@ enum Slot {
   kSlotS, kSlotInfo, kSlotDummy, kSlotWbuf, kSlotRbuf, kSlotBar, kSlotVout, kSlotSteps,
   kSlotConv, kSlotErr, kSlotCmag, kSlotEvals, kSlotSvout, kSlotLzW, kSlotLzState, kSlotMask,
   kSlotRowStats, kSlotFitRowStats, kSlotKeyCounts, kSlotKeyOff, kSlotMinRow, kSlotXs, kSlotXn,
-  kSlotLzXch, kNumSlots
+  kSlotLzXch, kSlotXm, kSlotUb, kSlotCm, kNumSlots
 };
 
 struct FitState {
@@ -73,6 +73,10 @@ struct FitState {
   bool have_dist = false;  // dist holds an assignment's distances (orders the next grouping)
   float2* xs = nullptr;    // half spectra of the series (per fit; NULL: transformed per call)
   float* xn = nullptr;     // their norms
+  float* xm = nullptr;     // their bound rows (with xs)
+  float* ub = nullptr;     // bounds of the current assignment [n][ub_ld] (with xs)
+  int ub_ld = 0;
+  float* cm = nullptr;     // centroid bound rows, fragment order (with xs)
   int *label = nullptr, *shift = nullptr, *changed = nullptr, *zero = nullptr;
   float *dist = nullptr, *cent = nullptr, *norm = nullptr;
   float2* spec = nullptr;
@@ -199,7 +203,7 @@ int check_series(int n, int m) {
 
 // Spectra, spectrum magnitudes, norms and all-zero flags of k rows.
 int spectra(tcx_ctx* c, const float* d_rows, int rows, int m, float2** spec, float** norm,
-            int** zero) {
+            int** zero, float* bmag = nullptr) {
   int F = fft_size(m);
   const float2* tw;
   float* mag;
@@ -208,7 +212,8 @@ int spectra(tcx_ctx* c, const float* d_rows, int rows, int m, float2** spec, flo
   TCX_TRY(ws(c, kSlotCmag, (size_t)rows * F, &mag));
   TCX_TRY(ws(c, kSlotCnorm, (size_t)rows, norm));
   TCX_TRY(ws(c, kSlotCzero, (size_t)rows, zero));
-  TCX_HIP(tcx::launch_spectra(F, d_rows, rows, m, tw, *spec, *norm, *zero, mag, c->stream));
+  TCX_HIP(tcx::launch_spectra(F, d_rows, rows, m, tw, *spec, *norm, *zero, mag, bmag,
+                              c->stream));
   return TCX_OK;
 }
 
@@ -218,7 +223,8 @@ int assign_core(tcx_ctx* c, const float* d_x, long n, int m, int k, const float2
                 const float* cnorm, int rev, int* label, float* dist, int* shift, float* dmat,
                 int* changed, const int* order = nullptr, long npairs = 0,
                 unsigned long long* evals = nullptr, const float* cmag = nullptr,
-                const float2* xs = nullptr, const float* xn = nullptr) {
+                const float2* xs = nullptr, const float* xn = nullptr,
+                const float* ub = nullptr, int ub_ld = 0) {
   int F = fft_size(m);
   const float2* tw;
   TCX_TRY(twiddles(c, F, &tw));
@@ -232,6 +238,8 @@ int assign_core(tcx_ctx* c, const float* d_x, long n, int m, int k, const float2
     p.evals = evals;
     p.xs = xs;
     p.xn = xn;
+    p.ub = ub;
+    p.ub_ld = ub_ld;
     TCX_HIP(tcx::launch_assign_prune(F, p, c->stream));
     return TCX_OK;
   }
@@ -580,7 +588,7 @@ int model_spectra(tcx_ctx* c, const float* d_cent, int k, int m, void* spec, flo
   const int F = fft_size(m);
   const float2* tw;
   TCX_TRY(twiddles(c, F, &tw));
-  TCX_HIP(launch_spectra(F, d_cent, k, m, tw, static_cast<float2*>(spec), norm, zero, mag,
+  TCX_HIP(launch_spectra(F, d_cent, k, m, tw, static_cast<float2*>(spec), norm, zero, mag, nullptr,
                          c->stream));
   return TCX_OK;
 }
@@ -861,19 +869,31 @@ int tcx_fit_begin(tcx_ctx* c, const float* d_x, int n, int m, int k, const tcx_f
   TCX_TRY(ws(c, kSlotDist, (size_t)n, &f.dist));
   f.have_dist = false;
   // The series do not change during the fit: transform them once for the
-  // pruned assignment (n * (F/2 + 1) complex values) when the budget allows.
+  // pruned assignment (n * (F/2 + 1) complex values) and keep their bound
+  // rows, so an assignment forms all n * k bounds as one product, when the
+  // budget allows.
   f.xs = nullptr;
   f.xn = nullptr;
+  f.xm = nullptr;
+  f.ub = nullptr;
+  f.cm = nullptr;
   {
     const int F = fft_size(m);
-    const size_t need = sizeof(float2) * (size_t)n * (F / 2 + 1) + sizeof(float) * (size_t)n;
+    const int kp = (k + 15) & ~15;
+    const size_t need = sizeof(float2) * (size_t)n * (F / 2 + 1) + sizeof(float) * (size_t)n +
+                        sizeof(float) * (size_t)tcx::bound_rows(n) * tcx::bound_kpad(F) +
+                        sizeof(float) * (size_t)n * kp;
     const bool fits = !c->budget || c->ws_bytes + c->user_bytes + need <= c->budget;
     if (k <= tcx::prune_max_k(F) && fits) {
       const float2* tw;
       TCX_TRY(twiddles(c, F, &tw));
       TCX_TRY(ws(c, kSlotXs, (size_t)n * (F / 2 + 1), &f.xs));
       TCX_TRY(ws(c, kSlotXn, (size_t)n, &f.xn));
-      TCX_HIP(tcx::launch_series_spec(F, d_x, n, m, tw, f.xs, f.xn, st));
+      TCX_TRY(ws(c, kSlotXm, (size_t)tcx::bound_rows(n) * tcx::bound_kpad(F), &f.xm));
+      TCX_TRY(ws(c, kSlotUb, (size_t)n * kp, &f.ub));
+      TCX_TRY(ws(c, kSlotCm, (size_t)tcx::bound_rows(k) * tcx::bound_kpad(F), &f.cm));
+      f.ub_ld = kp;
+      TCX_HIP(tcx::launch_series_spec(F, d_x, n, m, tw, f.xs, f.xn, f.xm, st));
     }
   }
   TCX_TRY(ws(c, kSlotCent, (size_t)k * m, &f.cent));
@@ -909,7 +929,7 @@ int tcx_fit_begin(tcx_ctx* c, const float* d_x, int n, int m, int k, const tcx_f
                              sizeof(float) * m, hipMemcpyDeviceToDevice, st));
     TCX_HIP(hipMemsetAsync(f.label, 0, sizeof(int) * n, st));
   }
-  TCX_TRY(spectra(c, f.cent, k, m, &f.spec, &f.norm, &f.zero));
+  TCX_TRY(spectra(c, f.cent, k, m, &f.spec, &f.norm, &f.zero, f.cm));
   if (o.init_mode == 0) TCX_HIP(tcx::launch_fill_int(f.zero, k, 1, st));  // centroids start at 0
   f.row_stats = nullptr;
   if (tcx::align_pack_uses_row_stats(m)) {
@@ -945,7 +965,7 @@ int fit_refine(tcx_ctx* c, FitState& f) {
                  f.split, f.tol, f.o.cal_select, f.row_stats, &rr,
                  f.have_dist ? f.dist : nullptr));
   TCX_HIP(hipMemcpyAsync(f.cent, cnew, sizeof(float) * f.k * f.m, hipMemcpyDeviceToDevice, st));
-  TCX_TRY(spectra(c, f.cent, f.k, f.m, &f.spec, &f.norm, &f.zero));
+  TCX_TRY(spectra(c, f.cent, f.k, f.m, &f.spec, &f.norm, &f.zero, f.cm));
   TCX_HIP(hipEventRecord(c->ev[1], st));
   f.ref_timed = true;  // read at the next synchronization (fit_assign)
   f.sum_class = rr.sum_class;
@@ -964,9 +984,10 @@ int fit_assign(tcx_ctx* c, FitState& f, int* h_changed) {
   TCX_HIP(tcx::launch_fill_int(reinterpret_cast<int*>(evals), 2, 0, st));
   TCX_HIP(hipEventRecord(c->ev[2], st));
   const int* order = f.have_order ? reinterpret_cast<const int*>(c->slot_ptr[kSlotPerm]) : nullptr;
+  if (f.xs) TCX_HIP(tcx::launch_bounds(fft_size(f.m), f.xm, f.n, f.cm, f.k, f.ub, f.ub_ld, st));
   TCX_TRY(assign_core(c, f.d_x, f.n, f.m, f.k, f.spec, f.norm, f.o.convention, f.label, f.dist,
                       f.shift, nullptr, f.changed, order, f.n_pad / 2, evals, nullptr, f.xs,
-                      f.xn));
+                      f.xn, f.ub, f.ub_ld));
   f.have_dist = true;
   TCX_HIP(hipEventRecord(c->ev[3], st));
   void* pin;

@@ -32,18 +32,37 @@ constexpr int kMaxFft = 8192;
 hipError_t launch_znorm(float* x, long rows, int m, int ddof, hipStream_t st);
 // Twiddle table (F entries) of the length-F transform, host side.
 void fft_twiddle_table(int F, float2* out);
+// bmag (may be NULL): rows (bound_rows(rows) x bound_kpad(F), fragment
+// order) of |Y[f]| / |y| for f <= F/2, zero padded (the centroid operand
+// of launch_bounds).
 hipError_t launch_spectra(int F, const float* y, int rows, int m, const float2* tw,
-                          float2* spec, float* norms, int* zero, float* mag, hipStream_t st);
+                          float2* spec, float* norms, int* zero, float* mag, float* bmag,
+                          hipStream_t st);
 
 // Exact pruned assignment (bound-and-prune on the spectra), k <= kPruneMaxK.
 constexpr int kPruneMaxK = 1024;
 constexpr int kLdsBytes = 160 * 1024;  // LDS of one workgroup on gfx950
 // Largest k the pruned assignment takes at FFT length F (LDS-bound).
 int prune_max_k(int F);
+// Columns of a series' bound row (launch_series_spec's xm): F/2+1
+// magnitudes padded to a multiple of 16 (the bound product's K step).
+__host__ __device__ constexpr int bound_kpad(int F) { return ((F / 2 + 1) + 15) & ~15; }
+// Bound rows are stored in MFMA fragment order: the 16 x 16 block of rows
+// 16 b .. 16 b + 15 and columns 16 s .. 16 s + 15 is 256 consecutive floats,
+// element (r, c) at 4 (r + 16 (c / 4)) + c % 4, so that lane l of a wave
+// reads its operand (row l % 16, columns 4 (l / 16) .. + 3) as the l-th
+// 16-byte vector of the block.  Rows are padded to a multiple of 16.
+__host__ __device__ constexpr long bound_frag(long row, int f, int kpad) {
+  return (((row >> 4) * (kpad >> 4) + (f >> 4)) * 64 + (row & 15) + 16 * ((f & 15) >> 2)) * 4 +
+         (f & 3);
+}
+__host__ __device__ constexpr long bound_rows(long rows) { return (rows + 15) & ~15L; }
 struct PruneArgs {
   const float* x = nullptr;
   const float2* xs = nullptr;    // per-series half spectra [n][F/2+1] (may be NULL: from x)
   const float* xn = nullptr;     // per-series norms (with xs)
+  const float* ub = nullptr;     // per-series bounds [n][ub_ld] of launch_bounds (with xs)
+  int ub_ld = 0;
   long n = 0;
   long npairs = 0;               // pairs of order[] (or ceil(n/2) without order)
   int m = 0, k = 0;
@@ -60,9 +79,16 @@ struct PruneArgs {
   unsigned long long* evals = nullptr;  // inverse transforms run (may be NULL)
 };
 hipError_t launch_assign_prune(int F, const PruneArgs& a, hipStream_t st);
-// Half spectra X_r[0 .. F/2] of every row and the row norms (k_series_spec).
+// Half spectra X_r[0 .. F/2] of every row and the row norms (k_series_spec);
+// with xm (may be NULL) also the bound rows (bound_rows(n) x bound_kpad(F),
+// fragment order): weight_f |X_r[f]| / (F |x_r|), weight 2 for
+// 0 < f < F/2, zero padded.
 hipError_t launch_series_spec(int F, const float* x, long n, int m, const float2* tw, float2* xs,
-                              float* xn, hipStream_t st);
+                              float* xn, float* xm, hipStream_t st);
+// ub[i][j] = sum_f xm[i][f] cm[j][f] (cm: launch_spectra's bmag rows), i < n,
+// j < k, row stride ld >= k: the pruned assignment's bounds (k_bounds).
+hipError_t launch_bounds(int F, const float* xm, long n, const float* cm, int k, float* ub,
+                         int ld, hipStream_t st);
 hipError_t launch_sbd(int F, int mode, const SbdArgs& a, hipStream_t st);
 hipError_t launch_apply_shift(float* x, long rows, int m, const int* shift, const int* skip,
                               hipStream_t st);
