@@ -153,6 +153,8 @@ typedef struct {
   long long sbd_evals;   /* inverse transforms run by the pruned assignment
                             (each covers up to two series x one centroid)   */
   long long sbd_pairs;   /* series x centroid pairs assigned                */
+  double ms_assign_bounds; /* part of ms_assign spent forming the pruning
+                              bounds of all series (HIP events)              */
 } tcx_fit_stats;
 
 /* Fill opts with defaults (max_iter 10, seed 0, CPU convention,

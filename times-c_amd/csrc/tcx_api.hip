@@ -68,7 +68,7 @@ struct FitState {
   bool have_order = false;   // the refine's cluster-sorted rows are valid
   const double2* row_stats = nullptr;  // per-row sums of d_x (fixed during the fit)
   long long evals = 0, pairs = 0;
-  float ms_ref = 0.f, ms_asg = 0.f, ms_asg_last = 0.f;
+  float ms_ref = 0.f, ms_asg = 0.f, ms_asg_last = 0.f, ms_bnd = 0.f;
   bool ref_timed = false;  // events 0..1 hold a refine not yet added to ms_ref
   bool have_dist = false;  // dist holds an assignment's distances (orders the next grouping)
   float2* xs = nullptr;    // half spectra of the series (per fit; NULL: transformed per call)
@@ -94,7 +94,7 @@ struct tcx_ctx {
   size_t ws_bytes = 0;
   size_t user_bytes = 0;
   std::map<void*, size_t> user_allocs;
-  hipEvent_t ev[4] = {};
+  hipEvent_t ev[5] = {};  // refine (0, 1), assignment (2, 3), bounds end (4)
   hipEvent_t coop_ev = nullptr;  // after this context's last multi-block Lanczos launch
   void* lz_xch = nullptr;        // exchange buffer (and its size) the generation below refers to
   size_t lz_xch_bytes = 0;
@@ -985,6 +985,7 @@ int fit_assign(tcx_ctx* c, FitState& f, int* h_changed) {
   TCX_HIP(hipEventRecord(c->ev[2], st));
   const int* order = f.have_order ? reinterpret_cast<const int*>(c->slot_ptr[kSlotPerm]) : nullptr;
   if (f.xs) TCX_HIP(tcx::launch_bounds(fft_size(f.m), f.xm, f.n, f.cm, f.k, f.ub, f.ub_ld, st));
+  TCX_HIP(hipEventRecord(c->ev[4], st));
   TCX_TRY(assign_core(c, f.d_x, f.n, f.m, f.k, f.spec, f.norm, f.o.convention, f.label, f.dist,
                       f.shift, nullptr, f.changed, order, f.n_pad / 2, evals, nullptr, f.xs,
                       f.xn, f.ub, f.ub_ld));
@@ -1011,6 +1012,8 @@ int fit_assign(tcx_ctx* c, FitState& f, int* h_changed) {
   TCX_HIP(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
   f.ms_asg += ms;
   f.ms_asg_last = ms;
+  TCX_HIP(hipEventElapsedTime(&ms, c->ev[2], c->ev[4]));
+  f.ms_bnd += ms;
   f.evals += (long long)h_evals;
   f.pairs += (long long)f.n * f.k;
   return TCX_OK;
@@ -1066,6 +1069,7 @@ int tcx_fit_query(tcx_ctx* c, tcx_fit_stats* stats) {
   stats->fallbacks = f.fallbacks;
   stats->ms_refine = f.ms_ref;
   stats->ms_assign = f.ms_asg;
+  stats->ms_assign_bounds = f.ms_bnd;
   stats->sbd_evals = f.evals;
   stats->sbd_pairs = f.pairs;
   stats->ms_total = std::chrono::duration<double, std::milli>(
@@ -1093,6 +1097,7 @@ int tcx_fit_end(tcx_ctx* c, int* h_labels, float* h_centroids, tcx_fit_stats* st
     stats->fallbacks = f.fallbacks;
     stats->ms_refine = f.ms_ref;
     stats->ms_assign = f.ms_asg;
+    stats->ms_assign_bounds = f.ms_bnd;
     stats->sbd_evals = f.evals;
     stats->sbd_pairs = f.pairs;
     stats->ms_total = std::chrono::duration<double, std::milli>(

@@ -70,6 +70,7 @@ class FitStats(ctypes.Structure):
         ("ms_assign", ctypes.c_double),
         ("sbd_evals", ctypes.c_longlong),
         ("sbd_pairs", ctypes.c_longlong),
+        ("ms_assign_bounds", ctypes.c_double),
     ]
 
     def as_dict(self):
