@@ -48,8 +48,8 @@ using tcx::fail;
 // Workspace slots (grow-only device buffers held by the context).
 enum Slot {
   kSlotCspec, kSlotCnorm, kSlotCzero, kSlotLabel, kSlotShift, kSlotDist, kSlotCent,
-  kSlotCentNew, kSlotChanged, kSlotBlkCount, kSlotBlkBase, kSlotCounts, kSlotOffsets,
-  kSlotPerm, kSlotBad, kSlotYhi, kSlotYlo, kSlotA0, kSlotItems, kSlotTiles, kSlotPart,
+  kSlotCentNew, kSlotBlkCount, kSlotBlkBase, kSlotGroupInfo,
+  kSlotPerm, kSlotYhi, kSlotYlo, kSlotA0, kSlotPlan, kSlotPart,
   kSlotS, kSlotInfo, kSlotDummy, kSlotWbuf, kSlotRbuf, kSlotBar, kSlotVout, kSlotSteps,
   kSlotConv, kSlotErr, kSlotCmag, kSlotEvals, kSlotSvout, kSlotLzW, kSlotLzState, kSlotMask,
   kSlotRowStats, kSlotFitRowStats, kSlotKeyCounts, kSlotKeyOff, kSlotMinRow, kSlotXs, kSlotXn,
@@ -77,7 +77,7 @@ struct FitState {
   float* ub = nullptr;     // bounds of the current assignment [n][ub_ld] (with xs)
   int ub_ld = 0;
   float* cm = nullptr;     // centroid bound rows, fragment order (with xs)
-  int *label = nullptr, *shift = nullptr, *changed = nullptr, *zero = nullptr;
+  int *label = nullptr, *shift = nullptr, *zero = nullptr;
   float *dist = nullptr, *cent = nullptr, *norm = nullptr;
   float2* spec = nullptr;
   std::chrono::steady_clock::time_point t_start;
@@ -102,6 +102,7 @@ struct tcx_ctx {
   // host copies kept alive until the stream has consumed them
   std::vector<tcx::SyrkItem> items;
   std::vector<tcx::SyrkTile> tiles;
+  std::vector<unsigned char> plan;  // items and tiles, uploaded as one copy
   int cus = 0;                 // compute units of the device
   long long wall_khz = 0;      // wall_clock64() rate
   void* pinned = nullptr;      // page-locked host area for small readbacks
@@ -456,12 +457,17 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
   long *offsets, *koff;
   TCX_TRY(ws(c, kSlotBlkCount, (size_t)(nblk * K), &blk_count));
   TCX_TRY(ws(c, kSlotBlkBase, (size_t)(nblk * K), &blk_base));
-  TCX_TRY(ws(c, kSlotCounts, (size_t)k, &counts));
+  // cluster offsets (k + 1), counts (k) and the bad-label flag in one
+  // buffer, in the layout of the pinned readback below (one copy)
+  const size_t ginfo_bytes = sizeof(long) * (k + 1) + sizeof(int) * (k + 2);
+  unsigned char* ginfo;
+  TCX_TRY(ws(c, kSlotGroupInfo, ginfo_bytes, &ginfo));
+  offsets = reinterpret_cast<long*>(ginfo);
+  counts = reinterpret_cast<int*>(offsets + k + 1);
+  bad = counts + k;
   TCX_TRY(ws(c, kSlotKeyCounts, (size_t)K, &kcounts));
-  TCX_TRY(ws(c, kSlotOffsets, (size_t)k + 1, &offsets));
   TCX_TRY(ws(c, kSlotKeyOff, (size_t)K, &koff));
   TCX_TRY(ws(c, kSlotMinRow, (size_t)k, &minrow));
-  TCX_TRY(ws(c, kSlotBad, 1, &bad));
   TCX_HIP(tcx::launch_fill_int(bad, 1, 0, st));
   TCX_HIP(tcx::launch_fill_int(minrow, k, 0x7fffffff, st));
   TCX_HIP(tcx::launch_group_count(d_label, n, k, d_dist, B, blk_count, minrow, bad, st));
@@ -485,13 +491,11 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
   TCX_HIP(tcx::launch_align_pack(d_x, m, perm, n_pad_max, offsets, minrow, k, d_shift, d_czero,
                                  row_stats, yhi, ylo, a0, st));
   void* pin;
-  TCX_TRY(pinned_area(c, sizeof(long) * (k + 1) + sizeof(int) * (k + 2), &pin));
+  TCX_TRY(pinned_area(c, ginfo_bytes, &pin));
   long* p_off = static_cast<long*>(pin);
   int* p_counts = reinterpret_cast<int*>(p_off + k + 1);
   int* p_bad = p_counts + k;
-  TCX_HIP(hipMemcpyAsync(p_counts, counts, sizeof(int) * k, hipMemcpyDeviceToHost, st));
-  TCX_HIP(hipMemcpyAsync(p_off, offsets, sizeof(long) * (k + 1), hipMemcpyDeviceToHost, st));
-  TCX_HIP(hipMemcpyAsync(p_bad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+  TCX_HIP(hipMemcpyAsync(pin, ginfo, ginfo_bytes, hipMemcpyDeviceToHost, st));
   TCX_HIP(hipStreamSynchronize(st));
   // out-of-range labels were left out of the grouping above
   if (*p_bad) return fail(TCX_ERR_ARG, "label outside [0, k)");
@@ -503,21 +507,25 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
   std::vector<long> seg_len(k);
   for (int j = 0; j < k; ++j) seg_len[j] = h_counts[j] > 0 ? h_off[j + 1] - h_off[j] : 0;
   plan_syrk(m, k, h_off.data(), seg_len.data(), c->cus, c->items, c->tiles);
-  tcx::SyrkItem* d_items;
-  tcx::SyrkTile* d_tiles;
+  // items and tiles in one buffer, uploaded by one copy
+  const size_t items_bytes = (sizeof(tcx::SyrkItem) * c->items.size() + 255) & ~(size_t)255;
+  const size_t plan_bytes = items_bytes + sizeof(tcx::SyrkTile) * c->tiles.size();
+  unsigned char* d_plan;
+  TCX_TRY(ws(c, kSlotPlan, plan_bytes, &d_plan));
+  tcx::SyrkItem* d_items = reinterpret_cast<tcx::SyrkItem*>(d_plan);
+  tcx::SyrkTile* d_tiles = reinterpret_cast<tcx::SyrkTile*>(d_plan + items_bytes);
   float *part, *S;
   tcx::PowerInfo* info;
-  TCX_TRY(ws(c, kSlotItems, c->items.size(), &d_items));
-  TCX_TRY(ws(c, kSlotTiles, c->tiles.size(), &d_tiles));
   const size_t nparts = c->tiles.empty() ? 1 : c->tiles.back().part0 + c->tiles.back().nparts;
   TCX_TRY(ws(c, kSlotPart, nparts * tcx::kSyrkTile * tcx::kSyrkTile, &part));
   TCX_TRY(ws(c, kSlotS, (size_t)k * m * m, &S));
   TCX_TRY(ws(c, kSlotInfo, (size_t)k, &info));
   if (!c->items.empty()) {
-    TCX_HIP(hipMemcpyAsync(d_items, c->items.data(), sizeof(tcx::SyrkItem) * c->items.size(),
-                           hipMemcpyHostToDevice, st));
-    TCX_HIP(hipMemcpyAsync(d_tiles, c->tiles.data(), sizeof(tcx::SyrkTile) * c->tiles.size(),
-                           hipMemcpyHostToDevice, st));
+    c->plan.resize(plan_bytes);
+    std::memcpy(c->plan.data(), c->items.data(), sizeof(tcx::SyrkItem) * c->items.size());
+    std::memcpy(c->plan.data() + items_bytes, c->tiles.data(),
+                sizeof(tcx::SyrkTile) * c->tiles.size());
+    TCX_HIP(hipMemcpyAsync(d_plan, c->plan.data(), plan_bytes, hipMemcpyHostToDevice, st));
   }
   TCX_HIP(tcx::launch_syrk(d_items, (int)c->items.size(), yhi, ylo, m, split, part, st));
   TCX_HIP(tcx::launch_syrk_reduce(d_tiles, (int)c->tiles.size(), part, m, S, st));
@@ -897,7 +905,6 @@ int tcx_fit_begin(tcx_ctx* c, const float* d_x, int n, int m, int k, const tcx_f
     }
   }
   TCX_TRY(ws(c, kSlotCent, (size_t)k * m, &f.cent));
-  TCX_TRY(ws(c, kSlotChanged, 1, &f.changed));
   std::vector<int> init;
   if (o.init_mode == 0) {
     init.resize(n);
@@ -964,7 +971,10 @@ int fit_refine(tcx_ctx* c, FitState& f) {
   TCX_TRY(refine(c, f.d_x, f.n, f.m, f.k, f.label, f.shift, f.zero, cnew, f.it, f.o.seed,
                  f.split, f.tol, f.o.cal_select, f.row_stats, &rr,
                  f.have_dist ? f.dist : nullptr));
-  TCX_HIP(hipMemcpyAsync(f.cent, cnew, sizeof(float) * f.k * f.m, hipMemcpyDeviceToDevice, st));
+  // the new centroids become the current ones: the two buffers swap roles
+  std::swap(c->slot_ptr[kSlotCent], c->slot_ptr[kSlotCentNew]);
+  std::swap(c->slot_bytes[kSlotCent], c->slot_bytes[kSlotCentNew]);
+  f.cent = cnew;
   TCX_TRY(spectra(c, f.cent, f.k, f.m, &f.spec, &f.norm, &f.zero, f.cm));
   TCX_HIP(hipEventRecord(c->ev[1], st));
   f.ref_timed = true;  // read at the next synchronization (fit_assign)
@@ -978,25 +988,29 @@ int fit_refine(tcx_ctx* c, FitState& f) {
 
 int fit_assign(tcx_ctx* c, FitState& f, int* h_changed) {
   hipStream_t st = c->stream;
-  TCX_HIP(tcx::launch_fill_int(f.changed, 1, 0, st));
+  // the evaluation count and the labels-changed flag in one buffer (one
+  // fill, one readback), on different 128-byte lines: the count takes an
+  // atomic add per block, the flag plain stores
+  constexpr int kChangedAt = 128 / sizeof(unsigned long long);
   unsigned long long* evals;
-  TCX_TRY(ws(c, kSlotEvals, 1, &evals));
-  TCX_HIP(tcx::launch_fill_int(reinterpret_cast<int*>(evals), 2, 0, st));
+  TCX_TRY(ws(c, kSlotEvals, kChangedAt + 1, &evals));
+  int* changed = reinterpret_cast<int*>(evals + kChangedAt);
+  TCX_HIP(tcx::launch_fill_int(reinterpret_cast<int*>(evals), 2 * (kChangedAt + 1), 0, st));
   TCX_HIP(hipEventRecord(c->ev[2], st));
   const int* order = f.have_order ? reinterpret_cast<const int*>(c->slot_ptr[kSlotPerm]) : nullptr;
   if (f.xs) TCX_HIP(tcx::launch_bounds(fft_size(f.m), f.xm, f.n, f.cm, f.k, f.ub, f.ub_ld, st));
   TCX_HIP(hipEventRecord(c->ev[4], st));
   TCX_TRY(assign_core(c, f.d_x, f.n, f.m, f.k, f.spec, f.norm, f.o.convention, f.label, f.dist,
-                      f.shift, nullptr, f.changed, order, f.n_pad / 2, evals, nullptr, f.xs,
+                      f.shift, nullptr, changed, order, f.n_pad / 2, evals, nullptr, f.xs,
                       f.xn, f.ub, f.ub_ld));
   f.have_dist = true;
   TCX_HIP(hipEventRecord(c->ev[3], st));
   void* pin;
-  TCX_TRY(pinned_area(c, 2 * sizeof(unsigned long long), &pin));
+  TCX_TRY(pinned_area(c, sizeof(unsigned long long) * (kChangedAt + 1), &pin));
   unsigned long long* p_evals = static_cast<unsigned long long*>(pin);
-  int* p_changed = reinterpret_cast<int*>(p_evals + 1);
-  TCX_HIP(hipMemcpyAsync(p_evals, evals, sizeof(*p_evals), hipMemcpyDeviceToHost, st));
-  TCX_HIP(hipMemcpyAsync(p_changed, f.changed, sizeof(int), hipMemcpyDeviceToHost, st));
+  int* p_changed = reinterpret_cast<int*>(p_evals + kChangedAt);
+  TCX_HIP(hipMemcpyAsync(p_evals, evals, sizeof(*p_evals) * kChangedAt + sizeof(int),
+                         hipMemcpyDeviceToHost, st));
   if (f.o.labels_per_iter && f.it < f.o.max_iter)
     TCX_HIP(hipMemcpyAsync(f.o.labels_per_iter + (size_t)f.it * f.n, f.label,
                            sizeof(int) * f.n, hipMemcpyDeviceToHost, st));
