@@ -5,6 +5,7 @@
 // Semantics: _preSBD_extract_shape + _extract_shape,
 // Code/test/test_timesC.py:127-204 (grouping order :228-243).
 #include <cfloat>
+#include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -550,41 +551,47 @@ __global__ __launch_bounds__(256) void k_align_pack_lds(
   // panel chunks (R/16 column chunks x tm row chunks), 16 bytes per lane:
   // lane ln packs columns qb..qb+7 (their parameters held in registers) at
   // t = 32 * r32 + ln % 32 for the row chunks r32 = w, w+4, ...
+  // y = ((a - mean_hi) - mean_lo) * isd in FP32: |y| <= sqrt(m - 1) for a
+  // row z-normalized with ddof 1, so nothing overflows; rows whose mean or
+  // 1/sd is not finite (constant rows, NaN input) get mean = isd = 0 and
+  // pack as zeros (nan_to_num of their zscore).  The BF16 halves are
+  // rounded to nearest even by the hardware conversion.
   const int tm = panel_blocks(m);
 #pragma unroll 1
   for (int cc = 0; cc < R / 16; ++cc) {
     const int qb = cc * 16 + 8 * (lane >> 5);
-    int c_row[8], c_sh[8];
-    double c_mean[8], c_isd[8];
+    int c_sh[8];
+    float c_mh[8], c_ml[8], c_isd[8];
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
-      c_row[v] = s_row[qb + v];
+      const double mean = s_mean[qb + v], isd = s_isd[qb + v];
+      const bool ok = s_row[qb + v] >= 0 && isfinite(mean) && isfinite(isd);
       c_sh[v] = s_sh[qb + v];
-      c_mean[v] = s_mean[qb + v];
-      c_isd[v] = s_isd[qb + v];
+      c_mh[v] = ok ? (float)mean : 0.f;
+      c_ml[v] = ok ? (float)(mean - (double)c_mh[v]) : 0.f;
+      c_isd[v] = ok ? (float)isd : 0.f;
     }
     for (int r32 = wid; r32 < tm; r32 += 4) {
       const int t = r32 * 32 + (lane & 31);
+      const bool tin = t < m;
+      float y[8];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int src = t - c_sh[c];
+        const bool in = (unsigned)src < (unsigned)m;  // else the zero fill
+        const float a = rows_s[(qb + c) * ms + (in ? src : 0)];
+        const float d = ((in ? a : 0.f) - c_mh[c]) - c_ml[c];
+        y[c] = tin ? d * c_isd[c] : 0.f;
+      }
       uint32_t h[4], l[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        uint16_t hh[2], ll[2];
-#pragma unroll
-        for (int v = 0; v < 2; ++v) {
-          const int c = 2 * u + v;
-          float f = 0.f;
-          if (c_row[c] >= 0 && t < m) {
-            const int src = t - c_sh[c];
-            const double a = (src >= 0 && src < m) ? (double)rows_s[(qb + c) * ms + src] : 0.0;
-            const double y = (a - c_mean[c]) * c_isd[c];
-            f = isnan(y) ? 0.f
-                         : (isinf(y) ? (y > 0 ? 3.402823466e38f : -3.402823466e38f) : (float)y);
-          }
-          hh[v] = f2bf_rne(f);
-          ll[v] = f2bf_rne(f - bf2f(hh[v]));
-        }
-        h[u] = (uint32_t)hh[0] | ((uint32_t)hh[1] << 16);
-        l[u] = (uint32_t)ll[0] | ((uint32_t)ll[1] << 16);
+        const __hip_bfloat162 hb = __float22bfloat162_rn(make_float2(y[2 * u], y[2 * u + 1]));
+        const float2 hf = __bfloat1622float2(hb);
+        const __hip_bfloat162 lb =
+            __float22bfloat162_rn(make_float2(y[2 * u] - hf.x, y[2 * u + 1] - hf.y));
+        h[u] = *reinterpret_cast<const uint32_t*>(&hb);
+        l[u] = *reinterpret_cast<const uint32_t*>(&lb);
       }
       const long off = (((p0 >> 4) + cc) * (long)tm + r32) * 512 + lane * 8;
       *reinterpret_cast<uint4*>(yhi + off) = make_uint4(h[0], h[1], h[2], h[3]);
