@@ -1856,12 +1856,13 @@ __global__ __launch_bounds__(kPowerThreads) void k_lz_finish(
 // CU.  Step it (q = q_it held whole by every member):
 //   A  w = M q on the member's rows (M q = S q - (r 1^T q + 1 r^T q)/m +
 //      sigma (1^T q)/m^2 1); partial dots Q^T w and |w|^2 published
-//   B  c = Q^T w (partials summed in member order); w -= Q c; partial dots
-//      of the new w published
-//   C  d = Q^T w; w -= Q d (classical Gram-Schmidt twice against the whole
-//      basis); the member's rows of w and its |w|^2 published
-//   D  beta = |w|, q_{it+1} = w / beta assembled from every member's rows.
-// alpha_it = c_it + d_it.  Every member sums the same values in the same
+//   B  c = Q^T w (partials summed in member order); w -= Q c (classical
+//      Gram-Schmidt against the whole basis)
+//   C  only when B left less than kLzCoopReorth of |w|^2 (|c|^2 against
+//      |w|^2): partial dots of the new w published, d = Q^T w, w -= Q d
+//   D  the member's rows of w published; beta = |w|, q_{it+1} = w / beta
+//      assembled from every member's rows.
+// alpha_it = c_it (+ d_it).  Every member sums the same values in the same
 // order, so alpha, beta and the acceptance test of y = T^I e1 (run by every
 // member) agree bit for bit and all members leave the loop at the same step.
 //
@@ -1883,6 +1884,10 @@ constexpr int kLzCoopWaves = kLzCoopThreads / kWave;
 constexpr int kLzCoopLQ = kLzMax + 1;    // basis columns per row (odd: spreads LDS banks)
 constexpr int kLzCoopPart = kLzMax + 2;  // partials per member: dots l <= it, then |w|^2
 constexpr int kLzCoopEpochBits = 10;
+// The second Gram-Schmidt pass runs when the first left less than this
+// fraction of |w|^2: one pass leaves components of about eps |w| along the
+// basis, i.e. eps |w| / |w - Q c| <= 100 eps relative to the new vector.
+constexpr double kLzCoopReorth = 1e-4;
 
 __device__ __forceinline__ double coop_sum(double v, double* red) {
   v = wave_sum_f64(v);
@@ -2238,16 +2243,25 @@ __global__ __launch_bounds__(kLzCoopThreads) void k_lz_coop(
     for (int l = tid; l <= it + 1; l += kLzCoopThreads)
       good = good && poll_sum(P1 + 2L * l, kLzCoopPart, G, tag, fail, gen, timeout, cf[l]);
     if (!all_ok(good)) return give_up();
-    const double nw2 = cf[it + 1], a1 = cf[it];
+    const double nw2 = cf[it + 1];
+    double a = cf[it];
+    // |w - Q c|^2 = |w|^2 - |c|^2 over the orthonormal basis; the same sum
+    // in the same order in every thread and member, so all take the same
+    // branch
+    double cc = 0.0;
+    for (int l = 0; l <= it; ++l) cc += cf[l] * cf[l];
+    const bool again = !(nw2 - cc >= kLzCoopReorth * nw2);
     coop_project(Qr, cf, nr, it, tmp, w);
-    tag = tag0 | (uint32_t)(3 + 3 * it);
-    coop_partials(Qr, w, nr, it, tmp, P2 + 2L * b * kLzCoopPart, tag);
-    // C
-    for (int l = tid; l <= it + 1; l += kLzCoopThreads)
-      good = good && poll_sum(P2 + 2L * l, kLzCoopPart, G, tag, fail, gen, timeout, cf[l]);
-    if (!all_ok(good)) return give_up();
-    const double a = a1 + cf[it];
-    coop_project(Qr, cf, nr, it, tmp, w);
+    if (again) {
+      // C
+      tag = tag0 | (uint32_t)(3 + 3 * it);
+      coop_partials(Qr, w, nr, it, tmp, P2 + 2L * b * kLzCoopPart, tag);
+      for (int l = tid; l <= it + 1; l += kLzCoopThreads)
+        good = good && poll_sum(P2 + 2L * l, kLzCoopPart, G, tag, fail, gen, timeout, cf[l]);
+      if (!all_ok(good)) return give_up();
+      a += cf[it];
+      coop_project(Qr, cf, nr, it, tmp, w);
+    }
     tag = tag0 | (uint32_t)(4 + 3 * it);
     for (int e = tid; e < nr; e += kLzCoopThreads) put_tagged(Wx + 2L * (e0 + e), w[e], tag);
     // D: every member's rows of w; |w|^2 from them (the same block sum in
