@@ -57,7 +57,7 @@ template <int F>
 __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_spectra(
     const float* __restrict__ y, int rows, int m, const float2* __restrict__ twg,
     float2* __restrict__ spec, float* __restrict__ norms, int* __restrict__ zero,
-    float* __restrict__ mag, float* __restrict__ bmag) {
+    float* __restrict__ mag, uint16_t* __restrict__ bmag) {
   using G = FftGeom<F>;
   constexpr int L = G::L, P = G::P;
   extern __shared__ float2 smem[];
@@ -97,9 +97,10 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_spectra(
 #pragma unroll
       for (int s = 0; s < P; ++s) {
         const int f = lane + L * s;
-        if (f <= F / 2) bmag[bound_frag(row, f, KP)] = sqrtf(v[s].x * v[s].x + v[s].y * v[s].y) * rn;
+        if (f <= F / 2)
+          bmag[bound_frag(row, f, KP)] = f16_bits_up(sqrtf(v[s].x * v[s].x + v[s].y * v[s].y) * rn);
       }
-      for (int f = F / 2 + 1 + lane; f < KP; f += L) bmag[bound_frag(row, f, KP)] = 0.f;
+      for (int f = F / 2 + 1 + lane; f < KP; f += L) bmag[bound_frag(row, f, KP)] = 0;
     }
     if (lane == 0) {
       norms[row] = sqrtf(ss);
@@ -361,7 +362,7 @@ __device__ __forceinline__ void partner_half(const float2 (&zx)[P], float2* buf,
 template <int F>
 __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_series_spec(
     const float* __restrict__ x, long n, int m, const float2* __restrict__ twg,
-    float2* __restrict__ xs, float* __restrict__ xn, float* __restrict__ xm) {
+    float2* __restrict__ xs, float* __restrict__ xn, uint16_t* __restrict__ xm) {
   using G = FftGeom<F>;
   constexpr int L = G::L, P = G::P, H = P / 2;
   constexpr long HS = F / 2 + 1;
@@ -401,10 +402,11 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_series_spec(
     partner_half<F, L, P>(zx, buf, lane, px, py);
     float2* oa = xs + ia * HS;
     float2* ob = xs + ib * HS;
-    // bound rows: weight * |X_f| / (F |x|), weight 2 inside, 1 at f = 0, F/2
+    // bound rows: weight * |X_f| / |x|, weight 2 inside, 1 at f = 0, F/2
+    // (the 1/F is applied to the product)
     constexpr int KP = bound_kpad(F);
-    const float ra = sa > 0.f ? 1.f / ((float)F * sqrtf(sa)) : 0.f;
-    const float rb = sb > 0.f ? 1.f / ((float)F * sqrtf(sb)) : 0.f;
+    const float ra = sa > 0.f ? 1.f / sqrtf(sa) : 0.f;
+    const float rb = sb > 0.f ? 1.f / sqrtf(sb) : 0.f;
 #pragma unroll
     for (int s = 0; s <= H; ++s) {
       const int f = lane + L * s;
@@ -416,85 +418,95 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_series_spec(
         if (hasb) ob[fo] = B;
         if (xm) {
           const float w = (fo == 0 || fo == F / 2) ? 1.f : 2.f;
-          xm[bound_frag(ia, fo, KP)] = w * sqrtf(A.x * A.x + A.y * A.y) * ra;
-          if (hasb) xm[bound_frag(ib, fo, KP)] = w * sqrtf(B.x * B.x + B.y * B.y) * rb;
+          xm[bound_frag(ia, fo, KP)] = f16_bits_up(w * sqrtf(A.x * A.x + A.y * A.y) * ra);
+          if (hasb) xm[bound_frag(ib, fo, KP)] = f16_bits_up(w * sqrtf(B.x * B.x + B.y * B.y) * rb);
         }
       }
     }
     if (xm)
       for (int f = F / 2 + 1 + lane; f < KP; f += L) {  // padding of the K dimension
-        xm[bound_frag(ia, f, KP)] = 0.f;
-        if (hasb) xm[bound_frag(ib, f, KP)] = 0.f;
+        xm[bound_frag(ia, f, KP)] = 0;
+        if (hasb) xm[bound_frag(ib, f, KP)] = 0;
       }
   }
 }
 
 // ------------------------------------------ bounds of the pruned assignment
-// ub[i][j] = sum_f xm[i][f] cm[j][f] for every series i and centroid j:
-// the bounds k_assign_prune otherwise forms per pair, as one FP32 product
-// of the per-fit magnitude rows xm (k_series_spec) and the centroids' rows
-// cm = |C_j[f]| / |c_j| (k_spectra), both in fragment order (bound_frag),
-// on v_mfma_f32_16x16x4f32.  A wave computes 16 series x 16 centroids per
-// tile and NT tiles per pass over its rows of xm.  For the 16 columns of a
-// step lane l reads the l-th 16-byte vector of the fragment block of each
-// operand (one contiguous 1 KB per wave): row l % 16 of A and column l % 16
-// of B at columns 4 (l / 16) + c, c < 4, and MFMA number c takes component
-// c.  Result element (i, j) is in lane j + 16 (i / 4), register i % 4.
-// Rows past n (or centroids past k) in the last block are computed from
-// whatever the padding holds and never stored: a result element depends on
-// its own row and column only.
+// ub[i][j] = sum_f xm[i][f] cm[j][f] / F for every series i and centroid j:
+// the bounds k_assign_prune otherwise forms per pair, as one product of the
+// per-fit magnitude rows xm (k_series_spec) and the centroids' rows cm =
+// |C_j[f]| / |c_j| (k_spectra), both binary16 rounded up in fragment order
+// (bound_frag), on v_mfma_f32_16x16x16_f16 with FP32 accumulation (the
+// products of two binary16 values are exact in FP32).  A wave computes 16
+// series x 16 centroids per tile and NT tiles per pass over its rows of xm;
+// for the 16 columns of a step lane l reads the l-th 8-byte vector of the
+// fragment block of each operand (one contiguous 512 B per wave).  Result
+// element (i, j) is in lane j + 16 (i / 4), register i % 4.  Rows past n (or
+// centroids past k) in the last block are computed from whatever the
+// padding holds and never stored: a result element depends on its own row
+// and column only.
+typedef _Float16 bound_h4 __attribute__((ext_vector_type(4)));
 typedef float bound_f4 __attribute__((ext_vector_type(4)));
-template <int NT, int NS>
-__global__ __launch_bounds__(256) void k_bounds(const float* __restrict__ xm, long n, int kpad,
-                                                const float* __restrict__ cm, int k, int ld,
-                                                float* __restrict__ ub) {
+template <int NT, int MT, int NS>
+__global__ __launch_bounds__(256) void k_bounds(const uint16_t* __restrict__ xm, long n, int kpad,
+                                                const uint16_t* __restrict__ cm, int k, int ld,
+                                                float scale, float* __restrict__ ub) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const long blk = (long)blockIdx.x * 4 + wid;
-  const long i0 = blk * 16;
-  if (i0 >= n) return;
+  const long blk0 = ((long)blockIdx.x * 4 + wid) * MT;  // first 16-row block of the wave
+  if (blk0 * 16 >= n) return;
+  const long nblk = (n + 15) >> 4;
   const int r = lane & 15, kq = lane >> 4;
   const int nsteps = kpad >> 4;
-  const float4* xa = reinterpret_cast<const float4*>(xm) + blk * nsteps * 64 + lane;
+  const bound_h4* xa[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const long b = blk0 + mt < nblk ? blk0 + mt : blk0;  // blocks past n: computed, not stored
+    xa[mt] = reinterpret_cast<const bound_h4*>(xm) + b * nsteps * 64 + lane;
+  }
   for (int t0 = 0; t0 * 16 < k; t0 += NT) {
-    bound_f4 acc[NT];
-    const float4* cb[NT];
+    bound_f4 acc[MT][NT];
+    const bound_h4* cb[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       const int jt = (t0 + t) * 16 < k ? t0 + t : t0;  // tiles past k: computed, not stored
-      cb[t] = reinterpret_cast<const float4*>(cm) + (long)jt * nsteps * 64 + lane;
-      acc[t] = bound_f4{0.f, 0.f, 0.f, 0.f};
+      cb[t] = reinterpret_cast<const bound_h4*>(cm) + (long)jt * nsteps * 64 + lane;
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) acc[mt][t] = bound_f4{0.f, 0.f, 0.f, 0.f};
     }
     for (int s0 = 0; s0 < nsteps; s0 += NS) {
-      float4 a[NS], bv[NT][NS];
+      bound_h4 a[MT][NS], bv[NT][NS];
 #pragma unroll
       for (int u = 0; u < NS; ++u) {
         const int s = s0 + u < nsteps ? s0 + u : nsteps - 1;  // clamped, dropped below
-        a[u] = xa[(long)s * 64];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) a[mt][u] = xa[mt][(long)s * 64];
 #pragma unroll
         for (int t = 0; t < NT; ++t) bv[t][u] = cb[t][(long)s * 64];
       }
 #pragma unroll
       for (int u = 0; u < NS; ++u) {
-        if (s0 + u >= nsteps) break;  // uniform
+        if (s0 + u < nsteps) {  // uniform
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, bv[t][u].x, acc[t], 0, 0, 0);
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, bv[t][u].y, acc[t], 0, 0, 0);
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].z, bv[t][u].z, acc[t], 0, 0, 0);
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].w, bv[t][u].w, acc[t], 0, 0, 0);
+          for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+              acc[mt][t] =
+                  __builtin_amdgcn_mfma_f32_16x16x16f16(a[mt][u], bv[t][u], acc[mt][t], 0, 0, 0);
         }
       }
     }
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const int j = (t0 + t) * 16 + r;
-      if (j >= k) continue;
+    for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const long i = i0 + 4 * kq + q;
-        if (i < n) ub[i * ld + j] = acc[t][q];
+      for (int t = 0; t < NT; ++t) {
+        const int j = (t0 + t) * 16 + r;
+        if (j >= k) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const long i = (blk0 + mt) * 16 + 4 * kq + q;
+          if (i < n) ub[i * ld + j] = acc[mt][t][q] * scale;
+        }
       }
-    }
   }
 }
 
@@ -778,7 +790,7 @@ static size_t sbd_smem() {
 template <int F>
 static hipError_t launch_spectra_F(const float* y, int rows, int m, const float2* tw,
                                    float2* spec, float* norms, int* zero, float* mag,
-                                   float* bmag, hipStream_t st) {
+                                   uint16_t* bmag, hipStream_t st) {
   using G = FftGeom<F>;
   int blocks = (rows + G::kGroups - 1) / G::kGroups;
   if (blocks < 1) blocks = 1;
@@ -838,7 +850,7 @@ static hipError_t launch_prune_F(const PruneArgs& a, hipStream_t st) {
 
 template <int F>
 static hipError_t launch_series_spec_F(const float* x, long n, int m, const float2* tw,
-                                       float2* xs, float* xn, float* xm, hipStream_t st) {
+                                       float2* xs, float* xn, uint16_t* xm, hipStream_t st) {
   using G = FftGeom<F>;
   long blocks = ((n + 1) / 2 + G::kGroups - 1) / G::kGroups;
   if (blocks < 1) blocks = 1;
@@ -871,7 +883,7 @@ void fft_twiddle_table(int F, float2* out) {
   X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
 
 hipError_t launch_spectra(int F, const float* y, int rows, int m, const float2* tw,
-                          float2* spec, float* norms, int* zero, float* mag, float* bmag,
+                          float2* spec, float* norms, int* zero, float* mag, uint16_t* bmag,
                           hipStream_t st) {
   switch (F) {
 #define X(FF) \
@@ -897,7 +909,7 @@ int prune_max_k(int F) {
 }
 
 hipError_t launch_series_spec(int F, const float* x, long n, int m, const float2* tw, float2* xs,
-                              float* xn, float* xm, hipStream_t st) {
+                              float* xn, uint16_t* xm, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   switch (F) {
 #define X(FF) \
@@ -910,21 +922,32 @@ hipError_t launch_series_spec(int F, const float* x, long n, int m, const float2
   }
 }
 
-hipError_t launch_bounds(int F, const float* xm, long n, const float* cm, int k, float* ub,
+hipError_t launch_bounds(int F, const uint16_t* xm, long n, const uint16_t* cm, int k, float* ub,
                          int ld, hipStream_t st) {
   if (n <= 0 || k <= 0) return hipSuccess;
   if (ld < k || F < 2) return hipErrorInvalidValue;
-  const long blocks = (n + 63) / 64;
   const int kpad = bound_kpad(F);
-  if (k <= 16)
-    hipLaunchKernelGGL((k_bounds<1, 8>), dim3((unsigned)blocks), dim3(256), 0, st, xm, n, kpad, cm,
-                       k, ld, ub);
+  const float scale = 1.f / (float)F;  // a power of two: exact
+  // MT row blocks of 16 per wave share each centroid operand load, NS
+  // steps of 16 columns in flight per batch (for one centroid tile the
+  // whole row up to 33 steps: F <= 1024 in one batch); 4 waves per block
+  auto grid = [&](int mt) { return dim3((unsigned)((n + 64L * mt - 1) / (64L * mt))); };
+  const int nsteps = kpad >> 4;
+  if (k <= 16 && nsteps <= 9)
+    hipLaunchKernelGGL((k_bounds<1, 1, 9>), grid(1), dim3(256), 0, st, xm, n, kpad, cm, k, ld,
+                       scale, ub);
+  else if (k <= 16 && nsteps <= 17)
+    hipLaunchKernelGGL((k_bounds<1, 1, 17>), grid(1), dim3(256), 0, st, xm, n, kpad, cm, k, ld,
+                       scale, ub);
+  else if (k <= 16)
+    hipLaunchKernelGGL((k_bounds<1, 1, 33>), grid(1), dim3(256), 0, st, xm, n, kpad, cm, k, ld,
+                       scale, ub);
   else if (k <= 32)
-    hipLaunchKernelGGL((k_bounds<2, 4>), dim3((unsigned)blocks), dim3(256), 0, st, xm, n, kpad, cm,
-                       k, ld, ub);
+    hipLaunchKernelGGL((k_bounds<2, 2, 11>), grid(2), dim3(256), 0, st, xm, n, kpad, cm, k, ld,
+                       scale, ub);
   else
-    hipLaunchKernelGGL((k_bounds<4, 2>), dim3((unsigned)blocks), dim3(256), 0, st, xm, n, kpad, cm,
-                       k, ld, ub);
+    hipLaunchKernelGGL((k_bounds<4, 1, 11>), grid(1), dim3(256), 0, st, xm, n, kpad, cm, k, ld,
+                       scale, ub);
   return hipGetLastError();
 }
 

@@ -73,10 +73,10 @@ struct FitState {
   bool have_dist = false;  // dist holds an assignment's distances (orders the next grouping)
   float2* xs = nullptr;    // half spectra of the series (per fit; NULL: transformed per call)
   float* xn = nullptr;     // their norms
-  float* xm = nullptr;     // their bound rows (with xs)
+  uint16_t* xm = nullptr;  // their bound rows, binary16 (with xs)
   float* ub = nullptr;     // bounds of the current assignment [n][ub_ld] (with xs)
   int ub_ld = 0;
-  float* cm = nullptr;     // centroid bound rows, fragment order (with xs)
+  uint16_t* cm = nullptr;  // centroid bound rows, binary16, fragment order (with xs)
   int *label = nullptr, *shift = nullptr, *zero = nullptr;
   float *dist = nullptr, *cent = nullptr, *norm = nullptr;
   float2* spec = nullptr;
@@ -204,7 +204,7 @@ int check_series(int n, int m) {
 
 // Spectra, spectrum magnitudes, norms and all-zero flags of k rows.
 int spectra(tcx_ctx* c, const float* d_rows, int rows, int m, float2** spec, float** norm,
-            int** zero, float* bmag = nullptr) {
+            int** zero, uint16_t* bmag = nullptr) {
   int F = fft_size(m);
   const float2* tw;
   float* mag;
@@ -889,7 +889,7 @@ int tcx_fit_begin(tcx_ctx* c, const float* d_x, int n, int m, int k, const tcx_f
     const int F = fft_size(m);
     const int kp = (k + 15) & ~15;
     const size_t need = sizeof(float2) * (size_t)n * (F / 2 + 1) + sizeof(float) * (size_t)n +
-                        sizeof(float) * (size_t)tcx::bound_rows(n) * tcx::bound_kpad(F) +
+                        sizeof(uint16_t) * (size_t)tcx::bound_rows(n) * tcx::bound_kpad(F) +
                         sizeof(float) * (size_t)n * kp;
     const bool fits = !c->budget || c->ws_bytes + c->user_bytes + need <= c->budget;
     if (k <= tcx::prune_max_k(F) && fits) {

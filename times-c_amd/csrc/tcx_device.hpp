@@ -52,6 +52,16 @@ __device__ __forceinline__ double ld_dev_f64(const uint64_t* p) {
 // device-scope store that completed is visible to every block.
 __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
+// IEEE binary16 bits of v >= 0 rounded toward +infinity: the nearest
+// value (hardware conversion), stepped to the next binary16 when it fell
+// below v (for v >= 0 the next larger bit pattern is the next value up).
+// NaN stays NaN.
+__device__ __forceinline__ uint16_t f16_bits_up(float v) {
+  const _Float16 h = (_Float16)v;
+  const uint16_t b = __builtin_bit_cast(uint16_t, h);
+  return (float)h < v ? (uint16_t)(b + 1) : b;
+}
+
 // Sync of an FFT lane group of L lanes: wave-local when the group lives in
 // one wave, a block barrier otherwise (then the group is the whole block).
 template <int L>

@@ -33,10 +33,10 @@ hipError_t launch_znorm(float* x, long rows, int m, int ddof, hipStream_t st);
 // Twiddle table (F entries) of the length-F transform, host side.
 void fft_twiddle_table(int F, float2* out);
 // bmag (may be NULL): rows (bound_rows(rows) x bound_kpad(F), fragment
-// order) of |Y[f]| / |y| for f <= F/2, zero padded (the centroid operand
-// of launch_bounds).
+// order) of |Y[f]| / |y| for f <= F/2, zero padded, as IEEE binary16 bits
+// rounded up (the centroid operand of launch_bounds).
 hipError_t launch_spectra(int F, const float* y, int rows, int m, const float2* tw,
-                          float2* spec, float* norms, int* zero, float* mag, float* bmag,
+                          float2* spec, float* norms, int* zero, float* mag, uint16_t* bmag,
                           hipStream_t st);
 
 // Exact pruned assignment (bound-and-prune on the spectra), k <= kPruneMaxK.
@@ -47,11 +47,12 @@ int prune_max_k(int F);
 // Columns of a series' bound row (launch_series_spec's xm): F/2+1
 // magnitudes padded to a multiple of 16 (the bound product's K step).
 __host__ __device__ constexpr int bound_kpad(int F) { return ((F / 2 + 1) + 15) & ~15; }
-// Bound rows are stored in MFMA fragment order: the 16 x 16 block of rows
-// 16 b .. 16 b + 15 and columns 16 s .. 16 s + 15 is 256 consecutive floats,
-// element (r, c) at 4 (r + 16 (c / 4)) + c % 4, so that lane l of a wave
-// reads its operand (row l % 16, columns 4 (l / 16) .. + 3) as the l-th
-// 16-byte vector of the block.  Rows are padded to a multiple of 16.
+// Bound rows are IEEE binary16 values in MFMA fragment order: the 16 x 16
+// block of rows 16 b .. 16 b + 15 and columns 16 s .. 16 s + 15 is 256
+// consecutive values, element (r, c) at 4 (r + 16 (c / 4)) + c % 4, so that
+// lane l of a wave reads its operand of v_mfma_f32_16x16x16_f16 (row l % 16,
+// columns 4 (l / 16) .. + 3) as the l-th 8-byte vector of the block.  Rows
+// are padded to a multiple of 16.
 __host__ __device__ constexpr long bound_frag(long row, int f, int kpad) {
   return (((row >> 4) * (kpad >> 4) + (f >> 4)) * 64 + (row & 15) + 16 * ((f & 15) >> 2)) * 4 +
          (f & 3);
@@ -81,13 +82,15 @@ struct PruneArgs {
 hipError_t launch_assign_prune(int F, const PruneArgs& a, hipStream_t st);
 // Half spectra X_r[0 .. F/2] of every row and the row norms (k_series_spec);
 // with xm (may be NULL) also the bound rows (bound_rows(n) x bound_kpad(F),
-// fragment order): weight_f |X_r[f]| / (F |x_r|), weight 2 for
-// 0 < f < F/2, zero padded.
+// fragment order, binary16 rounded up): weight_f |X_r[f]| / |x_r|, weight 2
+// for 0 < f < F/2, zero padded.
 hipError_t launch_series_spec(int F, const float* x, long n, int m, const float2* tw, float2* xs,
-                              float* xn, float* xm, hipStream_t st);
-// ub[i][j] = sum_f xm[i][f] cm[j][f] (cm: launch_spectra's bmag rows), i < n,
-// j < k, row stride ld >= k: the pruned assignment's bounds (k_bounds).
-hipError_t launch_bounds(int F, const float* xm, long n, const float* cm, int k, float* ub,
+                              float* xn, uint16_t* xm, hipStream_t st);
+// ub[i][j] = sum_f xm[i][f] cm[j][f] / F (cm: launch_spectra's bmag rows),
+// i < n, j < k, row stride ld >= k: the pruned assignment's bounds
+// (k_bounds).  Both operands are rounded up, so ub is never below the
+// product of the FP32 magnitudes.
+hipError_t launch_bounds(int F, const uint16_t* xm, long n, const uint16_t* cm, int k, float* ub,
                          int ld, hipStream_t st);
 hipError_t launch_sbd(int F, int mode, const SbdArgs& a, hipStream_t st);
 hipError_t launch_apply_shift(float* x, long rows, int m, const int* shift, const int* skip,
