@@ -27,7 +27,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per k_assign_prune launch at the default workload (n=100000,
 # m=512, k=16), from rocprofv3 FETCH_SIZE (x2) and WRITE_SIZE (x1) counter
 # runs calibrated on a plain 256 MiB copy: profiles/kshape_iteration_n100k_m512_k16.md
-ASSIGN_TRAFFIC_BYTES = {(100000, 512, 16): (444057 + 9988) * 1024}
+ASSIGN_TRAFFIC_BYTES = {(100000, 512, 16): (444021 + 9992) * 1024}
 
 
 def fft_size(m):
