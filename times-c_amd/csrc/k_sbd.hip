@@ -48,6 +48,52 @@ __global__ __launch_bounds__(256) void k_znorm(float* __restrict__ x, long rows,
   }
 }
 
+// The same for m % 4 == 0, m <= 256 * NV: the row is read once, as float4
+// vectors held in registers (lane l holds vectors l, l + 64, ...), and
+// written once; the sums are the same two-pass FP64 sums (in another
+// order).
+template <int NV>
+__global__ __launch_bounds__(256) void k_znorm_v(float* __restrict__ x, long rows, int m,
+                                                 int ddof) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= rows) return;
+  float4* r = reinterpret_cast<float4*>(x + row * (long)m);
+  const int nv = m >> 2;
+  float4 v[NV];
+#pragma unroll
+  for (int u = 0; u < NV; ++u) {
+    const int i = lane + kWave * u;
+    v[u] = i < nv ? r[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int u = 0; u < NV; ++u)
+    s += ((double)v[u].x + (double)v[u].y) + ((double)v[u].z + (double)v[u].w);
+  s = wave_sum_f64(s);
+  const double mean = s / (double)m;
+  double ss = 0.0;
+#pragma unroll
+  for (int u = 0; u < NV; ++u) {
+    if (lane + kWave * u < nv) {
+      const double a = (double)v[u].x - mean, b = (double)v[u].y - mean;
+      const double c = (double)v[u].z - mean, d = (double)v[u].w - mean;
+      ss += (a * a + b * b) + (c * c + d * d);
+    }
+  }
+  ss = wave_sum_f64(ss);
+  const double sd = sqrt(ss / (double)(m - ddof));
+  auto z = [&](float e) {
+    const double w = ((double)e - mean) / sd;
+    return isnan(w) ? 0.f : (isinf(w) ? (w > 0 ? 3.402823466e38f : -3.402823466e38f) : (float)w);
+  };
+#pragma unroll
+  for (int u = 0; u < NV; ++u) {
+    const int i = lane + kWave * u;
+    if (i < nv) r[i] = make_float4(z(v[u].x), z(v[u].y), z(v[u].z), z(v[u].w));
+  }
+}
+
 // ------------------------------------------------------------- spectra
 // Forward FFT of each row (zero padded to F, imaginary part 0), stored
 // lane-strided, plus the row's L2 norm, an all-zero flag and (mag != NULL)
@@ -1009,6 +1055,18 @@ hipError_t configure_sbd_kernels() {
 hipError_t launch_znorm(float* x, long rows, int m, int ddof, hipStream_t st) {
   if (rows <= 0) return hipSuccess;
   long blocks = (rows + 3) / 4;
+  if ((m & 3) == 0 && m <= 256 * 16 && (reinterpret_cast<uintptr_t>(x) & 15) == 0) {
+    if (m <= 256 * 2)
+      hipLaunchKernelGGL(k_znorm_v<2>, dim3((unsigned)blocks), dim3(256), 0, st, x, rows, m, ddof);
+    else if (m <= 256 * 4)
+      hipLaunchKernelGGL(k_znorm_v<4>, dim3((unsigned)blocks), dim3(256), 0, st, x, rows, m, ddof);
+    else if (m <= 256 * 8)
+      hipLaunchKernelGGL(k_znorm_v<8>, dim3((unsigned)blocks), dim3(256), 0, st, x, rows, m, ddof);
+    else
+      hipLaunchKernelGGL(k_znorm_v<16>, dim3((unsigned)blocks), dim3(256), 0, st, x, rows, m,
+                         ddof);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_znorm, dim3((unsigned)blocks), dim3(256), 0, st, x, rows, m, ddof);
   return hipGetLastError();
 }
