@@ -18,9 +18,12 @@ def _data(seed, n, m, k, sigma=0.3):
     return X, y
 
 
+# m = 77 takes the three-pass kernel; multiples of 4 up to 4096 the one-pass
+# register kernel (512, 1000, 2048, 4096: 2, 4, 8 and 16 float4 per lane)
+@pytest.mark.parametrize("m", [77, 512, 1000, 2048, 4096])
 @pytest.mark.parametrize("ddof", [0, 1])
-def test_znorm_matches_zscore(ctx, ddof):
-    X, _ = _data(1, 300, 77, 3)
+def test_znorm_matches_zscore(ctx, ddof, m):
+    X, _ = _data(1, 300, m, 3)
     X[5, :] = 2.0  # constant row -> zeros (nan_to_num)
     X[6, :] = 0.0
     d = ctx.to_device(X)
