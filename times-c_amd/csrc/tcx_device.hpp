@@ -92,19 +92,6 @@ __device__ __forceinline__ float readlane_f32(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// (v, key) <- the larger value, the smaller key among equal values; written
-// as selects so that no lane mask is saved or restored
-__device__ __forceinline__ void argmax_merge(float& v, int& key, float ov, int ok) {
-  const bool take = (ov > v) | ((ov == v) & (ok < key));
-  v = take ? ov : v;
-  key = take ? ok : key;
-}
-
-template <int W, int CTRL>
-__device__ __forceinline__ void argmax_step(float& v, int& key) {
-  argmax_merge(v, key, dpp_f32<CTRL>(v), dpp_i32<CTRL>(key));
-}
-
 template <int L>
 __device__ __forceinline__ float group_sum(float v, float* scratch) {
   constexpr int W = L < kWave ? L : kWave;
@@ -133,46 +120,69 @@ __device__ __forceinline__ float group_sum(float v, float* scratch) {
   return v;
 }
 
-// Max with the smallest key on ties (NumPy argmax: first index wins).
+// Max with the smallest key on ties (NumPy argmax: first index wins).  The
+// pair travels as one 64-bit word: the value mapped to an unsigned integer
+// of the same order (-0 folded into +0, so the two compare equal as floats
+// do) above the complemented key, so the largest word is the largest value
+// with the smallest key, and each reduction step is one 64-bit compare.  The
+// value is never NaN here (the lag scans skip NaN); a lane without a lag
+// holds (-inf, INT_MAX), which loses to every lag.
+__device__ __forceinline__ uint64_t argmax_pack(float v, int key) {
+  const uint32_t b = __float_as_uint(v + 0.0f);  // -0 + 0 = +0
+  const uint32_t o = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+  return ((uint64_t)o << 32) | (uint32_t)~key;
+}
+__device__ __forceinline__ void argmax_unpack(uint64_t p, float& v, int& key) {
+  const uint32_t o = (uint32_t)(p >> 32);
+  v = __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+  key = (int)~(uint32_t)p;
+}
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t p) {
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(p >> 32), CTRL, 0xF, 0xF,
+                                                         false);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)p, CTRL, 0xF, 0xF, false);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t p, int lane) {
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(p >> 32), lane);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)p, lane);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t max_u64(uint64_t a, uint64_t b) { return b > a ? b : a; }
+
 template <int L>
 __device__ __forceinline__ void group_argmax(float& v, int& key, float* scratch) {
   constexpr int W = L < kWave ? L : kWave;
-  if constexpr (W >= 2) argmax_step<W, 0xB1>(v, key);
-  if constexpr (W >= 4) argmax_step<W, 0x4E>(v, key);
-  if constexpr (W >= 8) argmax_step<W, 0x141>(v, key);
-  if constexpr (W >= 16) argmax_step<W, 0x140>(v, key);
-  if constexpr (W >= 32) {
-    float v0 = readlane_f32(v, 0), v1 = readlane_f32(v, 16);
-    int k0 = __builtin_amdgcn_readlane(key, 0), k1 = __builtin_amdgcn_readlane(key, 16);
-    argmax_merge(v0, k0, v1, k1);
-    float v2 = readlane_f32(v, 32), v3 = readlane_f32(v, 48);
-    int k2 = __builtin_amdgcn_readlane(key, 32), k3 = __builtin_amdgcn_readlane(key, 48);
-    argmax_merge(v2, k2, v3, k3);
-    if constexpr (W == 32) {
-      const bool hi = (threadIdx.x & 32) != 0;
-      v = hi ? v2 : v0;
-      key = hi ? k2 : k0;
-    } else {
-      argmax_merge(v0, k0, v2, k2);
-      v = v0;
-      key = k0;
-    }
+  uint64_t p = argmax_pack(v, key);
+  if constexpr (W >= 2) p = max_u64(p, dpp_u64<0xB1>(p));   // lane ^ 1
+  if constexpr (W >= 4) p = max_u64(p, dpp_u64<0x4E>(p));   // lane ^ 2
+  if constexpr (W >= 8) p = max_u64(p, dpp_u64<0x141>(p));  // mirror in 8 lanes
+  if constexpr (W >= 16) p = max_u64(p, dpp_u64<0x140>(p)); // mirror in 16 lanes
+  if constexpr (W == 32) {
+    const uint64_t a = max_u64(readlane_u64(p, 0), readlane_u64(p, 16));
+    const uint64_t b = max_u64(readlane_u64(p, 32), readlane_u64(p, 48));
+    p = (threadIdx.x & 32) ? b : a;
+  } else if constexpr (W == 64) {
+    p = max_u64(max_u64(readlane_u64(p, 0), readlane_u64(p, 16)),
+                max_u64(readlane_u64(p, 32), readlane_u64(p, 48)));
   }
   if constexpr (L > kWave) {
     constexpr int NW = L / kWave;
-    int wid = threadIdx.x / kWave;
-    int* ks = reinterpret_cast<int*>(scratch + NW);
+    const int wid = threadIdx.x / kWave;
+    uint32_t* hs = reinterpret_cast<uint32_t*>(scratch);
+    uint32_t* ls = hs + NW;
     __syncthreads();
     if ((threadIdx.x & (kWave - 1)) == 0) {
-      scratch[wid] = v;
-      ks[wid] = key;
+      hs[wid] = (uint32_t)(p >> 32);
+      ls[wid] = (uint32_t)p;
     }
     __syncthreads();
-    v = scratch[0];
-    key = ks[0];
+    p = ((uint64_t)hs[0] << 32) | ls[0];
 #pragma unroll
-    for (int w = 1; w < NW; ++w) argmax_merge(v, key, scratch[w], ks[w]);
+    for (int w = 1; w < NW; ++w) p = max_u64(p, ((uint64_t)hs[w] << 32) | ls[w]);
   }
+  argmax_unpack(p, v, key);
 }
 
 // Cross-lane moves of a double through DPP (VALU data-parallel primitives,
