@@ -147,6 +147,34 @@ def test_pruned_assign_equals_full_search(ctx, m, n, k):
     assert np.array_equal(sh, fsh)
 
 
+# (m, k) over the bound product's forms: row steps of 16 columns per
+# batch 9 (m = 100, F = 256), 17 (m = 200), 33 (m = 512), 33 + 32 (m = 700,
+# F = 2048); 1, 2 and 4 centroid tiles (k <= 16, <= 32, > 32)
+@pytest.mark.parametrize("m,k", [(100, 5), (100, 24), (100, 40), (200, 12), (512, 8), (700, 20)])
+def test_fit_assignment_equals_dense_search(ctx, m, k):
+    """The assignment inside a fit (series half spectra once per fit, the
+    bound rows in binary16 rounded up, the bound product per iteration)
+    labels every series as the dense search over the same centroids, up to
+    ties."""
+    n = 600
+    X, _ = _data(11 * m + k, n, m, k)
+    d = ctx.to_device(X)
+    try:
+        ctx.znorm(d, ddof=0)
+        ctx.fit_begin(d, k, max_iter=1 << 30, seed=3)
+        for _ in range(3):
+            ctx.fit_step()
+        res = ctx.fit_end(n, k, m)
+        dc = ctx.to_device(res["centroids"])
+        try:
+            flab, _, _, fdm = ctx.assign(d, dc, dmat=True)
+        finally:
+            dc.free()
+    finally:
+        d.free()
+    assert_labels_match(res["labels"], flab, fdm, what=f"fit assignment m={m} k={k}")
+
+
 @pytest.mark.parametrize("convention", [T.CONV_CPU, T.CONV_KSHAPE])
 @pytest.mark.parametrize("m", [16, 128, 700])
 def test_sbd_align_matches_oracle(ctx, convention, m):
