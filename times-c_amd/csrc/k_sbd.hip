@@ -758,7 +758,10 @@ __global__ __launch_bounds__(PruneGeom<F>::kBlock, PruneGeom<F>::kMinWaves) void
         sc = take ? v : sc;
         cj = take ? j : cj;
       }
-      group_argmax<L>(sc, cj, red);
+      if (L == kWave && k <= 16)  // the candidates sit in lanes 0..15
+        group_argmax_row0(sc, cj);
+      else
+        group_argmax<L>(sc, cj, red);
       if (sc == -INFINITY) break;
       const int j = cj;
       group_sync<L>();

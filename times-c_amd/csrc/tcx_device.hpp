@@ -181,6 +181,18 @@ __device__ __forceinline__ void group_argmax(float& v, int& key, float* scratch)
   argmax_unpack(p, v, key);
 }
 
+// group_argmax for a 64-lane group whose values all sit in lanes 0..15 (the
+// other lanes hold (-inf, INT_MAX)): the reduction stays inside the first
+// row of 16 lanes and lane 0's result is broadcast.
+__device__ __forceinline__ void group_argmax_row0(float& v, int& key) {
+  uint64_t p = argmax_pack(v, key);
+  p = max_u64(p, dpp_u64<0xB1>(p));
+  p = max_u64(p, dpp_u64<0x4E>(p));
+  p = max_u64(p, dpp_u64<0x141>(p));
+  p = max_u64(p, dpp_u64<0x140>(p));
+  argmax_unpack(readlane_u64(p, 0), v, key);
+}
+
 // Cross-lane moves of a double through DPP (VALU data-parallel primitives,
 // no LDS round trip): CTRL is the DPP control word.
 template <int CTRL>
