@@ -684,8 +684,7 @@ __global__ __launch_bounds__(PruneGeom<F>::kBlock, PruneGeom<F>::kMinWaves) void
       const float2* qb = xs + (hasb ? ib : ia) * HS + (F - L * H) - lane;
       na = xn[ia];
       nb = hasb ? xn[ib] : 0.f;
-      // two halves, so that only half of the loads are in flight at once
-      // (register pressure)
+      // all loads of the pair in flight at once (one latency per pair)
       auto assemble = [&](int s) {
         float2 A, B;
         if (s < H) {
@@ -700,10 +699,7 @@ __global__ __launch_bounds__(PruneGeom<F>::kBlock, PruneGeom<F>::kMinWaves) void
         zx[s] = make_float2(A.x - B.y, A.y + B.x);
       };
 #pragma unroll
-      for (int s = 0; s < H; ++s) assemble(s);
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int s = H; s < P; ++s) assemble(s);
+      for (int s = 0; s < P; ++s) assemble(s);
       // the bounds, computed for every series by k_bounds
       for (int j = lane; j < k; j += L) {
         ubs[j] = ubg[ia * ub_ld + j];
