@@ -57,6 +57,10 @@ int tcx_create(int device, size_t hbm_budget, tcx_ctx** out);
 int tcx_destroy(tcx_ctx* ctx);
 /* Bytes of workspace the context currently holds. */
 size_t tcx_workspace_bytes(const tcx_ctx* ctx);
+/* Upper bound of the workspace bytes one fit of n series of length m into k
+ * clusters grows on this context's device (what an HBM budget must leave
+ * for it).  0 for invalid sizes.  No reference counterpart. */
+size_t tcx_fit_workspace_bytes(const tcx_ctx* ctx, int n, int m, int k);
 
 /* Device memory helpers for hosts without their own allocator (tests,
  * the CLI).  Stand in for the cudaMalloc/cudaMemcpy/cudaFree calls of
@@ -139,6 +143,12 @@ typedef struct {
   double power_tol;      /* relative Rayleigh-quotient tolerance, 0 -> 1e-12 */
   int* labels_per_iter;  /* optional host buffer (max_iter*n): labels after
                             each iteration's assignment                      */
+  int stop_on_collapse;  /* 1: stop right after a refinement that leaves
+                            sum_class == 1 or sum_class <= k/2 non-empty
+                            clusters, without assigning, keeping the labels
+                            of the previous assignment (the GPU path's break,
+                            kshape.cuh:248-249, 454-455); 0: the CPU path,
+                            which has no such stop (default)                 */
 } tcx_fit_opts;
 
 typedef struct {
@@ -155,6 +165,7 @@ typedef struct {
   long long sbd_pairs;   /* series x centroid pairs assigned                */
   double ms_assign_bounds; /* part of ms_assign spent forming the pruning
                               bounds of all series (HIP events)              */
+  int collapsed;         /* 1 if the fit stopped on collapse (stop_on_collapse) */
 } tcx_fit_stats;
 
 /* Fill opts with defaults (max_iter 10, seed 0, CPU convention,
@@ -184,15 +195,29 @@ int tcx_fit_end(tcx_ctx* ctx, int* h_labels, float* h_centroids,
                 tcx_fit_stats* stats);
 /* Statistics of the fit in progress (no reference counterpart). */
 int tcx_fit_query(tcx_ctx* ctx, tcx_fit_stats* stats);
+/* State of the fit in progress, copied to host buffers (each may be NULL):
+ * labels, SBD distances and lags of the last assignment (n each; h_dist
+ * needs one assignment done) and the current centroids (k*m): for
+ * init_mode 0 those the last assignment used, for init_mode 1 those of the
+ * refinement that followed it.  No reference counterpart (the reference
+ * frees its centroids, kshape.cuh:321); for diagnostics and parity checks
+ * of a fit step by step. */
+int tcx_fit_snapshot(tcx_ctx* ctx, int* h_labels, float* h_dist, int* h_shift,
+                     float* h_centroids);
 
 /* Reference-shaped fit: one-based labels into h_idx, *iter_num in = cap,
- * out = iterations run, *sum_class out = non-empty clusters.
+ * out = iterations run (left unchanged when the fit stops on collapse, as
+ * the reference leaves it), *sum_class out = non-empty clusters after the
+ * last refinement.  Stops on collapse (stop_on_collapse = 1), like the
+ * reference's loop (kshape.cuh:248-249).
  * Replaces k_shape<float> (Code/include/kshape.cuh:197-324). */
 int tcx_kshape(tcx_ctx* ctx, const float* d_x, int* h_idx, int n, int m,
                int k, int* iter_num, int* sum_class, int cal_select);
 
 /* Centre-initialized fit (init_chose=1).  h_x is accepted for signature
- * parity and unused: the centres are rows of the z-normalized d_x.
+ * parity and unused: the centres are rows of the z-normalized d_x.  Stops
+ * on collapse after a refinement (kshape.cuh:454-455); *iter_num as for
+ * tcx_kshape.
  * Replaces k_shape_tsl<float> (Code/include/kshape.cuh:332-474). */
 int tcx_kshape_tsl(tcx_ctx* ctx, const float* d_x, const float* h_x,
                    int* h_idx, int n, int m, int k, int* iter_num,
@@ -278,18 +303,28 @@ typedef struct {
 } tcx_request;
 
 /* fit_workers + query_workers >= 1 worker contexts on `device`.
- * hbm_budget (0 = none) bounds the bytes of admitted requests' series and
- * results plus resident models. */
+ * hbm_budget (0 = none) bounds the device bytes of the resident models plus
+ * every worker's reservation (a running request's staging buffers and, for
+ * a fit, tcx_fit_workspace_bytes of its size) or held workspace (idle).  A
+ * request is admitted when it fits; idle workers release their workspace
+ * when that is what blocks it.  A request that cannot fit beside the
+ * resident models is refused (TCX_ERR_BUDGET). */
 int tcx_server_create(int device, int fit_workers, int query_workers, size_t hbm_budget,
                       tcx_server** out);
-/* Uploads k centroids (k*m, host) as a resident model; *model_id out. */
+/* Uploads k centroids (k*m, host) as a resident model; *model_id out.
+ * TCX_ERR_BUDGET when the model does not fit the budget left now (idle
+ * workers are asked to release their workspace; a retry may succeed). */
 int tcx_server_add_model(tcx_server* s, const float* h_centroids, int k, int m, int* model_id);
 /* Queues a request (the struct and its buffers must stay valid until it is
- * done).  Returns at once; errors found at submission are returned here. */
+ * done).  Returns at once; errors found at submission are returned here
+ * (TCX_ERR_BUDGET: the request needs more than the budget leaves beside the
+ * resident models; a queued request that a later model makes so fails with
+ * it). */
 int tcx_server_submit(tcx_server* s, tcx_request* req);
 /* Blocks until req is done; returns its status. */
 int tcx_server_wait(tcx_server* s, tcx_request* req);
-/* Finishes queued requests, stops the workers and frees the models. */
+/* Fails the requests still queued (TCX_ERR_ARG, done = 1), lets the running
+ * ones finish, stops the workers and frees the models. */
 int tcx_server_destroy(tcx_server* s);
 
 /* ------------------------------------------------------------- host I/O */

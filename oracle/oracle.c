@@ -715,29 +715,41 @@ void orc_refine(const double* X, long n, int m, int k, const int* labels,
 
 /* ----------------------------------------------------------- the loop */
 
+/* sum_class == 1 || sum_class <= k/2 (kshape.cuh:248, 454) */
+static int orc_collapsed(const int* counts, int k) {
+  int nonempty = 0;
+  for (int j = 0; j < k; ++j) nonempty += counts[j] > 0;
+  return nonempty == 1 || nonempty <= k / 2;
+}
+
 int orc_kshape(const double* X, long n, int m, int k, int init_mode,
                const int* init_labels, const int* init_rows, int max_iter,
                int convention, uint64_t seed, int exact_steps,
                int* labels_hist, double* centroids, int nthreads, int skip_rule,
-               int cal_select) {
+               int cal_select, int stop_on_collapse, int* collapsed) {
   int* lab = (int*)malloc(sizeof(int) * n);
   int* old = (int*)malloc(sizeof(int) * n);
+  int* counts = (int*)malloc(sizeof(int) * k);
   double* C = (double*)calloc((size_t)k * m, sizeof(double));
-  int it = 0;
+  int it = 0, stop = 0;
+  if (collapsed) *collapsed = 0;
   if (init_mode == 0) {
     /* test_timesC.py:217-264: refine, assign, stop when labels repeat */
     memcpy(lab, init_labels, sizeof(int) * n);
     for (it = 0; it < max_iter; ++it) {
       memcpy(old, lab, sizeof(int) * n);
-      orc_refine(X, n, m, k, lab, C, convention, seed, it, exact_steps, NULL,
+      orc_refine(X, n, m, k, lab, C, convention, seed, it, exact_steps, counts,
                  NULL, nthreads, skip_rule, cal_select);
+      /* the GPU path's break after extract_shape (kshape.cuh:248-249): no
+       * assignment, the labels stay those of the previous one */
+      if (stop_on_collapse && orc_collapsed(counts, k)) { stop = 1; break; }
       orc_assign(X, n, m, C, k, lab, NULL, NULL, NULL, nthreads);
       memcpy(labels_hist + (size_t)it * n, lab, sizeof(int) * n);
       if (memcmp(old, lab, sizeof(int) * n) == 0) { ++it; break; }
     }
   } else {
     /* centre initialisation, loop order of k_shape_tsl (kshape.cuh:376-461):
-     * assign, stop test, refine */
+     * assign, stop test, refine (then the collapse break, :454-455) */
     for (int j = 0; j < k; ++j)
       memcpy(C + (size_t)j * m, X + (long)init_rows[j] * m, sizeof(double) * m);
     for (it = 0; it < max_iter; ++it) {
@@ -745,11 +757,13 @@ int orc_kshape(const double* X, long n, int m, int k, int init_mode,
       memcpy(labels_hist + (size_t)it * n, lab, sizeof(int) * n);
       if (it > 0 && memcmp(old, lab, sizeof(int) * n) == 0) { ++it; break; }
       memcpy(old, lab, sizeof(int) * n);
-      orc_refine(X, n, m, k, lab, C, convention, seed, it, exact_steps, NULL,
+      orc_refine(X, n, m, k, lab, C, convention, seed, it, exact_steps, counts,
                  NULL, nthreads, skip_rule, cal_select);
+      if (stop_on_collapse && orc_collapsed(counts, k)) { stop = 1; ++it; break; }
     }
   }
+  if (collapsed) *collapsed = stop;
   if (centroids) memcpy(centroids, C, sizeof(double) * (size_t)k * m);
-  free(lab); free(old); free(C);
+  free(lab); free(old); free(counts); free(C);
   return it;
 }

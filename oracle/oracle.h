@@ -104,12 +104,16 @@ void orc_refine(const double* X, long n, int m, int k, const int* labels,
  *              the starting centroids; loop order assign -> stop test ->
  *              refine (the k_shape_tsl order, kshape.cuh:376-461).
  * labels_hist (max_iter*n) receives the labels after each iteration's
- * assignment; centroids (k*m) the final centroids; returns iterations run. */
+ * assignment; centroids (k*m) the final centroids; returns iterations run
+ * (assignments made).  stop_on_collapse != 0 adds the GPU path's break after
+ * a refinement leaving one non-empty cluster or at most k/2
+ * (kshape.cuh:248-249, 454-455); *collapsed (may be NULL) reports it. */
 int orc_kshape(const double* X, long n, int m, int k, int init_mode,
                const int* init_labels, const int* init_rows, int max_iter,
                int convention, uint64_t seed, int exact_steps,
                int* labels_hist, double* centroids, int nthreads,
-               int skip_rule, int cal_select);
+               int skip_rule, int cal_select, int stop_on_collapse,
+               int* collapsed);
 
 /* Top eigenpair of a symmetric m x m matrix (FP64): Householder
  * tridiagonalization, bisection for the largest eigenvalue, inverse

@@ -52,7 +52,7 @@ def load():
         "orc_power_v0": (None, [U, I, I, I, _P]),
         "orc_extract_shape": (I, [_P, L, I, _P, I, _P, ctypes.POINTER(ShapeInfo), I]),
         "orc_refine": (None, [_P, L, I, I, _P, _P, I, U, I, I, _P, _P, I, I, I]),
-        "orc_kshape": (I, [_P, L, I, I, I, _P, _P, I, I, U, I, _P, _P, I, I, I]),
+        "orc_kshape": (I, [_P, L, I, I, I, _P, _P, I, I, U, I, _P, _P, I, I, I, I, _P]),
         "orc_sum_pairwise": (D, [_P, L]),
         "orc_top_eig": (I, [_P, I, ctypes.POINTER(D), _P]),
     }
@@ -162,16 +162,19 @@ def refine(X, labels, C, convention=CONV_CPU, seed=0, it=0, exact_steps=False, n
 
 def kshape(X, k, init_mode=0, init_labels=None, init_rows=None, max_iter=10,
            convention=CONV_CPU, seed=0, exact_steps=False, nthreads=0, skip_rule=SKIP_ZERO,
-           cal_select=0):
+           cal_select=0, stop_on_collapse=False, return_collapsed=False):
     X = _f64(X)
     n, m = X.shape
     hist = np.zeros((max_iter, n), np.int32)
     cent = np.zeros((k, m))
     il = None if init_labels is None else np.ascontiguousarray(init_labels, np.int32)
     ir = None if init_rows is None else np.ascontiguousarray(init_rows, np.int32)
+    col = ctypes.c_int()
     it = load().orc_kshape(_p(X), n, m, k, init_mode, _p(il), _p(ir), max_iter, convention,
                            seed, int(exact_steps), _p(hist), _p(cent), nthreads, skip_rule,
-                           cal_select)
+                           cal_select, int(stop_on_collapse), ctypes.byref(col))
+    if return_collapsed:
+        return hist[:it], cent, bool(col.value)
     return hist[:it], cent
 
 

@@ -83,7 +83,8 @@ def test_cli_writes_the_fit_labels(ctx, tmp_path, init_chose):
     d = ctx.to_device(X)
     try:
         ctx.znorm(d, ddof=0)
-        res = ctx.fit(d, 3, max_iter=10, init_mode=init_chose, seed=0, labels_per_iter=True)
+        res = ctx.fit(d, 3, max_iter=10, init_mode=init_chose, seed=0, labels_per_iter=True,
+                      stop_on_collapse=True)
     finally:
         d.free()
     assert np.array_equal(cli, res["labels"] + 1)

@@ -83,9 +83,10 @@ def test_fit_with_eigenvector_centroids_matches_oracle(ctx, init_mode):
 
 
 def test_cli_reruns_with_eigenvector_centroids(ctx, tmp_path):
-    """A labeled file with k = 8 whose data has 2 classes: the first fit ends
-    with fewer than k/2 non-empty clusters, so the CLI reruns k_shape with
-    cal_select = 1 and writes that run's labels."""
+    """A labeled file with k = 8 whose data has 2 classes: the first fit stops
+    on collapse with fewer than k/2 non-empty clusters, so the CLI reruns
+    k_shape with cal_select = 1 (which stops on collapse too) and writes that
+    run's labels."""
     n, m, k = 200, 48, 8
     X, _ = T.generate(0, n, m, 2, 0.1)
     csv = str(tmp_path / "data.csv")
@@ -99,9 +100,9 @@ def test_cli_reruns_with_eigenvector_centroids(ctx, tmp_path):
     d = ctx.to_device(X)
     try:
         ctx.znorm(d, ddof=0)
-        first = ctx.fit(d, k, max_iter=10, seed=0)
-        second = ctx.fit(d, k, max_iter=10, seed=0, cal_select=1)
+        first = ctx.fit(d, k, max_iter=10, seed=0, stop_on_collapse=True)
+        second = ctx.fit(d, k, max_iter=10, seed=0, cal_select=1, stop_on_collapse=True)
     finally:
         d.free()
-    assert first["stats"]["sum_class"] < k // 2
+    assert first["stats"]["sum_class"] < k // 2 and first["stats"]["collapsed"]
     assert np.array_equal(cli, second["labels"] + 1)

@@ -134,3 +134,37 @@ def test_fit_tsl_cfg1_small(ctx):
     assert np.array_equal(got, hist)
     errs = centroid_errors(res["centroids"], cent)
     assert max(errs) <= CENT_RTOL, errs
+
+
+@pytest.mark.parametrize("init_mode", [0, 1])
+def test_fit_stops_on_collapse_like_the_gpu_loop(ctx, init_mode):
+    """Two-class data fitted with k = 4 collapses to exactly k/2 non-empty
+    clusters.  stop_on_collapse (the reference GPU loop's break,
+    kshape.cuh:248-249, 454-455) stops right after that refinement with the
+    labels of the assignment before it, as the oracle does; the
+    reference-shaped tcx_kshape / tcx_kshape_tsl stop there too, return
+    one-based labels, sum_class = k/2 and leave iter_num at the cap."""
+    import ctypes
+
+    n, m, k, seed = 60, 32, 4, 0  # seed 0: the reference-shaped entry points' default
+    d, Z, _ = _znormed(ctx, 2, n, m, 2, sigma=0.1)
+    kw = ({"init_labels": T.init_labels(seed, n, k)} if init_mode == 0
+          else {"init_rows": T.init_rows(seed, n, k)})
+    try:
+        res = ctx.fit(d, k, max_iter=10, init_mode=init_mode, seed=seed, labels_per_iter=True,
+                      stop_on_collapse=True, **kw)
+        it, sc = ctypes.c_int(10), ctypes.c_int(-1)
+        idx = np.empty(n, np.int32)
+        fn = ctx.lib.tcx_kshape if init_mode == 0 else ctx.lib.tcx_kshape_tsl
+        args = ([ctx.h, d.ptr] + ([None] if init_mode == 1 else []) +
+                [idx.ctypes.data, n, m, k, ctypes.byref(it), ctypes.byref(sc), 0])
+        T.check(fn(*args))
+    finally:
+        d.free()
+    hist, _, col = O.kshape(Z.astype(np.float64), k, init_mode=init_mode, max_iter=10, seed=seed,
+                            stop_on_collapse=True, return_collapsed=True, **kw)
+    assert col and res["stats"]["collapsed"] == 1
+    assert np.array_equal(res["labels_per_iter"], hist)
+    assert np.array_equal(res["labels"], hist[-1])
+    assert res["stats"]["sum_class"] == k // 2
+    assert np.array_equal(idx, hist[-1] + 1) and sc.value == k // 2 and it.value == 10

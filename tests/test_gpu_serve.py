@@ -82,12 +82,19 @@ def test_published_model_serves_the_fitted_centroids(ctx):
     assert np.array_equal(q["labels"], lab) and np.array_equal(q["dist"], dist)
 
 
-def test_admission_under_a_budget():
-    X = [T.generate(60 + i, 1000, 128, 3, 0.3)[0] for i in range(4)]  # 0.52 MB each
-    budget = 1_000_000  # room for one 0.524 MB request, not two
+def _fit_need(ctx, n, m, k):
+    """Bytes a fit request needs: staging (series, labels, shifts,
+    distances) plus the fit's workspace bound."""
+    return n * m * 4 + n * 12 + ctx.fit_workspace_bound(n, m, k)
+
+
+def test_admission_under_a_budget(ctx):
+    X = [T.generate(60 + i, 1000, 128, 3, 0.3)[0] for i in range(4)]
+    need = _fit_need(ctx, 1000, 128, 3)
+    budget = need + need // 2  # room for one such fit, not two
     with T.Server(0, fit_workers=3, query_workers=0, budget=budget) as srv:
         with pytest.raises(T.TimescError) as e:
-            srv.submit_fit(T.generate(1, 4000, 128, 3, 0.3)[0], 3)  # 2 MB > budget
+            srv.submit_fit(T.generate(1, 8000, 128, 3, 0.3)[0], 3)  # needs > budget
         assert e.value.code == T.ERR_BUDGET
         hs = [srv.submit_fit(x, 3, seed=1) for x in X]
         res = [srv.wait(h) for h in hs]
@@ -97,6 +104,61 @@ def test_admission_under_a_budget():
     spans = sorted((r["ms_queued"], r["ms_latency"]) for r in res)
     for (s0, e0), (s1, e1) in zip(spans, spans[1:]):
         assert s1 >= e0 - 1.0, spans
+
+
+def test_fit_workspace_bound_holds(ctx):
+    """The bound the admission reserves is never below what a fit grows."""
+    for i, (n, m, k) in enumerate([(1000, 128, 3), (3000, 700, 12), (257, 2, 2), (5000, 2048, 32),
+                                   (600, 1500, 4), (100000, 512, 16)]):
+        X, _ = T.generate(70 + i, n, m, k, 0.3)
+        with T.Context(0) as c:
+            d = c.to_device(X)
+            c.znorm(d, ddof=0)
+            before = c.workspace_bytes()
+            c.fit(d, k, max_iter=4, seed=i)
+            grown = c.workspace_bytes() - before
+            bound = c.fit_workspace_bound(n, m, k)
+            d.free()
+        assert 0 < grown <= bound, (n, m, k, grown, bound)
+
+
+def test_budget_below_one_fit_refuses_it_and_models_count(ctx):
+    """A fit whose workspace cannot fit the budget is refused at submission;
+    resident models reduce what is left, and a query that no longer fits is
+    refused too; the server then stops cleanly."""
+    n, m, k = 2000, 256, 4
+    need = _fit_need(ctx, n, m, k)
+    X, _ = T.generate(80, n, m, k, 0.3)
+    with T.Server(0, fit_workers=1, query_workers=1, budget=need - 1) as srv:
+        with pytest.raises(T.TimescError) as e:
+            srv.submit_fit(X, k)
+        assert e.value.code == T.ERR_BUDGET
+    C, _ = T.generate(81, 64, 1024, 8, 0.3)  # a 64-centroid model of m = 1024
+    Q, _ = T.generate(82, 300, 1024, 8, 0.3)
+    with T.Server(0, fit_workers=1, query_workers=1, budget=2_200_000) as srv:
+        mid = srv.add_model(C)  # 1.85 MB resident (centroids, spectra, twiddles)
+        with pytest.raises(T.TimescError) as e:
+            srv.submit_assign(mid, Q)  # 1.25 MB of staging no longer fits beside it
+        assert e.value.code == T.ERR_BUDGET
+        ok = srv.wait(srv.submit_assign(mid, Q[:50]))
+        assert ok["status"] == 0
+
+
+def test_destroy_fails_queued_requests():
+    """Requests still queued when the server stops fail with an error; the
+    running one finishes; destroy returns."""
+    fits = [T.generate(90 + i, 3000, 300, 8, 0.3)[0] for i in range(6)]
+    srv = T.Server(0, fit_workers=1, query_workers=0)
+    hs = [srv.submit_fit(X, 8, max_iter=10) for X in fits]
+    reqs = [h["req"] for h in hs]
+    srv.close()
+    done = [r.done for r in reqs]
+    status = [r.status for r in reqs]
+    assert all(done)
+    assert status[0] == 0 or status[0] == T.ERR_ARG
+    assert any(s == T.ERR_ARG for s in status[1:])
+    failed = [r.message.decode() for r in reqs if r.status == T.ERR_ARG]
+    assert all("stopped before the request ran" in msg for msg in failed)
 
 
 def test_request_errors_stay_with_their_request():

@@ -224,54 +224,104 @@ def test_power_v0_matches_engine():
 
 
 # ---------------------------------------------- the loop, restated in NumPy
-def np_kshape(X, k, init_labels, max_iter, seed):
-    """Independent restatement of _kshape (test_timesC.py:213-265) with the
-    engine's injected initial labels and power start vectors."""
+def np_refine(X, idx, C, seed, it):
+    """One refinement of _kshape (test_timesC.py:228-255) in NumPy: align each
+    cluster to its old centroid (all-zero rule), power iteration of exactly
+    min(m, 1000) steps from power_v0, sign fix, zscore.  Returns the new
+    centroids and the cluster sizes."""
+    n, m = X.shape
+    k = C.shape[0]
+    F = fft_size(m)
+    C = C.copy()
+    counts = np.array([np.sum(idx == j) for j in range(k)])
+    for j in range(k):
+        A = X[idx == j].copy()
+        if len(A) == 0:
+            C[j] = 0
+            continue
+        if np.any(C[j] != 0):  # :128-132, all-zero rule (see test below)
+            for q in range(len(A)):
+                cc = np.real(np.fft.ifft(np.fft.fft(A[q], F) * np.conj(np.fft.fft(C[j], F))))
+                cc = np.concatenate((cc[-(m - 1):], cc[:m]))
+                den = np.linalg.norm(A[q]) * np.linalg.norm(C[j])
+                cc = cc / den if den != 0 else np.zeros_like(cc)
+                A[q] = np_roll_zeropad(A[q], int(np.argmax(cc)) + 1 - m)
+        y = np_zscore(A, 1)
+        S = y.T @ y
+        P = np.eye(m) - np.full((m, m), 1.0 / m)
+        M = P @ S @ P
+        v = O.power_v0(seed, it, j, m)
+        v = v / np.linalg.norm(v)
+        for _ in range(min(m, 1000)):
+            v = M @ v
+            v = v / np.linalg.norm(v)
+        lam = (M @ v) @ v / (v @ v)
+        if math.sqrt(((A[0] - v) ** 2).sum()) >= math.sqrt(((A[0] + v) ** 2).sum()):
+            v = -v
+        assert lam > 0
+        C[j] = np_zscore(v, 1)
+    return C, counts
+
+
+def np_assign(X, C):
+    """dist = 1 - max_lag NCC, label = first argmin (test_timesC.py:258-260)."""
     n, m = X.shape
     F = fft_size(m)
+    Xf = np.fft.fft(X, F)
+    D = np.empty((n, C.shape[0]))
+    for j in range(C.shape[0]):
+        cc = np.real(np.fft.ifft(Xf * np.conj(np.fft.fft(C[j], F))))
+        cc = np.concatenate((cc[:, -(m - 1):], cc[:, :m]), axis=1)
+        den = np.linalg.norm(X, axis=1) * np.linalg.norm(C[j])
+        with np.errstate(invalid="ignore", divide="ignore"):
+            nc = np.where(den[:, None] != 0, cc / den[:, None], 0.0)
+        D[:, j] = 1 - nc.max(axis=1)
+    return D.argmin(axis=1).astype(np.int32)
+
+
+def collapsed(counts, k):
+    """sum_class == 1 || sum_class <= k/2 (kshape.cuh:248, 454)."""
+    s = int(np.sum(counts > 0))
+    return s == 1 or s <= k // 2
+
+
+def np_kshape(X, k, init_labels, max_iter, seed, stop_on_collapse=False):
+    """Independent restatement of _kshape (test_timesC.py:213-265) with the
+    engine's injected initial labels and power start vectors; with
+    stop_on_collapse, the GPU loop's break after extract_shape
+    (kshape.cuh:248-249)."""
+    n, m = X.shape
     idx = init_labels.copy()
     C = np.zeros((k, m))
     hist = []
     for it in range(max_iter):
         old = idx.copy()
-        for j in range(k):
-            A = X[idx == j].copy()
-            if len(A) == 0:
-                C[j] = 0
-                continue
-            if np.any(C[j] != 0):  # :128-132, all-zero rule (see test below)
-                for q in range(len(A)):
-                    cc = np.real(np.fft.ifft(np.fft.fft(A[q], F) * np.conj(np.fft.fft(C[j], F))))
-                    cc = np.concatenate((cc[-(m - 1):], cc[:m]))
-                    den = np.linalg.norm(A[q]) * np.linalg.norm(C[j])
-                    cc = cc / den if den != 0 else np.zeros_like(cc)
-                    A[q] = np_roll_zeropad(A[q], int(np.argmax(cc)) + 1 - m)
-            y = np_zscore(A, 1)
-            S = y.T @ y
-            P = np.eye(m) - np.full((m, m), 1.0 / m)
-            M = P @ S @ P
-            v = O.power_v0(seed, it, j, m)
-            v = v / np.linalg.norm(v)
-            for _ in range(min(m, 1000)):
-                v = M @ v
-                v = v / np.linalg.norm(v)
-            lam = (M @ v) @ v / (v @ v)
-            if math.sqrt(((A[0] - v) ** 2).sum()) >= math.sqrt(((A[0] + v) ** 2).sum()):
-                v = -v
-            assert lam > 0
-            C[j] = np_zscore(v, 1)
-        Xf = np.fft.fft(X, F)
-        D = np.empty((n, k))
-        for j in range(k):
-            cc = np.real(np.fft.ifft(Xf * np.conj(np.fft.fft(C[j], F))))
-            cc = np.concatenate((cc[:, -(m - 1):], cc[:, :m]), axis=1)
-            den = np.linalg.norm(X, axis=1) * np.linalg.norm(C[j])
-            with np.errstate(invalid="ignore", divide="ignore"):
-                nc = np.where(den[:, None] != 0, cc / den[:, None], 0.0)
-            D[:, j] = 1 - nc.max(axis=1)
-        idx = D.argmin(axis=1).astype(np.int32)
+        C, counts = np_refine(X, idx, C, seed, it)
+        if stop_on_collapse and collapsed(counts, k):
+            break
+        idx = np_assign(X, C)
         hist.append(idx.copy())
         if np.array_equal(old, idx):
+            break
+    return np.array(hist).reshape(-1, n), C
+
+
+def np_kshape_tsl(X, k, rows, max_iter, seed, stop_on_collapse=False):
+    """Independent restatement of k_shape_tsl's loop (kshape.cuh:376-461):
+    centres = the chosen rows (z-normalized, as X is), then per iteration
+    assign, stop when the labels equal the previous iteration's (from the
+    second iteration on), refine, and the collapse break (:454-455)."""
+    idx = None
+    C = X[rows].copy()
+    hist = []
+    for it in range(max_iter):
+        new = np_assign(X, C)
+        hist.append(new.copy())
+        if it > 0 and np.array_equal(idx, new):
+            break
+        idx = new
+        C, counts = np_refine(X, idx, C, seed, it)
+        if stop_on_collapse and collapsed(counts, k):
             break
     return np.array(hist), C
 
@@ -285,6 +335,78 @@ def test_kshape_matches_numpy_restatement():
     ref_hist, ref_cent = np_kshape(X, k, init, iters, 5)
     assert hist.shape == ref_hist.shape and np.array_equal(hist, ref_hist)
     assert np.abs(cent - ref_cent).max() <= 1e-8
+
+
+@pytest.mark.parametrize("seed,n,m,k", [(31, 80, 40, 3), (32, 120, 24, 4), (33, 64, 70, 2)])
+def test_kshape_tsl_matches_numpy_restatement(seed, n, m, k):
+    """init_mode 1 (k_shape_tsl, kshape.cuh:376-461; no CPU-path counterpart):
+    the oracle's loop equals a NumPy restatement built from brute-force
+    assignment and eigh-checked refinement, iteration by iteration."""
+    Xf, _ = T.generate(seed, n, m, k, 0.4)
+    X = O.zscore(Xf.astype(np.float64))
+    rows = T.init_rows(seed, n, k)
+    hist, cent = O.kshape(X, k, init_mode=1, init_rows=rows, max_iter=8, seed=seed,
+                          exact_steps=True)
+    ref_hist, ref_cent = np_kshape_tsl(X, k, rows, 8, seed)
+    assert hist.shape == ref_hist.shape and np.array_equal(hist, ref_hist)
+    assert np.abs(cent - ref_cent).max() <= 1e-8
+
+
+def test_kshape_tsl_first_assignment_is_brute_force_against_the_rows():
+    """The first labels of init_mode 1 are the brute-force SBD argmin against
+    the chosen rows themselves (O(m^2) correlation, no FFT)."""
+    n, m, k = 40, 20, 3
+    Xf, _ = T.generate(41, n, m, k, 0.5)
+    X = O.zscore(Xf.astype(np.float64))
+    rows = T.init_rows(41, n, k)
+    hist, _ = O.kshape(X, k, init_mode=1, init_rows=rows, max_iter=1)
+    want = [int(np.argmin([1 - direct_ncc(X[i], X[r]).max() for r in rows])) for i in range(n)]
+    assert np.array_equal(hist[0], want)
+
+
+def test_kshape_tsl_loop_order_known_answer():
+    """Noise-free, shift-free, separated classes with one centre row per
+    class: k_shape_tsl assigns the true labels, refines, assigns them again
+    and stops -- two iterations (assign first), where the label-initialized
+    loop started from the true labels stops after one (refine first)."""
+    n, m, k = 30, 40, 3
+    t = np.arange(m)
+    protos = [np.sin(2 * np.pi * f * t / m) for f in (1, 3, 7)]
+    y = np.repeat(np.arange(k), n // k).astype(np.int32)
+    rng = np.random.default_rng(0)
+    X = O.zscore(np.stack([rng.uniform(0.5, 2) * protos[c] + rng.normal() for c in y]))
+    rows = np.array([0, 10, 20], np.int32)
+    hist, _ = O.kshape(X, k, init_mode=1, init_rows=rows, max_iter=5)
+    assert len(hist) == 2 and np.array_equal(hist[0], y) and np.array_equal(hist[1], y)
+    hist0, _ = O.kshape(X, k, init_labels=y, max_iter=5)
+    assert len(hist0) == 1
+
+
+@pytest.mark.parametrize("init_mode", [0, 1])
+def test_collapse_stop_matches_numpy_restatement(init_mode):
+    """Two-class data fitted with k = 4: the clusters collapse to exactly
+    k/2 = 2 non-empty ones.  With the GPU loop's collapse break the fit stops
+    right after that refinement, keeping the labels of the assignment
+    before it; without it the loop runs on (the CPU path)."""
+    n, m, k = 60, 32, 4
+    Xf, _ = T.generate(2, n, m, 2, 0.1)
+    X = O.zscore(Xf.astype(np.float64))
+    if init_mode == 0:
+        init = T.init_labels(2, n, k)
+        hist, _, col = O.kshape(X, k, init_labels=init, max_iter=10, seed=2, exact_steps=True,
+                                stop_on_collapse=True, return_collapsed=True)
+        ref, _ = np_kshape(X, k, init, 10, 2, stop_on_collapse=True)
+    else:
+        rows = T.init_rows(2, n, k)
+        hist, _, col = O.kshape(X, k, init_mode=1, init_rows=rows, max_iter=10, seed=2,
+                                exact_steps=True, stop_on_collapse=True, return_collapsed=True)
+        ref, _ = np_kshape_tsl(X, k, rows, 10, 2, stop_on_collapse=True)
+    assert col
+    assert hist.shape == ref.shape and np.array_equal(hist, ref)
+    assert len(set(hist[-1])) == k // 2
+    full = O.kshape(X, k, init_mode=init_mode, max_iter=10, seed=2, exact_steps=True,
+                    **({"init_labels": init} if init_mode == 0 else {"init_rows": rows}))[0]
+    assert len(full) > len(hist)
 
 
 def test_sum_pairwise_is_numpy_sum_bit_exact():

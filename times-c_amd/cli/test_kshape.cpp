@@ -6,10 +6,13 @@
 // CLI k ignored, test_kshape.cu:24-30), 10 iterations (:21);
 // flag 1: unlabeled CSV, k from the command line, 20 iterations (:90).
 // init_chose 0: random initial labels (k_shape); 1: random initial centres
-// (k_shape_tsl).  When the fit ends with one non-empty cluster or fewer
-// than k/2, it is rerun as k_shape with eigenvector centroids (cal_select
-// 1), as test_kshape.cu:54-59 does.  Output: one-based labels "l,l,...,l,"
-// (readfile.cuh:120).
+// (k_shape_tsl).  Like the reference's k_shape and k_shape_tsl, a fit stops
+// right after a refinement that leaves one non-empty cluster or at most k/2
+// (kshape.cuh:248-249, 454-455), keeping the labels of the last assignment.
+// When the fit ends with one non-empty cluster or fewer than k/2, it is
+// rerun as k_shape with eigenvector centroids (cal_select 1), as
+// test_kshape.cu:54-59 does (exactly k/2 stops without a rerun).  Output:
+// one-based labels "l,l,...,l," (readfile.cuh:120).
 //
 // Optional trailing flags (no reference counterpart):
 //   --out=PATH          label file (default ../out/cukshape/result.txt)
@@ -25,6 +28,8 @@
 //   --budget=BYTES      HBM budget of the context
 //   --cpu-format        write zero-based labels one per line, the CPU
 //                       path's output (test_timesC.py:324-325)
+//   --no-collapse-stop  run every iteration like the CPU path (_kshape has
+//                       no collapse stop, test_timesC.py:223-264)
 // <file> may also be a binary matrix (tcx_write_binary, "TCXB0001" magic);
 // flag 0 then needs the file's labels.
 #include <chrono>
@@ -84,6 +89,7 @@ int main(int argc, char** argv) {
   bool cpu_format = false;
   tcx_fit_opts o;
   tcx_fit_opts_default(&o);
+  o.stop_on_collapse = 1;
   for (int i = 7; i < argc; ++i) {
     std::string v;
     if (opt(argv[i], "--out", &v)) out = v;
@@ -96,6 +102,7 @@ int main(int argc, char** argv) {
     else if (opt(argv[i], "--centroids", &v)) cent_path = v;
     else if (opt(argv[i], "--budget", &v)) budget = std::strtoull(v.c_str(), nullptr, 10);
     else if (std::strcmp(argv[i], "--cpu-format") == 0) cpu_format = true;
+    else if (std::strcmp(argv[i], "--no-collapse-stop") == 0) o.stop_on_collapse = 0;
     else {
       std::fprintf(stderr, "unknown option %s\n", argv[i]);
       return 2;
@@ -165,8 +172,8 @@ int main(int argc, char** argv) {
   tcx_fit_stats st;
   CHECKRC(tcx_kshape_fit(ctx, d, rows, cols, k, &o, labels.data(), cent.data(), &st));
   auto t2 = std::chrono::steady_clock::now();
-  std::printf("iterations %d converged %d sum_class %d power_steps_max %d\n", st.iterations,
-              st.converged, st.sum_class, st.power_steps_max);
+  std::printf("iterations %d converged %d collapsed %d sum_class %d power_steps_max %d\n",
+              st.iterations, st.converged, st.collapsed, st.sum_class, st.power_steps_max);
   std::printf("refine %.3f ms assign %.3f ms\n", st.ms_refine, st.ms_assign);
   std::printf("total time run on GPU: %f msecs.\n",
               std::chrono::duration<double, std::milli>(t2 - t0).count());
@@ -179,8 +186,8 @@ int main(int argc, char** argv) {
     o2.cal_select = 1;
     CHECKRC(tcx_kshape_fit(ctx, d, rows, cols, k, &o2, labels.data(), cent.data(), &st));
     t2 = std::chrono::steady_clock::now();
-    std::printf("iterations %d converged %d sum_class %d\n", st.iterations, st.converged,
-                st.sum_class);
+    std::printf("iterations %d converged %d collapsed %d sum_class %d\n", st.iterations,
+                st.converged, st.collapsed, st.sum_class);
     std::printf("total time run on GPU: %f msecs.\n",
                 std::chrono::duration<double, std::milli>(t2 - t0).count());
   }

@@ -5,10 +5,21 @@
 // device staging buffers for its requests' series and results.  Fit
 // workers take the oldest fit (or, when no fit waits, the oldest query);
 // query workers take only queries, so a burst of long fits cannot delay
-// short queries.  A request is admitted when its staging bytes fit in the
-// budget left after resident models and the requests already running; it
-// waits in its queue otherwise (first come, first served per queue).
-// No reference counterpart.
+// short queries.
+//
+// HBM budget.  A request needs its staging bytes plus, for a fit, the
+// workspace bound of a fit of its size (tcx_fit_workspace_bytes), for a
+// query the twiddle table.  Charged against the budget are the resident
+// models, every running worker's reservation and every idle worker's held
+// bytes (workspace slots and staging buffers stay allocated between
+// requests).  A worker takes the oldest request of its queues when the
+// charge with the request added fits the budget, releasing its own held
+// bytes first when only that makes it fit; when idle workers' held bytes
+// block it, those workers release them.  A request that cannot fit even
+// with every worker idle and empty (need > budget - resident models) is
+// refused at submission, or failed when a later model makes it so; since
+// running requests end and idle workers release, every queued request is
+// eventually admitted.  No reference counterpart.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -42,12 +53,17 @@ struct Model {
 struct Pending {
   tcx_request* req;
   Clock::time_point t_submit;
-  size_t bytes;
+  size_t bytes;  // what the request needs (staging + workspace bound)
 };
 
 struct Worker {
   tcx_ctx* ctx = nullptr;
   bool query_only = false;
+  // budget state, under the server's lock
+  bool busy = false;      // running a request (or releasing its workspace)
+  bool trim = false;      // asked to release its workspace while idle
+  size_t held = 0;        // bytes its context holds while idle
+  size_t reserved = 0;    // bytes charged while busy
   float* d_x = nullptr;
   size_t x_cap = 0;
   int* d_i = nullptr;  // labels, then shifts
@@ -60,7 +76,7 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
-size_t request_bytes(const tcx_request& r) {
+size_t staging_bytes(const tcx_request& r) {
   return (size_t)r.n * r.m * sizeof(float) + (size_t)r.n * 3 * sizeof(int);
 }
 
@@ -75,7 +91,8 @@ struct tcx_server {
   tcx_ctx* admin = nullptr;  // uploads models (under admin_mu)
   std::mutex admin_mu;
   int fit_workers = 0;
-  size_t budget = 0, used = 0;
+  size_t budget = 0;
+  size_t resident = 0;  // bytes of the resident models (and those being added)
   int running = 0;
   bool stop = false;
 };
@@ -108,6 +125,62 @@ int stage(Worker& w, long n, int m) {
   return TCX_OK;
 }
 
+void free_staging(Worker& w) {
+  if (w.d_x) tcx_free(w.ctx, w.d_x);
+  if (w.d_i) tcx_free(w.ctx, w.d_i);
+  if (w.d_f) tcx_free(w.ctx, w.d_f);
+  w.d_x = nullptr;
+  w.d_i = nullptr;
+  w.d_f = nullptr;
+  w.x_cap = w.n_cap = 0;
+}
+
+// Bytes charged against the budget, leaving out worker `except` (lock held).
+size_t charged(const tcx_server* s, const Worker* except) {
+  size_t b = s->resident;
+  for (const Worker* v : s->workers)
+    if (v != except) b += v->busy ? v->reserved : v->held;
+  return b;
+}
+
+// Asks the idle workers other than `except` that hold workspace to release
+// it (lock held).
+void request_trims(tcx_server* s, const Worker* except) {
+  bool any = false;
+  for (Worker* v : s->workers)
+    if (v != except && !v->busy && !v->trim && v->held > 0) {
+      v->trim = true;
+      any = true;
+    }
+  if (any) s->cv_work.notify_all();
+}
+
+// Fails the queued requests that can no longer fit the budget beside the
+// resident models (lock held).  Returns true if any was failed.
+bool fail_unfit(tcx_server* s) {
+  if (!s->budget) return false;
+  bool any = false;
+  for (auto* q : {&s->fitq, &s->asgq})
+    for (auto it = q->begin(); it != q->end();) {
+      if (it->bytes + s->resident > s->budget) {
+        tcx_request& r = *it->req;
+        r.status = TCX_ERR_BUDGET;
+        std::snprintf(r.message, sizeof(r.message),
+                      "request needs %zu bytes; the HBM budget leaves %zu beside the resident "
+                      "models", it->bytes, s->budget - std::min(s->budget, s->resident));
+        r.ms_queued = r.ms_latency = ms_between(it->t_submit, Clock::now());
+        r.done = 1;
+        it = q->erase(it);
+        any = true;
+      } else {
+        ++it;
+      }
+    }
+  return any;
+}
+
+int add_model(tcx_server* s, const float* h_centroids, int k, int m, int* model_id);
+
 int run_fit(tcx_server* s, Worker& w, tcx_request& r) {
   int rc = stage(w, r.n, r.m);
   if (rc == TCX_OK)
@@ -121,7 +194,15 @@ int run_fit(tcx_server* s, Worker& w, tcx_request& r) {
   }
   if (rc == TCX_OK) rc = tcx_kshape_fit(w.ctx, w.d_x, r.n, r.m, r.k, &r.opts, r.h_labels, hc,
                                         &r.stats);
-  if (rc == TCX_OK && r.publish_model) rc = tcx_server_add_model(s, hc, r.k, r.m, &r.model_out);
+  if (rc == TCX_OK && r.publish_model) {
+    {
+      // the fit is over: charge what this worker's context holds, not its
+      // reservation, while the model is added
+      std::lock_guard<std::mutex> lk(s->mu);
+      w.reserved = tcx::held_bytes(w.ctx);
+    }
+    rc = add_model(s, hc, r.k, r.m, &r.model_out);
+  }
   return rc;
 }
 
@@ -153,15 +234,35 @@ int run_assign(tcx_server* s, Worker& w, tcx_request& r) {
   return rc;
 }
 
-// Next admissible request for this worker (called with the lock held).
-bool take(tcx_server* s, Worker& w, Pending* out) {
-  auto fits = [&](const Pending& p) { return !s->budget || s->used + p.bytes <= s->budget; };
-  if (!w.query_only && !s->fitq.empty() && fits(s->fitq.front())) {
+// Takes the next admissible request for this worker (lock held).  Sets
+// *self_trim when the worker must release its own workspace first.
+bool take(tcx_server* s, Worker& w, Pending* out, bool* self_trim) {
+  auto admit = [&](const Pending& p) {
+    if (!s->budget) {
+      *self_trim = false;
+      w.reserved = 0;
+      return true;
+    }
+    const size_t others = charged(s, &w);
+    if (others + w.held + p.bytes <= s->budget) {
+      *self_trim = false;
+      w.reserved = w.held + p.bytes;
+      return true;
+    }
+    if (others + p.bytes <= s->budget) {
+      *self_trim = true;
+      w.reserved = p.bytes;
+      return true;
+    }
+    request_trims(s, &w);  // idle workers' held bytes may be what blocks it
+    return false;
+  };
+  if (!w.query_only && !s->fitq.empty() && admit(s->fitq.front())) {
     *out = s->fitq.front();
     s->fitq.pop_front();
     return true;
   }
-  if (!s->asgq.empty() && fits(s->asgq.front())) {
+  if (!s->asgq.empty() && admit(s->asgq.front())) {
     *out = s->asgq.front();
     s->asgq.pop_front();
     return true;
@@ -172,14 +273,33 @@ bool take(tcx_server* s, Worker& w, Pending* out) {
 void worker_main(tcx_server* s, Worker* w) {
   for (;;) {
     Pending p{nullptr, Clock::time_point(), 0};
+    bool self_trim = false, trim_only = false;
     {
       std::unique_lock<std::mutex> lk(s->mu);
       s->cv_work.wait(lk, [&] {
-        return (s->stop && s->fitq.empty() && s->asgq.empty()) || take(s, *w, &p);
+        if (s->stop) return true;
+        if (w->trim) return trim_only = true;
+        return take(s, *w, &p, &self_trim);
       });
-      if (!p.req) return;  // stopping with empty queues
-      s->used += p.bytes;
-      ++s->running;
+      if (!trim_only && !p.req) return;  // stopping (queued requests were failed)
+      w->trim = false;
+      w->busy = true;
+      if (trim_only) w->reserved = w->held;
+      else ++s->running;
+    }
+    if (trim_only || self_trim) {
+      free_staging(*w);
+      (void)tcx::trim_workspace(w->ctx);
+      tcx::clear_error();
+    }
+    if (trim_only) {
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        w->held = tcx::held_bytes(w->ctx);
+        w->busy = false;
+      }
+      s->cv_work.notify_all();  // bytes released
+      continue;
     }
     tcx_request& r = *p.req;
     const Clock::time_point ts = Clock::now();
@@ -196,7 +316,8 @@ void worker_main(tcx_server* s, Worker* w) {
     r.ms_latency = ms_between(p.t_submit, te);
     {
       std::lock_guard<std::mutex> lk(s->mu);
-      s->used -= p.bytes;
+      w->held = tcx::held_bytes(w->ctx);
+      w->busy = false;
       --s->running;
       r.done = 1;
     }
@@ -212,6 +333,53 @@ void free_model(tcx_ctx* c, Model& md) {
   if (md.d_norm) tcx_free(c, md.d_norm);
   if (md.d_zero) tcx_free(c, md.d_zero);
   md = Model();
+}
+
+int add_model(tcx_server* s, const float* h_centroids, int k, int m, int* model_id) {
+  if (!s || !h_centroids || !model_id || k < 1 || m < 2)
+    return tcx::fail(TCX_ERR_ARG, "add_model: bad arguments");
+  if (m > 4096) return tcx::fail(TCX_ERR_UNSUPPORTED, "add_model: series length > 4096");
+  const int F = tcx::fft_length(m);
+  Model md;
+  md.k = k;
+  md.m = m;
+  // centroids, spectra, magnitudes, norms, zero flags, and the upload
+  // context's twiddle table
+  md.bytes = (size_t)k * m * 4 + (size_t)k * F * 12 + (size_t)k * 8 + (size_t)F * 8;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);  // reserve the model's bytes
+    if (s->budget && charged(s, nullptr) + md.bytes > s->budget) {
+      request_trims(s, nullptr);
+      return tcx::fail(TCX_ERR_BUDGET, "add_model: the model (" + std::to_string(md.bytes) +
+                                           " bytes) does not fit the HBM budget left");
+    }
+    s->resident += md.bytes;
+  }
+  std::lock_guard<std::mutex> alk(s->admin_mu);  // one upload at a time
+  tcx_ctx* c = s->admin;
+  int rc = tcx_malloc(c, (void**)&md.d_cent, (size_t)k * m * 4);
+  if (rc == TCX_OK) rc = tcx_malloc(c, &md.d_spec, (size_t)k * F * 8);
+  if (rc == TCX_OK) rc = tcx_malloc(c, (void**)&md.d_mag, (size_t)k * F * 4);
+  if (rc == TCX_OK) rc = tcx_malloc(c, (void**)&md.d_norm, (size_t)k * 4);
+  if (rc == TCX_OK) rc = tcx_malloc(c, (void**)&md.d_zero, (size_t)k * 4);
+  if (rc == TCX_OK) rc = tcx_memcpy_h2d(c, md.d_cent, h_centroids, (size_t)k * m * 4);
+  if (rc == TCX_OK)
+    rc = tcx::model_spectra(c, md.d_cent, k, m, md.d_spec, md.d_mag, md.d_norm, md.d_zero);
+  if (rc == TCX_OK) rc = tcx::stream_sync(c);
+  bool failed_some = false;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (rc != TCX_OK) {
+      s->resident -= md.bytes;
+      free_model(c, md);
+      return rc;
+    }
+    s->models.push_back(md);
+    *model_id = (int)s->models.size() - 1;
+    failed_some = fail_unfit(s);
+  }
+  if (failed_some) s->cv_done.notify_all();
+  return TCX_OK;
 }
 
 }  // namespace
@@ -256,39 +424,7 @@ int tcx_server_create(int device, int fit_workers, int query_workers, size_t hbm
 
 int tcx_server_add_model(tcx_server* s, const float* h_centroids, int k, int m, int* model_id) {
   tcx::clear_error();
-  if (!s || !h_centroids || !model_id || k < 1 || m < 2)
-    return tcx::fail(TCX_ERR_ARG, "add_model: bad arguments");
-  const int F = tcx::fft_length(m);
-  Model md;
-  md.k = k;
-  md.m = m;
-  md.bytes = (size_t)k * m * 4 + (size_t)k * F * 12 + (size_t)k * 8;
-  {
-    std::lock_guard<std::mutex> lk(s->mu);  // reserve the model's bytes
-    if (s->budget && s->used + md.bytes > s->budget)
-      return tcx::fail(TCX_ERR_BUDGET, "add_model: the model exceeds the HBM budget");
-    s->used += md.bytes;
-  }
-  std::lock_guard<std::mutex> alk(s->admin_mu);  // one upload at a time
-  tcx_ctx* c = s->admin;
-  int rc = tcx_malloc(c, (void**)&md.d_cent, (size_t)k * m * 4);
-  if (rc == TCX_OK) rc = tcx_malloc(c, &md.d_spec, (size_t)k * F * 8);
-  if (rc == TCX_OK) rc = tcx_malloc(c, (void**)&md.d_mag, (size_t)k * F * 4);
-  if (rc == TCX_OK) rc = tcx_malloc(c, (void**)&md.d_norm, (size_t)k * 4);
-  if (rc == TCX_OK) rc = tcx_malloc(c, (void**)&md.d_zero, (size_t)k * 4);
-  if (rc == TCX_OK) rc = tcx_memcpy_h2d(c, md.d_cent, h_centroids, (size_t)k * m * 4);
-  if (rc == TCX_OK)
-    rc = tcx::model_spectra(c, md.d_cent, k, m, md.d_spec, md.d_mag, md.d_norm, md.d_zero);
-  if (rc == TCX_OK) rc = tcx::stream_sync(c);
-  std::lock_guard<std::mutex> lk(s->mu);
-  if (rc != TCX_OK) {
-    s->used -= md.bytes;
-    free_model(c, md);
-    return rc;
-  }
-  s->models.push_back(md);
-  *model_id = (int)s->models.size() - 1;
-  return TCX_OK;
+  return add_model(s, h_centroids, k, m, model_id);
 }
 
 int tcx_server_submit(tcx_server* s, tcx_request* r) {
@@ -303,14 +439,26 @@ int tcx_server_submit(tcx_server* s, tcx_request* r) {
     return tcx::fail(TCX_ERR_ARG, "submit: unknown request kind");
   if (!r->h_x || !r->h_labels || r->n < 1 || r->m < 2)
     return tcx::fail(TCX_ERR_ARG, "submit: bad series or label buffer");
-  Pending p{r, Clock::now(), request_bytes(*r)};
+  if (r->m > 4096) return tcx::fail(TCX_ERR_UNSUPPORTED, "submit: series length > 4096");
+  if (r->kind == TCX_REQ_FIT && (r->k < 1 || r->k > r->n))
+    return tcx::fail(TCX_ERR_ARG, "submit: fit needs 1 <= k <= n");
+  size_t need = staging_bytes(*r);
+  if (r->kind == TCX_REQ_FIT)
+    need += tcx_fit_workspace_bytes(s->admin, r->n, r->m, r->k);
+  else
+    need += (size_t)tcx::fft_length(r->m) * 8;  // the worker's twiddle table
+  Pending p{r, Clock::now(), need};
   {
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) return tcx::fail(TCX_ERR_ARG, "submit: server is stopping");
     if (r->kind == TCX_REQ_FIT && s->fit_workers == 0)
       return tcx::fail(TCX_ERR_ARG, "submit: the server has no fit workers");
-    if (s->budget && p.bytes > s->budget)
-      return tcx::fail(TCX_ERR_BUDGET, "submit: request exceeds the HBM budget");
+    if (s->budget && p.bytes + s->resident > s->budget)
+      return tcx::fail(TCX_ERR_BUDGET,
+                       "submit: request needs " + std::to_string(p.bytes) +
+                           " bytes; the HBM budget leaves " +
+                           std::to_string(s->budget - std::min(s->budget, s->resident)) +
+                           " beside the resident models");
     (r->kind == TCX_REQ_FIT ? s->fitq : s->asgq).push_back(p);
   }
   s->cv_work.notify_all();
@@ -330,16 +478,25 @@ int tcx_server_destroy(tcx_server* s) {
   tcx::clear_error();
   if (!s) return TCX_OK;
   {
+    // requests still queued fail; running ones finish
     std::lock_guard<std::mutex> lk(s->mu);
     s->stop = true;
+    for (auto* q : {&s->fitq, &s->asgq}) {
+      for (Pending& p : *q) {
+        p.req->status = TCX_ERR_ARG;
+        std::snprintf(p.req->message, sizeof(p.req->message),
+                      "the server stopped before the request ran");
+        p.req->done = 1;
+      }
+      q->clear();
+    }
   }
   s->cv_work.notify_all();
+  s->cv_done.notify_all();
   int rc = TCX_OK;
   for (Worker* w : s->workers) {
     w->th.join();
-    if (w->d_x) tcx_free(w->ctx, w->d_x);
-    if (w->d_i) tcx_free(w->ctx, w->d_i);
-    if (w->d_f) tcx_free(w->ctx, w->d_f);
+    free_staging(*w);
     int r = tcx_destroy(w->ctx);
     if (rc == TCX_OK) rc = r;
     delete w;

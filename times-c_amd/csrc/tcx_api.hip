@@ -64,6 +64,7 @@ struct FitState {
   double tol = 1e-12;
   int split = 1;
   int it = 0, converged = 0, sum_class = 0, steps_max = 0, fallbacks = 0;
+  int collapsed = 0;  // stopped on collapse (opts.stop_on_collapse)
   long n_pad = 0;
   bool have_order = false;   // the refine's cluster-sorted rows are valid
   const double2* row_stats = nullptr;  // per-row sums of d_x (fixed during the fit)
@@ -615,6 +616,70 @@ int stream_sync(tcx_ctx* c) {
   return TCX_OK;
 }
 
+size_t held_bytes(const tcx_ctx* c) { return c->ws_bytes + c->user_bytes; }
+
+int trim_workspace(tcx_ctx* c) {
+  TCX_TRY(set_dev(c));
+  if (c->fit.active) return fail(TCX_ERR_ARG, "trim: a fit is in progress");
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  for (int s = 0; s < kNumSlots; ++s)
+    if (c->slot_ptr[s]) {
+      TCX_HIP(hipFree(c->slot_ptr[s]));
+      c->ws_bytes -= c->slot_bytes[s];
+      c->slot_ptr[s] = nullptr;
+      c->slot_bytes[s] = 0;
+    }
+  for (auto& kv : c->twiddles) {
+    TCX_HIP(hipFree(kv.second));
+    c->ws_bytes -= sizeof(float2) * kv.first;
+  }
+  c->twiddles.clear();
+  // a new exchange buffer must be zeroed before its first use, even at the
+  // same address
+  c->lz_xch = nullptr;
+  c->lz_xch_bytes = 0;
+  c->fit = FitState();
+  return TCX_OK;
+}
+
+// Upper bound of the workspace bytes one fit of n series of length m into k
+// clusters allocates (every slot that tcx_fit_begin, the refinements and the
+// assignments grow, each rounded up to 256 bytes, plus the twiddle table).
+size_t fit_workspace_bound(long n, int m, int k, int cus) {
+  auto r = [](size_t b) { return (std::max<size_t>(b, 1) + 255) & ~(size_t)255; };
+  const size_t F = (size_t)fft_size(m), N = (size_t)n, Kc = (size_t)k, Mm = (size_t)m;
+  const size_t kp = (Kc + 15) & ~(size_t)15;
+  size_t b = 0;
+  // labels, shifts, distances; current and new centroids; centroid spectra
+  b += 3 * r(4 * N) + 2 * r(4 * Kc * Mm) + r(8 * Kc * F) + r(4 * Kc * F) + 2 * r(4 * Kc);
+  // series half spectra, norms, bound rows, bounds, centroid bound rows, row sums
+  b += r(8 * N * (F / 2 + 1)) + r(4 * N) + r(2 * (size_t)bound_rows(n) * bound_kpad((int)F)) +
+       r(4 * N * kp) + r(2 * (size_t)bound_rows(k) * bound_kpad((int)F)) + r(16 * N);
+  // grouping (with distances: the most bins)
+  const size_t B = (size_t)group_bins(n, k, true), keys = Kc * B;
+  const size_t nblk = (N + kGroupChunk - 1) / kGroupChunk;
+  b += 2 * r(4 * nblk * keys) + r(8 * (Kc + 1) + 4 * (Kc + 2)) + r(4 * keys) + r(8 * keys) +
+       r(4 * Kc);
+  // row order, Y panels (hi, lo), sign-fix rows
+  const size_t npad = std::max<size_t>((N + Kc * (kPackPad - 1)) / kPackPad * kPackPad, kPackPad);
+  b += r(4 * npad) + 2 * r(2 * (size_t)panel_blocks(m) * 32 * npad) + r(4 * Kc * Mm);
+  // S work list and partials: plan_syrk keeps (groups x tiles) within one
+  // round of the device (or one group per cluster) and a chunk of at least
+  // 256 columns
+  const size_t T = (Mm + kSyrkTile - 1) / kSyrkTile, tp = T * (T + 1) / 2;
+  const size_t groups = std::min(std::max((cus + 8 * tp) / tp, Kc), npad / 256 + Kc);
+  const size_t nparts = groups * tp;
+  b += r(sizeof(SyrkItem) * (nparts + 8 * tp) + 256 + sizeof(SyrkTile) * Kc * tp);
+  b += r(4 * nparts * kSyrkTile * kSyrkTile) + r(4 * Kc * Mm * Mm) + r(sizeof(PowerInfo) * Kc);
+  // power stage: basis, v, S v, r, w, counters, states, exchange words, mask
+  const size_t xch = Kc * (2 * (2 * (size_t)kLzCoopMaxG * (kLanczosMax + 2) + Mm) + kLzCoopMaxG);
+  b += r(8 * Kc * (kLanczosMax + 1) * Mm) + 4 * r(8 * Kc * Mm) + 2 * r(4 * Kc) +
+       r(sizeof(LzState) * Kc) + r(8 * xch + 4 * (Kc + 1)) + r(4 * Kc);
+  // evaluation count and changed flag; twiddles
+  b += r(8 * 17) + 8 * F;
+  return b;
+}
+
 }  // namespace tcx
 
 // ======================================================================= ABI
@@ -688,6 +753,11 @@ int tcx_destroy(tcx_ctx* c) {
 }
 
 size_t tcx_workspace_bytes(const tcx_ctx* c) { return c ? c->ws_bytes : 0; }
+
+size_t tcx_fit_workspace_bytes(const tcx_ctx* c, int n, int m, int k) {
+  if (!c || n < 1 || m < 2 || k < 1) return 0;
+  return tcx::fit_workspace_bound(n, m, k, c->cus);
+}
 
 int tcx_malloc(tcx_ctx* c, void** p, size_t bytes) {
   TCX_TRY(set_dev(c));
@@ -1039,10 +1109,22 @@ int tcx_fit_step(tcx_ctx* c, int* changed) {
   TCX_TRY(set_dev(c));
   FitState& f = c->fit;
   if (!f.active) return fail(TCX_ERR_ARG, "fit_step: no fit in progress (call tcx_fit_begin)");
+  if (f.collapsed) return fail(TCX_ERR_ARG, "fit_step: the fit stopped on collapse");
   int h_changed = 0;
+  // the GPU path's stop after a refinement that leaves too few non-empty
+  // clusters (kshape.cuh:248-249, 454-455)
+  auto collapse = [&f]() {
+    return f.o.stop_on_collapse && (f.sum_class == 1 || f.sum_class <= f.k / 2);
+  };
   if (f.o.init_mode == 0) {
     // test_timesC.py:223-264: refine, assign, stop when labels repeat
     TCX_TRY(fit_refine(c, f));
+    if (collapse()) {
+      // no assignment: the labels stay those of the previous one
+      f.collapsed = 1;
+      if (changed) *changed = 0;
+      return TCX_OK;
+    }
     TCX_TRY(fit_assign(c, f, &h_changed));
     if (!h_changed) f.converged = 1;
   } else {
@@ -1052,6 +1134,7 @@ int tcx_fit_step(tcx_ctx* c, int* changed) {
       f.converged = 1;
     else
       TCX_TRY(fit_refine(c, f));
+    if (!f.converged && collapse()) f.collapsed = 1;
   }
   ++f.it;
   if (changed) *changed = f.it == 1 && f.o.init_mode == 1 ? 1 : h_changed;
@@ -1086,8 +1169,28 @@ int tcx_fit_query(tcx_ctx* c, tcx_fit_stats* stats) {
   stats->ms_assign_bounds = f.ms_bnd;
   stats->sbd_evals = f.evals;
   stats->sbd_pairs = f.pairs;
+  stats->collapsed = f.collapsed;
   stats->ms_total = std::chrono::duration<double, std::milli>(
                         std::chrono::steady_clock::now() - f.t_start).count();
+  return TCX_OK;
+}
+
+int tcx_fit_snapshot(tcx_ctx* c, int* h_labels, float* h_dist, int* h_shift, float* h_centroids) {
+  TCX_TRY(set_dev(c));
+  const FitState& f = c->fit;
+  if (!f.active) return fail(TCX_ERR_ARG, "fit_snapshot: no fit in progress");
+  if (h_dist && !f.have_dist) return fail(TCX_ERR_ARG, "fit_snapshot: no assignment yet");
+  hipStream_t st = c->stream;
+  if (h_labels)
+    TCX_HIP(hipMemcpyAsync(h_labels, f.label, sizeof(int) * f.n, hipMemcpyDeviceToHost, st));
+  if (h_dist)
+    TCX_HIP(hipMemcpyAsync(h_dist, f.dist, sizeof(float) * f.n, hipMemcpyDeviceToHost, st));
+  if (h_shift)
+    TCX_HIP(hipMemcpyAsync(h_shift, f.shift, sizeof(int) * f.n, hipMemcpyDeviceToHost, st));
+  if (h_centroids)
+    TCX_HIP(hipMemcpyAsync(h_centroids, f.cent, sizeof(float) * f.k * f.m, hipMemcpyDeviceToHost,
+                           st));
+  TCX_HIP(hipStreamSynchronize(st));
   return TCX_OK;
 }
 
@@ -1114,6 +1217,7 @@ int tcx_fit_end(tcx_ctx* c, int* h_labels, float* h_centroids, tcx_fit_stats* st
     stats->ms_assign_bounds = f.ms_bnd;
     stats->sbd_evals = f.evals;
     stats->sbd_pairs = f.pairs;
+    stats->collapsed = f.collapsed;
     stats->ms_total = std::chrono::duration<double, std::milli>(
                           std::chrono::steady_clock::now() - f.t_start).count();
   }
@@ -1126,7 +1230,7 @@ int tcx_kshape_fit(tcx_ctx* c, const float* d_x, int n, int m, int k, const tcx_
   if (!h_labels) return fail(TCX_ERR_ARG, "fit: null label buffer");
   TCX_TRY(tcx_fit_begin(c, d_x, n, m, k, opts));
   FitState& f = c->fit;
-  while (f.it < f.o.max_iter && !f.converged) {
+  while (f.it < f.o.max_iter && !f.converged && !f.collapsed) {
     int rc = tcx_fit_step(c, nullptr);
     if (rc != TCX_OK) {
       f.active = false;
@@ -1143,10 +1247,13 @@ int tcx_kshape(tcx_ctx* c, const float* d_x, int* h_idx, int n, int m, int k, in
   tcx_fit_opts_default(&o);
   o.max_iter = *iter_num;
   o.cal_select = cal_select;
+  o.stop_on_collapse = 1;  // kshape.cuh:248-249
   tcx_fit_stats s;
   TCX_TRY(tcx_kshape_fit(c, d_x, n, m, k, &o, h_idx, nullptr, &s));
   for (int i = 0; i < n; ++i) h_idx[i] += 1;  // one-based, kshape.cuh:91
-  *iter_num = s.iterations;
+  // the reference sets iter_num only when the labels stop changing
+  // (kshape.cuh:297); after a collapse it keeps the cap
+  if (!s.collapsed) *iter_num = s.iterations;
   if (sum_class) *sum_class = s.sum_class;
   return TCX_OK;
 }
@@ -1160,10 +1267,11 @@ int tcx_kshape_tsl(tcx_ctx* c, const float* d_x, const float* h_x, int* h_idx, i
   o.max_iter = *iter_num;
   o.init_mode = 1;
   o.cal_select = cal_select;
+  o.stop_on_collapse = 1;  // kshape.cuh:454-455
   tcx_fit_stats s;
   TCX_TRY(tcx_kshape_fit(c, d_x, n, m, k, &o, h_idx, nullptr, &s));
   for (int i = 0; i < n; ++i) h_idx[i] += 1;
-  *iter_num = s.iterations;
+  if (!s.collapsed) *iter_num = s.iterations;
   if (sum_class) *sum_class = s.sum_class;
   return TCX_OK;
 }

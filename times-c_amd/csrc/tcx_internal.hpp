@@ -21,5 +21,14 @@ int assign_resident(tcx_ctx* c, const float* d_x, long n, int m, int k, const vo
                     int* shift);
 // Waits for the context's stream.
 int stream_sync(tcx_ctx* c);
+// Device bytes the context holds: workspace slots, twiddle tables and
+// tcx_malloc allocations.
+size_t held_bytes(const tcx_ctx* c);
+// Frees every workspace slot and twiddle table of an idle context (no fit
+// in progress); the next call grows them again.
+int trim_workspace(tcx_ctx* c);
+// Upper bound of the workspace one fit of (n, m, k) grows on a device with
+// `cus` compute units.
+size_t fit_workspace_bound(long n, int m, int k, int cus);
 
 }  // namespace tcx

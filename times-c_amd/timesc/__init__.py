@@ -7,8 +7,11 @@ library has not been built: there is no CPU fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
+import itertools
 import os
+import weakref
 
 import numpy as np
 
@@ -23,11 +26,12 @@ S_SPLIT_BF16, S_BF16 = 0, 1
 
 # every symbol include/timesc.h declares
 EXPORTS = [
-    "tcx_last_error", "tcx_create", "tcx_destroy", "tcx_workspace_bytes", "tcx_malloc",
+    "tcx_last_error", "tcx_create", "tcx_destroy", "tcx_workspace_bytes",
+    "tcx_fit_workspace_bytes", "tcx_malloc",
     "tcx_free", "tcx_memcpy_h2d", "tcx_memcpy_d2h", "tcx_memset", "tcx_synchronize",
     "tcx_znorm", "tcx_ncc", "tcx_sbd_align", "tcx_assign", "tcx_extract_shape",
     "tcx_fit_opts_default", "tcx_kshape_fit", "tcx_fit_begin", "tcx_fit_step", "tcx_fit_end",
-    "tcx_fit_query",
+    "tcx_fit_query", "tcx_fit_snapshot",
     "tcx_kshape", "tcx_kshape_tsl", "tcx_batch_create", "tcx_batch_run", "tcx_batch_destroy",
     "tcx_server_create", "tcx_server_add_model", "tcx_server_submit", "tcx_server_wait",
     "tcx_server_destroy",
@@ -55,6 +59,7 @@ class FitOpts(ctypes.Structure):
         ("cal_select", ctypes.c_int),
         ("power_tol", ctypes.c_double),
         ("labels_per_iter", ctypes.POINTER(ctypes.c_int)),
+        ("stop_on_collapse", ctypes.c_int),
     ]
 
 
@@ -71,6 +76,7 @@ class FitStats(ctypes.Structure):
         ("sbd_evals", ctypes.c_longlong),
         ("sbd_pairs", ctypes.c_longlong),
         ("ms_assign_bounds", ctypes.c_double),
+        ("collapsed", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -108,6 +114,27 @@ class Request(ctypes.Structure):
 
 
 _lib = None
+
+# Every open Context, Batch and Server, closed at interpreter exit (servers
+# and batches first) so that no device context outlives the runtime.
+_live = weakref.WeakValueDictionary()
+_live_seq = itertools.count()
+
+
+def _register(obj, rank):
+    _live[(rank, next(_live_seq))] = obj
+
+
+@atexit.register
+def _close_all():
+    for key in sorted(list(_live.keys())):
+        obj = _live.get(key)
+        if obj is not None:
+            try:
+                obj.close()
+            except Exception:
+                pass
+
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -151,6 +178,7 @@ def load():
         "tcx_create": (_I, [_I, _Z, ctypes.POINTER(_P)]),
         "tcx_destroy": (_I, [_P]),
         "tcx_workspace_bytes": (_Z, [_P]),
+        "tcx_fit_workspace_bytes": (_Z, [_P, _I, _I, _I]),
         "tcx_malloc": (_I, [_P, ctypes.POINTER(_P), _Z]),
         "tcx_free": (_I, [_P, _P]),
         "tcx_memcpy_h2d": (_I, [_P, _P, _P, _Z]),
@@ -169,6 +197,7 @@ def load():
         "tcx_fit_step": (_I, [_P, ctypes.POINTER(_I)]),
         "tcx_fit_end": (_I, [_P, _P, _P, ctypes.POINTER(FitStats)]),
         "tcx_fit_query": (_I, [_P, ctypes.POINTER(FitStats)]),
+        "tcx_fit_snapshot": (_I, [_P, _P, _P, _P, _P]),
         "tcx_kshape": (_I, [_P, _P, _P, _I, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), _I]),
         "tcx_kshape_tsl": (_I, [_P, _P, _P, _P, _I, _I, _I, ctypes.POINTER(_I),
                                 ctypes.POINTER(_I), _I]),
@@ -341,6 +370,7 @@ class Context:
         h = ctypes.c_void_p()
         check(self.lib.tcx_create(device, budget, ctypes.byref(h)))
         self.h = h
+        _register(self, 2)
 
     def close(self):
         if self.h:
@@ -355,6 +385,10 @@ class Context:
 
     def workspace_bytes(self):
         return self.lib.tcx_workspace_bytes(self.h)
+
+    def fit_workspace_bound(self, n, m, k):
+        """Upper bound of the workspace bytes one fit of (n, m, k) grows."""
+        return self.lib.tcx_fit_workspace_bytes(self.h, n, m, k)
 
     def empty(self, shape, dtype):
         shape = (shape,) if np.isscalar(shape) else tuple(shape)
@@ -432,17 +466,33 @@ class Context:
                                          dcent.ptr, n, m, k, it, cal_select, ctypes.byref(sc)))
         return sc.value
 
-    def _opts(self, max_iter, init_mode, seed, convention, s_precision, power_tol):
+    def _opts(self, n, max_iter=10, init_mode=0, seed=0, init_labels=None, init_rows=None,
+              convention=CONV_CPU, s_precision=S_SPLIT_BF16, power_tol=1e-12, cal_select=0,
+              stop_on_collapse=False, labels_per_iter=False):
+        """FitOpts and the host arrays it points to (kept alive by the caller)."""
         o = FitOpts()
         self.lib.tcx_fit_opts_default(ctypes.byref(o))
         o.max_iter, o.init_mode, o.seed = max_iter, init_mode, seed
+        o.cal_select = cal_select
         o.convention, o.s_precision, o.power_tol = convention, s_precision, power_tol
-        return o
+        o.stop_on_collapse = int(stop_on_collapse)
+        keep = {}
+        if init_labels is not None:
+            keep["il"] = np.ascontiguousarray(init_labels, np.int32)
+            o.init_labels = keep["il"].ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        if init_rows is not None:
+            keep["ir"] = np.ascontiguousarray(init_rows, np.int32)
+            o.init_rows = keep["ir"].ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        if labels_per_iter:
+            keep["hist"] = np.zeros((max_iter, n), np.int32)
+            o.labels_per_iter = keep["hist"].ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        return o, keep
 
-    def fit_begin(self, dx, k, max_iter=10, init_mode=0, seed=0, convention=CONV_CPU,
-                  s_precision=S_SPLIT_BF16, power_tol=1e-12):
+    def fit_begin(self, dx, k, **kw):
+        """Starts a stepwise fit (tcx_fit_begin); keyword options as for fit()."""
         n, m = dx.shape
-        o = self._opts(max_iter, init_mode, seed, convention, s_precision, power_tol)
+        o, keep = self._opts(n, **kw)
+        self._fit_keep = keep  # labels_per_iter is written by every step
         check(self.lib.tcx_fit_begin(self.h, dx.ptr, n, m, k, ctypes.byref(o)))
 
     def fit_step(self):
@@ -455,6 +505,17 @@ class Context:
         check(self.lib.tcx_fit_query(self.h, ctypes.byref(st)))
         return st.as_dict()
 
+    def fit_snapshot(self, n, k, m):
+        """Labels, distances and lags of the fit's last assignment and its
+        current centroids."""
+        labels = np.empty(n, np.int32)
+        dist = np.empty(n, np.float32)
+        shift = np.empty(n, np.int32)
+        cent = np.empty((k, m), np.float32)
+        check(self.lib.tcx_fit_snapshot(self.h, _ptr(labels), _ptr(dist), _ptr(shift),
+                                        _ptr(cent)))
+        return labels, dist, shift, cent
+
     def fit_end(self, n, k, m):
         labels = np.empty(n, np.int32)
         cent = np.empty((k, m), np.float32)
@@ -462,36 +523,21 @@ class Context:
         check(self.lib.tcx_fit_end(self.h, _ptr(labels), _ptr(cent), ctypes.byref(st)))
         return {"labels": labels, "centroids": cent, "stats": st.as_dict()}
 
-    def fit(self, dx, k, max_iter=10, init_mode=0, seed=0, init_labels=None, init_rows=None,
-            convention=CONV_CPU, s_precision=S_SPLIT_BF16, power_tol=1e-12,
-            labels_per_iter=False, cal_select=0):
+    def fit(self, dx, k, **kw):
+        """Full fit (tcx_kshape_fit).  Options: max_iter, init_mode, seed,
+        init_labels, init_rows, convention, s_precision, power_tol, cal_select,
+        stop_on_collapse, labels_per_iter (also return the labels after every
+        iteration)."""
         n, m = dx.shape
-        o = FitOpts()
-        self.lib.tcx_fit_opts_default(ctypes.byref(o))
-        o.max_iter, o.init_mode, o.seed = max_iter, init_mode, seed
-        o.cal_select = cal_select
-        o.convention, o.s_precision, o.power_tol = convention, s_precision, power_tol
-        keep = []
-        if init_labels is not None:
-            il = np.ascontiguousarray(init_labels, np.int32)
-            keep.append(il)
-            o.init_labels = il.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
-        if init_rows is not None:
-            ir = np.ascontiguousarray(init_rows, np.int32)
-            keep.append(ir)
-            o.init_rows = ir.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
-        hist = None
-        if labels_per_iter:
-            hist = np.zeros((max_iter, n), np.int32)
-            o.labels_per_iter = hist.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        o, keep = self._opts(n, **kw)
         labels = np.empty(n, np.int32)
         cent = np.empty((k, m), np.float32)
         st = FitStats()
         check(self.lib.tcx_kshape_fit(self.h, dx.ptr, n, m, k, ctypes.byref(o), _ptr(labels),
                                       _ptr(cent), ctypes.byref(st)))
         res = {"labels": labels, "centroids": cent, "stats": st.as_dict()}
-        if hist is not None:
-            res["labels_per_iter"] = hist[: st.iterations]
+        if "hist" in keep:
+            res["labels_per_iter"] = keep["hist"][: st.iterations]
         return res
 
 
@@ -503,6 +549,7 @@ class Batch:
         h = ctypes.c_void_p()
         check(self.lib.tcx_batch_create(device, workers, budget, ctypes.byref(h)))
         self.h = h
+        _register(self, 1)
 
     def close(self):
         if self.h:
@@ -567,6 +614,7 @@ class Server:
         check(self.lib.tcx_server_create(device, fit_workers, query_workers, budget,
                                          ctypes.byref(h)))
         self.h = h
+        _register(self, 0)
 
     def close(self):
         if self.h:
