@@ -377,18 +377,19 @@ def capture_fit(ctx, dx, n, m, k, seed):
     ctx.fit_begin(dx, k, max_iter=ITERS, seed=seed)
     import timesc
 
-    prev = (timesc.init_labels(seed, n, k), np.zeros((k, m), np.float32))
+    prev = (timesc.init_labels(seed, n, k), np.zeros((k, m), np.float32), None)
     q0 = ctx.fit_query()
     out = []
     for _ in range(ITERS):
         changed = ctx.fit_step()
         q1 = ctx.fit_query()
-        lab, dist, _, C = ctx.fit_snapshot(n, k, m)
-        out.append({"prev_lab": prev[0], "prev_C": prev[1], "lab": lab, "dist": dist, "C": C,
+        lab, dist, shift, C = ctx.fit_snapshot(n, k, m)
+        out.append({"prev_lab": prev[0], "prev_C": prev[1], "prev_shift": prev[2], "lab": lab,
+                    "dist": dist, "C": C,
                     "refine_ms": q1["ms_refine"] - q0["ms_refine"],
                     "assign_ms": q1["ms_assign"] - q0["ms_assign"],
                     "evals": q1["sbd_evals"] - q0["sbd_evals"]})
-        prev, q0 = (lab, C), q1
+        prev, q0 = (lab, C, shift), q1
         if not changed:
             break
     ctx.fit_end(n, k, m)
@@ -398,7 +399,8 @@ def capture_fit(ctx, dx, n, m, k, seed):
 def accuracy(Z, k, steps, seed):
     """Checker leg (FP64 oracle): centroid relative L2 error of iterations 1,
     2 and the last, each refined by the oracle from the GPU's own previous
-    labels and centroids over all series, and the SBD distance error of the
+    labels, centroids and alignment lags (the lags differ from the oracle's
+    own only at NCC ties) over all series, and the SBD distance error of the
     GPU's label on a sample of rows over every iteration."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
@@ -411,7 +413,7 @@ def accuracy(Z, k, steps, seed):
     for t in sorted({0, min(1, len(steps) - 1), len(steps) - 1}):
         st = steps[t]
         Co, _, _ = oracle_py.refine(Z64, st["prev_lab"], st["prev_C"].astype(np.float64),
-                                    seed=seed, it=t)
+                                    seed=seed, it=t, shifts=st["prev_shift"])
         errs = [float(np.linalg.norm(g - o) / np.linalg.norm(o)) if np.any(o) else
                 float(np.linalg.norm(g)) for g, o in zip(st["C"], Co)]
         cent[f"iteration_{t + 1}"] = max(errs)
@@ -423,7 +425,8 @@ def accuracy(Z, k, steps, seed):
     return {"max_sbd_err": sbd, "centroid_rel_l2": max(cent.values()),
             "centroid_rel_l2_per_iteration": cent,
             "note": f"FP64 oracle: SBD error on {rows.size} sampled rows over every iteration; "
-                    "centroids refined from the GPU's own previous state over all rows"}
+                    "centroids refined from the GPU's own previous labels, centroids and "
+                    "alignment lags over all rows"}
 
 
 def main():

@@ -655,6 +655,15 @@ void orc_refine(const double* X, long n, int m, int k, const int* labels,
                 double* C, int convention, uint64_t seed, int iter,
                 int exact_steps, int* counts, orc_shape_info* infos,
                 int nthreads, int skip_rule, int cal_select) {
+  orc_refine_shifts(X, n, m, k, labels, C, convention, seed, iter, exact_steps,
+                    counts, infos, nthreads, skip_rule, cal_select, NULL);
+}
+
+void orc_refine_shifts(const double* X, long n, int m, int k, const int* labels,
+                       double* C, int convention, uint64_t seed, int iter,
+                       int exact_steps, int* counts, orc_shape_info* infos,
+                       int nthreads, int skip_rule, int cal_select,
+                       const int* shifts) {
   set_threads(nthreads);
   int F = orc_fft_size(m);
   const double *tc, *ts;
@@ -695,7 +704,8 @@ void orc_refine(const double* X, long n, int m, int k, const int* labels,
 #pragma omp for schedule(dynamic, 8)
         for (long p = 0; p < c; ++p) {
           double* a = A + p * (long)m;
-          int s = orc_align_shift(a, cj, m, convention);
+          int s = shifts ? shifts[members[off[j] + p]]
+                         : orc_align_shift(a, cj, m, convention);
           orc_roll_zeropad(a, m, s, tmp);
           memcpy(a, tmp, sizeof(double) * m);
         }

@@ -98,6 +98,15 @@ void orc_refine(const double* X, long n, int m, int k, const int* labels,
                 int exact_steps, int* counts, orc_shape_info* infos,
                 int nthreads, int skip_rule, int cal_select);
 
+/* orc_refine with the alignment shift of every row given (shifts, n values;
+ * NULL: computed as _preSBD_extract_shape does).  Lets a checker refine
+ * from the exact shifts an FP32 engine chose where two lags tie. */
+void orc_refine_shifts(const double* X, long n, int m, int k, const int* labels,
+                       double* C, int convention, uint64_t seed, int iter,
+                       int exact_steps, int* counts, orc_shape_info* infos,
+                       int nthreads, int skip_rule, int cal_select,
+                       const int* shifts);
+
 /* Full k-Shape loop (test_timesC.py:213-265) on z-normalized X.
  * init_mode 0: init_labels (n) are the starting labels, centroids start at 0.
  * init_mode 1: init_rows (k) are row indices whose (z-normalized) rows are

@@ -52,6 +52,7 @@ def load():
         "orc_power_v0": (None, [U, I, I, I, _P]),
         "orc_extract_shape": (I, [_P, L, I, _P, I, _P, ctypes.POINTER(ShapeInfo), I]),
         "orc_refine": (None, [_P, L, I, I, _P, _P, I, U, I, I, _P, _P, I, I, I]),
+        "orc_refine_shifts": (None, [_P, L, I, I, _P, _P, I, U, I, I, _P, _P, I, I, I, _P]),
         "orc_kshape": (I, [_P, L, I, I, I, _P, _P, I, I, U, I, _P, _P, I, I, I, I, _P]),
         "orc_sum_pairwise": (D, [_P, L]),
         "orc_top_eig": (I, [_P, I, ctypes.POINTER(D), _P]),
@@ -146,7 +147,9 @@ def sum_pairwise(a):
 
 
 def refine(X, labels, C, convention=CONV_CPU, seed=0, it=0, exact_steps=False, nthreads=0,
-           skip_rule=SKIP_ZERO, cal_select=0):
+           skip_rule=SKIP_ZERO, cal_select=0, shifts=None):
+    """One refinement; shifts (n, optional): the alignment shift of every row
+    instead of the one the oracle computes."""
     X = _f64(X)
     n, m = X.shape
     C = _f64(C).copy()
@@ -154,9 +157,10 @@ def refine(X, labels, C, convention=CONV_CPU, seed=0, it=0, exact_steps=False, n
     lab = np.ascontiguousarray(labels, np.int32)
     counts = np.empty(k, np.int32)
     infos = (ShapeInfo * k)()
-    load().orc_refine(_p(X), n, m, k, _p(lab), _p(C), convention, seed, it, int(exact_steps),
-                      _p(counts), ctypes.cast(infos, ctypes.c_void_p), nthreads, skip_rule,
-                      cal_select)
+    sh = None if shifts is None else np.ascontiguousarray(shifts, np.int32)
+    load().orc_refine_shifts(_p(X), n, m, k, _p(lab), _p(C), convention, seed, it,
+                             int(exact_steps), _p(counts), ctypes.cast(infos, ctypes.c_void_p),
+                             nthreads, skip_rule, cal_select, _p(sh))
     return C, counts, [{name: getattr(i, name) for name, _ in i._fields_} for i in infos]
 
 

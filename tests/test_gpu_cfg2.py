@@ -116,12 +116,33 @@ def test_cfg2_sbd_error_against_fp64_oracle(cfg2_fit):
 
 @pytest.mark.parametrize("it", [0, 1])
 def test_cfg2_iteration_matches_oracle_stepwise(cfg2_fit, it):
+    """The refinement of iteration it+1 from the GPU's previous labels,
+    centroids and alignment lags (the previous assignment's), and the
+    assignment against the GPU's new centroids.  A lag the GPU chose may
+    differ from the oracle's own only where the two lags' FP64 NCC values tie
+    (within 1e-6); the oracle then refines from the GPU's lags."""
     steps = cfg2_fit["steps"]
     assert len(steps) > it  # from random labels the fit runs past iteration 2
     Z64 = cfg2_fit["Z"].astype(np.float64)
     prev_lab = cfg2_fit["init"] if it == 0 else steps[it - 1]["lab"]
     prev_C = np.zeros((K, M)) if it == 0 else steps[it - 1]["C"].astype(np.float64)
-    Co, counts, _ = O.refine(Z64, prev_lab, prev_C, seed=SEED, it=it)
+    shifts = None
+    if it > 0:
+        shifts = steps[it - 1]["shift"]
+        ties = 0
+        for j in range(K):
+            rows = np.nonzero(prev_lab == j)[0]
+            if rows.size == 0 or not np.any(prev_C[j]):
+                continue
+            _, _, own = O.assign(Z64[rows], prev_C[j:j + 1])
+            for q in np.nonzero(own != shifts[rows])[0]:
+                cc = O.ncc(Z64[rows[q]], prev_C[j])
+                a, b = cc[own[q] + M - 1], cc[shifts[rows[q]] + M - 1]
+                assert abs(a - b) <= SBD_TIE, (j, int(rows[q]), a, b)
+                ties += 1
+        print(f"iteration {it + 1}: {ties} alignment lags differ from the oracle's by a tie",
+              flush=True)
+    Co, counts, _ = O.refine(Z64, prev_lab, prev_C, seed=SEED, it=it, shifts=shifts)
     errs = centroid_errors(steps[it]["C"], Co)
     print(f"iteration {it + 1}: centroid rel-L2 max {max(errs):.3e}", flush=True)
     assert max(errs) <= CENT_RTOL, errs

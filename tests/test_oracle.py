@@ -452,6 +452,22 @@ def test_literal_skip_rule_on_an_exact_zero_sum_centroid():
     assert not np.allclose(aligned, unaligned)
 
 
+def test_refine_with_given_shifts_equals_own_shifts():
+    """orc_refine_shifts with every row's own alignment shift (against its
+    cluster's old centroid) gives exactly orc_refine's result."""
+    n, m, k = 50, 30, 3
+    Xf, _ = T.generate(13, n, m, k, 0.3)
+    X = O.zscore(Xf.astype(np.float64))
+    lab = T.init_labels(13, n, k)
+    C = O.zscore(X[T.init_rows(13, n, k)], 1)
+    own = np.array([O.align_shift(X[i], C[lab[i]]) for i in range(n)], np.int32)
+    a, _, _ = O.refine(X, lab, C, seed=1, it=1)
+    b, _, _ = O.refine(X, lab, C, seed=1, it=1, shifts=own)
+    assert np.array_equal(a, b)
+    c, _, _ = O.refine(X, lab, C, seed=1, it=1, shifts=np.zeros(n, np.int32))
+    assert not np.allclose(a, c)
+
+
 def test_kshape_true_labels_are_a_fixed_point():
     """Noise-free, shift-free, well-separated classes: starting from the true
     labels the labels never change."""
