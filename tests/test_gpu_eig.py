@@ -82,20 +82,23 @@ def test_fit_with_eigenvector_centroids_matches_oracle(ctx, init_mode):
     assert max(errs) <= CENT_RTOL, errs
 
 
-def test_cli_reruns_with_eigenvector_centroids(ctx, tmp_path):
+@pytest.mark.parametrize("sigma,rerun", [(0.05, True), (0.1, False)])
+def test_cli_collapse_stop_and_rerun(ctx, tmp_path, sigma, rerun):
     """A labeled file with k = 8 whose data has 2 classes: the first fit stops
-    on collapse with fewer than k/2 non-empty clusters, so the CLI reruns
-    k_shape with cal_select = 1 (which stops on collapse too) and writes that
-    run's labels."""
+    on collapse (kshape.cuh:248-249).  With sigma 0.05 it leaves 3 < k/2
+    non-empty clusters, so the CLI reruns k_shape with cal_select = 1 (which
+    stops on collapse too) and writes that run's labels; with sigma 0.1 it
+    leaves exactly k/2 = 4, where test_kshape.cu:54 (`<`) does not rerun and
+    the labels of the assignment before the collapse are written."""
     n, m, k = 200, 48, 8
-    X, _ = T.generate(0, n, m, 2, 0.1)
+    X, _ = T.generate(0, n, m, 2, sigma)
     csv = str(tmp_path / "data.csv")
     T.write_labeled_csv(csv, X, np.arange(n, dtype=np.int32) % k)
     out = str(tmp_path / "result.txt")
     r = subprocess.run([T.CLI_PATH, "-d", "0", csv, "0", "0", "0", "--out=" + out],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    assert "rerun k_shape with cal_select=1" in r.stdout, r.stdout
+    assert ("rerun k_shape with cal_select=1" in r.stdout) == rerun, r.stdout
     cli = np.array([int(v) for v in open(out).read()[:-1].split(",")])
     d = ctx.to_device(X)
     try:
@@ -104,5 +107,10 @@ def test_cli_reruns_with_eigenvector_centroids(ctx, tmp_path):
         second = ctx.fit(d, k, max_iter=10, seed=0, cal_select=1, stop_on_collapse=True)
     finally:
         d.free()
-    assert first["stats"]["sum_class"] < k // 2 and first["stats"]["collapsed"]
-    assert np.array_equal(cli, second["labels"] + 1)
+    assert first["stats"]["collapsed"]
+    if rerun:
+        assert first["stats"]["sum_class"] < k // 2
+        assert np.array_equal(cli, second["labels"] + 1)
+    else:
+        assert first["stats"]["sum_class"] == k // 2
+        assert np.array_equal(cli, first["labels"] + 1)
