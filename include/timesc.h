@@ -108,6 +108,20 @@ int tcx_assign(tcx_ctx* ctx, const float* d_x, const float* d_centroids,
                int n, int m, int k, int* d_label, float* d_dist, int* d_shift,
                float* d_dmat);
 
+/* tcx_assign without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No reference counterpart.  Same arguments and
+ * results as tcx_assign, on the context's stream; the centroid spectra and
+ * the FFT twiddle tables live in a static device array of 3.3 MB per GPU
+ * (held until the process ends) instead of context workspace: k <= 1024 and
+ * k * F <= 262144 (F = 1 << (2m-1).bit_length()), else TCX_ERR_ARG.  The
+ * first call on a device writes the twiddle tables there.  Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (other threads,
+ * other contexts) race and must be serialized by the caller. */
+int tcx_assign_noalloc(tcx_ctx* ctx, const float* d_x, const float* d_centroids,
+                       int n, int m, int k, int* d_label, float* d_dist,
+                       int* d_shift, float* d_dmat);
+
 /* Shape extraction: group rows by d_label (zero-based), align each member
  * by d_shift[i] (ignored for clusters whose old centroid is all zero; a
  * NULL d_shift means: compute each member's shift against its own old

@@ -219,21 +219,21 @@ int spectra(tcx_ctx* c, const float* d_rows, int rows, int m, float2** spec, flo
   return TCX_OK;
 }
 
-// Assignment.  Without a distance matrix the exact pruned kernel runs,
-// pairing the rows of `order` (npairs pairs, -1 = hole) or rows 2p, 2p+1.
-int assign_core(tcx_ctx* c, const float* d_x, long n, int m, int k, const float2* cspec,
-                const float* cnorm, int rev, int* label, float* dist, int* shift, float* dmat,
-                int* changed, const int* order = nullptr, long npairs = 0,
-                unsigned long long* evals = nullptr, const float* cmag = nullptr,
-                const float2* xs = nullptr, const float* xn = nullptr,
-                const float* ub = nullptr, int ub_ld = 0) {
-  int F = fft_size(m);
-  const float2* tw;
-  TCX_TRY(twiddles(c, F, &tw));
+// Assignment with the twiddle table tw of F = fft_size(m) and the
+// centroid magnitudes cmag given.  Without a distance matrix the exact
+// pruned kernel runs, pairing the rows of `order` (npairs pairs, -1 = hole)
+// or rows 2p, 2p+1.
+int assign_core_tw(tcx_ctx* c, const float2* tw, const float* d_x, long n, int m, int k,
+                   const float2* cspec, const float* cnorm, const float* cmag, int rev, int* label,
+                   float* dist, int* shift, float* dmat, int* changed,
+                   const int* order = nullptr, long npairs = 0,
+                   unsigned long long* evals = nullptr, const float2* xs = nullptr,
+                   const float* xn = nullptr, const float* ub = nullptr, int ub_ld = 0) {
+  const int F = fft_size(m);
   if (!dmat && k <= tcx::prune_max_k(F)) {
     tcx::PruneArgs p;
     p.x = d_x; p.n = n; p.m = m; p.k = k; p.cspec = cspec; p.cnorm = cnorm; p.tw = tw;
-    p.cmag = cmag ? cmag : reinterpret_cast<const float*>(c->slot_ptr[kSlotCmag]);
+    p.cmag = cmag;
     p.rev = rev; p.label = label; p.dist = dist; p.shift = shift; p.changed = changed;
     p.order = order;
     p.npairs = order ? npairs : (n + 1) / 2;
@@ -250,6 +250,64 @@ int assign_core(tcx_ctx* c, const float* d_x, long n, int m, int k, const float2
   a.rev = rev; a.label = label; a.dist = dist; a.shift = shift; a.dmat = dmat;
   a.changed = changed;
   TCX_HIP(tcx::launch_sbd(F, tcx::kModeAssign, a, c->stream));
+  return TCX_OK;
+}
+
+// The same with the context's twiddle table and, without cmag, the
+// magnitudes of the context's last spectra() call.
+int assign_core(tcx_ctx* c, const float* d_x, long n, int m, int k, const float2* cspec,
+                const float* cnorm, int rev, int* label, float* dist, int* shift, float* dmat,
+                int* changed, const int* order = nullptr, long npairs = 0,
+                unsigned long long* evals = nullptr, const float* cmag = nullptr,
+                const float2* xs = nullptr, const float* xn = nullptr,
+                const float* ub = nullptr, int ub_ld = 0) {
+  const float2* tw;
+  TCX_TRY(twiddles(c, fft_size(m), &tw));
+  if (!cmag) cmag = reinterpret_cast<const float*>(c->slot_ptr[kSlotCmag]);
+  return assign_core_tw(c, tw, d_x, n, m, k, cspec, cnorm, cmag, rev, label, dist, shift, dmat,
+                        changed, order, npairs, evals, xs, xn, ub, ub_ld);
+}
+
+// ------------------------------------------------ allocation-free assignment
+// Device scratch of tcx_assign_noalloc, one copy per GPU for the life of the
+// process: the twiddle tables of every FFT length F = 4 .. 8192 (table F at
+// entry F - 4, written once per device at set-up), then the centroid
+// spectra [k][F], magnitudes [k][F], norms [k] and zero flags [k] of the
+// current call, for k <= kNaMaxK and k * F <= kNaMaxKF (3.3 MB in all).
+constexpr long kNaMaxKF = 1L << 18;
+constexpr int kNaMaxK = 1024;
+constexpr size_t kNaTwEntries = 16384;  // 4 + 8 + ... + 8192 = 16380, rounded up
+constexpr size_t kNaSpecOff = sizeof(float2) * kNaTwEntries;
+constexpr size_t kNaMagOff = kNaSpecOff + sizeof(float2) * kNaMaxKF;
+constexpr size_t kNaNormOff = kNaMagOff + sizeof(float) * kNaMaxKF;
+constexpr size_t kNaZeroOff = kNaNormOff + sizeof(float) * kNaMaxK;
+constexpr size_t kNaBytes = kNaZeroOff + sizeof(int) * kNaMaxK;
+constexpr int kNaMaxDevices = 64;
+
+__device__ __align__(256) unsigned char scratch[kNaBytes];
+
+std::mutex g_na_mu;
+unsigned char* g_na_base[kNaMaxDevices];  // device address of each GPU's scratch
+float2 g_na_tw_host[kNaTwEntries];        // staging of the twiddle tables
+
+// Address of the current device's scratch (the device is current).  The
+// first call on a device looks the address up and writes the twiddle
+// tables; no device memory is allocated.
+int noalloc_scratch(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kNaMaxDevices) return fail(TCX_ERR_UNSUPPORTED, "device index >= 64");
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (!g_na_base[dev]) {
+    void* p = nullptr;
+    TCX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    for (int F = 4; F <= tcx::kMaxFft; F *= 2) {
+      float2* t = g_na_tw_host + (F - 4);
+      for (int i = 0; i < F; ++i) t[i] = make_float2(1.f, 0.f);
+      tcx::fft_twiddle_table(F, t);
+    }
+    TCX_HIP(hipMemcpy(p, g_na_tw_host, sizeof(g_na_tw_host), hipMemcpyHostToDevice));
+    g_na_base[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_na_base[dev];
   return TCX_OK;
 }
 
@@ -867,6 +925,33 @@ int tcx_assign(tcx_ctx* c, const float* d_x, const float* d_centroids, int n, in
   int* zero;
   TCX_TRY(spectra(c, d_centroids, k, m, &spec, &norm, &zero));
   TCX_TRY(assign_core(c, d_x, n, m, k, spec, norm, 0, d_label, d_dist, d_shift, d_dmat, nullptr));
+  TCX_HIP(hipStreamSynchronize(c->stream));
+  return TCX_OK;
+}
+
+int tcx_assign_noalloc(tcx_ctx* c, const float* d_x, const float* d_centroids, int n, int m,
+                       int k, int* d_label, float* d_dist, int* d_shift, float* d_dmat) {
+  TCX_TRY(set_dev(c));
+  TCX_TRY(check_series(n, m));
+  if (!d_x || !d_centroids || !d_label || k < 1)
+    return fail(TCX_ERR_ARG, "assign_noalloc: bad arguments");
+  const int F = fft_size(m);
+  if (k > kNaMaxK || (long)k * F > kNaMaxKF)
+    return fail(TCX_ERR_ARG, "assign_noalloc: k = " + std::to_string(k) + " and F = " +
+                                 std::to_string(F) + " exceed the static scratch (k <= " +
+                                 std::to_string(kNaMaxK) + ", k * F <= " +
+                                 std::to_string(kNaMaxKF) + ")");
+  unsigned char* base;
+  TCX_TRY(noalloc_scratch(c->device, &base));
+  const float2* tw = reinterpret_cast<const float2*>(base) + (F - 4);
+  float2* spec = reinterpret_cast<float2*>(base + kNaSpecOff);
+  float* mag = reinterpret_cast<float*>(base + kNaMagOff);
+  float* norm = reinterpret_cast<float*>(base + kNaNormOff);
+  int* zero = reinterpret_cast<int*>(base + kNaZeroOff);
+  TCX_HIP(tcx::launch_spectra(F, d_centroids, k, m, tw, spec, norm, zero, mag, nullptr,
+                              c->stream));
+  TCX_TRY(assign_core_tw(c, tw, d_x, n, m, k, spec, norm, mag, 0, d_label, d_dist, d_shift,
+                         d_dmat, nullptr));
   TCX_HIP(hipStreamSynchronize(c->stream));
   return TCX_OK;
 }

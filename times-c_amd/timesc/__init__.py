@@ -29,7 +29,8 @@ EXPORTS = [
     "tcx_last_error", "tcx_create", "tcx_destroy", "tcx_workspace_bytes",
     "tcx_fit_workspace_bytes", "tcx_malloc",
     "tcx_free", "tcx_memcpy_h2d", "tcx_memcpy_d2h", "tcx_memset", "tcx_synchronize",
-    "tcx_znorm", "tcx_ncc", "tcx_sbd_align", "tcx_assign", "tcx_extract_shape",
+    "tcx_znorm", "tcx_ncc", "tcx_sbd_align", "tcx_assign", "tcx_assign_noalloc",
+    "tcx_extract_shape",
     "tcx_fit_opts_default", "tcx_kshape_fit", "tcx_fit_begin", "tcx_fit_step", "tcx_fit_end",
     "tcx_fit_query", "tcx_fit_snapshot",
     "tcx_kshape", "tcx_kshape_tsl", "tcx_batch_create", "tcx_batch_run", "tcx_batch_destroy",
@@ -189,6 +190,7 @@ def load():
         "tcx_ncc": (_I, [_P, _P, _P, _P, _I, _I, _I]),
         "tcx_sbd_align": (_I, [_P, _P, _P, _I, _I, _I]),
         "tcx_assign": (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
+        "tcx_assign_noalloc": (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
         "tcx_extract_shape": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, ctypes.POINTER(_I)]),
         "tcx_fit_opts_default": (None, [ctypes.POINTER(FitOpts)]),
         "tcx_kshape_fit": (_I, [_P, _P, _I, _I, _I, ctypes.POINTER(FitOpts), _P, _P,
@@ -438,16 +440,20 @@ class Context:
         rows, m = dmembers.shape
         check(self.lib.tcx_sbd_align(self.h, dmembers.ptr, dcenter.ptr, rows, m, convention))
 
-    def assign(self, dx, dc, dmat=False):
+    def assign(self, dx, dc, dmat=False, noalloc=False):
+        """tcx_assign (noalloc: tcx_assign_noalloc) of the rows of dx against
+        the centroids dc; returns labels, distances, lags (and the distance
+        matrix with dmat)."""
         n, m = dx.shape
         k = dc.shape[0]
         lab = self.empty(n, np.int32)
         dist = self.empty(n, np.float32)
         sh = self.empty(n, np.int32)
         dm = self.empty((n, k), np.float32) if dmat else None
+        fn = self.lib.tcx_assign_noalloc if noalloc else self.lib.tcx_assign
         try:
-            check(self.lib.tcx_assign(self.h, dx.ptr, dc.ptr, n, m, k, lab.ptr, dist.ptr,
-                                      sh.ptr, dm.ptr if dm else None))
+            check(fn(self.h, dx.ptr, dc.ptr, n, m, k, lab.ptr, dist.ptr, sh.ptr,
+                     dm.ptr if dm else None))
             out = (lab.numpy(), dist.numpy(), sh.numpy())
             if dm:
                 out = out + (dm.numpy(),)
