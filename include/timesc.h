@@ -275,6 +275,21 @@ int tcx_batch_create(int device, int workers, size_t hbm_budget, tcx_batch** out
  * when every job succeeded, else the status of the first failed job (each
  * job's own status is in jobs[i].status).  One batch runs at a time. */
 int tcx_batch_run(tcx_batch* b, tcx_job* jobs, int njobs, int order);
+/* The same jobs through the grouped executor: each phase of an iteration
+ * (grouping, align + pack, S = Y^T Y, power stage, centroid spectra, bound
+ * product, pruned assignment) runs as one launch over every job in
+ * progress, with three host synchronizations per iteration for the whole
+ * batch, on the first worker's stream.  Jobs are admitted while their
+ * device memory fits the batch's hbm_budget (0: 80% of the free device
+ * memory) and join as others finish.  A job's results do not depend on the
+ * other jobs; they equal those of tcx_kshape_fit to rounding (the power
+ * stage is the one-block-per-cluster Lanczos up to m = 768 and the
+ * two-launch form above; S takes one K chunk per tile).  Jobs with
+ * init_mode 1, cal_select 1 or k beyond the pruned assignment's limit run on
+ * the worker pool as in tcx_batch_run.  Returns as tcx_batch_run; ms_refine
+ * and ms_assign of a grouped job are the wall times of the grouped phases
+ * it took part in (shared with the other jobs).  No reference counterpart. */
+int tcx_batch_run_grouped(tcx_batch* b, tcx_job* jobs, int njobs);
 int tcx_batch_destroy(tcx_batch* b);
 
 /* -------------------------------------------------------- serving loop */

@@ -17,12 +17,14 @@
 #include <vector>
 
 #include "tcx_host.hpp"
+#include "tcx_internal.hpp"
 #include "timesc.h"
 
 using Clock = std::chrono::steady_clock;
 
 struct tcx_batch {
   std::vector<tcx_ctx*> ctx;
+  size_t budget = 0;
   std::vector<std::thread> threads;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -84,6 +86,34 @@ void worker_main(tcx_batch* b, int w) {
   }
 }
 
+// Runs jobs[idx] on the worker pool (the caller set b->running); returns
+// when every one has finished and clears b->running.
+void run_pool(tcx_batch* b, tcx_job* jobs, const std::vector<int>& idx, Clock::time_point t0) {
+  {
+    std::unique_lock<std::mutex> lk(b->mu);
+    b->jobs = jobs;
+    b->pending = (int)idx.size();
+    b->t0 = t0;
+    for (int i : idx) {
+      jobs[i].status = -1;
+      b->queue.push_back(i);
+    }
+  }
+  b->cv_work.notify_all();
+  std::unique_lock<std::mutex> lk(b->mu);
+  b->cv_done.wait(lk, [&] { return b->pending == 0; });
+  b->running = false;
+  b->jobs = nullptr;
+}
+
+int first_failure(const tcx_job* jobs, int njobs) {
+  for (int i = 0; i < njobs; ++i)
+    if (jobs[i].status != TCX_OK)
+      return tcx::fail(jobs[i].status,
+                       "batch: job " + std::to_string(i) + " failed: " + jobs[i].message);
+  return TCX_OK;
+}
+
 double job_cost(const tcx_job& j) {
   int F = 1;
   while (F < 2 * j.m) F <<= 1;  // >= the FFT length
@@ -110,6 +140,7 @@ int tcx_batch_create(int device, int workers, size_t hbm_budget, tcx_batch** out
     }
     b->ctx.push_back(c);
   }
+  b->budget = hbm_budget;
   for (int w = 0; w < workers; ++w) b->threads.emplace_back(worker_main, b, w);
   *out = b;
   return TCX_OK;
@@ -133,26 +164,50 @@ int tcx_batch_run(tcx_batch* b, tcx_job* jobs, int njobs, int order) {
     std::unique_lock<std::mutex> lk(b->mu);
     if (b->running) return tcx::fail(TCX_ERR_ARG, "batch: a run is already in progress");
     b->running = true;
-    b->jobs = jobs;
-    b->pending = njobs;
-    b->t0 = Clock::now();
-    for (int i : idx) {
-      jobs[i].status = -1;
-      b->queue.push_back(i);
-    }
   }
-  b->cv_work.notify_all();
+  run_pool(b, jobs, idx, Clock::now());
+  return first_failure(jobs, njobs);
+}
+
+int tcx_batch_run_grouped(tcx_batch* b, tcx_job* jobs, int njobs) {
+  tcx::clear_error();
+  if (!b) return tcx::fail(TCX_ERR_ARG, "batch: null batch");
+  if (njobs < 0 || (njobs > 0 && !jobs)) return tcx::fail(TCX_ERR_ARG, "batch: bad job list");
   {
     std::unique_lock<std::mutex> lk(b->mu);
-    b->cv_done.wait(lk, [&] { return b->pending == 0; });
-    b->running = false;
-    b->jobs = nullptr;
+    if (b->running) return tcx::fail(TCX_ERR_ARG, "batch: a run is already in progress");
+    b->running = true;
   }
-  for (int i = 0; i < njobs; ++i)
-    if (jobs[i].status != TCX_OK)
-      return tcx::fail(jobs[i].status,
-                       "batch: job " + std::to_string(i) + " failed: " + jobs[i].message);
-  return TCX_OK;
+  const Clock::time_point t0 = Clock::now();
+  std::vector<int> grouped, rest;
+  for (int i = 0; i < njobs; ++i) {
+    const char* why = nullptr;
+    (tcx::grouped_supported(jobs[i], &why) ? grouped : rest).push_back(i);
+  }
+  int rc = TCX_OK;
+  if (!grouped.empty()) {
+    rc = tcx::grouped_fit(b->ctx[0], jobs, grouped.data(), (int)grouped.size(), b->budget, t0);
+    if (rc != TCX_OK) {
+      const std::string msg = tcx_last_error();
+      for (int i : grouped)
+        if (jobs[i].status == -1) {
+          jobs[i].status = rc;
+          std::snprintf(jobs[i].message, sizeof(jobs[i].message), "%s", msg.c_str());
+        }
+    }
+  }
+  // options the grouped executor does not take (centre initialisation,
+  // eigenvector centroids, k beyond the pruned assignment): worker pool
+  if (!rest.empty()) {
+    std::stable_sort(rest.begin(), rest.end(),
+                     [&](int a, int c) { return job_cost(jobs[a]) > job_cost(jobs[c]); });
+    run_pool(b, jobs, rest, t0);
+  } else {
+    std::lock_guard<std::mutex> lk(b->mu);
+    b->running = false;
+  }
+  if (rc != TCX_OK) return tcx::fail(rc, "batch: grouped run failed");
+  return first_failure(jobs, njobs);
 }
 
 int tcx_batch_destroy(tcx_batch* b) {

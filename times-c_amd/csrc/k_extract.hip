@@ -53,7 +53,8 @@ __device__ __forceinline__ int group_key(int l, long row, const float* dist, int
 
 // Per block of kGroupChunk rows: histogram of the K = k * B keys and the
 // lowest row of each cluster (atomicMin into minrow, INT_MAX when empty).
-__global__ __launch_bounds__(256) void k_group_count(const int* __restrict__ label, long n,
+__device__ __forceinline__ void group_count_body(const int bid, const int nblk,
+    const int* __restrict__ label, long n,
                                                      int k, const float* __restrict__ dist, int B,
                                                      int* __restrict__ blk_count,
                                                      int* __restrict__ minrow, int lds_min,
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(256) void k_group_count(const int* __restrict__ lab
   if (lds_min)
     for (int j = threadIdx.x; j < k; j += blockDim.x) mn[j] = 0x7fffffff;
   __syncthreads();
-  const long b0 = (long)blockIdx.x * kGroupChunk;
+  const long b0 = (long)bid * kGroupChunk;
   const long b1 = b0 + kGroupChunk < n ? b0 + kGroupChunk : n;
   for (long i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
     const int l = label[i];
@@ -80,24 +81,33 @@ __global__ __launch_bounds__(256) void k_group_count(const int* __restrict__ lab
     }
   }
   __syncthreads();
-  for (int j = threadIdx.x; j < K; j += blockDim.x) blk_count[(long)blockIdx.x * K + j] = cnt[j];
+  for (int j = threadIdx.x; j < K; j += blockDim.x) blk_count[(long)bid * K + j] = cnt[j];
   if (lds_min)
     for (int j = threadIdx.x; j < k; j += blockDim.x)
       if (mn[j] != 0x7fffffff) atomicMin(&minrow[j], mn[j]);
+}
+
+__global__ __launch_bounds__(256) void k_group_count(const int* __restrict__ label, long n,
+                                                     int k, const float* __restrict__ dist, int B,
+                                                     int* __restrict__ blk_count,
+                                                     int* __restrict__ minrow, int lds_min,
+                                                     int* __restrict__ bad) {
+  group_count_body(blockIdx.x, gridDim.x, label, n, k, dist, B, blk_count, minrow, lds_min, bad);
 }
 
 // Exclusive scan over blocks for one key per block (blockIdx.x = key):
 // blk_base[b][key] = sum of blk_count[b'][key] for b' < b; counts[key] =
 // total.  Each of the 256 threads scans a contiguous run of blocks, then the
 // run totals are scanned in LDS.
-__global__ __launch_bounds__(256) void k_group_scan(int nblk, int k,
+__device__ __forceinline__ void group_scan_body(const int bid, const int nblk,
+    int nrowblk, int k,
                                                     const int* __restrict__ blk_count,
                                                     int* __restrict__ blk_base,
                                                     int* __restrict__ counts) {
   __shared__ int tot[256];
-  const int j = blockIdx.x, t = threadIdx.x;
-  const int per = (nblk + 255) / 256;
-  const int b0 = t * per, b1 = b0 + per < nblk ? b0 + per : nblk;
+  const int j = bid, t = threadIdx.x;
+  const int per = (nrowblk + 255) / 256;
+  const int b0 = t * per, b1 = b0 + per < nrowblk ? b0 + per : nrowblk;
   int run = 0;
   for (int b = b0; b < b1; ++b) run += blk_count[(long)b * k + j];
   tot[t] = run;
@@ -116,17 +126,38 @@ __global__ __launch_bounds__(256) void k_group_scan(int nblk, int k,
   if (t == 255) counts[j] = tot[255];
 }
 
+__global__ __launch_bounds__(256) void k_group_scan(int nrowblk, int k,
+                                                    const int* __restrict__ blk_count,
+                                                    int* __restrict__ blk_base,
+                                                    int* __restrict__ counts) {
+  group_scan_body(blockIdx.x, gridDim.x, nrowblk, k, blk_count, blk_base, counts);
+}
+
 // One block of 1024 threads: cluster sizes from the key counts, padded
 // cluster offsets (each cluster's columns start at a multiple of kPackPad;
 // the k sizes are scanned 1024 at a time) and the offset of every key
 // inside its cluster.  Phase 1 runs one wave per cluster (lane b holds bin
 // b, B <= 64) and keeps the in-cluster exclusive prefix in koff; phase 3
 // adds the cluster offsets.
+__device__ __forceinline__ void group_offsets_body(int k, int B,
+                                                   const int* __restrict__ kcounts,
+                                                   int* __restrict__ counts,
+                                                   long* __restrict__ offsets,
+                                                   long* __restrict__ koff);
+
 __global__ __launch_bounds__(1024) void k_group_offsets(int k, int B,
                                                         const int* __restrict__ kcounts,
                                                         int* __restrict__ counts,
                                                         long* __restrict__ offsets,
                                                         long* __restrict__ koff) {
+  group_offsets_body(k, B, kcounts, counts, offsets, koff);
+}
+
+__device__ __forceinline__ void group_offsets_body(int k, int B,
+                                                   const int* __restrict__ kcounts,
+                                                   int* __restrict__ counts,
+                                                   long* __restrict__ offsets,
+                                                   long* __restrict__ koff) {
   __shared__ long tot[1024];
   __shared__ long carry;
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t >> 6;
@@ -212,7 +243,7 @@ __global__ __launch_bounds__(64) void k_group_place(const int* __restrict__ labe
 constexpr int kPlaceWaves = kGroupChunk / kWave;
 constexpr int kPlaceParMaxK = 2048;  // 8 * K * 4 bytes <= 64 KB
 
-__global__ __launch_bounds__(kGroupChunk) void k_group_place_par(
+__device__ __forceinline__ void group_place_par_body(const int bid, const int nblk,
     const int* __restrict__ label, long n, int k, const float* __restrict__ dist, int B,
     const int* __restrict__ blk_base, const long* __restrict__ koff, int* __restrict__ perm) {
   extern __shared__ int cnt[];  // [kPlaceWaves][K]: counts, then bases
@@ -220,7 +251,7 @@ __global__ __launch_bounds__(kGroupChunk) void k_group_place_par(
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
   int* mine = cnt + w * K;
   for (int j = lane; j < K; j += kWave) mine[j] = 0;
-  const long row = (long)blockIdx.x * kGroupChunk + threadIdx.x;
+  const long row = (long)bid * kGroupChunk + threadIdx.x;
   int key = -1;
   if (row < n) {
     const int l = label[row];
@@ -240,7 +271,7 @@ __global__ __launch_bounds__(kGroupChunk) void k_group_place_par(
   __syncthreads();
   // 2. exclusive scan over the slices, from the block's base
   for (int j = threadIdx.x; j < K; j += kGroupChunk) {
-    int run = blk_base[(long)blockIdx.x * K + j];
+    int run = blk_base[(long)bid * K + j];
 #pragma unroll
     for (int v = 0; v < kPlaceWaves; ++v) {
       const int c = cnt[v * K + j];
@@ -251,6 +282,12 @@ __global__ __launch_bounds__(kGroupChunk) void k_group_place_par(
   __syncthreads();
   // 3. placement
   if (key >= 0) perm[koff[key] + mine[key] + __popcll(peers & below)] = (int)row;
+}
+
+__global__ __launch_bounds__(kGroupChunk) void k_group_place_par(
+    const int* __restrict__ label, long n, int k, const float* __restrict__ dist, int B,
+    const int* __restrict__ blk_base, const long* __restrict__ koff, int* __restrict__ perm) {
+  group_place_par_body(blockIdx.x, gridDim.x, label, n, k, dist, B, blk_base, koff, perm);
 }
 
 // ------------------------------------------------------------ align + pack
@@ -266,10 +303,11 @@ __device__ __forceinline__ int cluster_of(long p, const long* offsets, int k) {
 // FP64 sum and sum of squares of every row of x (one wave per row).  The
 // shape extraction derives each aligned member's mean and variance from
 // these and the few values its shift drops, so it reads x once per refine.
-__global__ __launch_bounds__(256) void k_row_stats(const float* __restrict__ x, long rows, int m,
+__device__ __forceinline__ void row_stats_body(const int bid, const int nblk,
+    const float* __restrict__ x, long rows, int m,
                                                    double2* __restrict__ stats) {
   const int lane = threadIdx.x & (kWave - 1);
-  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long row = (long)bid * 4 + (threadIdx.x >> 6);
   if (row >= rows) return;
   const float* r = x + row * (long)m;
   double s = 0.0, q = 0.0;
@@ -281,6 +319,11 @@ __global__ __launch_bounds__(256) void k_row_stats(const float* __restrict__ x, 
   s = wave_sum_f64(s);
   q = wave_sum_f64(q);
   if (lane == 0) stats[row] = make_double2(s, q);
+}
+
+__global__ __launch_bounds__(256) void k_row_stats(const float* __restrict__ x, long rows, int m,
+                                                   double2* __restrict__ stats) {
+  row_stats_body(blockIdx.x, gridDim.x, x, rows, m, stats);
 }
 
 // Block of 64 packed columns.  For each column (a member of cluster cl):
@@ -295,7 +338,7 @@ __global__ __launch_bounds__(256) void k_row_stats(const float* __restrict__ x, 
 // (variance below 1e-8 of the mean square, or the kept sum of squares below
 // 1e-8 of the row's) the row is summed again in two passes.  x is then read
 // once, to pack.
-__global__ __launch_bounds__(256) void k_align_pack(
+__device__ __forceinline__ void align_pack_body(const int bid, const int nblk,
     const float* __restrict__ x, int m, const int* __restrict__ perm, long n_pad,
     const long* __restrict__ offsets, const int* __restrict__ minrow, int k,
     const int* __restrict__ shift,
@@ -305,7 +348,7 @@ __global__ __launch_bounds__(256) void k_align_pack(
   __shared__ int s_row[64], s_sh[64], s_first[64];
   __shared__ __align__(16) uint16_t t_hi[64][72], t_lo[64][72];  // 144-byte rows
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const long p0 = (long)blockIdx.x * 64;
+  const long p0 = (long)bid * 64;
   if (p0 >= offsets[k]) return;  // past the padded end (grid sized for the maximum)
   // each column's row, shift and (for a cluster's first member) cluster,
   // resolved by the 64 lanes of wave 0 at once
@@ -433,6 +476,15 @@ __global__ __launch_bounds__(256) void k_align_pack(
   }
 }
 
+__global__ __launch_bounds__(256) void k_align_pack(
+    const float* __restrict__ x, int m, const int* __restrict__ perm, long n_pad,
+    const long* __restrict__ offsets, const int* __restrict__ minrow, int k,
+    const int* __restrict__ shift,
+    const int* __restrict__ czero, const double2* __restrict__ stats,
+    uint16_t* __restrict__ yhi, uint16_t* __restrict__ ylo, float* __restrict__ a0) {
+  align_pack_body(blockIdx.x, gridDim.x, x, m, perm, n_pad, offsets, minrow, k, shift, czero, stats, yhi, ylo, a0);
+}
+
 // The same result as k_align_pack for m <= kPackLdsMaxM, from R whole rows
 // staged in LDS (R = 16, 32 or 64 packed columns per block, R * (m + 4)
 // floats): every gathered row is read from HBM once, the statistics of the
@@ -448,7 +500,7 @@ __host__ __device__ constexpr int pack_lds_rows(int m) {
 }
 
 template <int R>
-__global__ __launch_bounds__(256) void k_align_pack_lds(
+__device__ __forceinline__ void align_pack_lds_body(const int bid, const int nblk,
     const float* __restrict__ x, int m, const int* __restrict__ perm,
     const long* __restrict__ offsets, const int* __restrict__ minrow, int k,
     const int* __restrict__ shift,
@@ -459,7 +511,7 @@ __global__ __launch_bounds__(256) void k_align_pack_lds(
   __shared__ int s_row[64], s_sh[64];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int ms = m + 4;
-  const long p0 = (long)blockIdx.x * R;
+  const long p0 = (long)bid * R;
   if (p0 >= offsets[k]) return;  // past the padded end (grid sized for the maximum)
   int first = -1;  // lane q < R of wave 0: cluster whose lowest member is column q
   if (wid == 0 && lane < R) {
@@ -600,6 +652,16 @@ __global__ __launch_bounds__(256) void k_align_pack_lds(
   }
 }
 
+template <int R>
+__global__ __launch_bounds__(256) void k_align_pack_lds(
+    const float* __restrict__ x, int m, const int* __restrict__ perm,
+    const long* __restrict__ offsets, const int* __restrict__ minrow, int k,
+    const int* __restrict__ shift,
+    const int* __restrict__ czero, uint16_t* __restrict__ yhi, uint16_t* __restrict__ ylo,
+    float* __restrict__ a0) {
+  align_pack_lds_body<R>(blockIdx.x, gridDim.x, x, m, perm, offsets, minrow, k, shift, czero, yhi, ylo, a0);
+}
+
 // --------------------------------------------------------------- S = Y^T Y
 // One 256x256 tile of S_j over packed columns [p0, p1) per block: four waves
 // in a 2x2 arrangement, each a 128x128 quarter as 4x4 MFMA 32x32x16 BF16
@@ -610,11 +672,12 @@ __global__ __launch_bounds__(256) void k_align_pack_lds(
 // tiles the lower-left quarter (the mirror of the upper-right one) is not
 // computed.
 template <bool kSplit>
-__global__ __launch_bounds__(256) void k_syrk(const SyrkItem* __restrict__ items,
+__device__ __forceinline__ void syrk_body(const int bid, const int nblk,
+    const SyrkItem* __restrict__ items,
                                               const uint16_t* __restrict__ yhi,
                                               const uint16_t* __restrict__ ylo, int m,
                                               float* __restrict__ part) {
-  const SyrkItem it = items[blockIdx.x];
+  const SyrkItem it = items[bid];
   if (it.cluster < 0) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wi = wave >> 1, wj = wave & 1;
@@ -697,17 +760,26 @@ __global__ __launch_bounds__(256) void k_syrk(const SyrkItem* __restrict__ items
       }
 }
 
+template <bool kSplit>
+__global__ __launch_bounds__(256) void k_syrk(const SyrkItem* __restrict__ items,
+                                              const uint16_t* __restrict__ yhi,
+                                              const uint16_t* __restrict__ ylo, int m,
+                                              float* __restrict__ part) {
+  syrk_body<kSplit>(blockIdx.x, gridDim.x, items, yhi, ylo, m, part);
+}
+
 // S_j from the K-chunk partials of each tile, summed in a fixed order.  One
 // block per 64x64 sub-block (16 per tile, the lower ones of diagonal tiles
 // skipped): 16-byte partial loads, four rows per thread in flight, the sums
 // staged in LDS; then the entries with i <= j and their mirrors are stored
 // as coalesced row segments, so S_j is exactly symmetric.
-__global__ __launch_bounds__(256) void k_syrk_reduce(const SyrkTile* __restrict__ tiles,
+__device__ __forceinline__ void syrk_reduce_body(const int bid, const int nblk,
+    const SyrkTile* __restrict__ tiles,
                                                      const float* __restrict__ part, int m,
                                                      float* __restrict__ S) {
   constexpr int kSub = kSyrkTile / 64;
-  const SyrkTile tl = tiles[blockIdx.x / (kSub * kSub)];
-  const int sb = blockIdx.x % (kSub * kSub), si = sb / kSub, sj = sb % kSub;
+  const SyrkTile tl = tiles[bid / (kSub * kSub)];
+  const int sb = bid % (kSub * kSub), si = sb / kSub, sj = sb % kSub;
   if (tl.ti == tl.tj && si > sj) return;
   const int gi0 = tl.ti * kSyrkTile + si * 64, gj0 = tl.tj * kSyrkTile + sj * 64;
   if (gi0 >= m || gj0 >= m) return;
@@ -748,6 +820,12 @@ __global__ __launch_bounds__(256) void k_syrk_reduce(const SyrkTile* __restrict_
     const int hi = gi0 + col, hj = gj0 + r;  // mirror of entry (hi, hj)
     if (hi < m && hj < m && hi <= hj) Sj[(long)hj * m + hi] = tr[col][r];
   }
+}
+
+__global__ __launch_bounds__(256) void k_syrk_reduce(const SyrkTile* __restrict__ tiles,
+                                                     const float* __restrict__ part, int m,
+                                                     float* __restrict__ S) {
+  syrk_reduce_body(blockIdx.x, gridDim.x, tiles, part, m, S);
 }
 
 // ---------------------------------------------------------- power iteration
@@ -961,7 +1039,7 @@ __device__ void sign_fix(const float* a0, double* c, double* w, int m, double* s
 // only reports fallback = 1 (the caller reruns it in eig_mode); otherwise,
 // and when no eigenvector converged, the power method keeps iterating
 // towards the top eigenvector.  mask (may be NULL): clusters to process.
-__global__ __launch_bounds__(kPowerThreads) void k_power(
+__device__ __forceinline__ void power_body(const int bid, const int nblk,
     const float* __restrict__ S, int m, const int* __restrict__ counts,
     const float* __restrict__ a0all, uint64_t seed, int iter, int max_steps, double tol,
     const double* __restrict__ vin, const double* __restrict__ svin,
@@ -975,7 +1053,7 @@ __global__ __launch_bounds__(kPowerThreads) void k_power(
   double* scratch = sh + 3 * m;  // 8 * kPowerWaves doubles
   int* iscratch = reinterpret_cast<int*>(scratch + 6 * kPowerWaves);
   double* part = scratch + 8 * kPowerWaves;  // sym_part_len(m) doubles
-  const int j = blockIdx.x;
+  const int j = bid;
   if (mask && !mask[j]) return;
   const float* Sj = S + (long)j * m * m;
   const float* a0 = a0all + (long)j * m;
@@ -1122,6 +1200,16 @@ __global__ __launch_bounds__(kPowerThreads) void k_power(
     inf.flipped = flipped;
     info[j] = inf;
   }
+}
+
+__global__ __launch_bounds__(kPowerThreads) void k_power(
+    const float* __restrict__ S, int m, const int* __restrict__ counts,
+    const float* __restrict__ a0all, uint64_t seed, int iter, int max_steps, double tol,
+    const double* __restrict__ vin, const double* __restrict__ svin,
+    const double* __restrict__ rsum, const int* __restrict__ steps_in,
+    const int* __restrict__ conv_in, int eig_mode, const int* __restrict__ mask,
+    int defer_fallback, float* __restrict__ c_out, PowerInfo* __restrict__ info) {
+  power_body(blockIdx.x, gridDim.x, S, m, counts, a0all, seed, iter, max_steps, tol, vin, svin, rsum, steps_in, conv_in, eig_mode, mask, defer_fallback, c_out, info);
 }
 
 // ----------------------------------------------------- Lanczos power result
@@ -1433,7 +1521,7 @@ __device__ double tridiag_top_eig_wave(const double* alpha, const double* beta, 
 
 // Fused form for small m (S_j resident in L2): one block per cluster runs
 // every step, with the symmetric matvec done by the block itself.
-__global__ __launch_bounds__(kPowerThreads) void k_power_lanczos(
+__device__ __forceinline__ void power_lanczos_body(const int bid, const int nblk,
     const float* __restrict__ S, int m, const int* __restrict__ counts, uint64_t seed, int iter,
     int I, double lz_tol, double* __restrict__ Qg, double* __restrict__ vout,
     double* __restrict__ rout, int* __restrict__ steps_out, int* __restrict__ conv_out) {
@@ -1451,7 +1539,7 @@ __global__ __launch_bounds__(kPowerThreads) void k_power_lanczos(
   int* flag = reinterpret_cast<int*>(yp + kLzMax);  // 2 ints (padded to 1 double)
   double* part = yp + kLzMax + 1;     // sym_part_len(m)
   double* sq = part + sym_part_len(m);  // kLzSqDoubles: acceptance test
-  const int j = blockIdx.x;
+  const int j = bid;
   if (counts[j] == 0) return;
   const float* Sj = S + (long)j * m * m;
   double* Q = Qg + (long)j * (kLzMax + 1) * m;
@@ -1587,6 +1675,13 @@ __global__ __launch_bounds__(kPowerThreads) void k_power_lanczos(
   }
 }
 
+__global__ __launch_bounds__(kPowerThreads) void k_power_lanczos(
+    const float* __restrict__ S, int m, const int* __restrict__ counts, uint64_t seed, int iter,
+    int I, double lz_tol, double* __restrict__ Qg, double* __restrict__ vout,
+    double* __restrict__ rout, int* __restrict__ steps_out, int* __restrict__ conv_out) {
+  power_lanczos_body(blockIdx.x, gridDim.x, S, m, counts, seed, iter, I, lz_tol, Qg, vout, rout, steps_out, conv_out);
+}
+
 // --------------------------------------------- Lanczos, one step per launch
 // The Lanczos stage runs as a host-driven sequence of two launches per step
 // so that the symmetric matvec S_j q_j of every cluster spreads over the
@@ -1606,11 +1701,11 @@ constexpr int kMvThreads = 256;
 constexpr int kMvRowsPerWave = 4;
 constexpr int kMvRows = kMvRowsPerWave * (kMvThreads / kWave);  // rows per block
 
-__global__ __launch_bounds__(kMvThreads) void k_cluster_matvec(
+__device__ __forceinline__ void cluster_matvec_body(const int bid, const int nblk,
     const float* __restrict__ S, int m, int k, const double* __restrict__ x, long xstride,
     double* __restrict__ y, const LzState* __restrict__ state) {
   const int panels = (m + kMvRows - 1) / kMvRows;
-  const int j = blockIdx.x / panels, p = blockIdx.x % panels;
+  const int j = bid / panels, p = bid % panels;
   if (j >= k || (state && state[j].done)) return;
   extern __shared__ double xs[];  // m doubles
   const double* xj = x ? x + (long)j * xstride : nullptr;
@@ -1650,12 +1745,18 @@ __global__ __launch_bounds__(kMvThreads) void k_cluster_matvec(
   }
 }
 
+__global__ __launch_bounds__(kMvThreads) void k_cluster_matvec(
+    const float* __restrict__ S, int m, int k, const double* __restrict__ x, long xstride,
+    double* __restrict__ y, const LzState* __restrict__ state) {
+  cluster_matvec_body(blockIdx.x, gridDim.x, S, m, k, x, xstride, y, state);
+}
+
 // Starts the Lanczos process of every cluster from r = S 1 (in rbuf).
-__global__ __launch_bounds__(kPowerThreads) void k_lz_setup(
+__device__ __forceinline__ void lz_setup_body(const int bid, const int nblk,
     int m, const int* __restrict__ counts, const int* __restrict__ mask, uint64_t seed, int iter,
     const double* __restrict__ rbuf, double* __restrict__ Qg, LzState* __restrict__ state) {
   __shared__ double scratch[2 * kPowerWaves];
-  const int j = blockIdx.x;
+  const int j = bid;
   double* Q = Qg + (long)j * (kLzMax + 1) * m;
   double sg = 0.0, n0 = 0.0;
   for (int i = threadIdx.x; i < m; i += kPowerThreads) {
@@ -1674,8 +1775,14 @@ __global__ __launch_bounds__(kPowerThreads) void k_lz_setup(
   }
 }
 
+__global__ __launch_bounds__(kPowerThreads) void k_lz_setup(
+    int m, const int* __restrict__ counts, const int* __restrict__ mask, uint64_t seed, int iter,
+    const double* __restrict__ rbuf, double* __restrict__ Qg, LzState* __restrict__ state) {
+  lz_setup_body(blockIdx.x, gridDim.x, m, counts, mask, seed, iter, rbuf, Qg, state);
+}
+
 // One Lanczos step of every active cluster, given w = S q_it in wbuf.
-__global__ __launch_bounds__(kPowerThreads) void k_lz_step(
+__device__ __forceinline__ void lz_step_body(const int bid, const int nblk,
     int m, int mode, int I, double lz_tol, const double* __restrict__ rbuf,
     const double* __restrict__ wbuf, double* __restrict__ Qg, LzState* __restrict__ state) {
   extern __shared__ double lz_sh[];
@@ -1686,7 +1793,7 @@ __global__ __launch_bounds__(kPowerThreads) void k_lz_step(
   double* scratch = qp + m;       // 8 * kPowerWaves
   double* cf = scratch + 8 * kPowerWaves;  // kLzMax + 1
   double* part = cf + kLzMax + 1;          // sym_part_len(m)
-  const int j = blockIdx.x;
+  const int j = bid;
   LzState* stp = state + j;
   if (stp->done) return;
   const int it = stp->it;
@@ -1826,12 +1933,18 @@ __global__ __launch_bounds__(kPowerThreads) void k_lz_step(
   }
 }
 
+__global__ __launch_bounds__(kPowerThreads) void k_lz_step(
+    int m, int mode, int I, double lz_tol, const double* __restrict__ rbuf,
+    const double* __restrict__ wbuf, double* __restrict__ Qg, LzState* __restrict__ state) {
+  lz_step_body(blockIdx.x, gridDim.x, m, mode, I, lz_tol, rbuf, wbuf, Qg, state);
+}
+
 // v = Q y of every cluster (zeros when its Lanczos run was not accepted).
-__global__ __launch_bounds__(kPowerThreads) void k_lz_finish(
+__device__ __forceinline__ void lz_finish_body(const int bid, const int nblk,
     int m, const int* __restrict__ counts, const double* __restrict__ Qg,
     const LzState* __restrict__ state, double* __restrict__ vout, int* __restrict__ steps_out,
     int* __restrict__ conv_out) {
-  const int j = blockIdx.x;
+  const int j = bid;
   const LzState& z = state[j];
   const int len = counts[j] == 0 ? 0 : z.len;
   const int conv = counts[j] != 0 && z.conv;
@@ -1846,6 +1959,13 @@ __global__ __launch_bounds__(kPowerThreads) void k_lz_finish(
     steps_out[j] = len;
     conv_out[j] = conv;
   }
+}
+
+__global__ __launch_bounds__(kPowerThreads) void k_lz_finish(
+    int m, const int* __restrict__ counts, const double* __restrict__ Qg,
+    const LzState* __restrict__ state, double* __restrict__ vout, int* __restrict__ steps_out,
+    int* __restrict__ conv_out) {
+  lz_finish_body(blockIdx.x, gridDim.x, m, counts, Qg, state, vout, steps_out, conv_out);
 }
 
 // ------------------------------------ Lanczos, several blocks per cluster
@@ -2348,6 +2468,136 @@ __global__ __launch_bounds__(kLzCoopThreads) void k_lz_coop(
 }
 
 // ---------------------------------------------------------------- launchers
+// --------------------------------------------------------- grouped forms
+// The kernels above for many jobs in one launch (GroupTable): each block
+// finds its job and runs the body as that job's block bid of nblk.
+#define TCX_GROUP_JOB(T)                                                       \
+  const int gj_ = group_job(first, njobs, blockIdx.x);                         \
+  const T a = jobs[gj_];                                                       \
+  const int bid = blockIdx.x - first[gj_], nblk = first[gj_ + 1] - first[gj_];
+
+// Fills of many segments: blockIdx.y is the segment.
+__global__ void k_fill_segs(const FillSeg* __restrict__ segs) {
+  const FillSeg sg = segs[blockIdx.y];
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < sg.count;
+       i += (long)gridDim.x * blockDim.x)
+    sg.p[i] = sg.value;
+}
+
+__global__ __launch_bounds__(256) void k_group_count_group(const GroupCountJob* __restrict__ jobs,
+                                                           const int* __restrict__ first,
+                                                           int njobs) {
+  TCX_GROUP_JOB(GroupCountJob)
+  group_count_body(bid, nblk, a.label, a.n, a.k, a.dist, a.B, a.blk_count, a.minrow, a.lds_min,
+                   a.bad);
+}
+
+__global__ __launch_bounds__(256) void k_group_scan_group(const GroupScanJob* __restrict__ jobs,
+                                                          const int* __restrict__ first,
+                                                          int njobs) {
+  TCX_GROUP_JOB(GroupScanJob)
+  group_scan_body(bid, nblk, a.nrowblk, a.K, a.blk_count, a.blk_base, a.counts);
+}
+
+__global__ __launch_bounds__(1024) void k_group_offsets_group(
+    const GroupOffsetsJob* __restrict__ jobs, const int* __restrict__ first, int njobs) {
+  TCX_GROUP_JOB(GroupOffsetsJob)
+  (void)bid;
+  (void)nblk;
+  group_offsets_body(a.k, a.B, a.kcounts, a.counts, a.offsets, a.koff);
+}
+
+__global__ __launch_bounds__(kGroupChunk) void k_group_place_group(
+    const GroupPlaceJob* __restrict__ jobs, const int* __restrict__ first, int njobs) {
+  TCX_GROUP_JOB(GroupPlaceJob)
+  group_place_par_body(bid, nblk, a.label, a.n, a.k, a.dist, a.B, a.blk_base, a.koff, a.perm);
+}
+
+__global__ __launch_bounds__(256) void k_row_stats_group(const RowStatsJob* __restrict__ jobs,
+                                                         const int* __restrict__ first,
+                                                         int njobs) {
+  TCX_GROUP_JOB(RowStatsJob)
+  row_stats_body(bid, nblk, a.x, a.rows, a.m, a.stats);
+}
+
+__global__ __launch_bounds__(256) void k_align_pack_group(const AlignPackJob* __restrict__ jobs,
+                                                          const int* __restrict__ first,
+                                                          int njobs) {
+  TCX_GROUP_JOB(AlignPackJob)
+  align_pack_body(bid, nblk, a.x, a.m, a.perm, a.n_pad, a.offsets, a.minrow, a.k, a.shift,
+                  a.czero, a.stats, a.yhi, a.ylo, a.a0);
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_align_pack_lds_group(
+    const AlignPackJob* __restrict__ jobs, const int* __restrict__ first, int njobs) {
+  TCX_GROUP_JOB(AlignPackJob)
+  align_pack_lds_body<R>(bid, nblk, a.x, a.m, a.perm, a.offsets, a.minrow, a.k, a.shift,
+                         a.czero, a.yhi, a.ylo, a.a0);
+}
+
+// One item (tile of one job's cluster) per block.
+template <bool kSplit>
+__global__ __launch_bounds__(256) void k_syrk_group(const SyrkJob* __restrict__ jobs,
+                                                    const GroupSyrkItem* __restrict__ items) {
+  const GroupSyrkItem* gi = items + blockIdx.x;
+  const SyrkJob jb = jobs[gi->job];
+  syrk_body<kSplit>(0, 1, &gi->it, jb.yhi, jb.ylo, jb.m, jb.part);
+}
+
+__global__ __launch_bounds__(256) void k_syrk_reduce_group(const SyrkJob* __restrict__ jobs,
+                                                           const GroupSyrkTile* __restrict__ tiles) {
+  constexpr int kSub = kSyrkTile / 64;
+  const GroupSyrkTile* gt = tiles + blockIdx.x / (kSub * kSub);
+  const SyrkJob jb = jobs[gt->job];
+  syrk_reduce_body(blockIdx.x % (kSub * kSub), kSub * kSub, &gt->t, jb.part, jb.m, jb.S);
+}
+
+__global__ __launch_bounds__(kPowerThreads) void k_power_lanczos_group(
+    const PowerLanczosJob* __restrict__ jobs, const int* __restrict__ first, int njobs,
+    double lz_tol) {
+  TCX_GROUP_JOB(PowerLanczosJob)
+  power_lanczos_body(bid, nblk, a.S, a.m, a.counts, a.seed, a.iter, a.I, lz_tol, a.Q, a.vout,
+                     a.rout, a.steps_out, a.conv_out);
+}
+
+__global__ __launch_bounds__(kMvThreads) void k_cluster_matvec_group(
+    const ClusterMatvecJob* __restrict__ jobs, const int* __restrict__ first, int njobs) {
+  TCX_GROUP_JOB(ClusterMatvecJob)
+  cluster_matvec_body(bid, nblk, a.S, a.m, a.k, a.x, a.xstride, a.y, a.state);
+}
+
+__global__ __launch_bounds__(kPowerThreads) void k_lz_setup_group(const LzJob* __restrict__ jobs,
+                                                                  const int* __restrict__ first,
+                                                                  int njobs) {
+  TCX_GROUP_JOB(LzJob)
+  lz_setup_body(bid, nblk, a.m, a.counts, a.mask, a.seed, a.iter, a.rbuf, a.Q, a.state);
+}
+
+__global__ __launch_bounds__(kPowerThreads) void k_lz_step_group(const LzJob* __restrict__ jobs,
+                                                                 const int* __restrict__ first,
+                                                                 int njobs, int mode,
+                                                                 double lz_tol) {
+  TCX_GROUP_JOB(LzJob)
+  lz_step_body(bid, nblk, a.m, mode, a.I, lz_tol, a.rbuf, a.wbuf, a.Q, a.state);
+}
+
+__global__ __launch_bounds__(kPowerThreads) void k_lz_finish_group(const LzJob* __restrict__ jobs,
+                                                                   const int* __restrict__ first,
+                                                                   int njobs) {
+  TCX_GROUP_JOB(LzJob)
+  lz_finish_body(bid, nblk, a.m, a.counts, a.Q, a.state, a.vout, a.steps_out, a.conv_out);
+}
+
+__global__ __launch_bounds__(kPowerThreads) void k_power_group(const PowerJob* __restrict__ jobs,
+                                                               const int* __restrict__ first,
+                                                               int njobs) {
+  TCX_GROUP_JOB(PowerJob)
+  power_body(bid, nblk, a.S, a.m, a.counts, a.a0, a.seed, a.iter, a.max_steps, a.tol, a.vin,
+             a.svin, a.rsum, a.steps_in, a.conv_in, a.eig_mode, a.mask, a.defer_fallback, a.c_out,
+             a.info);
+}
+
 hipError_t launch_fill_int(int* p, long count, int value, hipStream_t st) {
   if (count <= 0) return hipSuccess;
   long blocks = (count + 255) / 256;
@@ -2570,6 +2820,164 @@ hipError_t launch_lz_finish(int m, int k, const int* counts, const double* Q,
   return hipGetLastError();
 }
 
+// ------------------------------------------------ grouped-form launchers
+hipError_t launch_fill_segs(const FillSeg* segs, int nseg, long max_count, hipStream_t st) {
+  if (nseg <= 0 || max_count <= 0) return hipSuccess;
+  long blocks = (max_count + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  for (int s0 = 0; s0 < nseg; s0 += 65535) {  // grid y limit
+    const int ns = nseg - s0 < 65535 ? nseg - s0 : 65535;
+    hipLaunchKernelGGL(k_fill_segs, dim3((unsigned)blocks, (unsigned)ns), dim3(256), 0, st,
+                       segs + s0);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+size_t group_count_lds(int k, int B, int* lds_min) {
+  const size_t lds = sizeof(int) * ((size_t)k * B + k);
+  *lds_min = lds <= 65536;
+  return *lds_min ? lds : sizeof(int) * (size_t)k * B;
+}
+
+#define TCX_GROUP_ARGS(T) static_cast<const T*>(g.jobs), g.first, g.njobs
+
+hipError_t launch_group_count_group(const GroupTable& g, hipStream_t st) {
+  if (g.blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_group_count_group, dim3(g.blocks), dim3(256), g.lds, st,
+                     TCX_GROUP_ARGS(GroupCountJob));
+  return hipGetLastError();
+}
+
+hipError_t launch_group_scan_group(const GroupTable& g, hipStream_t st) {
+  if (g.blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_group_scan_group, dim3(g.blocks), dim3(256), 0, st,
+                     TCX_GROUP_ARGS(GroupScanJob));
+  return hipGetLastError();
+}
+
+hipError_t launch_group_offsets_group(const GroupTable& g, hipStream_t st) {
+  if (g.blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_group_offsets_group, dim3(g.blocks), dim3(1024), 0, st,
+                     TCX_GROUP_ARGS(GroupOffsetsJob));
+  return hipGetLastError();
+}
+
+size_t group_place_lds(int k, int B) { return sizeof(int) * kPlaceWaves * (size_t)k * B; }
+
+hipError_t launch_group_place_group(const GroupTable& g, hipStream_t st) {
+  if (g.blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_group_place_group, dim3(g.blocks), dim3(kGroupChunk), g.lds, st,
+                     TCX_GROUP_ARGS(GroupPlaceJob));
+  return hipGetLastError();
+}
+
+hipError_t launch_row_stats_group(const GroupTable& g, hipStream_t st) {
+  if (g.blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_row_stats_group, dim3(g.blocks), dim3(256), 0, st,
+                     TCX_GROUP_ARGS(RowStatsJob));
+  return hipGetLastError();
+}
+
+int align_pack_form(int m) { return m <= kPackLdsMaxM ? pack_lds_rows(m) : 0; }
+long align_pack_cols(int form) { return form ? form : 64; }
+size_t align_pack_lds(int form, int m) { return form ? sizeof(float) * (size_t)form * (m + 4) : 0; }
+
+hipError_t launch_align_pack_group(int form, const GroupTable& g, hipStream_t st) {
+  if (g.blocks <= 0) return hipSuccess;
+  switch (form) {
+    case 0:
+      hipLaunchKernelGGL(k_align_pack_group, dim3(g.blocks), dim3(256), 0, st,
+                         TCX_GROUP_ARGS(AlignPackJob));
+      break;
+    case 16:
+      hipLaunchKernelGGL(k_align_pack_lds_group<16>, dim3(g.blocks), dim3(256), g.lds, st,
+                         TCX_GROUP_ARGS(AlignPackJob));
+      break;
+    case 32:
+      hipLaunchKernelGGL(k_align_pack_lds_group<32>, dim3(g.blocks), dim3(256), g.lds, st,
+                         TCX_GROUP_ARGS(AlignPackJob));
+      break;
+    case 64:
+      hipLaunchKernelGGL(k_align_pack_lds_group<64>, dim3(g.blocks), dim3(256), g.lds, st,
+                         TCX_GROUP_ARGS(AlignPackJob));
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_syrk_group(const SyrkJob* jobs, const GroupSyrkItem* items, int nitems,
+                             int split, hipStream_t st) {
+  if (nitems <= 0) return hipSuccess;
+  if (split)
+    hipLaunchKernelGGL(k_syrk_group<true>, dim3(nitems), dim3(256), 0, st, jobs, items);
+  else
+    hipLaunchKernelGGL(k_syrk_group<false>, dim3(nitems), dim3(256), 0, st, jobs, items);
+  return hipGetLastError();
+}
+
+hipError_t launch_syrk_reduce_group(const SyrkJob* jobs, const GroupSyrkTile* tiles, int ntiles,
+                                    hipStream_t st) {
+  if (ntiles <= 0) return hipSuccess;
+  constexpr int kSub = kSyrkTile / 64;
+  hipLaunchKernelGGL(k_syrk_reduce_group, dim3((unsigned)ntiles * kSub * kSub), dim3(256), 0, st,
+                     jobs, tiles);
+  return hipGetLastError();
+}
+
+size_t power_lanczos_lds(int m) { return lanczos_smem(m); }
+
+hipError_t launch_power_lanczos_group(const GroupTable& g, double lz_tol, hipStream_t st) {
+  if (g.blocks <= 0) return hipSuccess;
+  if (g.lds > lanczos_smem(kFusedLanczosMaxM)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_power_lanczos_group, dim3(g.blocks), dim3(kPowerThreads), g.lds, st,
+                     TCX_GROUP_ARGS(PowerLanczosJob), lz_tol);
+  return hipGetLastError();
+}
+
+long cluster_matvec_blocks(int m, int k) { return (long)((m + kMvRows - 1) / kMvRows) * k; }
+
+hipError_t launch_cluster_matvec_group(const GroupTable& g, hipStream_t st) {
+  if (g.blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cluster_matvec_group, dim3(g.blocks), dim3(kMvThreads), g.lds, st,
+                     TCX_GROUP_ARGS(ClusterMatvecJob));
+  return hipGetLastError();
+}
+
+hipError_t launch_lz_setup_group(const GroupTable& g, hipStream_t st) {
+  if (g.blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lz_setup_group, dim3(g.blocks), dim3(kPowerThreads), 0, st,
+                     TCX_GROUP_ARGS(LzJob));
+  return hipGetLastError();
+}
+
+hipError_t launch_lz_step_group(const GroupTable& g, int mode, double lz_tol, hipStream_t st) {
+  if (g.blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lz_step_group, dim3(g.blocks), dim3(kPowerThreads), g.lds, st,
+                     TCX_GROUP_ARGS(LzJob), mode, lz_tol);
+  return hipGetLastError();
+}
+
+hipError_t launch_lz_finish_group(const GroupTable& g, hipStream_t st) {
+  if (g.blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lz_finish_group, dim3(g.blocks), dim3(kPowerThreads), 0, st,
+                     TCX_GROUP_ARGS(LzJob));
+  return hipGetLastError();
+}
+
+size_t power_lds(int m) { return power_smem(m); }
+size_t lz_step_lds(int m) { return lz_step_smem(m); }
+
+hipError_t launch_power_group(const GroupTable& g, hipStream_t st) {
+  if (g.blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_power_group, dim3(g.blocks), dim3(kPowerThreads), g.lds, st,
+                     TCX_GROUP_ARGS(PowerJob));
+  return hipGetLastError();
+}
+
 hipError_t configure_extract_kernels() {
   hipError_t e = hipFuncSetAttribute((const void*)k_power,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -2600,6 +3008,37 @@ hipError_t configure_extract_kernels() {
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void*)k_align_pack_lds<16>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, kPackLdsMaxBytes);
+  // grouped forms
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_power_group, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)power_smem(4096));
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_power_lanczos_group,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lanczos_smem(kFusedLanczosMaxM));
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_lz_step_group,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lz_step_smem(4096));
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_cluster_matvec_group,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(double) * 4096));
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_align_pack_lds_group<64>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kPackLdsMaxBytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_align_pack_lds_group<32>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kPackLdsMaxBytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_align_pack_lds_group<16>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kPackLdsMaxBytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_group_count_group,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_group_place_group,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(int) * kPlaceWaves * kPlaceParMaxK));
   return e;
 }
 

@@ -7,6 +7,7 @@
 // the lane group that produced it reads it back with coalesced loads.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "tcx_device.hpp"
@@ -18,10 +19,11 @@ namespace tcx {
 // One wave per row, FP64 accumulation of mean and of the centred sum of
 // squares; (x - mean) / sqrt(ss / (m - ddof)), NaN -> 0 (nan_to_num).
 // Semantics of zscore, Code/test/test_timesC.py:42-52.
-__global__ __launch_bounds__(256) void k_znorm(float* __restrict__ x, long rows,
+__device__ __forceinline__ void znorm_body(const int bid, const int nblk,
+    float* __restrict__ x, long rows,
                                                int m, int ddof) {
   const int lane = threadIdx.x & (kWave - 1);
-  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long row = (long)bid * 4 + (threadIdx.x >> 6);
   if (row >= rows) return;
   float* r = x + row * (long)m;
   double s = 0.0;
@@ -48,15 +50,21 @@ __global__ __launch_bounds__(256) void k_znorm(float* __restrict__ x, long rows,
   }
 }
 
+__global__ __launch_bounds__(256) void k_znorm(float* __restrict__ x, long rows,
+                                               int m, int ddof) {
+  znorm_body(blockIdx.x, gridDim.x, x, rows, m, ddof);
+}
+
 // The same for m % 4 == 0, m <= 256 * NV: the row is read once, as float4
 // vectors held in registers (lane l holds vectors l, l + 64, ...), and
 // written once; the sums are the same two-pass FP64 sums (in another
 // order).
 template <int NV>
-__global__ __launch_bounds__(256) void k_znorm_v(float* __restrict__ x, long rows, int m,
+__device__ __forceinline__ void znorm_v_body(const int bid, const int nblk,
+    float* __restrict__ x, long rows, int m,
                                                  int ddof) {
   const int lane = threadIdx.x & (kWave - 1);
-  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long row = (long)bid * 4 + (threadIdx.x >> 6);
   if (row >= rows) return;
   float4* r = reinterpret_cast<float4*>(x + row * (long)m);
   const int nv = m >> 2;
@@ -94,13 +102,19 @@ __global__ __launch_bounds__(256) void k_znorm_v(float* __restrict__ x, long row
   }
 }
 
+template <int NV>
+__global__ __launch_bounds__(256) void k_znorm_v(float* __restrict__ x, long rows, int m,
+                                                 int ddof) {
+  znorm_v_body<NV>(blockIdx.x, gridDim.x, x, rows, m, ddof);
+}
+
 // ------------------------------------------------------------- spectra
 // Forward FFT of each row (zero padded to F, imaginary part 0), stored
 // lane-strided, plus the row's L2 norm, an all-zero flag and (mag != NULL)
 // the spectrum's magnitudes lane-major: |X[lane + L*s]| at lane*P + s, so a
 // lane reads its values as 16-byte vectors.
 template <int F>
-__global__ __launch_bounds__(FftGeom<F>::kBlock) void k_spectra(
+__device__ __forceinline__ void spectra_body(const int bid, const int nblk,
     const float* __restrict__ y, int rows, int m, const float2* __restrict__ twg,
     float2* __restrict__ spec, float* __restrict__ norms, int* __restrict__ zero,
     float* __restrict__ mag, uint16_t* __restrict__ bmag) {
@@ -113,7 +127,7 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_spectra(
   __syncthreads();
   const int g = threadIdx.x / L, lane = threadIdx.x % L;
   float2* buf = smem + F + g * fft_buf_len<F>();
-  for (int row = blockIdx.x * G::kGroups + g; row < rows; row += gridDim.x * G::kGroups) {
+  for (int row = bid * G::kGroups + g; row < rows; row += nblk * G::kGroups) {
     const float* yr = y + (long)row * m;
     float2 v[P];
     float ss = 0.f;
@@ -153,6 +167,14 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_spectra(
       if (zero) zero[row] = fnz == 0.f;
     }
   }
+}
+
+template <int F>
+__global__ __launch_bounds__(FftGeom<F>::kBlock) void k_spectra(
+    const float* __restrict__ y, int rows, int m, const float2* __restrict__ twg,
+    float2* __restrict__ spec, float* __restrict__ norms, int* __restrict__ zero,
+    float* __restrict__ mag, uint16_t* __restrict__ bmag) {
+  spectra_body<F>(blockIdx.x, gridDim.x, y, rows, m, twg, spec, norms, zero, mag, bmag);
 }
 
 // ------------------------------------------------ fused SBD / assignment
@@ -406,7 +428,7 @@ __device__ __forceinline__ void partner_half(const float2 (&zx)[P], float2* buf,
 // X_r[F-f] = conj(X_r[f])), and xn[r] = |x_r|.  Rows are transformed in
 // natural pairs packed as x_a + i x_b and split with partner_half.
 template <int F>
-__global__ __launch_bounds__(FftGeom<F>::kBlock) void k_series_spec(
+__device__ __forceinline__ void series_spec_body(const int bid, const int nblk,
     const float* __restrict__ x, long n, int m, const float2* __restrict__ twg,
     float2* __restrict__ xs, float* __restrict__ xn, uint16_t* __restrict__ xm) {
   using G = FftGeom<F>;
@@ -420,8 +442,8 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_series_spec(
   const int g = threadIdx.x / L, lane = threadIdx.x % L;
   float2* buf = smem + F + g * fft_buf_len<F>();
   const long npairs = (n + 1) / 2;
-  for (long pr = (long)blockIdx.x * G::kGroups + g; pr < npairs;
-       pr += (long)gridDim.x * G::kGroups) {
+  for (long pr = (long)bid * G::kGroups + g; pr < npairs;
+       pr += (long)nblk * G::kGroups) {
     const long ia = 2 * pr, ib = 2 * pr + 1;
     const bool hasb = ib < n;
     const float* xa = x + ia * (long)m;
@@ -477,6 +499,13 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_series_spec(
   }
 }
 
+template <int F>
+__global__ __launch_bounds__(FftGeom<F>::kBlock) void k_series_spec(
+    const float* __restrict__ x, long n, int m, const float2* __restrict__ twg,
+    float2* __restrict__ xs, float* __restrict__ xn, uint16_t* __restrict__ xm) {
+  series_spec_body<F>(blockIdx.x, gridDim.x, x, n, m, twg, xs, xn, xm);
+}
+
 // ------------------------------------------ bounds of the pruned assignment
 // ub[i][j] = sum_f xm[i][f] cm[j][f] / F for every series i and centroid j:
 // the bounds k_assign_prune otherwise forms per pair, as one product of the
@@ -494,19 +523,20 @@ __global__ __launch_bounds__(FftGeom<F>::kBlock) void k_series_spec(
 typedef _Float16 bound_h4 __attribute__((ext_vector_type(4)));
 typedef float bound_f4 __attribute__((ext_vector_type(4)));
 template <int NT, int MT, int NS>
-__global__ __launch_bounds__(256) void k_bounds(const uint16_t* __restrict__ xm, long n, int kpad,
+__device__ __forceinline__ void bounds_body(const int bid, const int nblk,
+    const uint16_t* __restrict__ xm, long n, int kpad,
                                                 const uint16_t* __restrict__ cm, int k, int ld,
                                                 float scale, float* __restrict__ ub) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const long blk0 = ((long)blockIdx.x * 4 + wid) * MT;  // first 16-row block of the wave
+  const long blk0 = ((long)bid * 4 + wid) * MT;  // first 16-row block of the wave
   if (blk0 * 16 >= n) return;
-  const long nblk = (n + 15) >> 4;
+  const long nrb = (n + 15) >> 4;  // row blocks of 16
   const int r = lane & 15, kq = lane >> 4;
   const int nsteps = kpad >> 4;
   const bound_h4* xa[MT];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt) {
-    const long b = blk0 + mt < nblk ? blk0 + mt : blk0;  // blocks past n: computed, not stored
+    const long b = blk0 + mt < nrb ? blk0 + mt : blk0;  // blocks past n: computed, not stored
     xa[mt] = reinterpret_cast<const bound_h4*>(xm) + b * nsteps * 64 + lane;
   }
   for (int t0 = 0; t0 * 16 < k; t0 += NT) {
@@ -554,6 +584,13 @@ __global__ __launch_bounds__(256) void k_bounds(const uint16_t* __restrict__ xm,
         }
       }
   }
+}
+
+template <int NT, int MT, int NS>
+__global__ __launch_bounds__(256) void k_bounds(const uint16_t* __restrict__ xm, long n, int kpad,
+                                                const uint16_t* __restrict__ cm, int k, int ld,
+                                                float scale, float* __restrict__ ub) {
+  bounds_body<NT, MT, NS>(blockIdx.x, gridDim.x, xm, n, kpad, cm, k, ld, scale, ub);
 }
 
 // Bounds of every centroid for one series pair (group of L lanes) from the
@@ -633,7 +670,7 @@ struct PruneGeom {
 };
 
 template <int F, bool kPre>
-__global__ __launch_bounds__(PruneGeom<F>::kBlock, PruneGeom<F>::kMinWaves) void k_assign_prune(
+__device__ __forceinline__ void assign_prune_body(const int bid, const int nblk,
     const float* __restrict__ x, const float2* __restrict__ xs, const float* __restrict__ xn,
     const float* __restrict__ ubg, int ub_ld, long n, long npairs, int m, int k,
     const float2* __restrict__ cspec, const float* __restrict__ cmag,
@@ -654,8 +691,8 @@ __global__ __launch_bounds__(PruneGeom<F>::kBlock, PruneGeom<F>::kMinWaves) void
   unsigned long long my_evals = 0;
   int s_lo, s_hi;
   lag_rows(lane, L, m, F, s_lo, s_hi);
-  for (long pr = (long)blockIdx.x * G::kGroups + g; pr < npairs;
-       pr += (long)gridDim.x * G::kGroups) {
+  for (long pr = (long)bid * G::kGroups + g; pr < npairs;
+       pr += (long)nblk * G::kGroups) {
     long ia = order ? order[2 * pr] : 2 * pr;
     long ib = order ? (2 * pr + 1 < 2 * npairs ? order[2 * pr + 1] : -1) : 2 * pr + 1;
     if (ia >= n) ia = -1;
@@ -803,6 +840,17 @@ __global__ __launch_bounds__(PruneGeom<F>::kBlock, PruneGeom<F>::kMinWaves) void
   if (evals && lane == 0 && my_evals) atomicAdd(evals, my_evals);
 }
 
+template <int F, bool kPre>
+__global__ __launch_bounds__(PruneGeom<F>::kBlock, PruneGeom<F>::kMinWaves) void k_assign_prune(
+    const float* __restrict__ x, const float2* __restrict__ xs, const float* __restrict__ xn,
+    const float* __restrict__ ubg, int ub_ld, long n, long npairs, int m, int k,
+    const float2* __restrict__ cspec, const float* __restrict__ cmag,
+    const float* __restrict__ cnorm, const float2* __restrict__ twg, int rev,
+    const int* __restrict__ order, int* __restrict__ label, float* __restrict__ dist,
+    int* __restrict__ shift, int* __restrict__ changed, unsigned long long* __restrict__ evals) {
+  assign_prune_body<F, kPre>(blockIdx.x, gridDim.x, x, xs, xn, ubg, ub_ld, n, npairs, m, k, cspec, cmag, cnorm, twg, rev, order, label, dist, shift, changed, evals);
+}
+
 // ---------------------------------------------------------- zero-fill shift
 // out[t] = in[t - s] (0 outside), s = shift[row]; skipped when *skip != 0.
 // roll_zeropad, test_timesC.py:55-77.  In place: one block per row, the row
@@ -823,6 +871,61 @@ __global__ __launch_bounds__(256) void k_apply_shift(float* __restrict__ x, long
     }
     __syncthreads();
   }
+}
+
+// --------------------------------------------------------- grouped forms
+// The kernels above for many jobs in one launch (GroupTable): each block
+// finds its job and runs the body as that job's block bid of nblk.
+#define TCX_GROUP_JOB(T)                                                       \
+  const int gj_ = group_job(first, njobs, blockIdx.x);                         \
+  const T a = jobs[gj_];                                                       \
+  const int bid = blockIdx.x - first[gj_], nblk = first[gj_ + 1] - first[gj_];
+
+__global__ __launch_bounds__(256) void k_znorm_group(const ZnormJob* __restrict__ jobs,
+                                                     const int* __restrict__ first, int njobs) {
+  TCX_GROUP_JOB(ZnormJob)
+  znorm_body(bid, nblk, a.x, a.rows, a.m, a.ddof);
+}
+
+template <int NV>
+__global__ __launch_bounds__(256) void k_znorm_v_group(const ZnormJob* __restrict__ jobs,
+                                                       const int* __restrict__ first, int njobs) {
+  TCX_GROUP_JOB(ZnormJob)
+  znorm_v_body<NV>(bid, nblk, a.x, a.rows, a.m, a.ddof);
+}
+
+template <int F>
+__global__ __launch_bounds__(FftGeom<F>::kBlock) void k_spectra_group(
+    const SpectraJob* __restrict__ jobs, const int* __restrict__ first, int njobs,
+    const float2* __restrict__ twg) {
+  TCX_GROUP_JOB(SpectraJob)
+  spectra_body<F>(bid, nblk, a.y, a.rows, a.m, twg, a.spec, a.norms, a.zero, a.mag, a.bmag);
+}
+
+template <int F>
+__global__ __launch_bounds__(FftGeom<F>::kBlock) void k_series_spec_group(
+    const SeriesSpecJob* __restrict__ jobs, const int* __restrict__ first, int njobs,
+    const float2* __restrict__ twg) {
+  TCX_GROUP_JOB(SeriesSpecJob)
+  series_spec_body<F>(bid, nblk, a.x, a.n, a.m, twg, a.xs, a.xn, a.xm);
+}
+
+template <int NT, int MT, int NS>
+__global__ __launch_bounds__(256) void k_bounds_group(const BoundsJob* __restrict__ jobs,
+                                                      const int* __restrict__ first, int njobs,
+                                                      int kpad, float scale) {
+  TCX_GROUP_JOB(BoundsJob)
+  bounds_body<NT, MT, NS>(bid, nblk, a.xm, a.n, kpad, a.cm, a.k, a.ld, scale, a.ub);
+}
+
+template <int F>
+__global__ __launch_bounds__(PruneGeom<F>::kBlock, PruneGeom<F>::kMinWaves) void k_assign_prune_group(
+    const PruneJob* __restrict__ jobs, const int* __restrict__ first, int njobs,
+    const float2* __restrict__ twg) {
+  TCX_GROUP_JOB(PruneJob)
+  assign_prune_body<F, true>(bid, nblk, a.x, a.xs, a.xn, a.ub, a.ub_ld, a.n, a.npairs, a.m, a.k,
+                             a.cspec, a.cmag, a.cnorm, twg, a.rev, a.order, a.label, a.dist,
+                             a.shift, a.changed, a.evals);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -1045,7 +1148,17 @@ hipError_t configure_sbd_kernels() {
                             (int)prune_smem<FF>(prune_max_k_F<FF>()));                        \
   if (e == hipSuccess)                                                                        \
     e = hipFuncSetAttribute((const void*)k_series_spec<FF>,                                   \
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sbd_smem<FF>());
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sbd_smem<FF>()); \
+  if (e == hipSuccess)                                                                        \
+    e = hipFuncSetAttribute((const void*)k_series_spec_group<FF>,                             \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sbd_smem<FF>()); \
+  if (e == hipSuccess)                                                                        \
+    e = hipFuncSetAttribute((const void*)k_spectra_group<FF>,                                 \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sbd_smem<FF>()); \
+  if (e == hipSuccess)                                                                        \
+    e = hipFuncSetAttribute((const void*)k_assign_prune_group<FF>,                            \
+                            hipFuncAttributeMaxDynamicSharedMemorySize,                       \
+                            (int)prune_smem<FF>(prune_max_k_F<FF>()));
   TCX_FOR_EACH_F(X)
 #undef X
   return e;
@@ -1067,6 +1180,197 @@ hipError_t launch_znorm(float* x, long rows, int m, int ddof, hipStream_t st) {
     return hipGetLastError();
   }
   hipLaunchKernelGGL(k_znorm, dim3((unsigned)blocks), dim3(256), 0, st, x, rows, m, ddof);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------ grouped-form launchers
+long znorm_blocks(long rows) { return (rows + 3) / 4; }
+
+int znorm_vectors(const float* x, int m) {
+  if ((m & 3) != 0 || m > 256 * 16 || (reinterpret_cast<uintptr_t>(x) & 15) != 0) return 0;
+  return m <= 256 * 2 ? 2 : (m <= 256 * 4 ? 4 : (m <= 256 * 8 ? 8 : 16));
+}
+
+hipError_t launch_znorm_group(const GroupTable& g, int nv, hipStream_t st) {
+  if (g.blocks <= 0) return hipSuccess;
+  const ZnormJob* j = static_cast<const ZnormJob*>(g.jobs);
+  switch (nv) {
+    case 0:
+      hipLaunchKernelGGL(k_znorm_group, dim3(g.blocks), dim3(256), 0, st, j, g.first, g.njobs);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_znorm_v_group<2>, dim3(g.blocks), dim3(256), 0, st, j, g.first,
+                         g.njobs);
+      break;
+    case 4:
+      hipLaunchKernelGGL(k_znorm_v_group<4>, dim3(g.blocks), dim3(256), 0, st, j, g.first,
+                         g.njobs);
+      break;
+    case 8:
+      hipLaunchKernelGGL(k_znorm_v_group<8>, dim3(g.blocks), dim3(256), 0, st, j, g.first,
+                         g.njobs);
+      break;
+    case 16:
+      hipLaunchKernelGGL(k_znorm_v_group<16>, dim3(g.blocks), dim3(256), 0, st, j, g.first,
+                         g.njobs);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int F>
+static long groups_blocks(long units) {  // launch_spectra_F / launch_*_F block counts
+  long b = (units + FftGeom<F>::kGroups - 1) / FftGeom<F>::kGroups;
+  return b < 1 ? 1 : b;
+}
+
+long series_spec_blocks(int F, long n) {
+  switch (F) {
+#define X(FF) \
+  case FF:    \
+    return std::min(groups_blocks<FF>((n + 1) / 2), 8192L);
+    TCX_FOR_EACH_F(X)
+#undef X
+    default:
+      return 0;
+  }
+}
+
+long spectra_blocks(int F, long rows) {
+  switch (F) {
+#define X(FF) \
+  case FF:    \
+    return std::min(groups_blocks<FF>(rows), 4096L);
+    TCX_FOR_EACH_F(X)
+#undef X
+    default:
+      return 0;
+  }
+}
+
+long prune_blocks(int F, long npairs) {
+  switch (F) {
+#define X(FF)                                                                   \
+  case FF: {                                                                    \
+    long b = (npairs + PruneGeom<FF>::kGroups - 1) / PruneGeom<FF>::kGroups;     \
+    return b < 1 ? 1 : (b > 8192 ? 8192 : b);                                   \
+  }
+    TCX_FOR_EACH_F(X)
+#undef X
+    default:
+      return 0;
+  }
+}
+
+size_t prune_lds(int F, int k) {
+  switch (F) {
+#define X(FF) \
+  case FF:    \
+    return prune_smem<FF>(k);
+    TCX_FOR_EACH_F(X)
+#undef X
+    default:
+      return 0;
+  }
+}
+
+hipError_t launch_series_spec_group(int F, const GroupTable& g, const float2* tw, hipStream_t st) {
+  if (g.blocks <= 0) return hipSuccess;
+  const SeriesSpecJob* j = static_cast<const SeriesSpecJob*>(g.jobs);
+  switch (F) {
+#define X(FF)                                                                                    \
+  case FF:                                                                                       \
+    hipLaunchKernelGGL((k_series_spec_group<FF>), dim3(g.blocks), dim3(FftGeom<FF>::kBlock),      \
+                       sbd_smem<FF>(), st, j, g.first, g.njobs, tw);                            \
+    break;
+    TCX_FOR_EACH_F(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_spectra_group(int F, const GroupTable& g, const float2* tw, hipStream_t st) {
+  if (g.blocks <= 0) return hipSuccess;
+  const SpectraJob* j = static_cast<const SpectraJob*>(g.jobs);
+  switch (F) {
+#define X(FF)                                                                                    \
+  case FF:                                                                                       \
+    hipLaunchKernelGGL((k_spectra_group<FF>), dim3(g.blocks), dim3(FftGeom<FF>::kBlock),          \
+                       sbd_smem<FF>(), st, j, g.first, g.njobs, tw);                            \
+    break;
+    TCX_FOR_EACH_F(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// The forms of launch_bounds: 0 <1,1,9>, 1 <1,1,17>, 2 <1,1,33> (k <= 16),
+// 3 <2,2,11> (k <= 32), 4 <4,1,11>.
+int bounds_form(int F, int k) {
+  const int nsteps = bound_kpad(F) >> 4;
+  if (k <= 16) return nsteps <= 9 ? 0 : (nsteps <= 17 ? 1 : 2);
+  return k <= 32 ? 3 : 4;
+}
+
+long bounds_blocks(int form, long n) {
+  const long mt = form == 3 ? 2 : 1;
+  return (n + 64 * mt - 1) / (64 * mt);
+}
+
+hipError_t launch_bounds_group(int F, int form, const GroupTable& g, hipStream_t st) {
+  if (g.blocks <= 0) return hipSuccess;
+  const BoundsJob* j = static_cast<const BoundsJob*>(g.jobs);
+  const int kpad = bound_kpad(F);
+  const float scale = 1.f / (float)F;
+  switch (form) {
+    case 0:
+      hipLaunchKernelGGL((k_bounds_group<1, 1, 9>), dim3(g.blocks), dim3(256), 0, st, j, g.first,
+                         g.njobs, kpad, scale);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_bounds_group<1, 1, 17>), dim3(g.blocks), dim3(256), 0, st, j, g.first,
+                         g.njobs, kpad, scale);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_bounds_group<1, 1, 33>), dim3(g.blocks), dim3(256), 0, st, j, g.first,
+                         g.njobs, kpad, scale);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_bounds_group<2, 2, 11>), dim3(g.blocks), dim3(256), 0, st, j, g.first,
+                         g.njobs, kpad, scale);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_bounds_group<4, 1, 11>), dim3(g.blocks), dim3(256), 0, st, j, g.first,
+                         g.njobs, kpad, scale);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_assign_prune_group(int F, const GroupTable& g, const float2* tw,
+                                     hipStream_t st) {
+  if (g.blocks <= 0) return hipSuccess;
+  const PruneJob* j = static_cast<const PruneJob*>(g.jobs);
+  switch (F) {
+#define X(FF)                                                                                    \
+  case FF:                                                                                       \
+    if (g.lds > prune_smem<FF>(prune_max_k_F<FF>())) return hipErrorInvalidValue;                \
+    hipLaunchKernelGGL((k_assign_prune_group<FF>), dim3(g.blocks), dim3(PruneGeom<FF>::kBlock),   \
+                       g.lds, st, j, g.first, g.njobs, tw);                                     \
+    break;
+    TCX_FOR_EACH_F(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -676,6 +676,17 @@ int stream_sync(tcx_ctx* c) {
 
 size_t held_bytes(const tcx_ctx* c) { return c->ws_bytes + c->user_bytes; }
 
+void* stream_of(tcx_ctx* c) { return c->stream; }
+int device_of(const tcx_ctx* c) { return c->device; }
+int cus_of(const tcx_ctx* c) { return c->cus; }
+int twiddle_table(tcx_ctx* c, int F, const void** tw) {
+  TCX_TRY(set_dev(c));
+  const float2* t;
+  TCX_TRY(twiddles(c, F, &t));
+  *tw = t;
+  return TCX_OK;
+}
+
 int trim_workspace(tcx_ctx* c) {
   TCX_TRY(set_dev(c));
   if (c->fit.active) return fail(TCX_ERR_ARG, "trim: a fit is in progress");

@@ -24,6 +24,21 @@ __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
   return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 
+// ------------------------------------------------------ grouped launches
+// Job of block b in a grouped launch: the largest j with first[j] <= b
+// (first: prefix of the jobs' block counts, jobs without blocks skipped).
+__device__ __forceinline__ int group_job(const int* __restrict__ first, int njobs, int b) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
 // ------------------------------------------------------- synchronisation
 // LDS exchange inside one wavefront: a wave's LDS operations complete in
 // issue order, so only the compiler must be kept from reordering.

@@ -3,6 +3,8 @@
 #pragma once
 #include <stddef.h>
 
+#include <chrono>
+
 #include "timesc.h"
 
 namespace tcx {
@@ -30,5 +32,21 @@ int trim_workspace(tcx_ctx* c);
 // Upper bound of the workspace one fit of (n, m, k) grows on a device with
 // `cus` compute units.
 size_t fit_workspace_bound(long n, int m, int k, int cus);
+// The context's stream (a hipStream_t), device and compute units, and its
+// twiddle table of FFT length F (a device float2 array, grown on first use).
+void* stream_of(tcx_ctx* c);
+int device_of(const tcx_ctx* c);
+int cus_of(const tcx_ctx* c);
+int twiddle_table(tcx_ctx* c, int F, const void** tw);
+
+// Grouped executor of a batch (host_multi.hip): whether a job can run
+// there (why: the reason when not), and the run of jobs[idx[0..count)] on
+// the context's stream, admitting jobs while their device memory fits
+// hbm_budget (0: most of the free device memory).  Every job gets its
+// status, results and timings (from t0); returns TCX_OK unless the run
+// itself failed.
+bool grouped_supported(const tcx_job& j, const char** why);
+int grouped_fit(tcx_ctx* c, tcx_job* jobs, const int* idx, int count, size_t hbm_budget,
+                std::chrono::steady_clock::time_point t0);
 
 }  // namespace tcx

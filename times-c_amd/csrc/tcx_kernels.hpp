@@ -239,4 +239,257 @@ hipError_t launch_lz_finish(int m, int k, const int* counts, const double* Q,
                             hipStream_t st);
 hipError_t configure_extract_kernels();
 
+// ----------------------------------------------------- grouped launches
+// One launch runs a kernel's body for many independent jobs (the fits of a
+// batch): job j owns the blocks [first[j], first[j+1]) of the grid and runs
+// them as blocks 0 .. first[j+1] - first[j] - 1 of a launch of its own, so a
+// job's results do not depend on the other jobs of the launch.  `jobs` and
+// `first` are device arrays (njobs entries, njobs + 1 prefix values).
+struct GroupTable {
+  const void* jobs = nullptr;  // device array of the kernel's job struct
+  const int* first = nullptr;  // device prefix of blocks per job
+  int njobs = 0;
+  int blocks = 0;              // first[njobs]
+  size_t lds = 0;              // dynamic LDS of the launch (largest job's)
+};
+
+struct FillSeg {  // p[0 .. count) = value
+  int* p;
+  long count;
+  int value;
+};
+hipError_t launch_fill_segs(const FillSeg* segs, int nseg, long max_count, hipStream_t st);
+
+struct ZnormJob {
+  float* x;
+  long rows;
+  int m, ddof;
+};
+// Rows of every job z-normalized (jobs with m % 4 == 0 and 16-byte aligned
+// rows in groups by vector count nv: 0 = the scalar form, else 2, 4, 8, 16).
+hipError_t launch_znorm_group(const GroupTable& g, int nv, hipStream_t st);
+int znorm_vectors(const float* x, int m);  // nv of launch_znorm for these rows
+long znorm_blocks(long rows);
+
+struct SeriesSpecJob {
+  const float* x;
+  long n;
+  int m;
+  float2* xs;
+  float* xn;
+  uint16_t* xm;
+};
+long series_spec_blocks(int F, long n);
+hipError_t launch_series_spec_group(int F, const GroupTable& g, const float2* tw, hipStream_t st);
+
+struct SpectraJob {
+  const float* y;
+  int rows, m;
+  float2* spec;
+  float* norms;
+  int* zero;
+  float* mag;
+  uint16_t* bmag;
+};
+long spectra_blocks(int F, long rows);
+hipError_t launch_spectra_group(int F, const GroupTable& g, const float2* tw, hipStream_t st);
+
+struct BoundsJob {
+  const uint16_t* xm;
+  long n;
+  const uint16_t* cm;
+  int k, ld;
+  float* ub;
+};
+// Bound-product kernel form for (F, k): 0..4; blocks of a job of n rows.
+int bounds_form(int F, int k);
+long bounds_blocks(int form, long n);
+hipError_t launch_bounds_group(int F, int form, const GroupTable& g, hipStream_t st);
+
+struct PruneJob {
+  const float* x;
+  const float2* xs;
+  const float* xn;
+  const float* ub;
+  int ub_ld;
+  long n, npairs;
+  int m, k;
+  const float2* cspec;
+  const float* cmag;
+  const float* cnorm;
+  int rev;
+  const int* order;
+  int* label;
+  float* dist;
+  int* shift;
+  int* changed;
+  unsigned long long* evals;
+};
+long prune_blocks(int F, long npairs);
+size_t prune_lds(int F, int k);
+hipError_t launch_assign_prune_group(int F, const GroupTable& g, const float2* tw, hipStream_t st);
+
+struct GroupCountJob {
+  const int* label;
+  long n;
+  int k;
+  const float* dist;
+  int B;
+  int* blk_count;
+  int* minrow;
+  int lds_min;
+  int* bad;
+};
+size_t group_count_lds(int k, int B, int* lds_min);
+hipError_t launch_group_count_group(const GroupTable& g, hipStream_t st);
+
+struct GroupScanJob {
+  int nrowblk, K;
+  const int* blk_count;
+  int* blk_base;
+  int* counts;
+};
+hipError_t launch_group_scan_group(const GroupTable& g, hipStream_t st);
+
+struct GroupOffsetsJob {
+  int k, B;
+  const int* kcounts;
+  int* counts;
+  long* offsets;
+  long* koff;
+};
+hipError_t launch_group_offsets_group(const GroupTable& g, hipStream_t st);
+
+struct GroupPlaceJob {
+  const int* label;
+  long n;
+  int k;
+  const float* dist;
+  int B;
+  const int* blk_base;
+  const long* koff;
+  int* perm;
+};
+size_t group_place_lds(int k, int B);
+hipError_t launch_group_place_group(const GroupTable& g, hipStream_t st);
+
+struct RowStatsJob {
+  const float* x;
+  long rows;
+  int m;
+  double2* stats;
+};
+hipError_t launch_row_stats_group(const GroupTable& g, hipStream_t st);
+
+struct AlignPackJob {
+  const float* x;
+  int m;
+  const int* perm;
+  long n_pad;
+  const long* offsets;
+  const int* minrow;
+  int k;
+  const int* shift;
+  const int* czero;
+  const double2* stats;
+  uint16_t* yhi;
+  uint16_t* ylo;
+  float* a0;
+};
+// Align + pack form for m: R = 16, 32, 64 (rows staged in LDS) or 0 (from
+// HBM with per-row sums, m > 1024); columns per block; dynamic LDS bytes.
+int align_pack_form(int m);
+long align_pack_cols(int form);
+size_t align_pack_lds(int form, int m);
+hipError_t launch_align_pack_group(int form, const GroupTable& g, hipStream_t st);
+
+// S = Y^T Y of many jobs: one item (tile) per block, the job's panels,
+// length and partials from syrk_jobs[job].
+struct SyrkJob {
+  const uint16_t* yhi;
+  const uint16_t* ylo;
+  int m;
+  float* part;
+  float* S;
+};
+struct GroupSyrkItem {
+  int job;
+  SyrkItem it;
+};
+struct GroupSyrkTile {
+  int job;
+  SyrkTile t;
+};
+hipError_t launch_syrk_group(const SyrkJob* jobs, const GroupSyrkItem* items, int nitems,
+                             int split, hipStream_t st);
+hipError_t launch_syrk_reduce_group(const SyrkJob* jobs, const GroupSyrkTile* tiles,
+                                    int ntiles, hipStream_t st);
+
+struct PowerLanczosJob {
+  const float* S;
+  int m;
+  const int* counts;
+  uint64_t seed;
+  int iter, I;
+  double* Q;
+  double* vout;
+  double* rout;
+  int* steps_out;
+  int* conv_out;
+};
+size_t power_lanczos_lds(int m);
+hipError_t launch_power_lanczos_group(const GroupTable& g, double lz_tol, hipStream_t st);
+
+struct ClusterMatvecJob {
+  const float* S;
+  int m, k;
+  const double* x;
+  long xstride;
+  double* y;
+  const LzState* state;
+};
+long cluster_matvec_blocks(int m, int k);
+hipError_t launch_cluster_matvec_group(const GroupTable& g, hipStream_t st);
+
+struct LzJob {  // k_lz_setup / k_lz_step / k_lz_finish of one job
+  int m;
+  const int* counts;
+  const int* mask;
+  uint64_t seed;
+  int iter, I;
+  const double* rbuf;
+  const double* wbuf;
+  double* Q;
+  LzState* state;
+  double* vout;
+  int* steps_out;
+  int* conv_out;
+};
+size_t lz_step_lds(int m);
+hipError_t launch_lz_setup_group(const GroupTable& g, hipStream_t st);
+hipError_t launch_lz_step_group(const GroupTable& g, int mode, double lz_tol, hipStream_t st);
+hipError_t launch_lz_finish_group(const GroupTable& g, hipStream_t st);
+
+struct PowerJob {
+  const float* S;
+  int m;
+  const int* counts;
+  const float* a0;
+  uint64_t seed;
+  int iter, max_steps;
+  double tol;
+  const double* vin;
+  const double* svin;
+  const double* rsum;
+  const int* steps_in;
+  const int* conv_in;
+  int eig_mode;
+  const int* mask;
+  int defer_fallback;
+  float* c_out;
+  PowerInfo* info;
+};
+hipError_t launch_power_group(const GroupTable& g, hipStream_t st);
+size_t power_lds(int m);
+
 }  // namespace tcx

@@ -33,7 +33,8 @@ EXPORTS = [
     "tcx_extract_shape",
     "tcx_fit_opts_default", "tcx_kshape_fit", "tcx_fit_begin", "tcx_fit_step", "tcx_fit_end",
     "tcx_fit_query", "tcx_fit_snapshot",
-    "tcx_kshape", "tcx_kshape_tsl", "tcx_batch_create", "tcx_batch_run", "tcx_batch_destroy",
+    "tcx_kshape", "tcx_kshape_tsl", "tcx_batch_create", "tcx_batch_run",
+    "tcx_batch_run_grouped", "tcx_batch_destroy",
     "tcx_server_create", "tcx_server_add_model", "tcx_server_submit", "tcx_server_wait",
     "tcx_server_destroy",
     "tcx_read_labeled_csv", "tcx_read_unlabeled_csv", "tcx_write_labels",
@@ -205,6 +206,7 @@ def load():
                                 ctypes.POINTER(_I), _I]),
         "tcx_batch_create": (_I, [_I, _I, _Z, ctypes.POINTER(_P)]),
         "tcx_batch_run": (_I, [_P, ctypes.POINTER(Job), _I, _I]),
+        "tcx_batch_run_grouped": (_I, [_P, ctypes.POINTER(Job), _I]),
         "tcx_batch_destroy": (_I, [_P]),
         "tcx_server_create": (_I, [_I, _I, _I, _Z, ctypes.POINTER(_P)]),
         "tcx_server_add_model": (_I, [_P, _P, _I, _I, ctypes.POINTER(_I)]),
@@ -568,12 +570,15 @@ class Batch:
     def __exit__(self, *a):
         self.close()
 
-    def run(self, specs, order=ORDER_GIVEN, raise_on_error=True):
+    def run(self, specs, order=ORDER_GIVEN, raise_on_error=True, grouped=False):
         """specs: dicts with d (DeviceArray [n, m]), k and optional
         znorm_ddof (default -1 = already normalized), max_iter, init_mode,
-        seed, init_labels, init_rows, convention.  Returns one dict per job
-        (labels, centroids, stats, status, message, worker, ms_queued,
-        ms_run, ms_latency)."""
+        seed, init_labels, init_rows, convention, cal_select, s_precision,
+        stop_on_collapse, labels_per_iter.  grouped: tcx_batch_run_grouped
+        (one launch per phase over all jobs) instead of the worker pool.
+        Returns one dict per job (labels, centroids, stats, status, message,
+        worker, ms_queued, ms_run, ms_latency, and labels_per_iter when
+        asked)."""
         jobs = (Job * len(specs))()
         keep = []
         outs = []
@@ -588,6 +593,14 @@ class Batch:
             j.opts.init_mode = sp.get("init_mode", 0)
             j.opts.seed = sp.get("seed", 0)
             j.opts.convention = sp.get("convention", CONV_CPU)
+            j.opts.cal_select = sp.get("cal_select", 0)
+            j.opts.s_precision = sp.get("s_precision", S_SPLIT_BF16)
+            j.opts.stop_on_collapse = int(sp.get("stop_on_collapse", False))
+            hist = None
+            if sp.get("labels_per_iter"):
+                hist = np.zeros((j.opts.max_iter, n), np.int32)
+                keep.append(hist)
+                j.opts.labels_per_iter = hist.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
             for key, field in (("init_labels", "init_labels"), ("init_rows", "init_rows")):
                 if sp.get(key) is not None:
                     a = np.ascontiguousarray(sp[key], np.int32)
@@ -597,16 +610,21 @@ class Batch:
             cent = np.empty((k, m), np.float32)
             keep += [lab, cent]
             j.h_labels, j.h_centroids = _ptr(lab), _ptr(cent)
-            outs.append((lab, cent))
-        rc = self.lib.tcx_batch_run(self.h, jobs, len(specs), order)
+            outs.append((lab, cent, hist))
+        if grouped:
+            rc = self.lib.tcx_batch_run_grouped(self.h, jobs, len(specs))
+        else:
+            rc = self.lib.tcx_batch_run(self.h, jobs, len(specs), order)
         if rc != OK and raise_on_error:
             check(rc)
         res = []
-        for j, (lab, cent) in zip(jobs, outs):
-            res.append({"labels": lab, "centroids": cent, "stats": j.stats.as_dict(),
-                        "status": j.status, "message": j.message.decode(), "worker": j.worker,
-                        "ms_queued": j.ms_queued, "ms_run": j.ms_run,
-                        "ms_latency": j.ms_latency})
+        for j, (lab, cent, hist) in zip(jobs, outs):
+            r = {"labels": lab, "centroids": cent, "stats": j.stats.as_dict(),
+                 "status": j.status, "message": j.message.decode(), "worker": j.worker,
+                 "ms_queued": j.ms_queued, "ms_run": j.ms_run, "ms_latency": j.ms_latency}
+            if hist is not None:
+                r["labels_per_iter"] = hist[: j.stats.iterations]
+            res.append(r)
         return res
 
 
