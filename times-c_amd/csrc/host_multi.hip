@@ -115,7 +115,10 @@ struct GJob {
   double *Q = nullptr, *v = nullptr, *sv = nullptr, *r = nullptr, *w = nullptr;
   int *steps = nullptr, *conv = nullptr, *mask = nullptr;
   LzState* lzs = nullptr;
+  int* scount = nullptr;  // counts of the S-form clusters, 0 elsewhere
   std::vector<int> mask_h;
+  std::vector<char> fat, thin, mask_sel;  // S form, Gram form, eigenvector rerun
+  int nfat = 0;
 };
 
 // Carves a job's buffers out of its arena (base NULL: sizes only); returns
@@ -165,6 +168,7 @@ size_t layout(GJob& j, unsigned char* base) {
   take(j.conv, k);
   take(j.mask, k);
   take(j.lzs, k);
+  take(j.scount, k);
   return off;
 }
 
@@ -288,6 +292,7 @@ struct Batch {
   unsigned char *ginfo_h = nullptr, *info_h = nullptr, *evc_h = nullptr;
   size_t ginfo_bytes = 0, info_bytes = 0, evc_bytes = 0;
   int* done_h = nullptr;  // Lanczos done flags of the two-launch form
+  int* done_d = nullptr;
   size_t done_cap = 0;
   std::vector<GroupSyrkItem> items;
   std::vector<GroupSyrkTile> tiles;
@@ -300,6 +305,7 @@ struct Batch {
     for (unsigned char* p : {ginfo_h, info_h, evc_h})
       if (p) (void)hipHostFree(p);
     if (done_h) (void)hipHostFree(done_h);
+    if (done_d) (void)hipFree(done_d);
   }
 
   long* offsets(GJob& j) { return reinterpret_cast<long*>(ginfo_d + j.ginfo_off); }
@@ -430,44 +436,64 @@ int refine_group_pack(Batch& b, const std::vector<GJob*>& act) {
 }
 
 // The two-launch Lanczos (grid-wide matvec + per-cluster step) of the jobs
-// `set` in one grouped launch per step; mode 0: M^I v0, mode 1: top
-// eigenvector of the masked clusters.  A job takes steps while
-// s < min(m, kLanczosMax) and some cluster of it is not done, as the
+// `set` in one grouped launch per step; mode 0: M^I v0 of the clusters
+// with sel[c] (their count in lzc), mode 1: top eigenvector of the masked
+// clusters (sel = the mask).  A job takes steps while s < min(m,
+// kLanczosMax) and some selected cluster of it is not done, as the
 // single-fit loop does.  Ends with v, S v, r, the step counts and
-// acceptance flags of every job.
+// acceptance flags of the selected clusters.
 int power_split(Batch& b, const std::vector<GJob*>& set, int mode) {
   if (set.empty()) return TCX_OK;
   int jmax_all = 0;
-  for (GJob* j : set) jmax_all = std::max(jmax_all, std::min(j->m, kLanczosMax));
+  long ndone = 0;
+  for (GJob* j : set) {
+    jmax_all = std::max(jmax_all, std::min(j->m, kLanczosMax));
+    ndone += j->k;
+  }
+  if ((size_t)ndone > b.done_cap) {
+    if (b.done_h) GX_HIP(hipHostFree(b.done_h));
+    if (b.done_d) GX_HIP(hipFree(b.done_d));
+    b.done_h = nullptr;
+    b.done_d = nullptr;
+    b.done_cap = 0;
+    GX_HIP(hipHostMalloc((void**)&b.done_h, sizeof(int) * ndone, hipHostMallocDefault));
+    if (hipMalloc((void**)&b.done_d, sizeof(int) * ndone) != hipSuccess)
+      return fail(TCX_ERR_NOMEM, "grouped batch: done flags allocation failed");
+    b.done_cap = (size_t)ndone;
+  }
   Launch<ClusterMatvecJob> mv_r, mv_v;
   std::vector<Launch<ClusterMatvecJob>> mv_w(jmax_all);
   std::vector<Launch<LzJob>> lz_s(jmax_all);
   Launch<LzJob> lz;
-  long ndone = 0;
+  long at = 0;
   for (GJob* j : set) {
     const long blocks = cluster_matvec_blocks(j->m, j->k);
+    const long blocks1 = cluster_matvec_blocks(j->m, 1);
     const size_t lds = sizeof(double) * j->m;
     const long qs = (long)(kLanczosMax + 1) * j->m;
+    const long mm = (long)j->m * j->m;
     const int jmax = std::min(j->m, kLanczosMax);
     const int steps = j->m > 1000 ? 1000 : j->m;
-    const LzJob lj{j->m, b.counts(*j), mode ? j->mask : nullptr, j->o.seed, j->it, steps, j->r,
-                   j->w, j->Q, j->lzs, j->v, j->steps, j->conv};
-    mv_r.add(ClusterMatvecJob{j->S, j->m, j->k, nullptr, 0, j->r, nullptr}, blocks, lds);
+    const std::vector<char>& sel = mode ? j->mask_sel : j->fat;
+    const LzJob lj{j->m, mode ? b.counts(*j) : j->scount, mode ? j->mask : nullptr, j->o.seed,
+                   j->it, steps, j->r, j->w, j->Q, j->lzs, j->v, j->steps, j->conv,
+                   b.done_d + at};
+    for (int c = 0; c < j->k; ++c) {
+      if (!sel[c]) continue;
+      mv_r.add(ClusterMatvecJob{j->S + c * mm, j->m, 1, nullptr, 0, j->r + (long)c * j->m,
+                                nullptr},
+               blocks1, lds);
+      mv_v.add(ClusterMatvecJob{j->S + c * mm, j->m, 1, j->v + (long)c * j->m, j->m,
+                                j->sv + (long)c * j->m, nullptr},
+               blocks1, lds);
+    }
     for (int s = 0; s < jmax; ++s) {
       mv_w[s].add(ClusterMatvecJob{j->S, j->m, j->k, j->Q + (long)s * j->m, qs, j->w, j->lzs},
                   blocks, lds);
       lz_s[s].add(lj, j->k, lz_step_lds(j->m));
     }
-    mv_v.add(ClusterMatvecJob{j->S, j->m, j->k, j->v, j->m, j->sv, nullptr}, blocks, lds);
     lz.add(lj, j->k);
-    ndone += j->k;
-  }
-  if ((size_t)ndone > b.done_cap) {
-    if (b.done_h) GX_HIP(hipHostFree(b.done_h));
-    b.done_h = nullptr;
-    b.done_cap = 0;
-    GX_HIP(hipHostMalloc((void**)&b.done_h, sizeof(int) * ndone, hipHostMallocDefault));
-    b.done_cap = (size_t)ndone;
+    at += j->k;
   }
   b.up.reset();
   mv_r.stage(b.up);
@@ -486,13 +512,9 @@ int power_split(Batch& b, const std::vector<GJob*>& set, int mode) {
     bool check = mode == 0 ? lz_power_check(len) : (len >= 8 && (len & 7) == 0);
     for (GJob* j : set) check |= len == std::min(j->m, kLanczosMax);
     if (!check) continue;
-    // stop once every job has reached its step limit or has every cluster done
-    long at = 0;
-    for (GJob* j : set) {
-      GX_HIP(hipMemcpy2DAsync(b.done_h + at, sizeof(int), &j->lzs[0].done, sizeof(LzState),
-                              sizeof(int), j->k, hipMemcpyDeviceToHost, b.st));
-      at += j->k;
-    }
+    // stop once every job has reached its step limit or has every cluster
+    // done (one readback of all the done flags)
+    GX_HIP(hipMemcpyAsync(b.done_h, b.done_d, sizeof(int) * ndone, hipMemcpyDeviceToHost, b.st));
     GX_HIP(hipStreamSynchronize(b.st));
     bool all = true;
     at = 0;
@@ -512,14 +534,66 @@ int power_split(Batch& b, const std::vector<GJob*>& set, int mode) {
   return TCX_OK;
 }
 
-// S = Y^T Y of every job (one K chunk per tile), the power stage and the
-// centroids; then the power stage's status read back.  Jobs whose power
-// result fails the sign / residual tests rerun those clusters through the
-// eigenvector path, as the single-fit refine does.
+// A cluster takes the Gram form when its padded segment nd satisfies
+// 2 nd <= m: every Lanczos step then moves at most a quarter of S's values.
+bool gram_form(int m, long nd, int count) {
+  return count > 0 && m <= kGramMaxM && nd > 0 && 2 * nd <= m;
+}
+
+// S = Y^T Y items (one K chunk per tile) of the clusters with sel[c].
+void add_syrk_items(Batch& b, int q, const GJob& j, const long* off, const std::vector<char>& sel) {
+  const int T = (j.m + kSyrkTile - 1) / kSyrkTile;
+  int part = 0;
+  for (int c = 0; c < j.k; ++c) {
+    if (!sel[c]) continue;
+    for (int ti = 0; ti < T; ++ti)
+      for (int tj = ti; tj < T; ++tj) {
+        b.items.push_back(GroupSyrkItem{q, SyrkItem{c, ti, tj, part, off[c], off[c + 1]}});
+        b.tiles.push_back(GroupSyrkTile{q, SyrkTile{c, ti, tj, part, 1}});
+        ++part;
+      }
+  }
+}
+
+// S of the selected clusters: syrk by precision, then the reduction.
+int launch_syrk_items(Batch& b, const std::vector<GJob*>& jobs, const std::vector<SyrkJob>& sjobs,
+                      size_t* sj_off, size_t by_off[2], size_t* tl_off,
+                      std::vector<GroupSyrkItem> by[2]) {
+  by[0].clear();
+  by[1].clear();
+  for (const GroupSyrkItem& gi : b.items) by[jobs[gi.job]->split].push_back(gi);
+  *sj_off = b.up.put_vec(sjobs);
+  by_off[0] = b.up.put_vec(by[0]);
+  by_off[1] = b.up.put_vec(by[1]);
+  *tl_off = b.up.put_vec(b.tiles);
+  return TCX_OK;
+}
+
+int run_syrk(Batch& b, size_t sj_off, const size_t by_off[2], size_t tl_off,
+             const std::vector<GroupSyrkItem> by[2]) {
+  const SyrkJob* dsj = reinterpret_cast<const SyrkJob*>(b.up.dev(sj_off));
+  for (int sp = 0; sp < 2; ++sp)
+    GX_HIP(launch_syrk_group(dsj, reinterpret_cast<const GroupSyrkItem*>(b.up.dev(by_off[sp])),
+                             (int)by[sp].size(), sp, b.st));
+  GX_HIP(launch_syrk_reduce_group(dsj, reinterpret_cast<const GroupSyrkTile*>(b.up.dev(tl_off)),
+                                  (int)b.tiles.size(), b.st));
+  return TCX_OK;
+}
+
+// The power stage of every job: S (fat clusters) or the Gram matrix (thin
+// clusters), the Lanczos evaluation of M^I v0, the centroids; then the
+// power stage's status read back.  Clusters whose power result fails the
+// sign / residual tests rerun through the eigenvector path on S (computed
+// then for thin ones), as the single-fit refine does.
 int refine_power(Batch& b, const std::vector<GJob*>& act, double* ms_power) {
   b.items.clear();
   b.tiles.clear();
   std::vector<SyrkJob> sjobs;
+  std::vector<GramItem> gitems;
+  std::vector<GramVecJob> gstart, gmap;
+  std::vector<GramLzJob> glz;
+  size_t lds_start = 0, lds_map = 0, lds_lz = 0;
+  Launch<FillSeg> fills;
   for (size_t q = 0; q < act.size(); ++q) {
     GJob* j = act[q];
     const long* off = reinterpret_cast<const long*>(b.ginfo_h + j->ginfo_off);
@@ -528,19 +602,38 @@ int refine_power(Batch& b, const std::vector<GJob*>& act, double* ms_power) {
     if (bad) return fail(TCX_ERR_ARG, "grouped batch: label outside [0, k)");
     j->n_pad = off[j->k];
     j->nonempty = 0;
-    for (int c = 0; c < j->k; ++c) j->nonempty += cnt[c] > 0;
-    sjobs.push_back(SyrkJob{j->yhi, j->ylo, j->m, j->part, j->S});
-    const int T = (j->m + kSyrkTile - 1) / kSyrkTile;
-    int part = 0;
+    j->fat.assign(j->k, 0);
+    j->thin.assign(j->k, 0);
+    j->nfat = 0;
+    const long mm = (long)j->m * j->m;
+    const int steps = j->m > 1000 ? 1000 : j->m;
     for (int c = 0; c < j->k; ++c) {
-      if (cnt[c] == 0) continue;
+      j->nonempty += cnt[c] > 0;
+      const long nd = off[c + 1] - off[c];
+      j->thin[c] = gram_form(j->m, nd, cnt[c]);
+      j->fat[c] = cnt[c] > 0 && !j->thin[c];
+      j->nfat += j->fat[c];
+      fills.add(FillSeg{j->scount + c, 1, j->fat[c] ? cnt[c] : 0}, 1);
+      if (!j->thin[c]) continue;
+      const int T = (int)(nd / 64);
       for (int ti = 0; ti < T; ++ti)
-        for (int tj = ti; tj < T; ++tj) {
-          b.items.push_back(GroupSyrkItem{(int)q, SyrkItem{c, ti, tj, part, off[c], off[c + 1]}});
-          b.tiles.push_back(GroupSyrkTile{(int)q, SyrkTile{c, ti, tj, part, 1}});
-          ++part;
-        }
+        for (int tj = ti; tj < T; ++tj)
+          gitems.push_back(GramItem{j->yhi, j->ylo, j->S + c * mm, off[c], j->m, (int)nd, ti, tj});
+      double* rc = j->r + (long)c * j->m;
+      double* wc = j->w + (long)c * j->m;
+      gstart.push_back(GramVecJob{j->yhi, j->ylo, off[c], j->m, (int)nd, j->o.seed, j->it, c, rc,
+                                  nullptr, nullptr, nullptr, nullptr});
+      glz.push_back(GramLzJob{j->S + c * mm, (int)nd, steps - 1, rc,
+                              j->Q + (long)c * (kLanczosMax + 1) * j->m, wc, wc + nd,
+                              j->steps + c, j->conv + c});
+      gmap.push_back(GramVecJob{j->yhi, j->ylo, off[c], j->m, (int)nd, j->o.seed, j->it, c,
+                                nullptr, wc, j->v + (long)c * j->m, j->sv + (long)c * j->m, rc});
+      lds_start = std::max(lds_start, gram_start_lds(j->m));
+      lds_map = std::max(lds_map, gram_map_lds(j->m, (int)nd));
+      lds_lz = std::max(lds_lz, gram_lanczos_lds((int)nd));
     }
+    sjobs.push_back(SyrkJob{j->yhi, j->ylo, j->m, j->part, j->S});
+    add_syrk_items(b, (int)q, *j, off, j->fat);
   }
   const auto t0 = Clock::now();
   Launch<PowerLanczosJob> pl;
@@ -549,44 +642,55 @@ int refine_power(Batch& b, const std::vector<GJob*>& act, double* ms_power) {
   std::vector<GJob*> split_set;
   for (GJob* j : act) {
     const int steps = j->m > 1000 ? 1000 : j->m;
-    if (j->m <= kFusedLanczosMaxM) {
-      pl.add(PowerLanczosJob{j->S, j->m, b.counts(*j), j->o.seed, j->it, steps, j->Q, j->v, j->r,
-                             j->steps, j->conv},
-             j->k, power_lanczos_lds(j->m));
-      sv.add(ClusterMatvecJob{j->S, j->m, j->k, j->v, j->m, j->sv, nullptr},
-             cluster_matvec_blocks(j->m, j->k), sizeof(double) * j->m);
-    } else {
-      split_set.push_back(j);
+    const long mm = (long)j->m * j->m;
+    if (j->nfat > 0) {
+      if (j->m <= kFusedLanczosMaxM) {
+        pl.add(PowerLanczosJob{j->S, j->m, j->scount, j->o.seed, j->it, steps, j->Q, j->v, j->r,
+                               j->steps, j->conv},
+               j->k, power_lanczos_lds(j->m));
+        for (int c = 0; c < j->k; ++c)
+          if (j->fat[c])
+            sv.add(ClusterMatvecJob{j->S + c * mm, j->m, 1, j->v + (long)c * j->m, j->m,
+                                    j->sv + (long)c * j->m, nullptr},
+                   cluster_matvec_blocks(j->m, 1), sizeof(double) * j->m);
+      } else {
+        split_set.push_back(j);
+      }
     }
     pw.add(PowerJob{j->S, j->m, b.counts(*j), j->a0, j->o.seed, j->it, steps, j->tol, j->v, j->sv,
                     j->r, j->steps, j->conv, 0, nullptr, 1, j->cent_new, b.info(*j)},
            j->k, power_lds(j->m));
   }
-  // items by precision (a job's precision is fixed for the run)
-  std::vector<GroupSyrkItem> by[2];
-  for (const GroupSyrkItem& gi : b.items) by[act[gi.job]->split].push_back(gi);
   b.up.reset();
-  const size_t sj_off = b.up.put_vec(sjobs);
-  const size_t by_off[2] = {b.up.put_vec(by[0]), b.up.put_vec(by[1])};
-  const size_t tl_off = b.up.put_vec(b.tiles);
+  size_t sj_off, by_off[2], tl_off;
+  std::vector<GroupSyrkItem> by[2];
+  GX_TRY(launch_syrk_items(b, act, sjobs, &sj_off, by_off, &tl_off, by));
+  const size_t fill_off = b.up.put_vec(fills.jobs);
+  const size_t gi_off = b.up.put_vec(gitems), gs_off = b.up.put_vec(gstart);
   pl.stage(b.up);
   sv.stage(b.up);
-  pw.stage(b.up);
   GX_TRY(b.up.flush(b.st));
-  const SyrkJob* dsj = reinterpret_cast<const SyrkJob*>(b.up.dev(sj_off));
-  for (int sp = 0; sp < 2; ++sp)
-    GX_HIP(launch_syrk_group(dsj, reinterpret_cast<const GroupSyrkItem*>(b.up.dev(by_off[sp])),
-                             (int)by[sp].size(), sp, b.st));
-  GX_HIP(launch_syrk_reduce_group(dsj, reinterpret_cast<const GroupSyrkTile*>(b.up.dev(tl_off)),
-                                  (int)b.tiles.size(), b.st));
+  if (!fills.jobs.empty())
+    GX_HIP(launch_fill_segs(reinterpret_cast<const FillSeg*>(b.up.dev(fill_off)),
+                            (int)fills.jobs.size(), 1, b.st));
+  GX_TRY(run_syrk(b, sj_off, by_off, tl_off, by));
+  GX_HIP(launch_gram_tiles(reinterpret_cast<const GramItem*>(b.up.dev(gi_off)), (int)gitems.size(),
+                           b.st));
+  GX_HIP(launch_gram_start(reinterpret_cast<const GramVecJob*>(b.up.dev(gs_off)),
+                           (int)gstart.size(), lds_start, b.st));
   GX_HIP(launch_power_lanczos_group(pl.table(b.up), kLzPowerTol, b.st));
   GX_HIP(launch_cluster_matvec_group(sv.table(b.up), b.st));  // S v
-  if (!split_set.empty()) {
-    GX_TRY(power_split(b, split_set, 0));  // uploads its own tables
-    b.up.reset();
-    pw.stage(b.up);
-    GX_TRY(b.up.flush(b.st));
-  }
+  // the two-launch form first: its finishing pass writes v, steps and conv
+  // of every cluster of its jobs, the Gram form's included
+  GX_TRY(power_split(b, split_set, 0));  // uploads its own tables
+  b.up.reset();
+  const size_t gl_off = b.up.put_vec(glz), gm_off = b.up.put_vec(gmap);
+  pw.stage(b.up);
+  GX_TRY(b.up.flush(b.st));
+  GX_HIP(launch_gram_lanczos(reinterpret_cast<const GramLzJob*>(b.up.dev(gl_off)), (int)glz.size(),
+                             lds_lz, kLzPowerTol, b.st));
+  GX_HIP(launch_gram_map(reinterpret_cast<const GramVecJob*>(b.up.dev(gm_off)), (int)gmap.size(),
+                         lds_map, b.st));
   GX_HIP(launch_power_group(pw.table(b.up), b.st));
   GX_HIP(hipMemcpyAsync(b.info_h, b.info_d, b.info_bytes, hipMemcpyDeviceToHost, b.st));
   GX_HIP(hipStreamSynchronize(b.st));
@@ -596,10 +700,12 @@ int refine_power(Batch& b, const std::vector<GJob*>& act, double* ms_power) {
     const PowerInfo* h = reinterpret_cast<const PowerInfo*>(b.info_h + j->info_off);
     bool any = false;
     j->mask_h.assign(j->k, 0);
+    j->mask_sel.assign(j->k, 0);
     for (int c = 0; c < j->k; ++c) {
       j->fallbacks += h[c].fallback;
       if (h[c].fallback && !h[c].empty && !h[c].zero) {
         j->mask_h[c] = 1;
+        j->mask_sel[c] = 1;
         any = true;
       }
     }
@@ -609,6 +715,22 @@ int refine_power(Batch& b, const std::vector<GJob*>& act, double* ms_power) {
       for (int c = 0; c < j->k; ++c) j->steps_max = std::max(j->steps_max, h[c].steps);
   }
   if (!redo.empty()) {
+    // S of the masked thin clusters (their S was never formed)
+    b.items.clear();
+    b.tiles.clear();
+    std::vector<SyrkJob> rjobs;
+    for (size_t q = 0; q < redo.size(); ++q) {
+      GJob* j = redo[q];
+      const long* off = reinterpret_cast<const long*>(b.ginfo_h + j->ginfo_off);
+      std::vector<char> need(j->k, 0);
+      for (int c = 0; c < j->k; ++c) need[c] = j->mask_sel[c] && j->thin[c];
+      rjobs.push_back(SyrkJob{j->yhi, j->ylo, j->m, j->part, j->S});
+      add_syrk_items(b, (int)q, *j, off, need);
+    }
+    b.up.reset();
+    GX_TRY(launch_syrk_items(b, redo, rjobs, &sj_off, by_off, &tl_off, by));
+    GX_TRY(b.up.flush(b.st));
+    GX_TRY(run_syrk(b, sj_off, by_off, tl_off, by));
     for (GJob* j : redo)
       GX_HIP(hipMemcpyAsync(j->mask, j->mask_h.data(), sizeof(int) * j->k, hipMemcpyHostToDevice,
                             b.st));

@@ -1519,12 +1519,22 @@ __device__ double tridiag_top_eig_wave(const double* alpha, const double* beta, 
   return theta;
 }
 
-// Fused form for small m (S_j resident in L2): one block per cluster runs
-// every step, with the symmetric matvec done by the block itself.
-__device__ __forceinline__ void power_lanczos_body(const int bid, const int nblk,
-    const float* __restrict__ S, int m, const int* __restrict__ counts, uint64_t seed, int iter,
-    int I, double lz_tol, double* __restrict__ Qg, double* __restrict__ vout,
-    double* __restrict__ rout, int* __restrict__ steps_out, int* __restrict__ conv_out) {
+// Fused form for small m (S_j resident in L2): one block runs every step of
+// one cluster, with the symmetric matvec done by the block itself.
+//
+// Two uses.  vstart NULL: Sj is the cluster's S (FP32, m x m), the operator
+// is M = P S P (rank-1 centring from r = S 1, written to rj), the start is
+// power_v0(seed, iter, j, .), and vj = M^I v0 / |.| (zero when the check
+// never passed: the caller's power method takes over).
+// vstart given: Sj is an already centred symmetric matrix (the Gram form of
+// a thin cluster), no centring, the start is vstart; vj = Sj^I vstart / |.|
+// and hj = Sj vj.  A zero start gives vj = hj = 0; when the check never
+// passes, plain power steps on Sj from vstart (at most I of them, stopping
+// when the Rayleigh quotient moves by <= 1e-12 relative) give vj.
+__device__ __forceinline__ void power_lanczos_cluster(const float* __restrict__ Sj, int m,
+    const double* __restrict__ vstart, uint64_t seed, int iter, int j, int I, double lz_tol,
+    double* __restrict__ Q, double* __restrict__ vj, double* __restrict__ rj,
+    double* __restrict__ hj, int* __restrict__ steps_j, int* __restrict__ conv_j) {
   extern __shared__ double lz_sh[];
   double* q = lz_sh;                  // m
   double* w = q + m;                  // m
@@ -1539,24 +1549,42 @@ __device__ __forceinline__ void power_lanczos_body(const int bid, const int nblk
   int* flag = reinterpret_cast<int*>(yp + kLzMax);  // 2 ints (padded to 1 double)
   double* part = yp + kLzMax + 1;     // sym_part_len(m)
   double* sq = part + sym_part_len(m);  // kLzSqDoubles: acceptance test
-  const int j = bid;
-  if (counts[j] == 0) return;
-  const float* Sj = S + (long)j * m * m;
-  double* Q = Qg + (long)j * (kLzMax + 1) * m;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int i = threadIdx.x; i < m; i += kPowerThreads) w[i] = 1.0;
-  __syncthreads();
-  sym_matvec_block(Sj, m, w, r, part);  // r = S 1
-  double sg = 0.0, n0 = 0.0;
-  for (int i = threadIdx.x; i < m; i += kPowerThreads) {
-    sg += r[i];
-    q[i] = power_v0(seed, iter, j, i);
-    qp[i] = 0.0;
-    n0 += q[i] * q[i];
-    rout[(long)j * m + i] = r[i];
+  double sigma = 0.0, n0 = 0.0;
+  if (vstart) {
+    for (int i = threadIdx.x; i < m; i += kPowerThreads) {
+      r[i] = 0.0;
+      q[i] = vstart[i];
+      qp[i] = 0.0;
+      n0 += q[i] * q[i];
+    }
+    n0 = sqrt(block_sum_f64(n0, scratch + kPowerWaves));
+    if (!(n0 > 0.0)) {  // every thread sees the same sum
+      for (int i = threadIdx.x; i < m; i += kPowerThreads) {
+        vj[i] = 0.0;
+        hj[i] = 0.0;
+      }
+      if (threadIdx.x == 0) {
+        *steps_j = 0;
+        *conv_j = 1;
+      }
+      return;
+    }
+  } else {
+    for (int i = threadIdx.x; i < m; i += kPowerThreads) w[i] = 1.0;
+    __syncthreads();
+    sym_matvec_block(Sj, m, w, r, part);  // r = S 1
+    double sg = 0.0;
+    for (int i = threadIdx.x; i < m; i += kPowerThreads) {
+      sg += r[i];
+      q[i] = power_v0(seed, iter, j, i);
+      qp[i] = 0.0;
+      n0 += q[i] * q[i];
+      rj[i] = r[i];
+    }
+    sigma = block_sum_f64(sg, scratch);
+    n0 = sqrt(block_sum_f64(n0, scratch + kPowerWaves));
   }
-  const double sigma = block_sum_f64(sg, scratch);
-  n0 = sqrt(block_sum_f64(n0, scratch + kPowerWaves));
   for (int i = threadIdx.x; i < m; i += kPowerThreads) {
     q[i] /= n0;
     Q[i] = q[i];
@@ -1663,16 +1691,60 @@ __device__ __forceinline__ void power_lanczos_body(const int bid, const int nblk
     }
     __syncthreads();
   }
-  // v = Q y
+  if (!vstart) {
+    // v = Q y
+    for (int e = threadIdx.x; e < m; e += kPowerThreads) {
+      double acc = 0.0;
+      for (int l = 0; l < len; ++l) acc += y[l] * Q[(long)l * m + e];
+      vj[e] = conv ? acc : 0.0;
+    }
+    if (threadIdx.x == 0) {
+      *steps_j = len;
+      *conv_j = conv;
+    }
+    return;
+  }
+  // Gram form: v (into q) = Q y, or the power steps when the check never
+  // passed; then h = Sj v
+  int steps = len;
+  if (conv) {
+    for (int e = threadIdx.x; e < m; e += kPowerThreads) {
+      double acc = 0.0;
+      for (int l = 0; l < len; ++l) acc += y[l] * Q[(long)l * m + e];
+      q[e] = acc;
+    }
+  } else {
+    for (int e = threadIdx.x; e < m; e += kPowerThreads) {
+      q[e] = vstart[e] / n0;
+      r[e] = 0.0;
+    }
+    __syncthreads();
+    int more = 0, conv2 = 0;
+    if (power_steps(Sj, m, q, r, 0.0, w, part, scratch, I, 1e-12, more, conv2))
+      for (int e = threadIdx.x; e < m; e += kPowerThreads) q[e] = 0.0;  // S v = 0
+    steps += more;
+  }
+  __syncthreads();
+  sym_matvec_block(Sj, m, q, w, part);  // h = Sj v (ends with a barrier)
   for (int e = threadIdx.x; e < m; e += kPowerThreads) {
-    double acc = 0.0;
-    for (int l = 0; l < len; ++l) acc += y[l] * Q[(long)l * m + e];
-    vout[(long)j * m + e] = conv ? acc : 0.0;
+    vj[e] = q[e];
+    hj[e] = w[e];
   }
   if (threadIdx.x == 0) {
-    steps_out[j] = len;
-    conv_out[j] = conv;
+    *steps_j = steps;
+    *conv_j = 1;
   }
+}
+
+__device__ __forceinline__ void power_lanczos_body(const int bid, const int nblk,
+    const float* __restrict__ S, int m, const int* __restrict__ counts, uint64_t seed, int iter,
+    int I, double lz_tol, double* __restrict__ Qg, double* __restrict__ vout,
+    double* __restrict__ rout, int* __restrict__ steps_out, int* __restrict__ conv_out) {
+  const int j = bid;
+  if (counts[j] == 0) return;
+  power_lanczos_cluster(S + (long)j * m * m, m, nullptr, seed, iter, j, I, lz_tol,
+                        Qg + (long)j * (kLzMax + 1) * m, vout + (long)j * m, rout + (long)j * m,
+                        nullptr, steps_out + j, conv_out + j);
 }
 
 __global__ __launch_bounds__(kPowerThreads) void k_power_lanczos(
@@ -2572,6 +2644,7 @@ __global__ __launch_bounds__(kPowerThreads) void k_lz_setup_group(const LzJob* _
                                                                   int njobs) {
   TCX_GROUP_JOB(LzJob)
   lz_setup_body(bid, nblk, a.m, a.counts, a.mask, a.seed, a.iter, a.rbuf, a.Q, a.state);
+  if (threadIdx.x == 0 && a.done_out) a.done_out[bid] = a.state[bid].done;  // thread 0 wrote it
 }
 
 __global__ __launch_bounds__(kPowerThreads) void k_lz_step_group(const LzJob* __restrict__ jobs,
@@ -2580,6 +2653,7 @@ __global__ __launch_bounds__(kPowerThreads) void k_lz_step_group(const LzJob* __
                                                                  double lz_tol) {
   TCX_GROUP_JOB(LzJob)
   lz_step_body(bid, nblk, a.m, mode, a.I, lz_tol, a.rbuf, a.wbuf, a.Q, a.state);
+  if (threadIdx.x == 0 && a.done_out) a.done_out[bid] = a.state[bid].done;  // thread 0 wrote it
 }
 
 __global__ __launch_bounds__(kPowerThreads) void k_lz_finish_group(const LzJob* __restrict__ jobs,
@@ -2841,6 +2915,231 @@ size_t group_count_lds(int k, int B, int* lds_min) {
   return *lds_min ? lds : sizeof(int) * (size_t)k * B;
 }
 
+
+// ------------------------------------------------ Gram form of thin clusters
+// For a cluster of c members with 2 c <= m (c counted to its padded
+// segment), M = P Y^T Y P = Yc^T Yc (Yc = Y P, rows centred) is evaluated
+// through the c x c Gram matrix G = Yc Yc^T = Y Y^T - s s^T / m (s = Y 1):
+// M^I v0 = Yc^T G^(I-1) (Yc v0), and M v = Yc^T (G g) for v = Yc^T g.  The
+// same algebra as the S form, on a matrix (m / c)^2 times smaller, so every
+// Lanczos step of a thin cluster moves c^2 instead of m^2 values.
+//   k_gram_tiles  G in 64 x 64 tiles from the packed panel (Y = hi + lo,
+//                 exact in FP32), FP32 MFMA 32x32x2 with FP32 accumulation,
+//                 the s s^T / m term in the epilogue; entries with a <= b
+//                 stored with their mirrors, so G is exactly symmetric
+//   k_gram_start  u0 = Y (v0 - mean(v0)) (FP64), v0 = power_v0(seed, iter,
+//                 cluster, .), into the cluster's r row
+//   (Lanczos)     power_lanczos_cluster in its Gram form: g = G^(I-1) u0
+//                 and h = G g, into the cluster's w row
+//   k_gram_map    v = P Y^T g, S v -> P Y^T h (= M v) and r = 0, the inputs
+//                 the power kernel takes for the S form (r = 0: no further
+//                 centring)
+typedef float f32x16g __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ void panel_load8(const uint16_t* __restrict__ yhi,
+                                            const uint16_t* __restrict__ ylo, long off,
+                                            float* y) {
+  const uint4 h = *reinterpret_cast<const uint4*>(yhi + off);
+  const uint4 l = *reinterpret_cast<const uint4*>(ylo + off);
+  const uint32_t hw[4] = {h.x, h.y, h.z, h.w}, lw[4] = {l.x, l.y, l.z, l.w};
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    y[2 * u] = __uint_as_float(hw[u] << 16) + __uint_as_float(lw[u] << 16);
+    y[2 * u + 1] = __uint_as_float(hw[u] & 0xffff0000u) + __uint_as_float(lw[u] & 0xffff0000u);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gram_tiles(const GramItem* __restrict__ items) {
+  const GramItem it = items[blockIdx.x];
+  constexpr int kLd = 64 + 4;
+  __shared__ float As[32 * kLd], Bs[32 * kLd];
+  __shared__ float ssA[64], ssB[64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool diag = it.ti == it.tj;
+  const int tm = panel_blocks(it.m);
+  // loader: chunk ch (16 members) of each side, lane ln: time ln % 32,
+  // members 8 * (ln / 32) .. + 7 of the chunk
+  const int ch = wave, tt = lane & 31, qo = ch * 16 + 8 * (lane >> 5);
+  const bool va = it.ti * 64 + ch * 16 < it.nd, vb = it.tj * 64 + ch * 16 < it.nd;
+  const long ca = (it.p0 >> 4) + it.ti * 4 + ch, cb = (it.p0 >> 4) + it.tj * 4 + ch;
+  float sa[8], sb[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) sa[e] = sb[e] = 0.f;
+  const int wi = wave >> 1, wj = wave & 1;
+  const bool active = !(diag && wi > wj);
+  f32x16g acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+  for (int r32 = 0; r32 < tm; ++r32) {
+    float ya[8], yb[8];
+    if (va) {
+      panel_load8(it.yhi, it.ylo, (ca * tm + r32) * 512 + lane * 8, ya);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) ya[e] = 0.f;
+    }
+    if (!diag && vb) {
+      panel_load8(it.yhi, it.ylo, (cb * tm + r32) * 512 + lane * 8, yb);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) yb[e] = diag ? ya[e] : 0.f;
+    }
+    __syncthreads();  // the previous step's operands are consumed
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      As[tt * kLd + qo + e] = ya[e];
+      Bs[tt * kLd + qo + e] = yb[e];
+      sa[e] += ya[e];
+      sb[e] += yb[e];
+    }
+    __syncthreads();
+    if (active) {
+      const float* ap = As + (lane >> 5) * kLd + wi * 32 + (lane & 31);
+      const float* bp = Bs + (lane >> 5) * kLd + wj * 32 + (lane & 31);
+#pragma unroll
+      for (int kk = 0; kk < 16; ++kk)
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ap[2 * kk * kLd], bp[2 * kk * kLd], acc, 0, 0,
+                                                   0);
+    }
+  }
+  // row sums: lanes of one half-wave hold the same 8 members at 32 times
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+      sa[e] += __shfl_xor(sa[e], o);
+      sb[e] += __shfl_xor(sb[e], o);
+    }
+  }
+  if (tt == 0) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      ssA[qo + e] = sa[e];
+      ssB[qo + e] = sb[e];
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const float inv_m = 1.0f / (float)it.m;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int i = wi * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+    const int jj = wj * 32 + (lane & 31);
+    const int a = it.ti * 64 + i, b = it.tj * 64 + jj;
+    if (a < it.nd && b < it.nd && a <= b) {
+      const float v = acc[r] - ssA[i] * ssB[jj] * inv_m;
+      it.G[(long)a * it.nd + b] = v;
+      it.G[(long)b * it.nd + a] = v;
+    }
+  }
+}
+
+// Sum over a block of 256 threads (4 waves); every thread gets the sum.
+__device__ __forceinline__ double block256_sum_f64(double v, double* red) {
+  v = wave_sum_f64(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ __launch_bounds__(256) void k_gram_start(const GramVecJob* __restrict__ jobs) {
+  const GramVecJob a = jobs[blockIdx.x];
+  extern __shared__ double gsh[];
+  double* vc = gsh;  // m
+  __shared__ double red[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double s = 0.0;
+  for (int t = threadIdx.x; t < a.m; t += 256) {
+    vc[t] = power_v0(a.seed, a.iter, a.cluster, t);
+    s += vc[t];
+  }
+  const double mean = block256_sum_f64(s, red) / (double)a.m;
+  for (int t = threadIdx.x; t < a.m; t += 256) vc[t] -= mean;
+  __syncthreads();
+  const int tm = panel_blocks(a.m), nch = a.nd / 16;
+  for (int cc = wave; cc < nch; cc += 4) {
+    double acc[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) acc[e] = 0.0;
+    const long base = ((a.p0 >> 4) + cc) * (long)tm * 512 + lane * 8;
+    for (int r32 = 0; r32 < tm; ++r32) {
+      const int t = r32 * 32 + (lane & 31);
+      if (t >= a.m) break;  // rows t >= m of the panel are zero
+      float y[8];
+      panel_load8(a.yhi, a.ylo, base + (long)r32 * 512, y);
+      const double x = vc[t];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) acc[e] += (double)y[e] * x;
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+#pragma unroll
+      for (int o = 1; o < 32; o <<= 1) acc[e] += __shfl_xor(acc[e], o);
+    if ((lane & 31) == 0)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) a.u0[cc * 16 + 8 * (lane >> 5) + e] = acc[e];
+  }
+}
+
+__global__ __launch_bounds__(kPowerThreads) void k_gram_lanczos(const GramLzJob* __restrict__ jobs,
+                                                                double lz_tol) {
+  const GramLzJob a = jobs[blockIdx.x];
+  power_lanczos_cluster(a.G, a.nd, a.u0, 0, 0, 0, a.I, lz_tol, a.Q, a.g, nullptr, a.h, a.steps,
+                        a.conv);
+}
+
+__global__ __launch_bounds__(256) void k_gram_map(const GramVecJob* __restrict__ jobs) {
+  const GramVecJob a = jobs[blockIdx.x];
+  extern __shared__ double gsh[];
+  double* g = gsh;          // nd
+  double* h = g + a.nd;     // nd
+  double* vt = h + a.nd;    // m
+  double* st = vt + a.m;    // m
+  __shared__ double red[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < a.nd; i += 256) {
+    g[i] = a.gh[i];
+    h[i] = a.gh[a.nd + i];
+  }
+  __syncthreads();
+  const int tm = panel_blocks(a.m), nch = a.nd / 16;
+  for (int r32 = wave; r32 < tm; r32 += 4) {
+    const int t = r32 * 32 + (lane & 31);
+    if (r32 * 32 >= a.m) break;
+    double av = 0.0, as = 0.0;
+    for (int cc = 0; cc < nch; ++cc) {
+      float y[8];
+      panel_load8(a.yhi, a.ylo, (((a.p0 >> 4) + cc) * (long)tm + r32) * 512 + lane * 8, y);
+      const int q = cc * 16 + 8 * (lane >> 5);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        av += (double)y[e] * g[q + e];
+        as += (double)y[e] * h[q + e];
+      }
+    }
+    av += __shfl_xor(av, 32);
+    as += __shfl_xor(as, 32);
+    if (lane < 32 && t < a.m) {
+      vt[t] = av;
+      st[t] = as;
+    }
+  }
+  __syncthreads();
+  double sv = 0.0, ss = 0.0;
+  for (int t = threadIdx.x; t < a.m; t += 256) {
+    sv += vt[t];
+    ss += st[t];
+  }
+  const double mv = block256_sum_f64(sv, red) / (double)a.m;
+  const double ms = block256_sum_f64(ss, red) / (double)a.m;
+  for (int t = threadIdx.x; t < a.m; t += 256) {
+    a.v[t] = vt[t] - mv;
+    a.sv[t] = st[t] - ms;
+    a.r[t] = 0.0;
+  }
+}
+
 #define TCX_GROUP_ARGS(T) static_cast<const T*>(g.jobs), g.first, g.njobs
 
 hipError_t launch_group_count_group(const GroupTable& g, hipStream_t st) {
@@ -2925,6 +3224,39 @@ hipError_t launch_syrk_reduce_group(const SyrkJob* jobs, const GroupSyrkTile* ti
   constexpr int kSub = kSyrkTile / 64;
   hipLaunchKernelGGL(k_syrk_reduce_group, dim3((unsigned)ntiles * kSub * kSub), dim3(256), 0, st,
                      jobs, tiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_gram_tiles(const GramItem* items, int nitems, hipStream_t st) {
+  if (nitems <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gram_tiles, dim3((unsigned)nitems), dim3(256), 0, st, items);
+  return hipGetLastError();
+}
+
+size_t gram_start_lds(int m) { return sizeof(double) * (size_t)m; }
+size_t gram_map_lds(int m, int nd) { return sizeof(double) * 2 * ((size_t)m + nd); }
+size_t gram_lanczos_lds(int nd) { return lanczos_smem(nd); }
+
+hipError_t launch_gram_start(const GramVecJob* jobs, int njobs, size_t lds, hipStream_t st) {
+  if (njobs <= 0) return hipSuccess;
+  if (lds > gram_start_lds(kGramMaxM)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_gram_start, dim3((unsigned)njobs), dim3(256), lds, st, jobs);
+  return hipGetLastError();
+}
+
+hipError_t launch_gram_lanczos(const GramLzJob* jobs, int njobs, size_t lds, double lz_tol,
+                               hipStream_t st) {
+  if (njobs <= 0) return hipSuccess;
+  if (lds > lanczos_smem(kGramMaxM / 2)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_gram_lanczos, dim3((unsigned)njobs), dim3(kPowerThreads), lds, st, jobs,
+                     lz_tol);
+  return hipGetLastError();
+}
+
+hipError_t launch_gram_map(const GramVecJob* jobs, int njobs, size_t lds, hipStream_t st) {
+  if (njobs <= 0) return hipSuccess;
+  if (lds > gram_map_lds(kGramMaxM, kGramMaxM / 2)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_gram_map, dim3((unsigned)njobs), dim3(256), lds, st, jobs);
   return hipGetLastError();
 }
 
@@ -3016,6 +3348,12 @@ hipError_t configure_extract_kernels() {
     e = hipFuncSetAttribute((const void*)k_power_lanczos_group,
                             hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lanczos_smem(kFusedLanczosMaxM));
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_gram_lanczos, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lanczos_smem(kGramMaxM / 2));
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_gram_map, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)gram_map_lds(kGramMaxM, kGramMaxM / 2));
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void*)k_lz_step_group,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lz_step_smem(4096));

@@ -440,6 +440,49 @@ struct PowerLanczosJob {
 size_t power_lanczos_lds(int m);
 hipError_t launch_power_lanczos_group(const GroupTable& g, double lz_tol, hipStream_t st);
 
+// Gram form of thin clusters (2 * segment <= m, m <= kGramMaxM): G = Yc Yc^T
+// in 64 x 64 tiles, the start u0 = Yc v0, the Lanczos stage on G, and the
+// map back to v = Yc^T g, M v = Yc^T h.  One item or job per block.
+constexpr int kGramMaxM = 4096;
+struct GramItem {
+  const uint16_t* yhi;
+  const uint16_t* ylo;
+  float* G;  // nd x nd, row stride nd
+  long p0;   // first packed column of the cluster (multiple of 64)
+  int m, nd, ti, tj;
+};
+struct GramVecJob {
+  const uint16_t* yhi;
+  const uint16_t* ylo;
+  long p0;
+  int m, nd;
+  uint64_t seed;
+  int iter, cluster;
+  double* u0;         // start: Yc v0 (nd)
+  const double* gh;   // map: g (nd), then h (nd)
+  double* v;          // map: m
+  double* sv;         // map: M v (m)
+  double* r;          // map: zeros (m)
+};
+struct GramLzJob {
+  const float* G;
+  int nd, I;
+  const double* u0;
+  double* Q;
+  double* g;
+  double* h;
+  int* steps;
+  int* conv;
+};
+hipError_t launch_gram_tiles(const GramItem* items, int nitems, hipStream_t st);
+size_t gram_start_lds(int m);
+size_t gram_map_lds(int m, int nd);
+size_t gram_lanczos_lds(int nd);
+hipError_t launch_gram_start(const GramVecJob* jobs, int njobs, size_t lds, hipStream_t st);
+hipError_t launch_gram_lanczos(const GramLzJob* jobs, int njobs, size_t lds, double lz_tol,
+                               hipStream_t st);
+hipError_t launch_gram_map(const GramVecJob* jobs, int njobs, size_t lds, hipStream_t st);
+
 struct ClusterMatvecJob {
   const float* S;
   int m, k;
@@ -464,6 +507,7 @@ struct LzJob {  // k_lz_setup / k_lz_step / k_lz_finish of one job
   double* vout;
   int* steps_out;
   int* conv_out;
+  int* done_out;  // may be NULL: per-cluster copy of state.done (one readback)
 };
 size_t lz_step_lds(int m);
 hipError_t launch_lz_setup_group(const GroupTable& g, hipStream_t st);
