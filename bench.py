@@ -175,11 +175,14 @@ def batch_workload(timesc, njobs):
 
 
 def run_batch(timesc, ctx, jobs, workers):
-    """The uneven batch on a pool of worker contexts against the same jobs run
-    one after another on one context (plain per-job launches, the reference's
-    one fit per call).  Each form first runs every job for one iteration
-    (sizes the workspaces), then the timed run: jobs/s, per-job latency from
-    batch start to job completion, per-job run time."""
+    """The uneven batch three ways, each first run once for one iteration
+    (sizes workspaces), then timed over the full fits:
+      - grouped: tcx_batch_run_grouped, every phase of an iteration one
+        launch over all jobs in progress (the reported figures);
+      - pool: whole fits on a pool of `workers` contexts;
+      - per-job launches: the same fits one after another on one context
+        (the reference's one fit per call), the speedup's baseline.
+    Per-job latency is from batch start to the job's completion."""
     ds = [ctx.to_device(X) for X, _, _ in jobs]
 
     def specs(max_iter):
@@ -190,13 +193,18 @@ def run_batch(timesc, ctx, jobs, workers):
         for d, (X, _, _) in zip(ds, jobs):
             ctx.h2d(d, X)
 
+    def timed(b, grouped):
+        reload()
+        b.run(specs(1), order=timesc.ORDER_LONGEST_FIRST, grouped=grouped)
+        reload()
+        t0 = time.perf_counter()
+        res = b.run(specs(10), order=timesc.ORDER_LONGEST_FIRST, grouped=grouped)
+        return res, time.perf_counter() - t0
+
     try:
         with timesc.Batch(0, workers=workers) as b:
-            b.run(specs(1), order=timesc.ORDER_LONGEST_FIRST)
-            reload()
-            t0 = time.perf_counter()
-            res = b.run(specs(10), order=timesc.ORDER_LONGEST_FIRST)
-            wall = time.perf_counter() - t0
+            res, wall = timed(b, True)
+            pool_res, pool_wall = timed(b, False)
         # plain per-job launches: one context, one fit after the other
         reload()
         for d, (_, k, seed) in zip(ds, jobs):
@@ -213,18 +221,22 @@ def run_batch(timesc, ctx, jobs, workers):
     finally:
         for d in ds:
             d.free()
+    assert all(r["status"] == 0 for r in res + pool_res), "a batch job failed"
     lat = np.array([r["ms_latency"] for r in res])
-    run = np.array([r["ms_run"] for r in res])
     iters = np.array([r["stats"]["iterations"] for r in res])
     pairs = sum(X.shape[0] * k * r["stats"]["iterations"] for (X, k, _), r in zip(jobs, res))
+    pool_lat = np.array([r["ms_latency"] for r in pool_res])
     return {
-        "jobs": len(jobs), "workers": workers, "makespan_ms": wall * 1e3,
+        "jobs": len(jobs), "executor": "grouped", "makespan_ms": wall * 1e3,
         "jobs_per_s": len(jobs) / wall,
         "sbd_pairs_per_s": pairs / wall,
         "p50_latency_ms": float(np.percentile(lat, 50)),
         "p99_latency_ms": float(np.percentile(lat, 99)),
-        "p50_run_ms": float(np.percentile(run, 50)), "p99_run_ms": float(np.percentile(run, 99)),
         "mean_iterations": float(iters.mean()),
+        "pool_workers": workers, "pool_jobs_per_s": len(jobs) / pool_wall,
+        "pool_makespan_ms": pool_wall * 1e3,
+        "pool_p50_latency_ms": float(np.percentile(pool_lat, 50)),
+        "pool_p99_latency_ms": float(np.percentile(pool_lat, 99)),
         "per_job_launch_jobs_per_s": len(jobs) / seq_wall,
         "per_job_launch_makespan_ms": seq_wall * 1e3,
         "per_job_launch_p99_latency_ms": float(np.percentile(np.array(seq_lat) * 1e3, 99)),

@@ -4,10 +4,12 @@ over all jobs in progress), through the C ABI:
   - a job's results do not depend on the other jobs: every job of a mixed
     batch equals the same job run alone through the executor, bit for bit;
   - parity: the labels after every iteration equal the FP64 oracle's and the
-    final centroids are within 1e-4 relative L2, over shapes that take every
-    kernel form (FFT lengths 16..4096, the fused and the two-launch power
-    stage, every align + pack form, both bound-product forms), on seeds
-    whose oracle trajectories hold no SBD or alignment-lag near-tie;
+    final centroids are within 1e-4 relative L2, over shapes that take
+    FFT lengths 16..4096, the S and the Gram form of the power stage, every
+    align + pack form and both bound-product forms, on seeds whose oracle
+    trajectories hold no SBD or alignment-lag near-tie; and the first
+    iteration of large clusters (every power-stage form) against the
+    oracle's refinement and assignment;
   - the collapse stop, and jobs the executor leaves to the worker pool
     (centre initialisation, eigenvector centroids) in the same call;
   - the 256-job uneven workload: every job equals itself alone through the
@@ -23,9 +25,12 @@ from parity import CENT_RTOL, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-# (n, m, k), seed: F = 16 .. 4096; m > 768 takes the two-launch power
-# stage, m > 1024 the align + pack from HBM; k > 16 the wider bound
-# product.  On these seeds the oracle's trajectory has no near-tie: in every
+# (n, m, k), seed: F = 16 .. 4096; clusters shorter than m (in padded
+# members) take the Gram form (one wave per cluster up to order 256, one
+# block above), the others the S form (one wave for m <= 256, the two-launch
+# form above: (1200, 300, 2)); m > 1024 takes the align + pack from HBM;
+# k > 16 the wider bound product.  On these seeds the oracle's trajectory
+# has no near-tie: in every
 # iteration the best and second-best SBD of every series, and the best and
 # second-best alignment lag of every member, differ by more than 2e-6
 # (checked on the CPU when the seeds were chosen), so FP32 must reproduce the
@@ -33,7 +38,8 @@ pytestmark = pytest.mark.gpu
 # z-normalized rows of length 2 are +-(1, -1), every assignment is a tie.)
 CASES = [((300, 30, 3), 710), ((500, 128, 5), 711), ((257, 5, 2), 712), ((400, 260, 17), 913),
          ((400, 700, 4), 1214), ((350, 900, 6), 1115), ((150, 1500, 3), 813),
-         ((2000, 96, 16), 1417), ((64, 33, 5), 718), ((200, 2048, 2), 1413)]
+         ((2000, 96, 16), 1417), ((64, 33, 5), 718), ((200, 2048, 2), 1413),
+         ((1200, 300, 2), 1818)]
 SHAPES = [c[0] for c in CASES]
 
 
@@ -44,7 +50,7 @@ def _specs(ctx, shapes, seeds, **kw):
         d = ctx.to_device(X)
         ds.append(d)
         Xs.append(X)
-        specs.append(dict(d=d, k=k, seed=seed, znorm_ddof=0, max_iter=10, **kw))
+        specs.append(dict(d=d, k=k, seed=seed, znorm_ddof=0, **{"max_iter": 10, **kw}))
     return ds, specs, Xs
 
 
@@ -87,6 +93,33 @@ def test_grouped_trajectories_match_oracle(ctx):
         assert max(errs) <= CENT_RTOL, ((n, m, k), errs)
 
 
+# First iteration (no alignment: the initial centroids are zero) of shapes
+# that take each power-stage form with large clusters: fat clusters with
+# m > 256 (the two-launch form, m <= 768 and above), thin clusters whose
+# Gram order exceeds 256 (one block per cluster), and the one-wave form
+# (Gram and S).  Free of alignment-lag ties, so any seed does.
+FIRST_ITER = [((1200, 300, 2), 1801), ((1700, 780, 2), 1802), ((2500, 900, 2), 1803),
+              ((1600, 1200, 2), 1804), ((700, 900, 2), 1805), ((600, 128, 4), 1806),
+              ((900, 700, 6), 1807)]
+
+
+def test_grouped_first_iteration_matches_oracle(ctx):
+    shapes = [c[0] for c in FIRST_ITER]
+    seeds = [c[1] for c in FIRST_ITER]
+    res, Xs = _run(ctx, shapes, seeds, max_iter=1)
+    from parity import assert_labels_match
+
+    for (n, m, k), seed, r, X in zip(shapes, seeds, res, Xs):
+        assert r["status"] == 0, r["message"]
+        assert r["stats"]["iterations"] == 1
+        Z = O.zscore(X.astype(np.float64))
+        cent, _, _ = O.refine(Z, T.init_labels(seed, n, k), np.zeros((k, m)), seed=seed, it=0)
+        errs = [rel_l2(a, b) for a, b in zip(r["centroids"], cent) if np.any(b)]
+        assert max(errs) <= CENT_RTOL, ((n, m, k), errs)
+        olab, _, _, odm = O.assign(Z, r["centroids"].astype(np.float64), dmat=True)
+        assert_labels_match(r["labels"], olab, odm, what=f"{(n, m, k)}")
+
+
 def test_grouped_collapse_stop_and_pool_jobs_in_one_call(ctx):
     """A two-class job with k = 4 stops on collapse like the single-fit
     engine; a centre-initialized job and an eigenvector-centroid job run on
@@ -123,9 +156,12 @@ def test_grouped_collapse_stop_and_pool_jobs_in_one_call(ctx):
 def test_grouped_256_uneven_jobs(ctx):
     """Every job: its final labels are the FP64 oracle's assignment against
     its own final centroids (up to SBD ties); 32 sampled jobs equal
-    themselves run alone through the executor; at least 90% end with the
-    single-fit engine's labels (the rest differ through FP32 ties met on
-    the way, in SBD or in an alignment lag)."""
+    themselves run alone through the executor; at least 85% end with the
+    single-fit engine's labels.  The rest differ through FP32 ties met on
+    the way, in SBD or in an alignment lag: the executor evaluates the power
+    iterate of a thin cluster through its Gram matrix, the single-fit engine
+    through S, so the two round differently, and ten iterations carry a
+    flipped near-tie forward."""
     from parity import assert_labels_match
 
     jobs = bench.batch_workload(T, 256)
@@ -165,4 +201,4 @@ def test_grouped_256_uneven_jobs(ctx):
             assert np.array_equal(r["labels"], a["labels"]), i
             assert np.array_equal(r["centroids"], a["centroids"]), i
     print(f"{same_labels} of {len(jobs)} jobs end with the single-fit engine's labels")
-    assert same_labels >= 0.9 * len(jobs)
+    assert same_labels >= 0.85 * len(jobs)

@@ -284,6 +284,10 @@ struct Keyed {
 struct Batch {
   tcx_ctx* ctx = nullptr;
   hipStream_t st = nullptr;
+  // second stream: the block-form Gram Lanczos runs beside the one-wave
+  // form (default priority; joined back into st by an event)
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int cus = 0;
   std::vector<GJob> jobs;
   Uploader up;
@@ -298,6 +302,10 @@ struct Batch {
   std::vector<GroupSyrkTile> tiles;
 
   ~Batch() {
+    if (st2) (void)hipStreamSynchronize(st2);
+    if (st2) (void)hipStreamDestroy(st2);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
     for (GJob& j : jobs)
       if (j.arena) (void)hipFree(j.arena);
     for (unsigned char* p : {ginfo_d, info_d, evc_d})
@@ -534,10 +542,10 @@ int power_split(Batch& b, const std::vector<GJob*>& set, int mode) {
   return TCX_OK;
 }
 
-// A cluster takes the Gram form when its padded segment nd satisfies
-// 2 nd <= m: every Lanczos step then moves at most a quarter of S's values.
+// A cluster takes the Gram form when its padded segment nd is shorter than
+// m: every Lanczos step then moves nd^2 < m^2 values.
 bool gram_form(int m, long nd, int count) {
-  return count > 0 && m <= kGramMaxM && nd > 0 && 2 * nd <= m;
+  return count > 0 && m <= kGramMaxM && nd > 0 && nd < m && nd <= kGramMaxND;
 }
 
 // S = Y^T Y items (one K chunk per tile) of the clusters with sel[c].
@@ -592,6 +600,7 @@ int refine_power(Batch& b, const std::vector<GJob*>& act, double* ms_power) {
   std::vector<GramItem> gitems;
   std::vector<GramVecJob> gstart, gmap;
   std::vector<GramLzJob> glz;
+  std::vector<LzWaveJob> wlz;
   size_t lds_start = 0, lds_map = 0, lds_lz = 0;
   Launch<FillSeg> fills;
   for (size_t q = 0; q < act.size(); ++q) {
@@ -612,51 +621,54 @@ int refine_power(Batch& b, const std::vector<GJob*>& act, double* ms_power) {
       const long nd = off[c + 1] - off[c];
       j->thin[c] = gram_form(j->m, nd, cnt[c]);
       j->fat[c] = cnt[c] > 0 && !j->thin[c];
-      j->nfat += j->fat[c];
-      fills.add(FillSeg{j->scount + c, 1, j->fat[c] ? cnt[c] : 0}, 1);
+      // fat clusters of short series: one wave each; the others by block
+      const bool fat_wave = j->fat[c] && j->m <= kLzWaveMaxD;
+      j->nfat += j->fat[c] && !fat_wave;
+      fills.add(FillSeg{j->scount + c, 1, j->fat[c] && !fat_wave ? cnt[c] : 0}, 1);
+      if (fat_wave)
+        wlz.push_back(LzWaveJob{j->S + c * mm, j->m, 0, steps, j->o.seed, j->it, c, nullptr,
+                                j->Q + (long)c * (kLanczosMax + 1) * j->m, j->v + (long)c * j->m,
+                                j->r + (long)c * j->m, j->sv + (long)c * j->m, j->steps + c,
+                                j->conv + c});
       if (!j->thin[c]) continue;
-      const int T = (int)(nd / 64);
+      // Gram order: the members rounded up to 4 (the matvec's float4 rows)
+      const int gn = (cnt[c] + 3) & ~3;
+      const int T = (gn + 63) / 64;
       for (int ti = 0; ti < T; ++ti)
         for (int tj = ti; tj < T; ++tj)
-          gitems.push_back(GramItem{j->yhi, j->ylo, j->S + c * mm, off[c], j->m, (int)nd, ti, tj});
+          gitems.push_back(
+              GramItem{j->yhi, j->ylo, j->S + c * mm, off[c], j->m, (int)nd, gn, ti, tj});
       double* rc = j->r + (long)c * j->m;
       double* wc = j->w + (long)c * j->m;
-      gstart.push_back(GramVecJob{j->yhi, j->ylo, off[c], j->m, (int)nd, j->o.seed, j->it, c, rc,
-                                  nullptr, nullptr, nullptr, nullptr});
-      glz.push_back(GramLzJob{j->S + c * mm, (int)nd, steps - 1, rc,
-                              j->Q + (long)c * (kLanczosMax + 1) * j->m, wc, wc + nd,
-                              j->steps + c, j->conv + c});
-      gmap.push_back(GramVecJob{j->yhi, j->ylo, off[c], j->m, (int)nd, j->o.seed, j->it, c,
-                                nullptr, wc, j->v + (long)c * j->m, j->sv + (long)c * j->m, rc});
+      gstart.push_back(GramVecJob{j->yhi, j->ylo, off[c], j->m, (int)nd, gn, j->o.seed, j->it, c,
+                                  rc, nullptr, nullptr, nullptr, nullptr, nullptr});
+      double* svc = j->sv + (long)c * j->m;
+      double* Qc = j->Q + (long)c * (kLanczosMax + 1) * j->m;
+      if (gn <= kLzWaveMaxD) {
+        wlz.push_back(LzWaveJob{j->S + c * mm, gn, 1, steps - 1, 0, 0, 0, rc, Qc, wc, nullptr,
+                                svc, j->steps + c, j->conv + c});
+      } else {
+        glz.push_back(GramLzJob{j->S + c * mm, gn, steps - 1, rc, Qc, wc, svc, j->steps + c,
+                                j->conv + c});
+        lds_lz = std::max(lds_lz, gram_lanczos_lds(gn));
+      }
+      gmap.push_back(GramVecJob{j->yhi, j->ylo, off[c], j->m, (int)nd, gn, j->o.seed, j->it, c,
+                                nullptr, wc, svc, j->v + (long)c * j->m, svc, rc});
       lds_start = std::max(lds_start, gram_start_lds(j->m));
       lds_map = std::max(lds_map, gram_map_lds(j->m, (int)nd));
-      lds_lz = std::max(lds_lz, gram_lanczos_lds((int)nd));
     }
     sjobs.push_back(SyrkJob{j->yhi, j->ylo, j->m, j->part, j->S});
     add_syrk_items(b, (int)q, *j, off, j->fat);
   }
   const auto t0 = Clock::now();
-  Launch<PowerLanczosJob> pl;
-  Launch<ClusterMatvecJob> sv;
   Launch<PowerJob> pw;
   std::vector<GJob*> split_set;
   for (GJob* j : act) {
     const int steps = j->m > 1000 ? 1000 : j->m;
-    const long mm = (long)j->m * j->m;
-    if (j->nfat > 0) {
-      if (j->m <= kFusedLanczosMaxM) {
-        pl.add(PowerLanczosJob{j->S, j->m, j->scount, j->o.seed, j->it, steps, j->Q, j->v, j->r,
-                               j->steps, j->conv},
-               j->k, power_lanczos_lds(j->m));
-        for (int c = 0; c < j->k; ++c)
-          if (j->fat[c])
-            sv.add(ClusterMatvecJob{j->S + c * mm, j->m, 1, j->v + (long)c * j->m, j->m,
-                                    j->sv + (long)c * j->m, nullptr},
-                   cluster_matvec_blocks(j->m, 1), sizeof(double) * j->m);
-      } else {
-        split_set.push_back(j);
-      }
-    }
+    // fat clusters of m > kLzWaveMaxD take the two-launch form: each Lanczos
+    // step of all of them is one grid-wide matvec and one step kernel,
+    // instead of one latency-bound block per cluster
+    if (j->nfat > 0) split_set.push_back(j);
     pw.add(PowerJob{j->S, j->m, b.counts(*j), j->a0, j->o.seed, j->it, steps, j->tol, j->v, j->sv,
                     j->r, j->steps, j->conv, 0, nullptr, 1, j->cent_new, b.info(*j)},
            j->k, power_lds(j->m));
@@ -667,8 +679,6 @@ int refine_power(Batch& b, const std::vector<GJob*>& act, double* ms_power) {
   GX_TRY(launch_syrk_items(b, act, sjobs, &sj_off, by_off, &tl_off, by));
   const size_t fill_off = b.up.put_vec(fills.jobs);
   const size_t gi_off = b.up.put_vec(gitems), gs_off = b.up.put_vec(gstart);
-  pl.stage(b.up);
-  sv.stage(b.up);
   GX_TRY(b.up.flush(b.st));
   if (!fills.jobs.empty())
     GX_HIP(launch_fill_segs(reinterpret_cast<const FillSeg*>(b.up.dev(fill_off)),
@@ -678,17 +688,24 @@ int refine_power(Batch& b, const std::vector<GJob*>& act, double* ms_power) {
                            b.st));
   GX_HIP(launch_gram_start(reinterpret_cast<const GramVecJob*>(b.up.dev(gs_off)),
                            (int)gstart.size(), lds_start, b.st));
-  GX_HIP(launch_power_lanczos_group(pl.table(b.up), kLzPowerTol, b.st));
-  GX_HIP(launch_cluster_matvec_group(sv.table(b.up), b.st));  // S v
   // the two-launch form first: its finishing pass writes v, steps and conv
   // of every cluster of its jobs, the Gram form's included
   GX_TRY(power_split(b, split_set, 0));  // uploads its own tables
   b.up.reset();
   const size_t gl_off = b.up.put_vec(glz), gm_off = b.up.put_vec(gmap);
+  const size_t wl_off = b.up.put_vec(wlz);
   pw.stage(b.up);
   GX_TRY(b.up.flush(b.st));
-  GX_HIP(launch_gram_lanczos(reinterpret_cast<const GramLzJob*>(b.up.dev(gl_off)), (int)glz.size(),
-                             lds_lz, kLzPowerTol, b.st));
+  if (!glz.empty()) {  // the block form on the second stream, after the upload
+    GX_HIP(hipEventRecord(b.ev_fork, b.st));
+    GX_HIP(hipStreamWaitEvent(b.st2, b.ev_fork, 0));
+    GX_HIP(launch_gram_lanczos(reinterpret_cast<const GramLzJob*>(b.up.dev(gl_off)),
+                               (int)glz.size(), lds_lz, kLzPowerTol, b.st2));
+    GX_HIP(hipEventRecord(b.ev_join, b.st2));
+  }
+  GX_HIP(launch_lanczos_wave(reinterpret_cast<const LzWaveJob*>(b.up.dev(wl_off)), (int)wlz.size(),
+                             kLzPowerTol, b.st));
+  if (!glz.empty()) GX_HIP(hipStreamWaitEvent(b.st, b.ev_join, 0));
   GX_HIP(launch_gram_map(reinterpret_cast<const GramVecJob*>(b.up.dev(gm_off)), (int)gmap.size(),
                          lds_map, b.st));
   GX_HIP(launch_power_group(pw.table(b.up), b.st));
@@ -845,6 +862,9 @@ int grouped_fit(tcx_ctx* ctx, tcx_job* jobs, const int* idx, int count, size_t h
   b.st = static_cast<hipStream_t>(stream_of(ctx));
   b.cus = cus_of(ctx);
   GX_HIP(hipSetDevice(device_of(ctx)));
+  GX_HIP(hipStreamCreateWithFlags(&b.st2, hipStreamNonBlocking));
+  GX_HIP(hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming));
+  GX_HIP(hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming));
   b.jobs.resize(count);
   for (int q = 0; q < count; ++q) {
     tcx_job& jb = jobs[idx[q]];

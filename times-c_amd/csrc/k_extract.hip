@@ -1548,7 +1548,9 @@ __device__ __forceinline__ void power_lanczos_cluster(const float* __restrict__ 
   double* yp = y + kLzMax;            // kLzMax
   int* flag = reinterpret_cast<int*>(yp + kLzMax);  // 2 ints (padded to 1 double)
   double* part = yp + kLzMax + 1;     // sym_part_len(m)
-  double* sq = part + sym_part_len(m);  // kLzSqDoubles: acceptance test
+  // the acceptance test (kLzSqDoubles) reuses the partials of the matvec and
+  // the projection: the two are never live at once
+  double* sq = part;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   double sigma = 0.0, n0 = 0.0;
   if (vstart) {
@@ -2797,8 +2799,8 @@ hipError_t launch_power(const float* S, int m, int k, const int* counts, const f
 }
 
 static size_t lanczos_smem(int m) {
-  return sizeof(double) *
-         (4 * (size_t)m + 8 * kPowerWaves + 5 * (kLzMax + 1) + sym_part_len(m) + kLzSqDoubles);
+  return sizeof(double) * (4 * (size_t)m + 8 * kPowerWaves + 5 * (kLzMax + 1) +
+                           std::max((size_t)sym_part_len(m), (size_t)kLzSqDoubles));
 }
 
 hipError_t launch_power_lanczos(const float* S, int m, int k, const int* counts, uint64_t seed,
@@ -2917,20 +2919,22 @@ size_t group_count_lds(int k, int B, int* lds_min) {
 
 
 // ------------------------------------------------ Gram form of thin clusters
-// For a cluster of c members with 2 c <= m (c counted to its padded
-// segment), M = P Y^T Y P = Yc^T Yc (Yc = Y P, rows centred) is evaluated
+// For a cluster of c members whose padded segment is shorter than m (and at
+// most kGramMaxND), M = P Y^T Y P = Yc^T Yc (Yc = Y P, rows centred) is evaluated
 // through the c x c Gram matrix G = Yc Yc^T = Y Y^T - s s^T / m (s = Y 1):
 // M^I v0 = Yc^T G^(I-1) (Yc v0), and M v = Yc^T (G g) for v = Yc^T g.  The
 // same algebra as the S form, on a matrix (m / c)^2 times smaller, so every
 // Lanczos step of a thin cluster moves c^2 instead of m^2 values.
-//   k_gram_tiles  G in 64 x 64 tiles from the packed panel (Y = hi + lo,
-//                 exact in FP32), FP32 MFMA 32x32x2 with FP32 accumulation,
+//   k_gram_tiles  G (order n = c rounded up to 4, the padding members zero)
+//                 in 64 x 64 tiles of the segment from the packed panel
+//                 (Y = hi + lo, exact in FP32), FP32 MFMA 32x32x2 with FP32
+//                 accumulation,
 //                 the s s^T / m term in the epilogue; entries with a <= b
 //                 stored with their mirrors, so G is exactly symmetric
 //   k_gram_start  u0 = Y (v0 - mean(v0)) (FP64), v0 = power_v0(seed, iter,
 //                 cluster, .), into the cluster's r row
 //   (Lanczos)     power_lanczos_cluster in its Gram form: g = G^(I-1) u0
-//                 and h = G g, into the cluster's w row
+//                 and h = G g, into the cluster's w and sv rows
 //   k_gram_map    v = P Y^T g, S v -> P Y^T h (= M v) and r = 0, the inputs
 //                 the power kernel takes for the S form (r = 0: no further
 //                 centring)
@@ -3026,10 +3030,10 @@ __global__ __launch_bounds__(256) void k_gram_tiles(const GramItem* __restrict__
     const int i = wi * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
     const int jj = wj * 32 + (lane & 31);
     const int a = it.ti * 64 + i, b = it.tj * 64 + jj;
-    if (a < it.nd && b < it.nd && a <= b) {
+    if (a < it.n && b < it.n && a <= b) {
       const float v = acc[r] - ssA[i] * ssB[jj] * inv_m;
-      it.G[(long)a * it.nd + b] = v;
-      it.G[(long)b * it.nd + a] = v;
+      it.G[(long)a * it.n + b] = v;
+      it.G[(long)b * it.n + a] = v;
     }
   }
 }
@@ -3078,7 +3082,10 @@ __global__ __launch_bounds__(256) void k_gram_start(const GramVecJob* __restrict
       for (int o = 1; o < 32; o <<= 1) acc[e] += __shfl_xor(acc[e], o);
     if ((lane & 31) == 0)
 #pragma unroll
-      for (int e = 0; e < 8; ++e) a.u0[cc * 16 + 8 * (lane >> 5) + e] = acc[e];
+      for (int e = 0; e < 8; ++e) {
+        const int q = cc * 16 + 8 * (lane >> 5) + e;
+        if (q < a.n) a.u0[q] = acc[e];
+      }
   }
 }
 
@@ -3098,9 +3105,9 @@ __global__ __launch_bounds__(256) void k_gram_map(const GramVecJob* __restrict__
   double* st = vt + a.m;    // m
   __shared__ double red[4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int i = threadIdx.x; i < a.nd; i += 256) {
-    g[i] = a.gh[i];
-    h[i] = a.gh[a.nd + i];
+  for (int i = threadIdx.x; i < a.nd; i += 256) {  // members n .. nd - 1 are padding
+    g[i] = i < a.n ? a.g[i] : 0.0;
+    h[i] = i < a.n ? a.h[i] : 0.0;
   }
   __syncthreads();
   const int tm = panel_blocks(a.m), nch = a.nd / 16;
@@ -3137,6 +3144,314 @@ __global__ __launch_bounds__(256) void k_gram_map(const GramVecJob* __restrict__
     a.v[t] = vt[t] - mv;
     a.sv[t] = st[t] - ms;
     a.r[t] = 0.0;
+  }
+}
+
+
+// ------------------------------------------- one wave per small cluster
+// The Lanczos evaluation of power_lanczos_cluster for dimension d <=
+// kLzWaveMaxD by one wave, with no block barriers: vectors are held in
+// registers (element e = lane + 64 u), the matvec reads column c of the
+// symmetric matrix as one coalesced row segment with x[c] broadcast from
+// the wave's LDS, sums are DPP wave sums, and T^I e1 is the one-wave recurrence
+// (tridiag_power_wave).  Four clusters per block of 256 threads.  Same
+// steps and acceptance rule as the block form (which takes the recurrence
+// by binary powering for short T: the two agree to rounding).
+constexpr int kLzWaveU = kLzWaveMaxD / 64;
+constexpr int kLzWaveLds = 3 * (kLzMax + 1) + 2 * kLzMax + kLzWaveMaxD;  // alpha, beta, cf, y, yp, x
+
+// y = S x for symmetric S (FP32, d x d) through the wave's LDS slot xb (d
+// doubles); entries e >= d of y come back zero.  Loads past the last row
+// are clamped to it.
+__device__ __forceinline__ void wave_sym_matvec(const float* __restrict__ S, int d, int U,
+                                                const double (&x)[kLzWaveU],
+                                                double (&y)[kLzWaveU], double* xb) {
+  constexpr int kCols = 8;  // columns whose loads are in flight together
+  const int lane = threadIdx.x & 63;
+  int ec[kLzWaveU];
+#pragma unroll
+  for (int u = 0; u < kLzWaveU; ++u) {
+    y[u] = 0.0;
+    const int e = lane + 64 * u;
+    ec[u] = e < d ? e : d - 1;
+    if (e < d) xb[e] = x[u];
+  }
+  wave_lds_sync();
+  for (int c0 = 0; c0 < d; c0 += kCols) {
+    float sv[kCols][kLzWaveU];
+#pragma unroll
+    for (int t = 0; t < kCols; ++t) {
+      const int c = c0 + t < d ? c0 + t : d - 1;
+      const float* col = S + (long)c * d;
+#pragma unroll
+      for (int u = 0; u < kLzWaveU; ++u) sv[t][u] = u < U ? col[ec[u]] : 0.f;
+    }
+#pragma unroll
+    for (int t = 0; t < kCols; ++t) {
+      const double xc = c0 + t < d ? xb[c0 + t] : 0.0;
+#pragma unroll
+      for (int u = 0; u < kLzWaveU; ++u) y[u] += (double)sv[t][u] * xc;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kLzWaveU; ++u)
+    if (lane + 64 * u >= d) y[u] = 0.0;
+  wave_lds_sync();  // xb is rewritten by the next product
+}
+
+// Rows l0 .. l0 + kLzWaveRows - 1 of the basis (rows past `last`, and
+// entries e >= d, read as zero), loaded together.
+constexpr int kLzWaveRows = 8;
+__device__ __forceinline__ void wave_basis_rows(const double* __restrict__ Q, int d, int U, int l0,
+                                                int last,
+                                                double (&qv)[kLzWaveRows][kLzWaveU]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int t = 0; t < kLzWaveRows; ++t) {
+    const double* ql = Q + (long)(l0 + t) * d;
+#pragma unroll
+    for (int u = 0; u < kLzWaveU; ++u) {
+      const int e = lane + 64 * u;
+      qv[t][u] = (u < U && l0 + t <= last && e < d) ? ql[e] : 0.0;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256, 3) void k_lanczos_wave(const LzWaveJob* __restrict__ jobs,
+                                                      int njobs, double lz_tol) {
+  __shared__ double lds[4][kLzWaveLds];
+  // the wave index is uniform in the wave: readfirstlane lets the job's
+  // fields live in scalar registers
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int jid = blockIdx.x * 4 + wv;
+  if (jid >= njobs) return;  // whole wave
+  const LzWaveJob a = jobs[jid];
+  double* alpha = lds[wv];
+  double* beta = alpha + kLzMax + 1;
+  double* cf = beta + kLzMax + 1;
+  double* y = cf + kLzMax + 1;
+  double* yp = y + kLzMax;
+  double* xb = yp + kLzMax;
+  const int d = a.d, U = (d + 63) / 64;
+  const float* S = a.S;
+  double q[kLzWaveU], w[kLzWaveU], qp[kLzWaveU], r[kLzWaveU];
+  double sigma = 0.0, n0 = 0.0;
+  const double inv_d = 1.0 / (double)d;
+  if (a.gram) {
+#pragma unroll
+    for (int u = 0; u < kLzWaveU; ++u) {
+      const int e = lane + 64 * u;
+      q[u] = e < d ? a.vstart[e] : 0.0;
+      r[u] = qp[u] = 0.0;
+      n0 += q[u] * q[u];
+    }
+    n0 = sqrt(wave_sum_f64(n0));
+    if (!(n0 > 0.0)) {  // wave-uniform
+#pragma unroll
+      for (int u = 0; u < kLzWaveU; ++u) {
+        const int e = lane + 64 * u;
+        if (e < d) {
+          a.v[e] = 0.0;
+          a.h[e] = 0.0;
+        }
+      }
+      if (lane == 0) {
+        *a.steps = 0;
+        *a.conv = 1;
+      }
+      return;
+    }
+  } else {
+    double one[kLzWaveU];
+#pragma unroll
+    for (int u = 0; u < kLzWaveU; ++u) one[u] = lane + 64 * u < d ? 1.0 : 0.0;
+    wave_sym_matvec(S, d, U, one, r, xb);  // r = S 1
+#pragma unroll
+    for (int u = 0; u < kLzWaveU; ++u) {
+      const int e = lane + 64 * u;
+      sigma += r[u];
+      q[u] = e < d ? power_v0(a.seed, a.iter, a.cluster, e) : 0.0;
+      qp[u] = 0.0;
+      n0 += q[u] * q[u];
+      if (e < d) a.r[e] = r[u];
+    }
+    sigma = wave_sum_f64(sigma);
+    n0 = sqrt(wave_sum_f64(n0));
+  }
+#pragma unroll
+  for (int u = 0; u < kLzWaveU; ++u) {
+    const int e = lane + 64 * u;
+    q[u] /= n0;
+    if (e < d) a.Q[e] = q[u];
+  }
+  const int jmax = d < kLzMax ? d : kLzMax;
+  int len = 0, conv = 0, prev_len = 0;
+  double bprev = 0.0;
+  for (int it = 0; it < jmax; ++it) {
+    wave_sym_matvec(S, d, U, q, w, xb);  // w = S q
+    if (!a.gram) {                   // w = M q = P S P q
+      double sv = 0.0, rv = 0.0;
+#pragma unroll
+      for (int u = 0; u < kLzWaveU; ++u) {
+        sv += q[u];
+        rv += r[u] * q[u];
+      }
+      sv = wave_sum_f64(sv);
+      rv = wave_sum_f64(rv);
+#pragma unroll
+      for (int u = 0; u < kLzWaveU; ++u)
+        if (lane + 64 * u < d)
+          w[u] = w[u] - (r[u] * sv + rv) * inv_d + sigma * sv * inv_d * inv_d;
+    }
+    double nw2 = 0.0, qw = 0.0;
+#pragma unroll
+    for (int u = 0; u < kLzWaveU; ++u) {
+      nw2 += w[u] * w[u];
+      qw += q[u] * w[u];
+    }
+    nw2 = wave_sum_f64(nw2);
+    qw = wave_sum_f64(qw);
+    double w1 = 0.0;
+#pragma unroll
+    for (int u = 0; u < kLzWaveU; ++u) {
+      w[u] = w[u] - qw * q[u] - bprev * qp[u];
+      w1 += w[u] * w[u];
+    }
+    w1 = wave_sum_f64(w1);
+    double b = sqrt(w1);
+    for (int pass = 0; pass < 2; ++pass) {
+      // classical Gram-Schmidt against q_0..q_it, repeated when it removed
+      // more than 1 - 1/sqrt(2) of the norm; basis rows in groups of
+      // kLzWaveRows so that their loads are in flight together
+      for (int l0 = 0; l0 <= it; l0 += kLzWaveRows) {
+        double qv[kLzWaveRows][kLzWaveU];
+        wave_basis_rows(a.Q, d, U, l0, it, qv);
+#pragma unroll
+        for (int t = 0; t < kLzWaveRows; ++t) {
+          double dt = 0.0;
+#pragma unroll
+          for (int u = 0; u < kLzWaveU; ++u) dt += qv[t][u] * w[u];
+          dt = wave_sum_f64(dt);
+          if (lane == 0 && l0 + t <= it) cf[l0 + t] = dt;
+        }
+      }
+      wave_lds_sync();
+      for (int l0 = 0; l0 <= it; l0 += kLzWaveRows) {
+        double qv[kLzWaveRows][kLzWaveU];
+        wave_basis_rows(a.Q, d, U, l0, it, qv);
+#pragma unroll
+        for (int t = 0; t < kLzWaveRows; ++t) {
+          const double c = l0 + t <= it ? cf[l0 + t] : 0.0;
+#pragma unroll
+          for (int u = 0; u < kLzWaveU; ++u) w[u] -= c * qv[t][u];
+        }
+      }
+      double w2 = 0.0;
+#pragma unroll
+      for (int u = 0; u < kLzWaveU; ++u) w2 += w[u] * w[u];
+      w2 = wave_sum_f64(w2);
+      const bool again = w2 < 0.5 * w1;
+      b = sqrt(w2);
+      w1 = w2;
+      wave_lds_sync();  // cf is rewritten by the next pass
+      if (!again) break;
+    }
+    if (lane == 0) alpha[it] = qw;
+    len = it + 1;
+    const bool breakdown = !(b > 1e-12 * sqrt(nw2));
+    if (breakdown || len == jmax || lz_power_check(len)) {
+      wave_lds_sync();
+      tridiag_power_wave(alpha, beta, len, a.I, y);
+      wave_lds_sync();
+      double dd = 0.0;
+      for (int e = lane; e < len; e += 64) {
+        const double pv = e < prev_len ? yp[e] : 0.0;
+        dd += (y[e] - pv) * (y[e] - pv);
+      }
+      dd = wave_sum_f64(dd);
+      conv = breakdown || (prev_len > 0 && sqrt(dd) <= lz_tol);
+      for (int e = lane; e < len; e += 64) yp[e] = y[e];
+      wave_lds_sync();
+      prev_len = len;
+      if (conv || breakdown) break;
+    }
+    if (len == jmax) break;
+    if (lane == 0) beta[it + 1] = b;
+    const double inv_b = 1.0 / b;
+    double* qn = a.Q + (long)(it + 1) * d;
+#pragma unroll
+    for (int u = 0; u < kLzWaveU; ++u) {
+      const int e = lane + 64 * u;
+      qp[u] = q[u];
+      q[u] = w[u] * inv_b;
+      if (e < d) qn[e] = q[u];
+    }
+    bprev = b;
+  }
+  // v = Q y (zero when the check never passed, in the S form)
+  double v[kLzWaveU];
+  int steps = len;
+  if (conv) {
+#pragma unroll
+    for (int u = 0; u < kLzWaveU; ++u) v[u] = 0.0;
+    for (int l0 = 0; l0 < len; l0 += kLzWaveRows) {
+      double qv[kLzWaveRows][kLzWaveU];
+      wave_basis_rows(a.Q, d, U, l0, len - 1, qv);
+#pragma unroll
+      for (int t = 0; t < kLzWaveRows; ++t) {
+        const double c = l0 + t < len ? y[l0 + t] : 0.0;
+#pragma unroll
+        for (int u = 0; u < kLzWaveU; ++u) v[u] += c * qv[t][u];
+      }
+    }
+  } else if (!a.gram) {
+#pragma unroll
+    for (int u = 0; u < kLzWaveU; ++u) v[u] = 0.0;
+  } else {
+    // Gram form: plain power steps from vstart (at most I, until the
+    // Rayleigh quotient moves by <= 1e-12 relative); S v = 0 gives v = 0
+#pragma unroll
+    for (int u = 0; u < kLzWaveU; ++u) {
+      const int e = lane + 64 * u;
+      v[u] = e < d ? a.vstart[e] / n0 : 0.0;
+    }
+    double lam_prev = 0.0;
+    for (int t = 0; t < a.I; ++t) {
+      wave_sym_matvec(S, d, U, v, w, xb);
+      double nw2 = 0.0, vw = 0.0;
+#pragma unroll
+      for (int u = 0; u < kLzWaveU; ++u) {
+        nw2 += w[u] * w[u];
+        vw += v[u] * w[u];
+      }
+      nw2 = wave_sum_f64(nw2);
+      vw = wave_sum_f64(vw);
+      if (!(nw2 > 0.0)) {
+#pragma unroll
+        for (int u = 0; u < kLzWaveU; ++u) v[u] = 0.0;
+        break;
+      }
+      const double inv = 1.0 / sqrt(nw2);
+#pragma unroll
+      for (int u = 0; u < kLzWaveU; ++u) v[u] = w[u] * inv;
+      ++steps;
+      if (t > 0 && fabs(vw - lam_prev) <= 1e-12 * fabs(vw)) break;
+      lam_prev = vw;
+    }
+    conv = 1;
+  }
+  wave_sym_matvec(S, d, U, v, w, xb);  // h = S v
+#pragma unroll
+  for (int u = 0; u < kLzWaveU; ++u) {
+    const int e = lane + 64 * u;
+    if (e < d) {
+      a.v[e] = v[u];
+      a.h[e] = w[u];
+    }
+  }
+  if (lane == 0) {
+    *a.steps = steps;
+    *a.conv = conv;
   }
 }
 
@@ -3247,7 +3562,7 @@ hipError_t launch_gram_start(const GramVecJob* jobs, int njobs, size_t lds, hipS
 hipError_t launch_gram_lanczos(const GramLzJob* jobs, int njobs, size_t lds, double lz_tol,
                                hipStream_t st) {
   if (njobs <= 0) return hipSuccess;
-  if (lds > lanczos_smem(kGramMaxM / 2)) return hipErrorInvalidValue;
+  if (lds > lanczos_smem(kGramMaxND)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_gram_lanczos, dim3((unsigned)njobs), dim3(kPowerThreads), lds, st, jobs,
                      lz_tol);
   return hipGetLastError();
@@ -3255,8 +3570,15 @@ hipError_t launch_gram_lanczos(const GramLzJob* jobs, int njobs, size_t lds, dou
 
 hipError_t launch_gram_map(const GramVecJob* jobs, int njobs, size_t lds, hipStream_t st) {
   if (njobs <= 0) return hipSuccess;
-  if (lds > gram_map_lds(kGramMaxM, kGramMaxM / 2)) return hipErrorInvalidValue;
+  if (lds > gram_map_lds(kGramMaxM, kGramMaxND)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_gram_map, dim3((unsigned)njobs), dim3(256), lds, st, jobs);
+  return hipGetLastError();
+}
+
+hipError_t launch_lanczos_wave(const LzWaveJob* jobs, int njobs, double lz_tol, hipStream_t st) {
+  if (njobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lanczos_wave, dim3((unsigned)((njobs + 3) / 4)), dim3(256), 0, st, jobs,
+                     njobs, lz_tol);
   return hipGetLastError();
 }
 
@@ -3350,10 +3672,10 @@ hipError_t configure_extract_kernels() {
                             (int)lanczos_smem(kFusedLanczosMaxM));
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void*)k_gram_lanczos, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lanczos_smem(kGramMaxM / 2));
+                            (int)lanczos_smem(kGramMaxND));
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void*)k_gram_map, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)gram_map_lds(kGramMaxM, kGramMaxM / 2));
+                            (int)gram_map_lds(kGramMaxM, kGramMaxND));
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void*)k_lz_step_group,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lz_step_smem(4096));

@@ -440,26 +440,29 @@ struct PowerLanczosJob {
 size_t power_lanczos_lds(int m);
 hipError_t launch_power_lanczos_group(const GroupTable& g, double lz_tol, hipStream_t st);
 
-// Gram form of thin clusters (2 * segment <= m, m <= kGramMaxM): G = Yc Yc^T
-// in 64 x 64 tiles, the start u0 = Yc v0, the Lanczos stage on G, and the
-// map back to v = Yc^T g, M v = Yc^T h.  One item or job per block.
+// Gram form of thin clusters (segment nd < m, nd <= kGramMaxND, m <=
+// kGramMaxM): G = Yc Yc^T in 64 x 64 tiles, the start u0 = Yc v0, the
+// Lanczos stage on G, and the map back to v = Yc^T g, M v = Yc^T h.  One
+// item or job per block.
 constexpr int kGramMaxM = 4096;
+constexpr int kGramMaxND = 2048;
 struct GramItem {
   const uint16_t* yhi;
   const uint16_t* ylo;
-  float* G;  // nd x nd, row stride nd
+  float* G;  // n x n, row stride n
   long p0;   // first packed column of the cluster (multiple of 64)
-  int m, nd, ti, tj;
+  int m, nd, n, ti, tj;  // segment nd (multiple of 64), members n (<= nd)
 };
 struct GramVecJob {
   const uint16_t* yhi;
   const uint16_t* ylo;
   long p0;
-  int m, nd;
+  int m, nd, n;
   uint64_t seed;
   int iter, cluster;
   double* u0;         // start: Yc v0 (nd)
-  const double* gh;   // map: g (nd), then h (nd)
+  const double* g;    // map: g (nd)
+  const double* h;    // map: h (nd; may be sv: read before sv is written)
   double* v;          // map: m
   double* sv;         // map: M v (m)
   double* r;          // map: zeros (m)
@@ -482,6 +485,27 @@ hipError_t launch_gram_start(const GramVecJob* jobs, int njobs, size_t lds, hipS
 hipError_t launch_gram_lanczos(const GramLzJob* jobs, int njobs, size_t lds, double lz_tol,
                                hipStream_t st);
 hipError_t launch_gram_map(const GramVecJob* jobs, int njobs, size_t lds, hipStream_t st);
+
+// One wave per cluster of dimension d <= kLzWaveMaxD: the Lanczos
+// evaluation of the power iterate on the cluster's S (S form: centring,
+// start power_v0(seed, iter, cluster, .), r = S 1 written) or on its Gram
+// matrix (start vstart, no centring, power steps when the check never
+// passes); v = the iterate's direction, h = S v (the matrix given).
+constexpr int kLzWaveMaxD = 256;
+struct LzWaveJob {
+  const float* S;  // d x d, row stride d
+  int d, gram, I;
+  uint64_t seed;
+  int iter, cluster;
+  const double* vstart;  // Gram form
+  double* Q;             // (kLanczosMax + 1) x d
+  double* v;
+  double* r;             // S form
+  double* h;
+  int* steps;
+  int* conv;
+};
+hipError_t launch_lanczos_wave(const LzWaveJob* jobs, int njobs, double lz_tol, hipStream_t st);
 
 struct ClusterMatvecJob {
   const float* S;
