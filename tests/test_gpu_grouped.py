@@ -3,6 +3,8 @@ over all jobs in progress), through the C ABI:
 
   - a job's results do not depend on the other jobs: every job of a mixed
     batch equals the same job run alone through the executor, bit for bit;
+  - the assignment phase replayed as a HIP graph equals the same launches
+    enqueued directly, every iteration;
   - parity: the labels after every iteration equal the FP64 oracle's and the
     final centroids are within 1e-4 relative L2, over shapes that take
     FFT lengths 16..4096, the S and the Gram form of the power stage, every
@@ -77,6 +79,24 @@ def test_grouped_jobs_equal_themselves_alone(ctx):
         assert np.array_equal(r["centroids"], a["centroids"]), shape
         assert r["stats"]["iterations"] == a["stats"]["iterations"], shape
         assert r["stats"]["sbd_evals"] == a["stats"]["sbd_evals"], shape
+
+
+def test_grouped_assignment_graph_equals_direct_launches(ctx, monkeypatch):
+    """The assignment phase replayed as a HIP graph (the default) gives the
+    same labels after every iteration and the same centroids as the same
+    launches enqueued directly (TCX_GROUPED_GRAPH=0): several replays per
+    job, with the refinement's eager work between them."""
+    shapes = [(300, 30, 3), (500, 128, 5), (400, 700, 4), (2000, 96, 16), (1200, 300, 2)]
+    seeds = [31, 32, 33, 34, 35]
+    graph, _ = _run(ctx, shapes, seeds, labels_per_iter=True)
+    monkeypatch.setenv("TCX_GROUPED_GRAPH", "0")
+    direct, _ = _run(ctx, shapes, seeds, labels_per_iter=True)
+    for shape, g, d in zip(shapes, graph, direct):
+        assert g["status"] == 0 and d["status"] == 0, (g["message"], d["message"])
+        assert g["stats"]["iterations"] >= 3, shape
+        assert np.array_equal(g["labels_per_iter"], d["labels_per_iter"]), shape
+        assert np.array_equal(g["centroids"], d["centroids"]), shape
+        assert g["stats"]["sbd_evals"] == d["stats"]["sbd_evals"], shape
 
 
 def test_grouped_trajectories_match_oracle(ctx):

@@ -20,13 +20,15 @@
 //
 // Jobs are admitted while their device memory (one arena each) fits the
 // budget; a finished job's arena is released and waiting jobs join at the
-// next iteration.  No reference counterpart (the reference runs one fit per
+// next iteration.  When every arena fits the budget at once they are carved
+// from one allocation.  No reference counterpart (the reference runs one fit per
 // process, Code/test/test_kshape.cu:13-80).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -93,13 +95,13 @@ struct GJob {
   long long evals = 0, pairs = 0;
   double ms_refine = 0, ms_assign = 0;
   Clock::time_point t_admit;
-  size_t arena_bytes = 0;
+  size_t arena_bytes = 0, slab_off = 0;
   unsigned char* arena = nullptr;
   // pools (offsets into the batch's readback pools)
   size_t ginfo_off = 0, info_off = 0, evc_off = 0;
   // device buffers (inside the arena)
   int *label = nullptr, *shift = nullptr, *zero = nullptr;
-  float *dist = nullptr, *cent = nullptr, *cent_new = nullptr, *mag = nullptr, *norm = nullptr;
+  float *dist = nullptr, *cent = nullptr, *mag = nullptr, *norm = nullptr;
   float2* spec = nullptr;
   uint16_t* cm = nullptr;
   float2* xs = nullptr;
@@ -135,7 +137,6 @@ size_t layout(GJob& j, unsigned char* base) {
   take(j.shift, n);
   take(j.dist, n);
   take(j.cent, k * m);
-  take(j.cent_new, k * m);
   take(j.spec, k * F);
   take(j.mag, k * F);
   take(j.norm, k);
@@ -298,6 +299,16 @@ struct Batch {
   int* done_h = nullptr;  // Lanczos done flags of the two-launch form
   int* done_d = nullptr;
   size_t done_cap = 0;
+  // every job's arena in one allocation when they all fit the budget (no
+  // allocation or free per job: hipFree waits for the device)
+  unsigned char* slab = nullptr;
+  unsigned char* out_h = nullptr;  // pinned staging of finished jobs' results
+  size_t out_cap = 0;
+  // the assignment phase as a graph, for the set of jobs it was captured
+  // with, and that phase's own table buffer
+  hipGraphExec_t asg_exec = nullptr;
+  std::vector<GJob*> asg_set;
+  Uploader up_asg;
   std::vector<GroupSyrkItem> items;
   std::vector<GroupSyrkTile> tiles;
 
@@ -307,13 +318,16 @@ struct Batch {
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
     for (GJob& j : jobs)
-      if (j.arena) (void)hipFree(j.arena);
+      if (j.arena && !slab) (void)hipFree(j.arena);
+    if (slab) (void)hipFree(slab);
     for (unsigned char* p : {ginfo_d, info_d, evc_d})
       if (p) (void)hipFree(p);
     for (unsigned char* p : {ginfo_h, info_h, evc_h})
       if (p) (void)hipHostFree(p);
     if (done_h) (void)hipHostFree(done_h);
     if (done_d) (void)hipFree(done_d);
+    if (out_h) (void)hipHostFree(out_h);
+    if (asg_exec) (void)hipGraphExecDestroy(asg_exec);
   }
 
   long* offsets(GJob& j) { return reinterpret_cast<long*>(ginfo_d + j.ginfo_off); }
@@ -670,7 +684,7 @@ int refine_power(Batch& b, const std::vector<GJob*>& act, double* ms_power) {
     // instead of one latency-bound block per cluster
     if (j->nfat > 0) split_set.push_back(j);
     pw.add(PowerJob{j->S, j->m, b.counts(*j), j->a0, j->o.seed, j->it, steps, j->tol, j->v, j->sv,
-                    j->r, j->steps, j->conv, 0, nullptr, 1, j->cent_new, b.info(*j)},
+                    j->r, j->steps, j->conv, 0, nullptr, 1, j->cent, b.info(*j)},
            j->k, power_lds(j->m));
   }
   b.up.reset();
@@ -757,7 +771,7 @@ int refine_power(Batch& b, const std::vector<GJob*>& act, double* ms_power) {
     for (GJob* j : redo) {
       const int steps = j->m > 1000 ? 1000 : j->m;
       pe.add(PowerJob{j->S, j->m, b.counts(*j), j->a0, j->o.seed, j->it, steps, j->tol, j->v,
-                      j->sv, j->r, j->steps, j->conv, 1, j->mask, 0, j->cent_new, b.info(*j)},
+                      j->sv, j->r, j->steps, j->conv, 1, j->mask, 0, j->cent, b.info(*j)},
              j->k, power_lds(j->m));
     }
     b.up.reset();
@@ -772,17 +786,27 @@ int refine_power(Batch& b, const std::vector<GJob*>& act, double* ms_power) {
     }
   }
   *ms_power = ms_between(t0, Clock::now());
-  for (GJob* j : act) {
-    std::swap(j->cent, j->cent_new);
-    j->sum_class = j->nonempty;
-  }
+  // the new centroids were written in place (nothing in the refinement
+  // reads the old ones: the alignment takes the assignment's shifts and
+  // zero flags), so the assignment's pointers stay fixed
+  for (GJob* j : act) j->sum_class = j->nonempty;
   return TCX_OK;
 }
 
 // --------------------------------------------------------------- assign
 // Spectra of the new centroids, the bound products and the pruned
 // assignments of every job; then the changed flags and evaluation counts.
-int assign_all(Batch& b, const std::vector<GJob*>& act) {
+// The phase reads only pointers fixed for a job's life (centroids are
+// refined in place, the pair count is read on the device), so while the
+// set of jobs in progress is unchanged it is one HIP graph, captured once
+// and replayed (no memset nodes: the counters are cleared by a fill
+// kernel).  TCX_GROUPED_GRAPH=0 enqueues the launches directly instead.
+bool graphs_enabled() {
+  const char* e = std::getenv("TCX_GROUPED_GRAPH");
+  return !(e && e[0] == '0');
+}
+
+int enqueue_assign(Batch& b, const std::vector<GJob*>& act) {
   Keyed<SpectraJob> sp;
   Keyed<BoundsJob> bd;
   Keyed<PruneJob> pr;
@@ -798,35 +822,72 @@ int assign_all(Batch& b, const std::vector<GJob*>& act) {
     const int form = bounds_form(j->F, j->k);
     bd.at((long)j->F * 8 + form)
         .add(BoundsJob{j->xm, j->n, j->cm, j->k, j->kp, j->ub}, bounds_blocks(form, j->n));
-    const long npairs = j->n_pad / 2;
-    pr.at(j->F).add(PruneJob{j->job->d_x, j->xs, j->xn, j->ub, j->kp, j->n, npairs, j->m, j->k,
+    const long npairs_max = j->n_pad_max / 2;
+    pr.at(j->F).add(PruneJob{j->job->d_x, j->xs, j->xn, j->ub, j->kp, j->n, npairs_max, j->m, j->k,
                              j->spec, j->mag, j->norm, j->o.convention, j->perm, j->label,
-                             j->dist, j->shift, b.changed(*j), b.evals(*j)},
-                    prune_blocks(j->F, npairs), prune_lds(j->F, j->k));
+                             j->dist, j->shift, b.changed(*j), b.evals(*j), b.offsets(*j) + j->k},
+                    prune_blocks(j->F, npairs_max), prune_lds(j->F, j->k));
   }
-  b.up.reset();
-  sp.stage(b.up);
-  bd.stage(b.up);
-  pr.stage(b.up);
-  const size_t fill_off = b.up.put_vec(fills.jobs);
-  GX_TRY(b.up.flush(b.st));
-  for (auto& e : sp.v) {
+  std::vector<std::pair<int, const void*>> tws;
+  std::vector<long> fs;
+  for (auto& e : sp.v) fs.push_back(e.first);
+  for (auto& e : pr.v) fs.push_back(e.first);
+  for (long F : fs) {
     const void* tw;
-    GX_TRY(twiddle_table(b.ctx, (int)e.first, &tw));
-    GX_HIP(launch_spectra_group((int)e.first, e.second.table(b.up), static_cast<const float2*>(tw),
-                                b.st));
+    GX_TRY(twiddle_table(b.ctx, (int)F, &tw));  // may allocate: never under capture
+    tws.emplace_back((int)F, tw);
   }
-  GX_HIP(launch_fill_segs(reinterpret_cast<const FillSeg*>(b.up.dev(fill_off)),
-                          (int)fills.jobs.size(), (long)(kEvcBytes / sizeof(int)), b.st));
+  auto tw_of = [&](long F) {
+    for (auto& t : tws)
+      if (t.first == F) return static_cast<const float2*>(t.second);
+    return static_cast<const float2*>(nullptr);
+  };
+  b.up_asg.reset();
+  sp.stage(b.up_asg);
+  bd.stage(b.up_asg);
+  pr.stage(b.up_asg);
+  const size_t fill_off = b.up_asg.put_vec(fills.jobs);
+  GX_TRY(b.up_asg.flush(b.st));
+  const bool capture = graphs_enabled();
+  if (capture) GX_HIP(hipStreamBeginCapture(b.st, hipStreamCaptureModeThreadLocal));
+  int rc = TCX_OK;
+  auto step = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && rc == TCX_OK)
+      rc = fail(TCX_ERR_HIP, std::string("grouped assignment: ") + what + ": " +
+                                 hipGetErrorString(e));
+  };
+  for (auto& e : sp.v)
+    step(launch_spectra_group((int)e.first, e.second.table(b.up_asg), tw_of(e.first), b.st),
+         "spectra");
+  step(launch_fill_segs(reinterpret_cast<const FillSeg*>(b.up_asg.dev(fill_off)),
+                        (int)fills.jobs.size(), (long)(kEvcBytes / sizeof(int)), b.st),
+       "counter fill");
   for (auto& e : bd.v)
-    GX_HIP(launch_bounds_group((int)(e.first / 8), (int)(e.first % 8), e.second.table(b.up), b.st));
-  for (auto& e : pr.v) {
-    const void* tw;
-    GX_TRY(twiddle_table(b.ctx, (int)e.first, &tw));
-    GX_HIP(launch_assign_prune_group((int)e.first, e.second.table(b.up),
-                                     static_cast<const float2*>(tw), b.st));
+    step(launch_bounds_group((int)(e.first / 8), (int)(e.first % 8), e.second.table(b.up_asg),
+                             b.st),
+         "bounds");
+  for (auto& e : pr.v)
+    step(launch_assign_prune_group((int)e.first, e.second.table(b.up_asg), tw_of(e.first), b.st),
+         "pruned assignment");
+  step(hipMemcpyAsync(b.evc_h, b.evc_d, b.evc_bytes, hipMemcpyDeviceToHost, b.st), "readback");
+  if (capture) {
+    hipGraph_t g = nullptr;
+    step(hipStreamEndCapture(b.st, &g), "end capture");  // ends the capture in every case
+    if (rc == TCX_OK) step(hipGraphInstantiate(&b.asg_exec, g, nullptr, nullptr, 0), "instantiate");
+    if (g) (void)hipGraphDestroy(g);
   }
-  GX_HIP(hipMemcpyAsync(b.evc_h, b.evc_d, b.evc_bytes, hipMemcpyDeviceToHost, b.st));
+  return rc;
+}
+
+int assign_all(Batch& b, const std::vector<GJob*>& act) {
+  if (!b.asg_exec || act != b.asg_set) {
+    if (b.asg_exec) GX_HIP(hipGraphExecDestroy(b.asg_exec));
+    b.asg_exec = nullptr;
+    b.asg_set.clear();
+    GX_TRY(enqueue_assign(b, act));
+    if (b.asg_exec) b.asg_set = act;
+  }
+  if (b.asg_exec) GX_HIP(hipGraphLaunch(b.asg_exec, b.st));
   GX_HIP(hipStreamSynchronize(b.st));
   return TCX_OK;
 }
@@ -926,6 +987,18 @@ int grouped_fit(tcx_ctx* ctx, tcx_job* jobs, const int* idx, int count, size_t h
     budget = free_b / 10 * 8;
   }
   size_t used = 0;
+  {
+    size_t total = 0;
+    for (GJob& j : b.jobs)
+      if (!j.done) {
+        j.slab_off = total;
+        total += align_up(j.arena_bytes);
+      }
+    if (total > 0 && total <= budget && hipMalloc((void**)&b.slab, total) != hipSuccess) {
+      (void)hipGetLastError();
+      b.slab = nullptr;  // per-job allocations instead
+    }
+  }
   for (;;) {
     // admit waiting jobs while their arenas fit (largest first, so the
     // long jobs start early)
@@ -939,8 +1012,8 @@ int grouped_fit(tcx_ctx* ctx, tcx_job* jobs, const int* idx, int count, size_t h
     for (GJob& j : b.jobs) none_active &= !(j.admitted && !j.done);
     for (GJob* j : waiting) {
       if (used + j->arena_bytes > budget && !(none_active && fresh.empty())) continue;
-      void* p = nullptr;
-      if (hipMalloc(&p, j->arena_bytes) != hipSuccess) {
+      void* p = b.slab ? b.slab + j->slab_off : nullptr;
+      if (!p && hipMalloc(&p, j->arena_bytes) != hipSuccess) {
         (void)hipGetLastError();
         if (none_active && fresh.empty()) {
           j->job->status = TCX_ERR_NOMEM;
@@ -1010,14 +1083,37 @@ int grouped_fit(tcx_ctx* ctx, tcx_job* jobs, const int* idx, int count, size_t h
       if (j->done) finished.push_back(j);
     }
     if (!finished.empty()) {
+      // results through a pinned staging area (a copy into pageable host
+      // memory is staged and synchronized by the runtime, one at a time)
+      size_t need = 0;
+      for (GJob* j : finished)
+        need += align_up(sizeof(int) * j->n) + align_up(sizeof(float) * j->k * j->m);
+      if (need > b.out_cap) {
+        GX_HIP(hipStreamSynchronize(b.st));
+        if (b.out_h) GX_HIP(hipHostFree(b.out_h));
+        b.out_h = nullptr;
+        b.out_cap = 0;
+        GX_HIP(hipHostMalloc((void**)&b.out_h, need * 2, hipHostMallocDefault));
+        b.out_cap = need * 2;
+      }
+      size_t at = 0;
       for (GJob* j : finished) {
-        GX_HIP(hipMemcpyAsync(j->job->h_labels, j->label, sizeof(int) * j->n,
+        GX_HIP(hipMemcpyAsync(b.out_h + at, j->label, sizeof(int) * j->n, hipMemcpyDeviceToHost,
+                              b.st));
+        at += align_up(sizeof(int) * j->n);
+        GX_HIP(hipMemcpyAsync(b.out_h + at, j->cent, sizeof(float) * j->k * j->m,
                               hipMemcpyDeviceToHost, b.st));
-        if (j->job->h_centroids)
-          GX_HIP(hipMemcpyAsync(j->job->h_centroids, j->cent, sizeof(float) * j->k * j->m,
-                                hipMemcpyDeviceToHost, b.st));
+        at += align_up(sizeof(float) * j->k * j->m);
       }
       GX_HIP(hipStreamSynchronize(b.st));
+      at = 0;
+      for (GJob* j : finished) {
+        std::memcpy(j->job->h_labels, b.out_h + at, sizeof(int) * j->n);
+        at += align_up(sizeof(int) * j->n);
+        if (j->job->h_centroids)
+          std::memcpy(j->job->h_centroids, b.out_h + at, sizeof(float) * j->k * j->m);
+        at += align_up(sizeof(float) * j->k * j->m);
+      }
       const auto te = Clock::now();
       for (GJob* j : finished) {
         tcx_job& jb = *j->job;
@@ -1036,7 +1132,7 @@ int grouped_fit(tcx_ctx* ctx, tcx_job* jobs, const int* idx, int count, size_t h
         jb.ms_queued = ms_between(t0, j->t_admit);
         jb.ms_run = ms_between(j->t_admit, te);
         jb.ms_latency = ms_between(t0, te);
-        GX_HIP(hipFree(j->arena));
+        if (!b.slab) GX_HIP(hipFree(j->arena));
         j->arena = nullptr;
         used -= j->arena_bytes;
       }

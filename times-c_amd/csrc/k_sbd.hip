@@ -923,7 +923,8 @@ __global__ __launch_bounds__(PruneGeom<F>::kBlock, PruneGeom<F>::kMinWaves) void
     const PruneJob* __restrict__ jobs, const int* __restrict__ first, int njobs,
     const float2* __restrict__ twg) {
   TCX_GROUP_JOB(PruneJob)
-  assign_prune_body<F, true>(bid, nblk, a.x, a.xs, a.xn, a.ub, a.ub_ld, a.n, a.npairs, a.m, a.k,
+  const long npairs = a.npad ? *a.npad / 2 : a.npairs;  // the grid covers the largest count
+  assign_prune_body<F, true>(bid, nblk, a.x, a.xs, a.xn, a.ub, a.ub_ld, a.n, npairs, a.m, a.k,
                              a.cspec, a.cmag, a.cnorm, twg, a.rev, a.order, a.label, a.dist,
                              a.shift, a.changed, a.evals);
 }

@@ -324,6 +324,7 @@ struct PruneJob {
   int* shift;
   int* changed;
   unsigned long long* evals;
+  const long* npad;  // may be NULL; else npairs = *npad / 2, read on the device
 };
 long prune_blocks(int F, long npairs);
 size_t prune_lds(int F, int k);
