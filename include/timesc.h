@@ -383,6 +383,37 @@ int tcx_write_binary(const char* path, const float* data, const int* labels, int
 int tcx_read_binary(const char* path, float** data, int** labels, int* rows, int* cols);
 void tcx_free_host(void* p);
 
+/* ---------------------------------------------------------- autotuner */
+
+/* Launch shapes that do not change results, measured on the caller's data
+ * and kept in a process-wide table that every context and the grouped
+ * executor consult: the pruned assignment's grid cap per (F, ceil(k/16))
+ * (TCX_TUNE_PRUNE_CAP, keys F and k/16) and the align + pack rows per block
+ * per m rounded up to 32, m <= 1024 (TCX_TUNE_PACK_ROWS, key m).  No
+ * counterpart in the reference (its launch shapes are fixed). */
+#define TCX_TUNE_PRUNE_CAP 0
+#define TCX_TUNE_PACK_ROWS 1
+typedef struct {
+  int fft;               /* F of the tuned shape                             */
+  int prune_cap;         /* chosen grid cap (0: the built-in one was fastest) */
+  int pack_rows;         /* chosen rows per block (0: the built-in choice)   */
+  double ms_default;     /* best of reps 3-iteration fits with the built-in shapes */
+  double ms_tuned;       /* the same with the chosen shapes                  */
+} tcx_tune_report;
+/* Times 3-iteration fits of d_x (n x m, z-normalized, k clusters, seed 1)
+ * under each candidate, best of `reps`, and keeps the fastest; every
+ * candidate must reproduce the labels and centroids of the built-in shapes
+ * (else TCX_ERR_NUMERIC, table unchanged). */
+int tcx_autotune(tcx_ctx* ctx, const float* d_x, int n, int m, int k, int reps,
+                 tcx_tune_report* report);
+int tcx_tune_set(int kind, int key0, int key1, int value);
+int tcx_tune_get(int kind, int key0, int key1, int* value);
+void tcx_tune_reset(void);
+/* Table file: lines "prune <F> <k/16> <cap>" and "pack <m32> <rows>", '#'
+ * comments; load checks every entry before applying any (TCX_ERR_IO). */
+int tcx_tune_save(const char* path);
+int tcx_tune_load(const char* path);
+
 /* --------------------------------------------- synthetic data (host) */
 
 /* Seeded generator of the benchmark workloads: per class a prototype

@@ -11,6 +11,7 @@
 #include <cmath>
 
 #include "tcx_device.hpp"
+#include "tcx_internal.hpp"
 #include "tcx_kernels.hpp"
 #include "tcx_rng.hpp"
 
@@ -2744,7 +2745,7 @@ hipError_t launch_align_pack(const float* x, int m, const int* perm, long n_pad,
                              uint16_t* ylo, float* a0, hipStream_t st) {
   if (n_pad <= 0) return hipSuccess;
   if (m <= kPackLdsMaxM) {
-    const int R = pack_lds_rows(m);
+    const int R = align_pack_form(m);
     const size_t lds = sizeof(float) * (size_t)R * (m + 4);
     const dim3 grid((unsigned)(n_pad / R));
     if (R == 64)
@@ -3494,7 +3495,14 @@ hipError_t launch_row_stats_group(const GroupTable& g, hipStream_t st) {
   return hipGetLastError();
 }
 
-int align_pack_form(int m) { return m <= kPackLdsMaxM ? pack_lds_rows(m) : 0; }
+static_assert(kTunePackMaxM == kPackLdsMaxM && kTunePackMaxBytes == kPackLdsMaxBytes,
+              "the tuner's align + pack limits are the kernel's");
+
+int align_pack_form(int m) {
+  if (m > kPackLdsMaxM) return 0;
+  const int tuned = tuned_pack_rows(m);  // every column is packed alike in any block
+  return tuned > 0 ? tuned : pack_lds_rows(m);
+}
 long align_pack_cols(int form) { return form ? form : 64; }
 size_t align_pack_lds(int form, int m) { return form ? sizeof(float) * (size_t)form * (m + 4) : 0; }
 

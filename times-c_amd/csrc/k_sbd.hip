@@ -11,6 +11,7 @@
 #include <cmath>
 
 #include "tcx_device.hpp"
+#include "tcx_internal.hpp"
 #include "tcx_kernels.hpp"
 
 namespace tcx {
@@ -981,9 +982,11 @@ static int prune_max_k_F() {
 template <int F>
 static hipError_t launch_prune_F(const PruneArgs& a, hipStream_t st) {
   using G = PruneGeom<F>;
+  const int tuned = tuned_prune_cap(F, a.k);  // the grid changes no result
+  const long cap = tuned > 0 ? tuned : 8192;
   long blocks = (a.npairs + G::kGroups - 1) / G::kGroups;
   if (blocks < 1) blocks = 1;
-  if (blocks > 8192) blocks = 8192;
+  if (blocks > cap) blocks = cap;
   if (a.xs && a.ub)
     hipLaunchKernelGGL((k_assign_prune<F, true>), dim3((unsigned)blocks), dim3(G::kBlock),
                        prune_smem<F>(a.k), st, a.x, a.xs, a.xn, a.ub, a.ub_ld, a.n, a.npairs, a.m,
@@ -1251,12 +1254,14 @@ long spectra_blocks(int F, long rows) {
   }
 }
 
-long prune_blocks(int F, long npairs) {
+long prune_blocks(int F, long npairs, int k) {
+  const int tuned = tuned_prune_cap(F, k);  // the grid changes no result
+  const long cap = tuned > 0 ? tuned : 8192;
   switch (F) {
 #define X(FF)                                                                   \
   case FF: {                                                                    \
     long b = (npairs + PruneGeom<FF>::kGroups - 1) / PruneGeom<FF>::kGroups;     \
-    return b < 1 ? 1 : (b > 8192 ? 8192 : b);                                   \
+    return b < 1 ? 1 : (b > cap ? cap : b);                                     \
   }
     TCX_FOR_EACH_F(X)
 #undef X

@@ -40,7 +40,22 @@ EXPORTS = [
     "tcx_read_labeled_csv", "tcx_read_unlabeled_csv", "tcx_write_labels",
     "tcx_write_labels_lines", "tcx_write_binary", "tcx_read_binary", "tcx_free_host",
     "tcx_generate", "tcx_init_labels", "tcx_init_rows", "tcx_power_v0",
+    "tcx_autotune", "tcx_tune_set", "tcx_tune_get", "tcx_tune_reset", "tcx_tune_save",
+    "tcx_tune_load",
 ]
+
+TUNE_PRUNE_CAP, TUNE_PACK_ROWS = 0, 1
+
+
+class TuneReport(ctypes.Structure):
+    """tcx_tune_report."""
+    _fields_ = [
+        ("fft", ctypes.c_int),
+        ("prune_cap", ctypes.c_int),
+        ("pack_rows", ctypes.c_int),
+        ("ms_default", ctypes.c_double),
+        ("ms_tuned", ctypes.c_double),
+    ]
 
 
 class TimescError(RuntimeError):
@@ -227,6 +242,12 @@ def load():
         "tcx_init_labels": (_I, [ctypes.c_uint64, _L, _I, _P]),
         "tcx_init_rows": (_I, [ctypes.c_uint64, _L, _I, _P]),
         "tcx_power_v0": (None, [ctypes.c_uint64, _I, _I, _I, _P]),
+        "tcx_autotune": (_I, [_P, _P, _I, _I, _I, _I, ctypes.POINTER(TuneReport)]),
+        "tcx_tune_set": (_I, [_I, _I, _I, _I]),
+        "tcx_tune_get": (_I, [_I, _I, _I, ctypes.POINTER(_I)]),
+        "tcx_tune_reset": (None, []),
+        "tcx_tune_save": (_I, [ctypes.c_char_p]),
+        "tcx_tune_load": (_I, [ctypes.c_char_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -366,6 +387,30 @@ class DeviceArray:
     def free(self):
         if self.ptr:
             self.ctx.free(self)
+
+
+def tune_set(kind, key0, key1, value):
+    """One table entry: (TUNE_PRUNE_CAP, F, ceil(k/16), cap) or
+    (TUNE_PACK_ROWS, m rounded up to 32, 0, rows)."""
+    check(load().tcx_tune_set(kind, key0, key1, value))
+
+
+def tune_get(kind, key0, key1=0):
+    v = ctypes.c_int()
+    check(load().tcx_tune_get(kind, key0, key1, ctypes.byref(v)))
+    return v.value
+
+
+def tune_reset():
+    load().tcx_tune_reset()
+
+
+def tune_save(path):
+    check(load().tcx_tune_save(str(path).encode()))
+
+
+def tune_load(path):
+    check(load().tcx_tune_load(str(path).encode()))
 
 
 class Context:
@@ -530,6 +575,14 @@ class Context:
         st = FitStats()
         check(self.lib.tcx_fit_end(self.h, _ptr(labels), _ptr(cent), ctypes.byref(st)))
         return {"labels": labels, "centroids": cent, "stats": st.as_dict()}
+
+    def autotune(self, dx, k, reps=3):
+        """tcx_autotune on the z-normalized series dx: the launch shapes that
+        do not change results, timed on 3-iteration fits (best of reps)."""
+        n, m = dx.shape
+        r = TuneReport()
+        check(self.lib.tcx_autotune(self.h, dx.ptr, n, m, k, reps, ctypes.byref(r)))
+        return {name: getattr(r, name) for name, _ in r._fields_}
 
     def fit(self, dx, k, **kw):
         """Full fit (tcx_kshape_fit).  Options: max_iter, init_mode, seed,
