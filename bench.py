@@ -39,6 +39,7 @@ PEAK_HBM_GBS = 8000.0
 # plain 256 MiB copy: profiles/kshape_iteration_n100k_m512_k16.md
 ASSIGN_TRAFFIC_BYTES = {(100000, 512, 16): (444021 + 9992) * 1024}
 ITERS = 10          # iterations of the configured fit (test_timesC.py:221)
+TUNE_TABLE = os.path.join(ROOT, "times-c_amd", "tune_mi355x.txt")
 ERR_SAMPLE = 2000   # rows of the SBD-error sample
 
 
@@ -349,6 +350,34 @@ def run_serve(timesc, models, reqs, qps, fit_workers, query_workers):
     }
 
 
+def grouped_batch_only(timesc, njobs):
+    """The uneven batch through the grouped executor alone (one untimed
+    one-iteration run, then the timed run), for profiling its kernels
+    without the other workloads in the trace; prints one JSON line."""
+    jobs = batch_workload(timesc, njobs)
+    ctx = timesc.Context(0)
+    ds = [ctx.to_device(X) for X, _, _ in jobs]
+    try:
+        with timesc.Batch(0, workers=1) as b:
+            for it in (1, ITERS):
+                for d, (X, _, _) in zip(ds, jobs):
+                    ctx.h2d(d, X)
+                t0 = time.perf_counter()
+                res = b.run([{"d": d, "k": k, "seed": seed, "znorm_ddof": 0, "max_iter": it}
+                             for d, (_, k, seed) in zip(ds, jobs)], grouped=True)
+                wall = time.perf_counter() - t0
+    finally:
+        for d in ds:
+            d.free()
+        ctx.close()
+    assert all(r["status"] == 0 for r in res), "a batch job failed"
+    print(json.dumps({"grouped_batch_jobs": njobs, "makespan_ms": wall * 1e3,
+                      "jobs_per_s": njobs / wall,
+                      "mean_iterations": float(np.mean([r["stats"]["iterations"]
+                                                        for r in res]))}))
+    return 0
+
+
 def copy_calibration(mib):
     """A plain copy of `mib` MiB (read once, written once), three launches."""
     import torch
@@ -454,6 +483,11 @@ def main():
     ap.add_argument("--cpu-one-core-sample", type=int, default=10000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-steady", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="built-in launch shapes")
+    ap.add_argument("--retune", action="store_true",
+                    help="tune the launch shapes in this run instead of loading the table")
+    ap.add_argument("--grouped-batch-only", action="store_true",
+                    help="only the uneven batch through the grouped executor (profiling)")
     ap.add_argument("--batch-jobs", type=int, default=256)
     ap.add_argument("--batch-workers", type=int, default=8)
     ap.add_argument("--no-batch", action="store_true")
@@ -471,12 +505,29 @@ def main():
         return copy_calibration(args.copy_calibration)
     import timesc
 
+    if args.grouped_batch_only:
+        if not args.no_tune and os.path.exists(TUNE_TABLE):
+            timesc.tune_load(TUNE_TABLE)
+        return grouped_batch_only(timesc, args.batch_jobs)
+
     n, m, k = args.n, args.m, args.k
     print(f"# generating n={n} m={m} k={k}", file=sys.stderr, flush=True)
     X, y = timesc.generate(args.seed, n, m, k, 0.5)
     ctx = timesc.Context(0)
     dx = ctx.to_device(X)
     ctx.znorm(dx, ddof=0)
+    # launch shapes that do not change results: the committed table of this
+    # GPU (times-c_amd/tune_mi355x.py wrote it), or a fresh tuning run
+    tuning = None
+    F = fft_size(m)
+    if args.retune:
+        print("# autotuning the launch shapes", file=sys.stderr, flush=True)
+        tuning = dict(ctx.autotune(dx, k, reps=3), source="tcx_autotune in this run")
+    elif not args.no_tune and os.path.exists(TUNE_TABLE):
+        timesc.tune_load(TUNE_TABLE)
+        tuning = {"source": os.path.relpath(TUNE_TABLE, ROOT), "fft": F,
+                  "prune_cap": timesc.tune_get(timesc.TUNE_PRUNE_CAP, F, (k + 15) // 16),
+                  "pack_rows": timesc.tune_get(timesc.TUNE_PACK_ROWS, m)}
     for _ in range(args.warmup):
         timed_fit(ctx, dx, n, m, k, args.seed)
     print(f"# warmup done ({args.warmup} fits)", file=sys.stderr, flush=True)
@@ -530,6 +581,7 @@ def main():
         "assign_pairs_per_s": n * k * iters / (sum(f["stats"]["ms_assign"] for f in fits) * 1e-3),
         "assign_eval_fraction": evals / (iters * ((n + 1) // 2) * k),
         "power_steps_max": st["power_steps_max"],
+        "tuning": tuning,
         "iteration_roofline": {
             "t_roof_us": roof_s * 1e6,
             "frac": roof_s / (ms_iter * 1e-3),
