@@ -36,8 +36,8 @@ PEAK_BF16_TFLOPS = 2516.6
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per k_assign_prune launch at n=100000, m=512, k=16, from
 # rocprofv3 FETCH_SIZE (x2) and WRITE_SIZE (x1) counter runs calibrated on a
-# plain 256 MiB copy: profiles/kshape_iteration_n100k_m512_k16.md
-ASSIGN_TRAFFIC_BYTES = {(100000, 512, 16): (444021 + 9992) * 1024}
+# plain 256 MiB copy: profiles/kshape_fit_n100k_m512_k16.md
+ASSIGN_TRAFFIC_BYTES = {(100000, 512, 16): (446166 + 9880) * 1024}
 ITERS = 10          # iterations of the configured fit (test_timesC.py:221)
 TUNE_TABLE = os.path.join(ROOT, "times-c_amd", "tune_mi355x.txt")
 ERR_SAMPLE = 2000   # rows of the SBD-error sample
@@ -526,7 +526,8 @@ def main():
     elif not args.no_tune and os.path.exists(TUNE_TABLE):
         timesc.tune_load(TUNE_TABLE)
         tuning = {"source": os.path.relpath(TUNE_TABLE, ROOT), "fft": F,
-                  "prune_cap": timesc.tune_get(timesc.TUNE_PRUNE_CAP, F, (k + 15) // 16),
+                  "prune_cap": timesc.tune_get(timesc.TUNE_PRUNE_CAP, F, (k + 15) // 16,
+                                               n.bit_length() - 1),
                   "pack_rows": timesc.tune_get(timesc.TUNE_PACK_ROWS, m)}
     for _ in range(args.warmup):
         timed_fit(ctx, dx, n, m, k, args.seed)

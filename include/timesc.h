@@ -387,10 +387,11 @@ void tcx_free_host(void* p);
 
 /* Launch shapes that do not change results, measured on the caller's data
  * and kept in a process-wide table that every context and the grouped
- * executor consult: the pruned assignment's grid cap per (F, ceil(k/16))
- * (TCX_TUNE_PRUNE_CAP, keys F and k/16) and the align + pack rows per block
- * per m rounded up to 32, m <= 1024 (TCX_TUNE_PACK_ROWS, key m).  No
- * counterpart in the reference (its launch shapes are fixed). */
+ * executor consult: the pruned assignment's grid cap per (F, ceil(k/16),
+ * floor(log2 n)) (TCX_TUNE_PRUNE_CAP, keys F, k/16, log2 n) and the
+ * align + pack rows per block per m rounded up to 32, m <= 1024
+ * (TCX_TUNE_PACK_ROWS, key m; the other keys 0).  No counterpart in the
+ * reference (its launch shapes are fixed). */
 #define TCX_TUNE_PRUNE_CAP 0
 #define TCX_TUNE_PACK_ROWS 1
 typedef struct {
@@ -406,11 +407,12 @@ typedef struct {
  * (else TCX_ERR_NUMERIC, table unchanged). */
 int tcx_autotune(tcx_ctx* ctx, const float* d_x, int n, int m, int k, int reps,
                  tcx_tune_report* report);
-int tcx_tune_set(int kind, int key0, int key1, int value);
-int tcx_tune_get(int kind, int key0, int key1, int* value);
+int tcx_tune_set(int kind, int key0, int key1, int key2, int value);
+int tcx_tune_get(int kind, int key0, int key1, int key2, int* value);
 void tcx_tune_reset(void);
-/* Table file: lines "prune <F> <k/16> <cap>" and "pack <m32> <rows>", '#'
- * comments; load checks every entry before applying any (TCX_ERR_IO). */
+/* Table file: lines "prune <F> <k/16> <log2 n> <cap>" and "pack <m32>
+ * <rows>", '#' comments; load checks every entry before applying any
+ * (TCX_ERR_IO). */
 int tcx_tune_save(const char* path);
 int tcx_tune_load(const char* path);
 

@@ -52,9 +52,10 @@ def test_forced_shapes_equal_builtin(ctx, n, m, k):
         for cap in (64, 100000):
             for r in rows or [0]:
                 T.tune_reset()
-                T.tune_set(T.TUNE_PRUNE_CAP, F, (k + 15) // 16, cap)
+                for nn, kk in ((n, k), (700, 3)):  # the fit and both grouped jobs
+                    T.tune_set(T.TUNE_PRUNE_CAP, F, (kk + 15) // 16, nn.bit_length() - 1, cap)
                 if r:
-                    T.tune_set(T.TUNE_PACK_ROWS, m32, 0, r)
+                    T.tune_set(T.TUNE_PACK_ROWS, m32, 0, 0, r)
                 print(f"n={n} m={m} k={k}: cap {cap}, rows {r}", flush=True)
                 _same(_fit(ctx, X, k), ref)
                 for a, b in zip(_grouped(ctx, data), ref_g):
@@ -80,13 +81,14 @@ def test_autotune_choices_reproduce_default(ctx, tmp_path):
         assert rep["prune_cap"] in (0, 1024, 2048, 4096, 8192, 16384, 32768)
         assert rep["pack_rows"] in (0, 16, 32, 64)
         assert 0 < rep["ms_tuned"] <= rep["ms_default"]
-        assert T.tune_get(T.TUNE_PRUNE_CAP, rep["fft"], 1) == rep["prune_cap"]
+        nb = n.bit_length() - 1
+        assert T.tune_get(T.TUNE_PRUNE_CAP, rep["fft"], 1, nb) == rep["prune_cap"]
         assert T.tune_get(T.TUNE_PACK_ROWS, m) == rep["pack_rows"]
         _same(_fit(ctx, X, k), ref)
         path = tmp_path / "tune.txt"
         T.tune_save(path)
         T.tune_reset()
         T.tune_load(path)
-        assert T.tune_get(T.TUNE_PRUNE_CAP, rep["fft"], 1) == rep["prune_cap"]
+        assert T.tune_get(T.TUNE_PRUNE_CAP, rep["fft"], 1, nb) == rep["prune_cap"]
     finally:
         T.tune_reset()

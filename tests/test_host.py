@@ -243,32 +243,33 @@ def test_tune_table_roundtrip_and_validation(tmp_path):
     table unchanged."""
     T.tune_reset()
     try:
-        T.tune_set(T.TUNE_PRUNE_CAP, 1024, 1, 4096)
-        T.tune_set(T.TUNE_PACK_ROWS, 512, 0, 16)
-        assert T.tune_get(T.TUNE_PRUNE_CAP, 1024, 1) == 4096
+        T.tune_set(T.TUNE_PRUNE_CAP, 1024, 1, 16, 4096)
+        T.tune_set(T.TUNE_PACK_ROWS, 512, 0, 0, 16)
+        assert T.tune_get(T.TUNE_PRUNE_CAP, 1024, 1, 16) == 4096
         assert T.tune_get(T.TUNE_PACK_ROWS, 500) == 16  # m rounded up to 32
-        assert T.tune_get(T.TUNE_PRUNE_CAP, 2048, 1) == 0  # no entry: built-in
+        assert T.tune_get(T.TUNE_PRUNE_CAP, 2048, 1, 16) == 0  # no entry: built-in
+        assert T.tune_get(T.TUNE_PRUNE_CAP, 1024, 1, 11) == 0  # another size bucket
         path = tmp_path / "tune.txt"
         T.tune_save(path)
         text = path.read_text()
-        assert "prune 1024 1 4096" in text and "pack 512 16" in text
+        assert "prune 1024 1 16 4096" in text and "pack 512 16" in text
         T.tune_reset()
-        assert T.tune_get(T.TUNE_PRUNE_CAP, 1024, 1) == 0
+        assert T.tune_get(T.TUNE_PRUNE_CAP, 1024, 1, 16) == 0
         T.tune_load(path)
-        assert T.tune_get(T.TUNE_PRUNE_CAP, 1024, 1) == 4096
+        assert T.tune_get(T.TUNE_PRUNE_CAP, 1024, 1, 16) == 4096
         assert T.tune_get(T.TUNE_PACK_ROWS, 512) == 16
-        for bad in ["prune 1000 1 4096\n",        # F not a power of two
+        for bad in ["prune 1000 1 16 4096\n",     # F not a power of two
                     "pack 512 0 24\n",             # rows not 16/32/64
                     "pack 1024 64\n",              # LDS of 64 rows at m = 1024 too large
                     "pack 2048 16\n",              # m beyond the LDS form
-                    "prune 1024 1\n", "tile 3 4 5\n"]:
+                    "prune 1024 1 16\n", "tile 3 4 5\n"]:
             p = tmp_path / "bad.txt"
-            p.write_text("prune 4096 2 2048\n" + bad)
+            p.write_text("prune 4096 2 12 2048\n" + bad)
             with pytest.raises(T.TimescError):
                 T.tune_load(p)
-            assert T.tune_get(T.TUNE_PRUNE_CAP, 4096, 2) == 0  # nothing applied
+            assert T.tune_get(T.TUNE_PRUNE_CAP, 4096, 2, 12) == 0  # nothing applied
         with pytest.raises(T.TimescError):
-            T.tune_set(T.TUNE_PACK_ROWS, 100, 0, 16)  # key not a multiple of 32
+            T.tune_set(T.TUNE_PACK_ROWS, 100, 0, 0, 16)  # key not a multiple of 32
         with pytest.raises(T.TimescError):
             T.tune_load(tmp_path / "missing.txt")
     finally:

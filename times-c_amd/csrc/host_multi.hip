@@ -826,7 +826,8 @@ int enqueue_assign(Batch& b, const std::vector<GJob*>& act) {
     pr.at(j->F).add(PruneJob{j->job->d_x, j->xs, j->xn, j->ub, j->kp, j->n, npairs_max, j->m, j->k,
                              j->spec, j->mag, j->norm, j->o.convention, j->perm, j->label,
                              j->dist, j->shift, b.changed(*j), b.evals(*j), b.offsets(*j) + j->k},
-                    prune_blocks(j->F, npairs_max, j->k), prune_lds(j->F, j->k));
+                    prune_blocks(j->F, npairs_max, j->k, j->n),
+                    prune_lds(j->F, j->k));
   }
   std::vector<std::pair<int, const void*>> tws;
   std::vector<long> fs;

@@ -3,15 +3,15 @@
 // Two choices, each measured on the caller's data with whole fits (host
 // clock around tcx_kshape_fit, which ends synchronized) and kept in a
 // process-wide table that every context and the grouped executor consult:
-//   - the pruned assignment's grid cap, per (F, ceil(k/16)): the kernel
-//     strides over series pairs, each pair evaluated by one group, so the
-//     grid changes only which block takes a pair;
+//   - the pruned assignment's grid cap, per (F, ceil(k/16), floor(log2 n)):
+//     the kernel strides over series pairs, each pair evaluated by one
+//     group, so the grid changes only which block takes a pair;
 //   - the align + pack rows per block (R = 16, 32, 64), per m rounded up to
 //     32 (m <= 1024): each packed column is computed the same way whatever
 //     block holds it.
 // The tuner checks that every candidate gives the labels and centroids of
 // the default, and fails otherwise.  Table file: one entry per line,
-// "prune <F> <kb> <cap>" or "pack <m32> <rows>"; '#' starts a comment.
+// "prune <F> <kb> <nb> <cap>" or "pack <m32> <rows>"; '#' starts a comment.
 // No reference counterpart (the reference has fixed launch shapes).
 #include <algorithm>
 #include <chrono>
@@ -20,6 +20,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -31,10 +32,18 @@ namespace tcx {
 namespace {
 
 std::mutex g_tune_mu;
-std::map<std::pair<int, int>, int> g_prune_cap;  // (F, kb) -> grid cap
+std::map<std::tuple<int, int, int>, int> g_prune_cap;  // (F, kb, nb) -> grid cap
 std::map<int, int> g_pack_rows;                  // m rounded up to 32 -> rows
 
 int kbucket(int k) { return (k + 15) / 16; }
+int nbucket(long n) {
+  int b = 0;
+  while (n > 1) {
+    n >>= 1;
+    ++b;
+  }
+  return b;
+}
 int mbucket(int m) { return (m + 31) / 32 * 32; }
 
 bool valid_cap(int cap) { return cap >= 64 && cap <= (1 << 20); }
@@ -65,9 +74,9 @@ int fit_ms(tcx_ctx* ctx, const float* d_x, int n, int m, int k, int iters, int r
 
 }  // namespace
 
-int tuned_prune_cap(int F, int k) {
+int tuned_prune_cap(int F, int k, long n) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  auto it = g_prune_cap.find({F, kbucket(k)});
+  auto it = g_prune_cap.find({F, kbucket(k), nbucket(n)});
   return it == g_prune_cap.end() ? 0 : it->second;
 }
 
@@ -89,12 +98,13 @@ void tcx_tune_reset(void) {
   g_pack_rows.clear();
 }
 
-int tcx_tune_set(int kind, int key0, int key1, int value) {
+int tcx_tune_set(int kind, int key0, int key1, int key2, int value) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   if (kind == TCX_TUNE_PRUNE_CAP) {
-    if (key0 < 4 || (key0 & (key0 - 1)) != 0 || key1 < 1 || !valid_cap(value))
+    if (key0 < 4 || (key0 & (key0 - 1)) != 0 || key1 < 1 || key2 < 0 || key2 > 40 ||
+        !valid_cap(value))
       return fail(TCX_ERR_ARG, "tune_set: bad pruned-assignment entry");
-    g_prune_cap[{key0, key1}] = value;
+    g_prune_cap[{key0, key1, key2}] = value;
     return TCX_OK;
   }
   if (kind == TCX_TUNE_PACK_ROWS) {
@@ -106,10 +116,11 @@ int tcx_tune_set(int kind, int key0, int key1, int value) {
   return fail(TCX_ERR_ARG, "tune_set: unknown kind");
 }
 
-int tcx_tune_get(int kind, int key0, int key1, int* value) {
+int tcx_tune_get(int kind, int key0, int key1, int key2, int* value) {
   if (!value) return fail(TCX_ERR_ARG, "tune_get: null value");
-  *value = kind == TCX_TUNE_PRUNE_CAP ? tuned_prune_cap(key0, key1 * 16)
-                                      : (kind == TCX_TUNE_PACK_ROWS ? tuned_pack_rows(key0) : -1);
+  *value = kind == TCX_TUNE_PRUNE_CAP
+               ? tuned_prune_cap(key0, key1 * 16, key2 >= 0 && key2 < 62 ? 1L << key2 : 0)
+               : (kind == TCX_TUNE_PACK_ROWS ? tuned_pack_rows(key0) : -1);
   return *value < 0 ? fail(TCX_ERR_ARG, "tune_get: unknown kind") : TCX_OK;
 }
 
@@ -120,7 +131,8 @@ int tcx_tune_save(const char* path) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   std::fprintf(f, "# times-c_amd launch-shape table\n");
   for (auto& e : g_prune_cap)
-    std::fprintf(f, "prune %d %d %d\n", e.first.first, e.first.second, e.second);
+    std::fprintf(f, "prune %d %d %d %d\n", std::get<0>(e.first), std::get<1>(e.first),
+                 std::get<2>(e.first), e.second);
   for (auto& e : g_pack_rows) std::fprintf(f, "pack %d %d\n", e.first, e.second);
   const bool ok = std::fclose(f) == 0;
   return ok ? TCX_OK : fail(TCX_ERR_IO, std::string("tune_save: write failed: ") + path);
@@ -138,13 +150,13 @@ int tcx_tune_load(const char* path) {
     char* h = std::strchr(line, '#');
     if (h) *h = '\0';
     char kind[16];
-    int a = 0, b = 0, c = 0;
-    const int got = std::sscanf(line, "%15s %d %d %d", kind, &a, &b, &c);
+    int a = 0, b = 0, c = 0, e = 0;
+    const int got = std::sscanf(line, "%15s %d %d %d %d", kind, &a, &b, &c, &e);
     if (got <= 0) continue;  // blank
-    if (got == 4 && std::strcmp(kind, "prune") == 0) {
-      rows.push_back({TCX_TUNE_PRUNE_CAP, a, b, c});
+    if (got == 5 && std::strcmp(kind, "prune") == 0) {
+      rows.push_back({TCX_TUNE_PRUNE_CAP, a, b, c, e});
     } else if (got == 3 && std::strcmp(kind, "pack") == 0) {
-      rows.push_back({TCX_TUNE_PACK_ROWS, a, 0, b});
+      rows.push_back({TCX_TUNE_PACK_ROWS, a, 0, 0, b});
     } else {
       rc = fail(TCX_ERR_IO, "tune_load: bad line " + std::to_string(lineno) + " of " + path);
       break;
@@ -157,12 +169,13 @@ int tcx_tune_load(const char* path) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (auto& r : rows) {
       const bool ok = r[0] == TCX_TUNE_PRUNE_CAP
-                          ? (r[1] >= 4 && (r[1] & (r[1] - 1)) == 0 && r[2] >= 1 && valid_cap(r[3]))
-                          : (r[1] >= 32 && r[1] % 32 == 0 && valid_rows(r[1], r[3]));
+                          ? (r[1] >= 4 && (r[1] & (r[1] - 1)) == 0 && r[2] >= 1 && r[3] >= 0 &&
+                             r[3] <= 40 && valid_cap(r[4]))
+                          : (r[1] >= 32 && r[1] % 32 == 0 && valid_rows(r[1], r[4]));
       if (!ok) return fail(TCX_ERR_IO, std::string("tune_load: invalid entry in ") + path);
     }
   }
-  for (auto& r : rows) tcx_tune_set(r[0], r[1], r[2], r[3]);
+  for (auto& r : rows) tcx_tune_set(r[0], r[1], r[2], r[3], r[4]);
   return TCX_OK;
 }
 
@@ -171,20 +184,20 @@ int tcx_autotune(tcx_ctx* ctx, const float* d_x, int n, int m, int k, int reps,
   if (!ctx || !d_x || n < 2 || m < 2 || k < 1 || reps < 1)
     return fail(TCX_ERR_ARG, "autotune: bad arguments");
   const int F = fft_length(m);
-  const int kb = (k + 15) / 16;
+  const int kb = (k + 15) / 16, nb = nbucket(n);
   const int iters = 3;
   std::vector<int> lab0, lab;
   std::vector<float> c0, c;
   // the shapes in force before tuning, as the reference for equality
-  int prev_cap = tuned_prune_cap(F, k), prev_rows = tuned_pack_rows(m);
+  int prev_cap = tuned_prune_cap(F, k, n), prev_rows = tuned_pack_rows(m);
   auto restore = [&]() {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    if (prev_cap) g_prune_cap[{F, kb}] = prev_cap; else g_prune_cap.erase({F, kb});
+    if (prev_cap) g_prune_cap[{F, kb, nb}] = prev_cap; else g_prune_cap.erase({F, kb, nb});
     if (prev_rows) g_pack_rows[mbucket(m)] = prev_rows; else g_pack_rows.erase(mbucket(m));
   };
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_prune_cap.erase({F, kb});
+    g_prune_cap.erase({F, kb, nb});
     g_pack_rows.erase(mbucket(m));
   }
   double ms_default = 0;
@@ -204,7 +217,7 @@ int tcx_autotune(tcx_ctx* ctx, const float* d_x, int n, int m, int k, int reps,
     if (cap > 2 * npairs) break;
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
-      g_prune_cap[{F, kb}] = cap;
+      g_prune_cap[{F, kb, nb}] = cap;
     }
     double ms = 0;
     rc = fit_ms(ctx, d_x, n, m, k, iters, reps, lab, c, &ms);
@@ -221,7 +234,7 @@ int tcx_autotune(tcx_ctx* ctx, const float* d_x, int n, int m, int k, int reps,
   }
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    if (best_cap) g_prune_cap[{F, kb}] = best_cap; else g_prune_cap.erase({F, kb});
+    if (best_cap) g_prune_cap[{F, kb, nb}] = best_cap; else g_prune_cap.erase({F, kb, nb});
   }
   // align + pack rows per block
   int best_rows = 0;

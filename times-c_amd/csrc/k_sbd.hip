@@ -982,7 +982,7 @@ static int prune_max_k_F() {
 template <int F>
 static hipError_t launch_prune_F(const PruneArgs& a, hipStream_t st) {
   using G = PruneGeom<F>;
-  const int tuned = tuned_prune_cap(F, a.k);  // the grid changes no result
+  const int tuned = tuned_prune_cap(F, a.k, a.n);  // the grid changes no result
   const long cap = tuned > 0 ? tuned : 8192;
   long blocks = (a.npairs + G::kGroups - 1) / G::kGroups;
   if (blocks < 1) blocks = 1;
@@ -1254,8 +1254,8 @@ long spectra_blocks(int F, long rows) {
   }
 }
 
-long prune_blocks(int F, long npairs, int k) {
-  const int tuned = tuned_prune_cap(F, k);  // the grid changes no result
+long prune_blocks(int F, long npairs, int k, long n) {
+  const int tuned = tuned_prune_cap(F, k, n);  // the grid changes no result
   const long cap = tuned > 0 ? tuned : 8192;
   switch (F) {
 #define X(FF)                                                                   \

@@ -10,9 +10,9 @@
 namespace tcx {
 
 // Launch shapes chosen by the autotuner (0: the built-in choice): the
-// pruned assignment's grid cap for (F, k), the align + pack rows per block
-// for m.  Neither changes results.
-int tuned_prune_cap(int F, int k);
+// pruned assignment's grid cap for (F, k, n), the align + pack rows per
+// block for m.  Neither changes results.
+int tuned_prune_cap(int F, int k, long n);
 int tuned_pack_rows(int m);
 constexpr int kTunePackMaxM = 1024;          // align + pack from LDS up to this m
 constexpr int kTunePackMaxBytes = 72 * 1024;  // its LDS per block at most

@@ -326,7 +326,7 @@ struct PruneJob {
   unsigned long long* evals;
   const long* npad;  // may be NULL; else npairs = *npad / 2, read on the device
 };
-long prune_blocks(int F, long npairs, int k);  // the grid cap may be tuned per (F, k)
+long prune_blocks(int F, long npairs, int k, long n);  // grid cap tuned per (F, k, n)
 size_t prune_lds(int F, int k);
 hipError_t launch_assign_prune_group(int F, const GroupTable& g, const float2* tw, hipStream_t st);
 

@@ -243,8 +243,8 @@ def load():
         "tcx_init_rows": (_I, [ctypes.c_uint64, _L, _I, _P]),
         "tcx_power_v0": (None, [ctypes.c_uint64, _I, _I, _I, _P]),
         "tcx_autotune": (_I, [_P, _P, _I, _I, _I, _I, ctypes.POINTER(TuneReport)]),
-        "tcx_tune_set": (_I, [_I, _I, _I, _I]),
-        "tcx_tune_get": (_I, [_I, _I, _I, ctypes.POINTER(_I)]),
+        "tcx_tune_set": (_I, [_I, _I, _I, _I, _I]),
+        "tcx_tune_get": (_I, [_I, _I, _I, _I, ctypes.POINTER(_I)]),
         "tcx_tune_reset": (None, []),
         "tcx_tune_save": (_I, [ctypes.c_char_p]),
         "tcx_tune_load": (_I, [ctypes.c_char_p]),
@@ -389,15 +389,15 @@ class DeviceArray:
             self.ctx.free(self)
 
 
-def tune_set(kind, key0, key1, value):
-    """One table entry: (TUNE_PRUNE_CAP, F, ceil(k/16), cap) or
-    (TUNE_PACK_ROWS, m rounded up to 32, 0, rows)."""
-    check(load().tcx_tune_set(kind, key0, key1, value))
+def tune_set(kind, key0, key1, key2, value):
+    """One table entry: (TUNE_PRUNE_CAP, F, ceil(k/16), floor(log2 n), cap)
+    or (TUNE_PACK_ROWS, m rounded up to 32, 0, 0, rows)."""
+    check(load().tcx_tune_set(kind, key0, key1, key2, value))
 
 
-def tune_get(kind, key0, key1=0):
+def tune_get(kind, key0, key1=0, key2=0):
     v = ctypes.c_int()
-    check(load().tcx_tune_get(kind, key0, key1, ctypes.byref(v)))
+    check(load().tcx_tune_get(kind, key0, key1, key2, ctypes.byref(v)))
     return v.value
 
 
