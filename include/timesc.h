@@ -43,6 +43,9 @@ extern "C" {
 /* precision of S = Y^T Y in shape extraction */
 #define TCX_S_SPLIT_BF16 0 /* hi+lo BF16 MFMA, FP32 accumulation (default)      */
 #define TCX_S_BF16 1       /* single BF16 MFMA product, FP32 accumulation        */
+#define TCX_S_AUTO 2       /* split-BF16 until a refinement's BF16 centroids are
+                              within s_budget of its split ones, then BF16 for
+                              the rest of the fit (checked at iterations 2..4)  */
 
 typedef struct tcx_ctx tcx_ctx;
 
@@ -152,7 +155,7 @@ typedef struct {
   const int* init_labels;/* optional host labels (n, zero-based), mode 0     */
   const int* init_rows;  /* optional host row indices (k), mode 1            */
   int convention;        /* TCX_CONV_CPU (default) or TCX_CONV_KSHAPE         */
-  int s_precision;       /* TCX_S_SPLIT_BF16 (default) or TCX_S_BF16          */
+  int s_precision;       /* TCX_S_SPLIT_BF16 (default), TCX_S_BF16, TCX_S_AUTO */
   int cal_select;        /* 0: power iteration, 1: top eigenvector (eigh)      */
   double power_tol;      /* relative Rayleigh-quotient tolerance, 0 -> 1e-12 */
   int* labels_per_iter;  /* optional host buffer (max_iter*n): labels after
@@ -163,6 +166,9 @@ typedef struct {
                             of the previous assignment (the GPU path's break,
                             kshape.cuh:248-249, 454-455); 0: the CPU path,
                             which has no such stop (default)                 */
+  double s_budget;       /* TCX_S_AUTO: the largest relative L2 difference
+                            between a refinement's BF16 and split-BF16
+                            centroids that switches S to BF16; 0 -> 1e-4    */
 } tcx_fit_opts;
 
 typedef struct {
@@ -180,6 +186,10 @@ typedef struct {
   double ms_assign_bounds; /* part of ms_assign spent forming the pruning
                               bounds of all series (HIP events)              */
   int collapsed;         /* 1 if the fit stopped on collapse (stop_on_collapse) */
+  int s_bf16_from;       /* iteration (0-based) from which S ran in BF16; -1:
+                            never (TCX_S_AUTO decides it, TCX_S_BF16 gives 0) */
+  double s_check_err;    /* TCX_S_AUTO: the last BF16 - split centroid
+                            difference measured (relative L2; 0: none)      */
 } tcx_fit_stats;
 
 /* Fill opts with defaults (max_iter 10, seed 0, CPU convention,

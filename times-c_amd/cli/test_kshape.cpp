@@ -22,7 +22,8 @@
 //                       CPU path's zscore; the reference GPU CLI used 1 --
 //                       NCC is scale invariant, so labels do not depend on it)
 //   --convention=cpu|kshape   alignment sign (default cpu)
-//   --precision=split|bf16    S = Y^T Y precision (default split)
+//   --precision=split|bf16|auto  S = Y^T Y precision (default split; auto: BF16
+//                             once its centroids are within 1e-4 of split)
 //   --dump-iters=PATH   labels after every iteration, one line each
 //   --centroids=PATH    final centroids, one CSV line each
 //   --budget=BYTES      HBM budget of the context
@@ -72,7 +73,7 @@ int main(int argc, char** argv) {
     std::fprintf(stderr,
                  "usage: %s -d <dev> <file> <k> <flag> <init_chose> [--out=PATH] "
                  "[--max-iter=N] [--seed=S] [--ddof=D] [--convention=cpu|kshape] "
-                 "[--precision=split|bf16] [--dump-iters=PATH] [--centroids=PATH] "
+                 "[--precision=split|bf16|auto] [--dump-iters=PATH] [--centroids=PATH] "
                  "[--budget=BYTES] [--cpu-format]\n",
                  argv[0]);
     return 2;
@@ -97,7 +98,8 @@ int main(int argc, char** argv) {
     else if (opt(argv[i], "--seed", &v)) seed = std::strtoull(v.c_str(), nullptr, 10);
     else if (opt(argv[i], "--ddof", &v)) ddof = std::atoi(v.c_str());
     else if (opt(argv[i], "--convention", &v)) o.convention = v == "kshape" ? TCX_CONV_KSHAPE : TCX_CONV_CPU;
-    else if (opt(argv[i], "--precision", &v)) o.s_precision = v == "bf16" ? TCX_S_BF16 : TCX_S_SPLIT_BF16;
+    else if (opt(argv[i], "--precision", &v))
+      o.s_precision = v == "bf16" ? TCX_S_BF16 : (v == "auto" ? TCX_S_AUTO : TCX_S_SPLIT_BF16);
     else if (opt(argv[i], "--dump-iters", &v)) dump = v;
     else if (opt(argv[i], "--centroids", &v)) cent_path = v;
     else if (opt(argv[i], "--budget", &v)) budget = std::strtoull(v.c_str(), nullptr, 10);

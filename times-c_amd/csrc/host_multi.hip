@@ -1122,6 +1122,7 @@ int grouped_fit(tcx_ctx* ctx, tcx_job* jobs, const int* idx, int count, size_t h
         jb.stats.iterations = j->it;
         jb.stats.converged = j->converged;
         jb.stats.collapsed = j->collapsed;
+        jb.stats.s_bf16_from = j->split ? -1 : 0;
         jb.stats.sum_class = j->sum_class;
         jb.stats.power_steps_max = j->steps_max;
         jb.stats.fallbacks = j->fallbacks;

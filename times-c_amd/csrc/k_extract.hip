@@ -471,7 +471,7 @@ __device__ __forceinline__ void align_pack_body(const int bid, const int nblk,
       const uint4 l = *reinterpret_cast<const uint4*>(&t_lo[tt][q]);
       const long off = (((p0 >> 4) + cc) * (long)tm + r32) * 512 + ln * 8;
       *reinterpret_cast<uint4*>(yhi + off) = h;
-      *reinterpret_cast<uint4*>(ylo + off) = l;
+      if (ylo) *reinterpret_cast<uint4*>(ylo + off) = l;  // NULL: BF16 S (hi only)
     }
     __syncthreads();
   }
@@ -648,7 +648,8 @@ __device__ __forceinline__ void align_pack_lds_body(const int bid, const int nbl
       }
       const long off = (((p0 >> 4) + cc) * (long)tm + r32) * 512 + lane * 8;
       *reinterpret_cast<uint4*>(yhi + off) = make_uint4(h[0], h[1], h[2], h[3]);
-      *reinterpret_cast<uint4*>(ylo + off) = make_uint4(l[0], l[1], l[2], l[3]);
+      if (ylo)  // NULL: BF16 S (hi only)
+        *reinterpret_cast<uint4*>(ylo + off) = make_uint4(l[0], l[1], l[2], l[3]);
     }
   }
 }

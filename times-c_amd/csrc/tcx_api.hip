@@ -53,7 +53,7 @@ enum Slot {
   kSlotS, kSlotInfo, kSlotDummy, kSlotWbuf, kSlotRbuf, kSlotBar, kSlotVout, kSlotSteps,
   kSlotConv, kSlotErr, kSlotCmag, kSlotEvals, kSlotSvout, kSlotLzW, kSlotLzState, kSlotMask,
   kSlotRowStats, kSlotFitRowStats, kSlotKeyCounts, kSlotKeyOff, kSlotMinRow, kSlotXs, kSlotXn,
-  kSlotLzXch, kSlotXm, kSlotUb, kSlotCm, kNumSlots
+  kSlotLzXch, kSlotXm, kSlotUb, kSlotCm, kSlotCheck, kSlotCheckInfo, kNumSlots
 };
 
 struct FitState {
@@ -65,6 +65,10 @@ struct FitState {
   int split = 1;
   int it = 0, converged = 0, sum_class = 0, steps_max = 0, fallbacks = 0;
   int collapsed = 0;  // stopped on collapse (opts.stop_on_collapse)
+  // TCX_S_AUTO: iteration from which S runs in BF16 (-1: not yet), checks
+  // run so far, and the accepted BF16 - split centroid difference
+  int s_bf16_from = -1, s_checks = 0;
+  double s_budget = 1e-4, s_check_err = 0.0;
   long n_pad = 0;
   bool have_order = false;   // the refine's cluster-sorted rows are valid
   const double2* row_stats = nullptr;  // per-row sums of d_x (fixed during the fit)
@@ -376,6 +380,7 @@ struct RefineResult {
   int sum_class = 0;
   int steps_max = 0;
   int fallbacks = 0;
+  double check_err = 0.0;  // with a BF16 check: max relative L2 centroid difference
 };
 
 // Multi-block Lanczos launches wait for each other: the members of one
@@ -496,7 +501,7 @@ int power_lanczos(tcx_ctx* c, const float* S, int m, int k, const int* counts, u
 int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_label,
            const int* d_shift, const int* d_czero, float* d_cout, int iter, uint64_t seed,
            int split, double tol, int cal_select, const double2* row_stats, RefineResult* res,
-           const float* d_dist = nullptr) {
+           const float* d_dist = nullptr, float* d_check = nullptr) {
   hipStream_t st = c->stream;
   if (!row_stats && tcx::align_pack_uses_row_stats(m)) {
     double2* rs;
@@ -545,7 +550,8 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
   float* a0;
   const size_t panel = (size_t)tcx::panel_blocks(m) * 32 * (size_t)n_pad_max;
   TCX_TRY(ws(c, kSlotYhi, panel, &yhi));
-  TCX_TRY(ws(c, kSlotYlo, panel, &ylo));
+  ylo = nullptr;  // a BF16 S needs the hi part only
+  if (split) TCX_TRY(ws(c, kSlotYlo, panel, &ylo));
   TCX_TRY(ws(c, kSlotA0, (size_t)k * m, &a0));
   TCX_HIP(tcx::launch_align_pack(d_x, m, perm, n_pad_max, offsets, minrow, k, d_shift, d_czero,
                                  row_stats, yhi, ylo, a0, st));
@@ -629,6 +635,44 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
       TCX_HIP(hipStreamSynchronize(st));
       h_info.assign(p_info, p_info + k);
     }
+  }
+  if (d_check && nonempty > 0) {
+    // the budget check of TCX_S_AUTO: the same S from one BF16 product (the
+    // hi panel only) through the same power stage; the largest relative L2
+    // difference between its centroids and the split ones above
+    TCX_HIP(tcx::launch_syrk(d_items, (int)c->items.size(), yhi, ylo, m, 0, part, st));
+    TCX_HIP(tcx::launch_syrk_reduce(d_tiles, (int)c->tiles.size(), part, m, S, st));
+    TCX_TRY(power_lanczos(c, S, m, k, counts, seed, iter, max_steps, 0, nullptr, &vin, &svin,
+                          &rsum, &steps_in, &conv_in));
+    tcx::PowerInfo* cinfo;
+    TCX_TRY(ws(c, kSlotCheckInfo, (size_t)k, &cinfo));
+    TCX_HIP(tcx::launch_power(S, m, k, counts, a0, seed, iter, max_steps, tol, vin, svin, rsum,
+                              steps_in, conv_in, 0, nullptr, 1, d_check, cinfo, st));
+    const size_t cb = sizeof(float) * (size_t)k * m;
+    TCX_TRY(pinned_area(c, 2 * cb + sizeof(tcx::PowerInfo) * k, &pin));
+    float* h_split = static_cast<float*>(pin);
+    float* h_bf = h_split + (size_t)k * m;
+    tcx::PowerInfo* h_cinfo = reinterpret_cast<tcx::PowerInfo*>(h_bf + (size_t)k * m);
+    TCX_HIP(hipMemcpyAsync(h_split, d_cout, cb, hipMemcpyDeviceToHost, st));
+    TCX_HIP(hipMemcpyAsync(h_bf, d_check, cb, hipMemcpyDeviceToHost, st));
+    TCX_HIP(hipMemcpyAsync(h_cinfo, cinfo, sizeof(tcx::PowerInfo) * k, hipMemcpyDeviceToHost, st));
+    TCX_HIP(hipStreamSynchronize(st));
+    double err = 0.0;
+    for (int j = 0; j < k; ++j) {
+      if (h_info[j].empty || h_info[j].zero) continue;
+      if (h_cinfo[j].fallback || h_cinfo[j].zero || h_info[j].fallback) {
+        err = HUGE_VAL;  // the BF16 result would take another branch
+        break;
+      }
+      double dd = 0.0, nn = 0.0;
+      for (int i = 0; i < m; ++i) {
+        const double a = h_bf[(size_t)j * m + i], b = h_split[(size_t)j * m + i];
+        dd += (a - b) * (a - b);
+        nn += b * b;
+      }
+      err = std::max(err, nn > 0.0 ? std::sqrt(dd / nn) : std::sqrt(dd));
+    }
+    if (res) res->check_err = err;
   }
   if (res) {
     res->n_pad = n_pad;
@@ -1033,8 +1077,10 @@ int tcx_fit_begin(tcx_ctx* c, const float* d_x, int n, int m, int k, const tcx_f
   if (o.init_mode == 1 && k > n) return fail(TCX_ERR_ARG, "fit: k > n with centre initialisation");
   if (o.convention != TCX_CONV_CPU && o.convention != TCX_CONV_KSHAPE)
     return fail(TCX_ERR_ARG, "fit: unknown convention");
-  if (o.s_precision != TCX_S_SPLIT_BF16 && o.s_precision != TCX_S_BF16)
+  if (o.s_precision != TCX_S_SPLIT_BF16 && o.s_precision != TCX_S_BF16 &&
+      o.s_precision != TCX_S_AUTO)
     return fail(TCX_ERR_ARG, "fit: unknown s_precision");
+  if (!(o.s_budget >= 0.0)) return fail(TCX_ERR_ARG, "fit: s_budget must be >= 0");
   if (o.cal_select != 0 && o.cal_select != 1)
     return fail(TCX_ERR_ARG, "fit: cal_select must be 0 (power iteration) or 1 (eigenvector)");
   hipStream_t st = c->stream;
@@ -1117,7 +1163,9 @@ int tcx_fit_begin(tcx_ctx* c, const float* d_x, int n, int m, int k, const tcx_f
   f.o.init_labels = nullptr;
   f.o.init_rows = nullptr;
   f.tol = o.power_tol > 0 ? o.power_tol : 1e-12;
-  f.split = o.s_precision == TCX_S_SPLIT_BF16;
+  f.split = o.s_precision != TCX_S_BF16;  // TCX_S_AUTO starts split
+  f.s_bf16_from = o.s_precision == TCX_S_BF16 ? 0 : -1;
+  f.s_budget = o.s_budget > 0 ? o.s_budget : 1e-4;
   f.d_x = d_x;
   f.n = n;
   f.m = m;
@@ -1132,11 +1180,26 @@ int fit_refine(tcx_ctx* c, FitState& f) {
   hipStream_t st = c->stream;
   float* cnew;
   TCX_TRY(ws(c, kSlotCentNew, (size_t)f.k * f.m, &cnew));
+  // TCX_S_AUTO: at iterations 2..4, until one passes, the refinement also
+  // forms the BF16 centroids and measures their difference from the split
+  // ones; within the budget, S runs in BF16 from the next iteration on
+  float* check = nullptr;
+  if (f.o.s_precision == TCX_S_AUTO && f.s_bf16_from < 0 && f.it >= 1 && f.s_checks < 3 &&
+      f.o.cal_select == 0)
+    TCX_TRY(ws(c, kSlotCheck, (size_t)f.k * f.m, &check));
   TCX_HIP(hipEventRecord(c->ev[0], st));
   RefineResult rr;
   TCX_TRY(refine(c, f.d_x, f.n, f.m, f.k, f.label, f.shift, f.zero, cnew, f.it, f.o.seed,
                  f.split, f.tol, f.o.cal_select, f.row_stats, &rr,
-                 f.have_dist ? f.dist : nullptr));
+                 f.have_dist ? f.dist : nullptr, check));
+  if (check) {
+    ++f.s_checks;
+    f.s_check_err = rr.check_err;
+    if (rr.check_err <= f.s_budget) {
+      f.split = 0;
+      f.s_bf16_from = f.it + 1;
+    }
+  }
   // the new centroids become the current ones: the two buffers swap roles
   std::swap(c->slot_ptr[kSlotCent], c->slot_ptr[kSlotCentNew]);
   std::swap(c->slot_bytes[kSlotCent], c->slot_bytes[kSlotCentNew]);
@@ -1266,6 +1329,8 @@ int tcx_fit_query(tcx_ctx* c, tcx_fit_stats* stats) {
   stats->sbd_evals = f.evals;
   stats->sbd_pairs = f.pairs;
   stats->collapsed = f.collapsed;
+  stats->s_bf16_from = f.s_bf16_from;
+  stats->s_check_err = f.s_check_err;
   stats->ms_total = std::chrono::duration<double, std::milli>(
                         std::chrono::steady_clock::now() - f.t_start).count();
   return TCX_OK;
@@ -1314,6 +1379,8 @@ int tcx_fit_end(tcx_ctx* c, int* h_labels, float* h_centroids, tcx_fit_stats* st
     stats->sbd_evals = f.evals;
     stats->sbd_pairs = f.pairs;
     stats->collapsed = f.collapsed;
+    stats->s_bf16_from = f.s_bf16_from;
+    stats->s_check_err = f.s_check_err;
     stats->ms_total = std::chrono::duration<double, std::milli>(
                           std::chrono::steady_clock::now() - f.t_start).count();
   }

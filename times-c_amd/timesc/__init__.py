@@ -22,7 +22,7 @@ CLI_PATH = os.path.join(PKG_ROOT, "bin", "test")
 
 OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_BUDGET, ERR_IO, ERR_UNSUPPORTED, ERR_NUMERIC = range(8)
 CONV_CPU, CONV_KSHAPE = 0, 1
-S_SPLIT_BF16, S_BF16 = 0, 1
+S_SPLIT_BF16, S_BF16, S_AUTO = 0, 1, 2
 
 # every symbol include/timesc.h declares
 EXPORTS = [
@@ -77,6 +77,7 @@ class FitOpts(ctypes.Structure):
         ("power_tol", ctypes.c_double),
         ("labels_per_iter", ctypes.POINTER(ctypes.c_int)),
         ("stop_on_collapse", ctypes.c_int),
+        ("s_budget", ctypes.c_double),
     ]
 
 
@@ -94,6 +95,8 @@ class FitStats(ctypes.Structure):
         ("sbd_pairs", ctypes.c_longlong),
         ("ms_assign_bounds", ctypes.c_double),
         ("collapsed", ctypes.c_int),
+        ("s_bf16_from", ctypes.c_int),
+        ("s_check_err", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -521,7 +524,7 @@ class Context:
 
     def _opts(self, n, max_iter=10, init_mode=0, seed=0, init_labels=None, init_rows=None,
               convention=CONV_CPU, s_precision=S_SPLIT_BF16, power_tol=1e-12, cal_select=0,
-              stop_on_collapse=False, labels_per_iter=False):
+              stop_on_collapse=False, labels_per_iter=False, s_budget=0.0):
         """FitOpts and the host arrays it points to (kept alive by the caller)."""
         o = FitOpts()
         self.lib.tcx_fit_opts_default(ctypes.byref(o))
@@ -529,6 +532,7 @@ class Context:
         o.cal_select = cal_select
         o.convention, o.s_precision, o.power_tol = convention, s_precision, power_tol
         o.stop_on_collapse = int(stop_on_collapse)
+        o.s_budget = s_budget
         keep = {}
         if init_labels is not None:
             keep["il"] = np.ascontiguousarray(init_labels, np.int32)
