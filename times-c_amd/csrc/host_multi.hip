@@ -705,6 +705,16 @@ int refine_power(Batch& b, const std::vector<GJob*>& act, double* ms_power) {
   // the two-launch form first: its finishing pass writes v, steps and conv
   // of every cluster of its jobs, the Gram form's included
   GX_TRY(power_split(b, split_set, 0));  // uploads its own tables
+  // largest clusters first: a launch lasts as long as its slowest block,
+  // so those start in the first wave of blocks (each cluster's result is
+  // independent of its position)
+  std::stable_sort(wlz.begin(), wlz.end(),
+                   [](const LzWaveJob& a, const LzWaveJob& c) { return a.d > c.d; });
+  // orders above kLzWaveSplitD (at the front) take four waves per cluster
+  size_t nw4 = 0;
+  while (nw4 < wlz.size() && wlz[nw4].d > kLzWaveSplitD) ++nw4;
+  std::stable_sort(glz.begin(), glz.end(),
+                   [](const GramLzJob& a, const GramLzJob& c) { return a.nd > c.nd; });
   b.up.reset();
   const size_t gl_off = b.up.put_vec(glz), gm_off = b.up.put_vec(gmap);
   const size_t wl_off = b.up.put_vec(wlz);
@@ -717,8 +727,9 @@ int refine_power(Batch& b, const std::vector<GJob*>& act, double* ms_power) {
                                (int)glz.size(), lds_lz, kLzPowerTol, b.st2));
     GX_HIP(hipEventRecord(b.ev_join, b.st2));
   }
-  GX_HIP(launch_lanczos_wave(reinterpret_cast<const LzWaveJob*>(b.up.dev(wl_off)), (int)wlz.size(),
-                             kLzPowerTol, b.st));
+  const LzWaveJob* dwl = reinterpret_cast<const LzWaveJob*>(b.up.dev(wl_off));
+  GX_HIP(launch_lanczos_wave(dwl, (int)nw4, 4, kLzPowerTol, b.st));
+  GX_HIP(launch_lanczos_wave(dwl + nw4, (int)(wlz.size() - nw4), 1, kLzPowerTol, b.st));
   if (!glz.empty()) GX_HIP(hipStreamWaitEvent(b.st, b.ev_join, 0));
   GX_HIP(launch_gram_map(reinterpret_cast<const GramVecJob*>(b.up.dev(gm_off)), (int)gmap.size(),
                          lds_map, b.st));

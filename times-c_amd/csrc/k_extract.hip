@@ -3165,9 +3165,15 @@ constexpr int kLzWaveLds = 3 * (kLzMax + 1) + 2 * kLzMax + kLzWaveMaxD;  // alph
 // y = S x for symmetric S (FP32, d x d) through the wave's LDS slot xb (d
 // doubles); entries e >= d of y come back zero.  Loads past the last row
 // are clamped to it.
+// kW > 1: the kW waves of the block share one cluster, wave `part` takes
+// the columns [part * d/kW, (part+1) * d/kW) and the partial products are
+// summed through `red` (kW x kLzWaveMaxD doubles) in wave order, so every
+// wave ends with the same y.
+template <int kW>
 __device__ __forceinline__ void wave_sym_matvec(const float* __restrict__ S, int d, int U,
                                                 const double (&x)[kLzWaveU],
-                                                double (&y)[kLzWaveU], double* xb) {
+                                                double (&y)[kLzWaveU], double* xb, int part,
+                                                double* red) {
   constexpr int kCols = 8;  // columns whose loads are in flight together
   const int lane = threadIdx.x & 63;
   int ec[kLzWaveU];
@@ -3179,21 +3185,41 @@ __device__ __forceinline__ void wave_sym_matvec(const float* __restrict__ S, int
     if (e < d) xb[e] = x[u];
   }
   wave_lds_sync();
-  for (int c0 = 0; c0 < d; c0 += kCols) {
+  const int share = (d + kW - 1) / kW;
+  const int cb = kW > 1 ? part * share : 0, ce = kW > 1 ? (cb + share < d ? cb + share : d) : d;
+  for (int c0 = cb; c0 < ce; c0 += kCols) {
     float sv[kCols][kLzWaveU];
 #pragma unroll
     for (int t = 0; t < kCols; ++t) {
-      const int c = c0 + t < d ? c0 + t : d - 1;
+      const int c = c0 + t < ce ? c0 + t : ce - 1;
       const float* col = S + (long)c * d;
 #pragma unroll
       for (int u = 0; u < kLzWaveU; ++u) sv[t][u] = u < U ? col[ec[u]] : 0.f;
     }
 #pragma unroll
     for (int t = 0; t < kCols; ++t) {
-      const double xc = c0 + t < d ? xb[c0 + t] : 0.0;
+      const double xc = c0 + t < ce ? xb[c0 + t] : 0.0;
 #pragma unroll
       for (int u = 0; u < kLzWaveU; ++u) y[u] += (double)sv[t][u] * xc;
     }
+  }
+  if constexpr (kW > 1) {
+#pragma unroll
+    for (int u = 0; u < kLzWaveU; ++u) {
+      const int e = lane + 64 * u;
+      if (e < d) red[part * kLzWaveMaxD + e] = y[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kLzWaveU; ++u) {
+      const int e = lane + 64 * u;
+      double t = 0.0;
+      if (e < d)
+#pragma unroll
+        for (int p = 0; p < kW; ++p) t += red[p * kLzWaveMaxD + e];
+      y[u] = t;
+    }
+    __syncthreads();  // red is rewritten by the next product
   }
 #pragma unroll
   for (int u = 0; u < kLzWaveU; ++u)
@@ -3219,14 +3245,23 @@ __device__ __forceinline__ void wave_basis_rows(const double* __restrict__ Q, in
   }
 }
 
+// kW = 1: four clusters per block, one wave each.  kW = 4: one cluster per
+// block, the four waves sharing each matvec by columns (the product's
+// latency chain is a quarter as long) and computing everything else alike;
+// wave 0 writes the results, every wave the (identical) basis rows it reads
+// back itself.
+template <int kW>
 __global__ __launch_bounds__(256, 3) void k_lanczos_wave(const LzWaveJob* __restrict__ jobs,
                                                       int njobs, double lz_tol) {
   __shared__ double lds[4][kLzWaveLds];
+  __shared__ double red[kW > 1 ? kW * kLzWaveMaxD : 1];
   // the wave index is uniform in the wave: readfirstlane lets the job's
   // fields live in scalar registers
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int jid = blockIdx.x * 4 + wv;
-  if (jid >= njobs) return;  // whole wave
+  const int jid = kW > 1 ? (int)blockIdx.x : (int)blockIdx.x * 4 + wv;
+  if (jid >= njobs) return;  // whole wave (kW = 1) or whole block
+  const int part = kW > 1 ? wv : 0;
+  const bool writer = part == 0;
   const LzWaveJob a = jobs[jid];
   double* alpha = lds[wv];
   double* beta = alpha + kLzMax + 1;
@@ -3252,12 +3287,12 @@ __global__ __launch_bounds__(256, 3) void k_lanczos_wave(const LzWaveJob* __rest
 #pragma unroll
       for (int u = 0; u < kLzWaveU; ++u) {
         const int e = lane + 64 * u;
-        if (e < d) {
+        if (e < d && writer) {
           a.v[e] = 0.0;
           a.h[e] = 0.0;
         }
       }
-      if (lane == 0) {
+      if (lane == 0 && writer) {
         *a.steps = 0;
         *a.conv = 1;
       }
@@ -3267,7 +3302,7 @@ __global__ __launch_bounds__(256, 3) void k_lanczos_wave(const LzWaveJob* __rest
     double one[kLzWaveU];
 #pragma unroll
     for (int u = 0; u < kLzWaveU; ++u) one[u] = lane + 64 * u < d ? 1.0 : 0.0;
-    wave_sym_matvec(S, d, U, one, r, xb);  // r = S 1
+    wave_sym_matvec<kW>(S, d, U, one, r, xb, part, red);  // r = S 1
 #pragma unroll
     for (int u = 0; u < kLzWaveU; ++u) {
       const int e = lane + 64 * u;
@@ -3275,7 +3310,7 @@ __global__ __launch_bounds__(256, 3) void k_lanczos_wave(const LzWaveJob* __rest
       q[u] = e < d ? power_v0(a.seed, a.iter, a.cluster, e) : 0.0;
       qp[u] = 0.0;
       n0 += q[u] * q[u];
-      if (e < d) a.r[e] = r[u];
+      if (e < d && writer) a.r[e] = r[u];
     }
     sigma = wave_sum_f64(sigma);
     n0 = sqrt(wave_sum_f64(n0));
@@ -3290,7 +3325,7 @@ __global__ __launch_bounds__(256, 3) void k_lanczos_wave(const LzWaveJob* __rest
   int len = 0, conv = 0, prev_len = 0;
   double bprev = 0.0;
   for (int it = 0; it < jmax; ++it) {
-    wave_sym_matvec(S, d, U, q, w, xb);  // w = S q
+    wave_sym_matvec<kW>(S, d, U, q, w, xb, part, red);  // w = S q
     if (!a.gram) {                   // w = M q = P S P q
       double sv = 0.0, rv = 0.0;
 #pragma unroll
@@ -3419,7 +3454,7 @@ __global__ __launch_bounds__(256, 3) void k_lanczos_wave(const LzWaveJob* __rest
     }
     double lam_prev = 0.0;
     for (int t = 0; t < a.I; ++t) {
-      wave_sym_matvec(S, d, U, v, w, xb);
+      wave_sym_matvec<kW>(S, d, U, v, w, xb, part, red);
       double nw2 = 0.0, vw = 0.0;
 #pragma unroll
       for (int u = 0; u < kLzWaveU; ++u) {
@@ -3442,16 +3477,16 @@ __global__ __launch_bounds__(256, 3) void k_lanczos_wave(const LzWaveJob* __rest
     }
     conv = 1;
   }
-  wave_sym_matvec(S, d, U, v, w, xb);  // h = S v
+  wave_sym_matvec<kW>(S, d, U, v, w, xb, part, red);  // h = S v
 #pragma unroll
   for (int u = 0; u < kLzWaveU; ++u) {
     const int e = lane + 64 * u;
-    if (e < d) {
+    if (e < d && writer) {
       a.v[e] = v[u];
       a.h[e] = w[u];
     }
   }
-  if (lane == 0) {
+  if (lane == 0 && writer) {
     *a.steps = steps;
     *a.conv = conv;
   }
@@ -3584,10 +3619,17 @@ hipError_t launch_gram_map(const GramVecJob* jobs, int njobs, size_t lds, hipStr
   return hipGetLastError();
 }
 
-hipError_t launch_lanczos_wave(const LzWaveJob* jobs, int njobs, double lz_tol, hipStream_t st) {
+hipError_t launch_lanczos_wave(const LzWaveJob* jobs, int njobs, int waves_per_cluster,
+                               double lz_tol, hipStream_t st) {
   if (njobs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_lanczos_wave, dim3((unsigned)((njobs + 3) / 4)), dim3(256), 0, st, jobs,
-                     njobs, lz_tol);
+  if (waves_per_cluster == 4)
+    hipLaunchKernelGGL(k_lanczos_wave<4>, dim3((unsigned)njobs), dim3(256), 0, st, jobs, njobs,
+                       lz_tol);
+  else if (waves_per_cluster == 1)
+    hipLaunchKernelGGL(k_lanczos_wave<1>, dim3((unsigned)((njobs + 3) / 4)), dim3(256), 0, st,
+                       jobs, njobs, lz_tol);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

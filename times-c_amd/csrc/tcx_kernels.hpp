@@ -506,7 +506,11 @@ struct LzWaveJob {
   int* steps;
   int* conv;
 };
-hipError_t launch_lanczos_wave(const LzWaveJob* jobs, int njobs, double lz_tol, hipStream_t st);
+// waves_per_cluster 1: four clusters per block; 4: one cluster per block,
+// each matvec shared by its four waves (for the larger orders)
+hipError_t launch_lanczos_wave(const LzWaveJob* jobs, int njobs, int waves_per_cluster,
+                               double lz_tol, hipStream_t st);
+constexpr int kLzWaveSplitD = 128;  // orders above this take four waves
 
 struct ClusterMatvecJob {
   const float* S;
