@@ -612,10 +612,10 @@ int refine_power(Batch& b, const std::vector<GJob*>& act, double* ms_power) {
   b.tiles.clear();
   std::vector<SyrkJob> sjobs;
   std::vector<GramItem> gitems;
-  std::vector<GramVecJob> gstart, gmap;
+  std::vector<GramVecJob> gmap;
   std::vector<GramLzJob> glz;
   std::vector<LzWaveJob> wlz;
-  size_t lds_start = 0, lds_map = 0, lds_lz = 0;
+  size_t lds_map = 0, lds_lz = 0;
   Launch<FillSeg> fills;
   for (size_t q = 0; q < act.size(); ++q) {
     GJob* j = act[q];
@@ -648,14 +648,13 @@ int refine_power(Batch& b, const std::vector<GJob*>& act, double* ms_power) {
       // Gram order: the members rounded up to 4 (the matvec's float4 rows)
       const int gn = (cnt[c] + 3) & ~3;
       const int T = (gn + 63) / 64;
-      for (int ti = 0; ti < T; ++ti)
-        for (int tj = ti; tj < T; ++tj)
-          gitems.push_back(
-              GramItem{j->yhi, j->ylo, j->S + c * mm, off[c], j->m, (int)nd, gn, ti, tj});
       double* rc = j->r + (long)c * j->m;
       double* wc = j->w + (long)c * j->m;
-      gstart.push_back(GramVecJob{j->yhi, j->ylo, off[c], j->m, (int)nd, gn, j->o.seed, j->it, c,
-                                  rc, nullptr, nullptr, nullptr, nullptr, nullptr});
+      // the diagonal tiles also form the start vector u0 (into r)
+      for (int ti = 0; ti < T; ++ti)
+        for (int tj = ti; tj < T; ++tj)
+          gitems.push_back(GramItem{j->yhi, j->ylo, j->S + c * mm, off[c], j->m, (int)nd, gn, ti,
+                                    tj, ti == tj ? rc : nullptr, j->o.seed, j->it, c});
       double* svc = j->sv + (long)c * j->m;
       double* Qc = j->Q + (long)c * (kLanczosMax + 1) * j->m;
       if (gn <= kLzWaveMaxD) {
@@ -666,9 +665,9 @@ int refine_power(Batch& b, const std::vector<GJob*>& act, double* ms_power) {
                                 j->conv + c});
         lds_lz = std::max(lds_lz, gram_lanczos_lds(gn));
       }
-      gmap.push_back(GramVecJob{j->yhi, j->ylo, off[c], j->m, (int)nd, gn, j->o.seed, j->it, c,
-                                nullptr, wc, svc, j->v + (long)c * j->m, svc, rc});
-      lds_start = std::max(lds_start, gram_start_lds(j->m));
+      gmap.push_back(
+          GramVecJob{j->yhi, j->ylo, off[c], j->m, (int)nd, gn, wc, svc, j->v + (long)c * j->m,
+                     svc, rc});
       lds_map = std::max(lds_map, gram_map_lds(j->m, (int)nd));
     }
     sjobs.push_back(SyrkJob{j->yhi, j->ylo, j->m, j->part, j->S});
@@ -692,7 +691,7 @@ int refine_power(Batch& b, const std::vector<GJob*>& act, double* ms_power) {
   std::vector<GroupSyrkItem> by[2];
   GX_TRY(launch_syrk_items(b, act, sjobs, &sj_off, by_off, &tl_off, by));
   const size_t fill_off = b.up.put_vec(fills.jobs);
-  const size_t gi_off = b.up.put_vec(gitems), gs_off = b.up.put_vec(gstart);
+  const size_t gi_off = b.up.put_vec(gitems);
   GX_TRY(b.up.flush(b.st));
   if (!fills.jobs.empty())
     GX_HIP(launch_fill_segs(reinterpret_cast<const FillSeg*>(b.up.dev(fill_off)),
@@ -700,8 +699,6 @@ int refine_power(Batch& b, const std::vector<GJob*>& act, double* ms_power) {
   GX_TRY(run_syrk(b, sj_off, by_off, tl_off, by));
   GX_HIP(launch_gram_tiles(reinterpret_cast<const GramItem*>(b.up.dev(gi_off)), (int)gitems.size(),
                            b.st));
-  GX_HIP(launch_gram_start(reinterpret_cast<const GramVecJob*>(b.up.dev(gs_off)),
-                           (int)gstart.size(), lds_start, b.st));
   // the two-launch form first: its finishing pass writes v, steps and conv
   // of every cluster of its jobs, the Gram form's included
   GX_TRY(power_split(b, split_set, 0));  // uploads its own tables

@@ -2933,8 +2933,9 @@ size_t group_count_lds(int k, int B, int* lds_min) {
 //                 accumulation,
 //                 the s s^T / m term in the epilogue; entries with a <= b
 //                 stored with their mirrors, so G is exactly symmetric
-//   k_gram_start  u0 = Y (v0 - mean(v0)) (FP64), v0 = power_v0(seed, iter,
-//                 cluster, .), into the cluster's r row
+//   (start)       u0 = Y (v0 - mean(v0)) (FP64), v0 = power_v0(seed, iter,
+//                 cluster, .), into the cluster's r row, formed by the
+//                 diagonal tiles from the rows they load
 //   (Lanczos)     power_lanczos_cluster in its Gram form: g = G^(I-1) u0
 //                 and h = G g, into the cluster's w and sv rows
 //   k_gram_map    v = P Y^T g, S v -> P Y^T h (= M v) and r = 0, the inputs
@@ -2976,6 +2977,22 @@ __global__ __launch_bounds__(256) void k_gram_tiles(const GramItem* __restrict__
   f32x16g acc;
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+  // the start vector of the tile's members (diagonal tiles with u0): the
+  // mean of v0 over the m times, then u0[a] = sum_t y_a[t] (v0[t] - mean)
+  __shared__ double vred[4];
+  double vmean = 0.0;
+  double su[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) su[e] = 0.0;
+  const bool start = it.u0 != nullptr;  // uniform: the whole block
+  if (start) {
+    double t0 = 0.0;
+    for (int t = tid; t < it.m; t += 256) t0 += power_v0(it.seed, it.iter, it.cluster, t);
+    t0 = wave_sum_f64(t0);
+    if (lane == 0) vred[wave] = t0;
+    __syncthreads();
+    vmean = ((vred[0] + vred[1]) + (vred[2] + vred[3])) / (double)it.m;
+  }
   for (int r32 = 0; r32 < tm; ++r32) {
     float ya[8], yb[8];
     if (va) {
@@ -2997,6 +3014,14 @@ __global__ __launch_bounds__(256) void k_gram_tiles(const GramItem* __restrict__
       Bs[tt * kLd + qo + e] = yb[e];
       sa[e] += ya[e];
       sb[e] += yb[e];
+    }
+    if (start) {
+      const int t = r32 * 32 + tt;
+      if (t < it.m) {  // rows t >= m of the panel are zero
+        const double vc = power_v0(it.seed, it.iter, it.cluster, t) - vmean;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) su[e] += (double)ya[e] * vc;
+      }
     }
     __syncthreads();
     if (active) {
@@ -3024,6 +3049,15 @@ __global__ __launch_bounds__(256) void k_gram_tiles(const GramItem* __restrict__
       ssB[qo + e] = sb[e];
     }
   }
+  if (start) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+#pragma unroll
+      for (int o = 1; o < 32; o <<= 1) su[e] += __shfl_xor(su[e], o);
+      const int a = it.ti * 64 + qo + e;
+      if (tt == 0 && a < it.n) it.u0[a] = su[e];
+    }
+  }
   __syncthreads();
   if (!active) return;
   const float inv_m = 1.0f / (float)it.m;
@@ -3047,48 +3081,6 @@ __device__ __forceinline__ double block256_sum_f64(double v, double* red) {
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
   return (red[0] + red[1]) + (red[2] + red[3]);
-}
-
-__global__ __launch_bounds__(256) void k_gram_start(const GramVecJob* __restrict__ jobs) {
-  const GramVecJob a = jobs[blockIdx.x];
-  extern __shared__ double gsh[];
-  double* vc = gsh;  // m
-  __shared__ double red[4];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double s = 0.0;
-  for (int t = threadIdx.x; t < a.m; t += 256) {
-    vc[t] = power_v0(a.seed, a.iter, a.cluster, t);
-    s += vc[t];
-  }
-  const double mean = block256_sum_f64(s, red) / (double)a.m;
-  for (int t = threadIdx.x; t < a.m; t += 256) vc[t] -= mean;
-  __syncthreads();
-  const int tm = panel_blocks(a.m), nch = a.nd / 16;
-  for (int cc = wave; cc < nch; cc += 4) {
-    double acc[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) acc[e] = 0.0;
-    const long base = ((a.p0 >> 4) + cc) * (long)tm * 512 + lane * 8;
-    for (int r32 = 0; r32 < tm; ++r32) {
-      const int t = r32 * 32 + (lane & 31);
-      if (t >= a.m) break;  // rows t >= m of the panel are zero
-      float y[8];
-      panel_load8(a.yhi, a.ylo, base + (long)r32 * 512, y);
-      const double x = vc[t];
-#pragma unroll
-      for (int e = 0; e < 8; ++e) acc[e] += (double)y[e] * x;
-    }
-#pragma unroll
-    for (int e = 0; e < 8; ++e)
-#pragma unroll
-      for (int o = 1; o < 32; o <<= 1) acc[e] += __shfl_xor(acc[e], o);
-    if ((lane & 31) == 0)
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int q = cc * 16 + 8 * (lane >> 5) + e;
-        if (q < a.n) a.u0[q] = acc[e];
-      }
-  }
 }
 
 __global__ __launch_bounds__(kPowerThreads) void k_gram_lanczos(const GramLzJob* __restrict__ jobs,
@@ -3592,16 +3584,8 @@ hipError_t launch_gram_tiles(const GramItem* items, int nitems, hipStream_t st) 
   return hipGetLastError();
 }
 
-size_t gram_start_lds(int m) { return sizeof(double) * (size_t)m; }
 size_t gram_map_lds(int m, int nd) { return sizeof(double) * 2 * ((size_t)m + nd); }
 size_t gram_lanczos_lds(int nd) { return lanczos_smem(nd); }
-
-hipError_t launch_gram_start(const GramVecJob* jobs, int njobs, size_t lds, hipStream_t st) {
-  if (njobs <= 0) return hipSuccess;
-  if (lds > gram_start_lds(kGramMaxM)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_gram_start, dim3((unsigned)njobs), dim3(256), lds, st, jobs);
-  return hipGetLastError();
-}
 
 hipError_t launch_gram_lanczos(const GramLzJob* jobs, int njobs, size_t lds, double lz_tol,
                                hipStream_t st) {

@@ -442,9 +442,9 @@ size_t power_lanczos_lds(int m);
 hipError_t launch_power_lanczos_group(const GroupTable& g, double lz_tol, hipStream_t st);
 
 // Gram form of thin clusters (segment nd < m, nd <= kGramMaxND, m <=
-// kGramMaxM): G = Yc Yc^T in 64 x 64 tiles, the start u0 = Yc v0, the
-// Lanczos stage on G, and the map back to v = Yc^T g, M v = Yc^T h.  One
-// item or job per block.
+// kGramMaxM): G = Yc Yc^T in 64 x 64 tiles (the diagonal ones also form
+// the start u0 = Yc v0), the Lanczos stage on G, and the map back to
+// v = Yc^T g, M v = Yc^T h.  One item or job per block.
 constexpr int kGramMaxM = 4096;
 constexpr int kGramMaxND = 2048;
 struct GramItem {
@@ -453,20 +453,22 @@ struct GramItem {
   float* G;  // n x n, row stride n
   long p0;   // first packed column of the cluster (multiple of 64)
   int m, nd, n, ti, tj;  // segment nd (multiple of 64), members n (<= nd)
+  // diagonal tiles also form the start u0 = Yc v0 of their 64 members
+  // (u0 NULL: not this tile), v0 = power_v0(seed, iter, cluster, .)
+  double* u0;
+  uint64_t seed;
+  int iter, cluster;
 };
-struct GramVecJob {
+struct GramVecJob {  // the map back of one thin cluster
   const uint16_t* yhi;
   const uint16_t* ylo;
   long p0;
   int m, nd, n;
-  uint64_t seed;
-  int iter, cluster;
-  double* u0;         // start: Yc v0 (nd)
-  const double* g;    // map: g (nd)
-  const double* h;    // map: h (nd; may be sv: read before sv is written)
-  double* v;          // map: m
-  double* sv;         // map: M v (m)
-  double* r;          // map: zeros (m)
+  const double* g;    // g (n)
+  const double* h;    // h (n; may be sv: read before sv is written)
+  double* v;          // m
+  double* sv;         // M v (m)
+  double* r;          // zeros (m)
 };
 struct GramLzJob {
   const float* G;
@@ -479,10 +481,8 @@ struct GramLzJob {
   int* conv;
 };
 hipError_t launch_gram_tiles(const GramItem* items, int nitems, hipStream_t st);
-size_t gram_start_lds(int m);
 size_t gram_map_lds(int m, int nd);
 size_t gram_lanczos_lds(int nd);
-hipError_t launch_gram_start(const GramVecJob* jobs, int njobs, size_t lds, hipStream_t st);
 hipError_t launch_gram_lanczos(const GramLzJob* jobs, int njobs, size_t lds, double lz_tol,
                                hipStream_t st);
 hipError_t launch_gram_map(const GramVecJob* jobs, int njobs, size_t lds, hipStream_t st);
