@@ -14,9 +14,13 @@ over all jobs in progress), through the C ABI:
     oracle's refinement and assignment;
   - the collapse stop, and jobs the executor leaves to the worker pool
     (centre initialisation, eigenvector centroids) in the same call;
-  - the 256-job uneven workload: every job equals itself alone through the
-    executor, and its final labels and centroids equal tcx_kshape_fit's
-    (to rounding; differing labels only at SBD ties)."""
+  - cohorts (batches of 32 jobs or more split over up to
+    TCX_GROUPED_COHORTS contexts, with a second stream created or borrowed
+    from an idle worker): every layout gives the one-cohort results bit for
+    bit;
+  - the 256-job uneven workload (two cohorts): every job equals itself alone
+    through the executor, and its final labels and centroids equal
+    tcx_kshape_fit's (to rounding; differing labels only at SBD ties)."""
 import numpy as np
 import pytest
 
@@ -173,6 +177,27 @@ def test_grouped_collapse_stop_and_pool_jobs_in_one_call(ctx):
         assert np.array_equal(r["centroids"], alone["centroids"])
 
 
+@pytest.mark.parametrize("workers,cohorts", [(2, None), (3, "2"), (4, None), (8, "3")])
+def test_grouped_cohorts_equal_one_cohort(ctx, monkeypatch, workers, cohorts):
+    """40 jobs: two cohorts with a created second stream (2 and 3 workers),
+    two with borrowed ones (4 workers), three with borrowed ones (8)."""
+    shapes = [(120 + 37 * i, (24, 64, 130, 300)[i % 4], 2 + i % 5) for i in range(40)]
+    seeds = [3000 + i for i in range(40)]
+    monkeypatch.setenv("TCX_GROUPED_COHORTS", "1")
+    ref, _ = _run(ctx, shapes, seeds, workers=1, labels_per_iter=True)
+    if cohorts is None:
+        monkeypatch.delenv("TCX_GROUPED_COHORTS")
+    else:
+        monkeypatch.setenv("TCX_GROUPED_COHORTS", cohorts)
+    print(f"{workers} workers, cohorts {cohorts or 'default'}", flush=True)
+    res, _ = _run(ctx, shapes, seeds, workers=workers, labels_per_iter=True)
+    for i, (a, b) in enumerate(zip(res, ref)):
+        assert a["status"] == 0 and b["status"] == 0, (i, a["message"], b["message"])
+        assert np.array_equal(a["labels_per_iter"], b["labels_per_iter"]), i
+        assert np.array_equal(a["centroids"], b["centroids"]), i
+        assert a["stats"]["iterations"] == b["stats"]["iterations"], i
+
+
 def test_grouped_256_uneven_jobs(ctx):
     """Every job: its final labels are the FP64 oracle's assignment against
     its own final centroids (up to SBD ties); 32 sampled jobs equal
@@ -189,7 +214,7 @@ def test_grouped_256_uneven_jobs(ctx):
     specs = [dict(d=d, k=k, seed=seed, znorm_ddof=0, max_iter=10)
              for d, (_, k, seed) in zip(ds, jobs)]
     try:
-        with T.Batch(0, workers=1) as b:
+        with T.Batch(0, workers=4) as b:  # two cohorts
             res = b.run(specs, grouped=True)
     finally:
         for d in ds:

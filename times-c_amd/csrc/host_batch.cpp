@@ -9,6 +9,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <deque>
 #include <mutex>
 #include <new>
@@ -169,6 +170,8 @@ int tcx_batch_run(tcx_batch* b, tcx_job* jobs, int njobs, int order) {
   return first_failure(jobs, njobs);
 }
 
+constexpr size_t kGroupedCohortMin = 32;  // fewer jobs: one cohort
+
 int tcx_batch_run_grouped(tcx_batch* b, tcx_job* jobs, int njobs) {
   tcx::clear_error();
   if (!b) return tcx::fail(TCX_ERR_ARG, "batch: null batch");
@@ -186,13 +189,53 @@ int tcx_batch_run_grouped(tcx_batch* b, tcx_job* jobs, int njobs) {
   }
   int rc = TCX_OK;
   if (!grouped.empty()) {
-    rc = tcx::grouped_fit(b->ctx[0], jobs, grouped.data(), (int)grouped.size(), b->budget, t0);
-    if (rc != TCX_OK) {
-      const std::string msg = tcx_last_error();
-      for (int i : grouped)
+    // Cohorts (default two, TCX_GROUPED_COHORTS) on as many contexts (their
+    // own streams), run from one host thread each: while one cohort is in a
+    // latency-bound phase (the Lanczos steps, the host's planning) the
+    // others' launches fill the GPU.  Jobs go to the cohort with the
+    // smaller summed cost, largest first; a job's results do not depend on
+    // its cohort.
+    int W = 1;
+    if (grouped.size() >= kGroupedCohortMin) {
+      const char* e = std::getenv("TCX_GROUPED_COHORTS");  // default 2
+      W = e ? std::max(1, std::atoi(e)) : 2;
+      W = std::min<int>(W, (int)b->ctx.size());
+    }
+    std::vector<std::vector<int>> cohort(W);
+    std::vector<double> load(W, 0.0);
+    std::vector<int> order = grouped;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int c) { return job_cost(jobs[a]) > job_cost(jobs[c]); });
+    for (int i : order) {
+      const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      cohort[w].push_back(i);
+      load[w] += job_cost(jobs[i]);
+    }
+    size_t budget = b->budget;
+    if (!budget) budget = tcx::device_free_bytes(b->ctx[0]) / 10 * 8;
+    std::vector<int> rcs(W, TCX_OK);
+    std::vector<std::string> msgs(W);
+    auto work = [&](int w) {
+      std::sort(cohort[w].begin(), cohort[w].end());  // the caller's order inside a cohort
+      // the second stream: an idle worker context's, when there are enough,
+      // so that the cohorts' streams are the first 2W streams created (the
+      // runtime maps streams onto its hardware queues in creation order)
+      void* second = b->ctx.size() >= (size_t)(2 * W) ? tcx::stream_of(b->ctx[W + w]) : nullptr;
+      rcs[w] = tcx::grouped_fit(b->ctx[w], jobs, cohort[w].data(), (int)cohort[w].size(),
+                                budget / W, t0, second);
+      if (rcs[w] != TCX_OK) msgs[w] = tcx_last_error();  // thread-local
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < W; ++w) th.emplace_back(work, w);
+    work(0);
+    for (std::thread& t : th) t.join();
+    for (int w = 0; w < W; ++w) {
+      if (rcs[w] == TCX_OK) continue;
+      if (rc == TCX_OK) rc = rcs[w];
+      for (int i : cohort[w])
         if (jobs[i].status == -1) {
-          jobs[i].status = rc;
-          std::snprintf(jobs[i].message, sizeof(jobs[i].message), "%s", msg.c_str());
+          jobs[i].status = rcs[w];
+          std::snprintf(jobs[i].message, sizeof(jobs[i].message), "%s", msgs[w].c_str());
         }
     }
   }

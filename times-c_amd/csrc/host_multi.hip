@@ -288,6 +288,7 @@ struct Batch {
   // second stream: the block-form Gram Lanczos runs beside the one-wave
   // form (default priority; joined back into st by an event)
   hipStream_t st2 = nullptr;
+  bool own_st2 = true;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int cus = 0;
   std::vector<GJob> jobs;
@@ -314,7 +315,7 @@ struct Batch {
 
   ~Batch() {
     if (st2) (void)hipStreamSynchronize(st2);
-    if (st2) (void)hipStreamDestroy(st2);
+    if (st2 && own_st2) (void)hipStreamDestroy(st2);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
     for (GJob& j : jobs)
@@ -926,13 +927,18 @@ int validate(const tcx_job& jb, const char** why) {
 bool grouped_supported(const tcx_job& j, const char** why) { return validate(j, why) < 0; }
 
 int grouped_fit(tcx_ctx* ctx, tcx_job* jobs, const int* idx, int count, size_t hbm_budget,
-                Clock::time_point t0) {
+                Clock::time_point t0, void* second_stream) {
   Batch b;
   b.ctx = ctx;
   b.st = static_cast<hipStream_t>(stream_of(ctx));
   b.cus = cus_of(ctx);
   GX_HIP(hipSetDevice(device_of(ctx)));
-  GX_HIP(hipStreamCreateWithFlags(&b.st2, hipStreamNonBlocking));
+  if (second_stream) {
+    b.st2 = static_cast<hipStream_t>(second_stream);  // the caller's (idle) stream
+    b.own_st2 = false;
+  } else {
+    GX_HIP(hipStreamCreateWithFlags(&b.st2, hipStreamNonBlocking));
+  }
   GX_HIP(hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming));
   GX_HIP(hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming));
   b.jobs.resize(count);

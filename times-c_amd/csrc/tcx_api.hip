@@ -720,6 +720,13 @@ int stream_sync(tcx_ctx* c) {
 
 size_t held_bytes(const tcx_ctx* c) { return c->ws_bytes + c->user_bytes; }
 
+size_t device_free_bytes(tcx_ctx* c) {
+  if (set_dev(c) != TCX_OK) return 0;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 0;
+  return free_b;
+}
+
 void* stream_of(tcx_ctx* c) { return c->stream; }
 int device_of(const tcx_ctx* c) { return c->device; }
 int cus_of(const tcx_ctx* c) { return c->cus; }

@@ -34,6 +34,8 @@ int stream_sync(tcx_ctx* c);
 // Device bytes the context holds: workspace slots, twiddle tables and
 // tcx_malloc allocations.
 size_t held_bytes(const tcx_ctx* c);
+// Free device memory of the context's GPU (0 when it cannot be read).
+size_t device_free_bytes(tcx_ctx* c);
 // Frees every workspace slot and twiddle table of an idle context (no fit
 // in progress); the next call grows them again.
 int trim_workspace(tcx_ctx* c);
@@ -52,9 +54,10 @@ int twiddle_table(tcx_ctx* c, int F, const void** tw);
 // the context's stream, admitting jobs while their device memory fits
 // hbm_budget (0: most of the free device memory).  Every job gets its
 // status, results and timings (from t0); returns TCX_OK unless the run
-// itself failed.
+// itself failed.  second_stream (may be NULL: one is created) carries the
+// block-form Gram Lanczos beside the context's stream.
 bool grouped_supported(const tcx_job& j, const char** why);
 int grouped_fit(tcx_ctx* c, tcx_job* jobs, const int* idx, int count, size_t hbm_budget,
-                std::chrono::steady_clock::time_point t0);
+                std::chrono::steady_clock::time_point t0, void* second_stream = nullptr);
 
 }  // namespace tcx
