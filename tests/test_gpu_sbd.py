@@ -126,7 +126,11 @@ def test_pruned_assign_matches_oracle(ctx, m, n, k):
         assert abs(cc[sh[i] + m - 1] - cc[osh[i] + m - 1]) <= SBD_TIE, (i, sh[i], osh[i])
 
 
-@pytest.mark.parametrize("m,n,k", [(96, 3001, 12), (2048, 257, 6)])
+# F = 256 (16 lanes), 1024 (one wave: m = 260 leaves 505 non-lag positions
+# over rows 4..11, m = 512 one), 4096 (256 lanes); k up to 64 keeps a
+# one-wave group's bounds in registers, k = 80 in LDS
+@pytest.mark.parametrize("m,n,k", [(96, 3001, 12), (2048, 257, 6), (260, 2001, 16),
+                                   (512, 3001, 16), (300, 1501, 64), (400, 801, 80)])
 def test_pruned_assign_equals_full_search(ctx, m, n, k):
     """Pruned and full assignment of the same inputs agree bit for bit on
     labels and distances (both pair rows 2p, 2p+1 in one transform)."""
@@ -141,6 +145,31 @@ def test_pruned_assign_equals_full_search(ctx, m, n, k):
         flab, fdist, fsh, _ = ctx.assign(d, dc, dmat=True)
         dc.free()
     finally:
+        d.free()
+    assert np.array_equal(lab, flab)
+    assert np.array_equal(dist, fdist)
+    assert np.array_equal(sh, fsh)
+
+
+@pytest.mark.parametrize("m", [260, 512])
+def test_pruned_assign_equals_full_search_on_ties(ctx, m):
+    """Series and centroids of +-1 values: many exactly equal correlations
+    (across lags and across centroids), so the first-key and first-label
+    rules decide; pruned and full search agree bit for bit."""
+    n, k = 1201, 12
+    rng = np.random.default_rng(m)
+    C = rng.choice([-1.0, 1.0], size=(k, m)).astype(np.float32)
+    X = C[rng.integers(0, k, n)] * rng.choice([-1.0, 1.0], size=(n, 1)).astype(np.float32)
+    flip = rng.random((n, m)) < 0.2
+    X = np.where(flip, -X, X).astype(np.float32)
+    X[::7, : m // 2] = 1.0  # constant halves: flat correlation plateaus
+    d = ctx.to_device(X)
+    dc = ctx.to_device(C)
+    try:
+        lab, dist, sh = ctx.assign(d, dc)
+        flab, fdist, fsh, _ = ctx.assign(d, dc, dmat=True)
+    finally:
+        dc.free()
         d.free()
     assert np.array_equal(lab, flab)
     assert np.array_equal(dist, fdist)

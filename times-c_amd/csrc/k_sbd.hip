@@ -778,28 +778,47 @@ __device__ __forceinline__ void assign_prune_body(const int bid, const int nblk,
     group_sync<L>();
     float bestA = INFINITY, bestB = hasb ? INFINITY : -INFINITY;
     int labA = 0, labB = 0, keyA = 0, keyB = 0;
+    // one-wave group with k <= 64: centroid j's two bounds stay in lane j's
+    // registers (-1: evaluated), the next candidate is a wave max and a ballot
+    const bool inreg = L == kWave && k <= kWave;  // uniform
+    float myA = -1.f, myB = -1.f;
+    if (inreg && lane < k) {
+      myA = ubs[lane];
+      myB = ubs[k + lane];
+    }
     for (int round = 0; round < k; ++round) {
       // candidate with the largest bound among centroids that can still win
       float sc = -INFINITY;
       int cj = 0x7fffffff;
-      for (int j = lane; j < k; j += L) {  // j increases: strict '>' keeps the first
-        const float ua = ubs[j], ub = ubs[k + j];
-        const bool oka = (ua >= 0.f) & (1.f - (ua * (1.f + kUbRel) + kUbAbs) <= bestA);
-        const bool okb = (ub >= 0.f) & (1.f - (ub * (1.f + kUbRel) + kUbAbs) <= bestB);
-        float v = oka ? ua : -INFINITY;
-        v = okb ? fmaxf(v, ub) : v;
-        const bool take = v > sc;
-        sc = take ? v : sc;
-        cj = take ? j : cj;
-      }
-      if (L == kWave && k <= 16)  // the candidates sit in lanes 0..15
-        group_argmax_row0(sc, cj);
-      else
+      if (inreg) {
+        const bool oka = (myA >= 0.f) & (1.f - (myA * (1.f + kUbRel) + kUbAbs) <= bestA);
+        const bool okb = (myB >= 0.f) & (1.f - (myB * (1.f + kUbRel) + kUbAbs) <= bestB);
+        float v = oka ? myA : -INFINITY;
+        v = okb ? fmaxf(v, myB) : v;
+        sc = wave_max(v);
+        if (sc > -INFINITY) cj = __builtin_ctzll(__ballot(v == sc));  // first j on ties
+      } else {
+        for (int j = lane; j < k; j += L) {  // j increases: strict '>' keeps the first
+          const float ua = ubs[j], ub = ubs[k + j];
+          const bool oka = (ua >= 0.f) & (1.f - (ua * (1.f + kUbRel) + kUbAbs) <= bestA);
+          const bool okb = (ub >= 0.f) & (1.f - (ub * (1.f + kUbRel) + kUbAbs) <= bestB);
+          float v = oka ? ua : -INFINITY;
+          v = okb ? fmaxf(v, ub) : v;
+          const bool take = v > sc;
+          sc = take ? v : sc;
+          cj = take ? j : cj;
+        }
         group_argmax<L>(sc, cj, red);
-      if (sc == -INFINITY) break;
+      }
+      if (!(sc > -INFINITY)) break;
       const int j = cj;
       group_sync<L>();
-      if (lane == 0) {
+      if (inreg) {
+        if (lane == j) {
+          myA = -1.f;  // evaluated
+          myB = -1.f;
+        }
+      } else if (lane == 0) {
         ubs[j] = -1.f;  // evaluated
         ubs[k + j] = -1.f;
       }
@@ -815,9 +834,18 @@ __device__ __forceinline__ void assign_prune_body(const int bid, const int nblk,
         for (int s = 0; s < P; ++s) z[s] = cmulc(zx[s], cjs[s * L + lane]);
         fft<F, L, true>(z, buf, tw, lane);
         float va, vb;
-        lag_argmax2<L, P>(z, s_lo, s_hi, rev, lane, m, F, va, ka, vb, kb);
-        group_argmax<L>(va, ka, red);
-        group_argmax<L>(vb, kb, red);
+        if constexpr (L == kWave) {
+          // m through an empty asm: the row tests are then a few scalar
+          // operations here instead of loop invariants held (and spilled)
+          // across the whole loop
+          int mo = m;
+          asm volatile("" : "+s"(mo));
+          lag_max_wave<P>(z, rev, lane, mo, F, va, ka, vb, kb);
+        } else {
+          lag_argmax2<L, P>(z, s_lo, s_hi, rev, lane, m, F, va, ka, vb, kb);
+          group_argmax<L>(va, ka, red);
+          group_argmax<L>(vb, kb, red);
+        }
         if (dena != 0.f) da = 1.f - (va / (float)F) / dena; else ka = 0;
         if (denb != 0.f) db = 1.f - (vb / (float)F) / denb; else kb = 0;
       }

@@ -510,4 +510,101 @@ __device__ __forceinline__ void lag_argmax2(const float2* z, int s_lo, int s_hi,
   kb = by > -INFINITY ? lag_key(lane + L * sy, m, F, rev) : 0x7fffffff;
 }
 
+// Order-preserving map of a float's bits to a signed integer (a < b as
+// floats => ord(a) < ord(b); -0 maps below +0); its own inverse.
+__device__ __forceinline__ int f32_ord(float v) {
+  const int b = __float_as_int(v);
+  return b ^ ((b >> 31) & 0x7fffffff);
+}
+__device__ __forceinline__ float ord_f32(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7fffffff)); }
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+
+// Wave maximum of v, which must not be NaN (every lane gets it), on the
+// integer images: no NaN canonicalization between the DPP steps.
+__device__ __forceinline__ float wave_max(float v) {
+  int k = f32_ord(v);
+  k = max(k, dpp_i32<0xB1>(k));   // lane ^ 1
+  k = max(k, dpp_i32<0x4E>(k));   // lane ^ 2
+  k = max(k, dpp_i32<0x141>(k));  // mirror in 8 lanes
+  k = max(k, dpp_i32<0x140>(k));  // mirror in 16 lanes
+  const int a = max(__builtin_amdgcn_readlane(k, 0), __builtin_amdgcn_readlane(k, 16));
+  const int b = max(__builtin_amdgcn_readlane(k, 32), __builtin_amdgcn_readlane(k, 48));
+  return ord_f32(max(a, b));
+}
+
+// lag_argmax2 followed by group_argmax of both results, for a one-wave
+// group (L = 64): the same values and keys, fewer instructions.  Each
+// series' maximum g is a per-lane max (NaN never wins, as the scan's '>'
+// never takes it) then a wave max; its key is the first lag holding g in
+// key order, found by one ballot per row visited in key order (rows
+// P/2..P-1 then 0..P/2-1, ascending lanes; mirrored when rev, whose keys
+// run the other way), so ties keep the smallest key; a row holding g is
+// then masked to its lags (tau in [m, F-m] is no lag).  g is first taken
+// over all F positions; only if no lag holds it (every lag below some
+// non-lag: the non-lags hold the circular wrap, about 0) is it taken again
+// over the lags alone.
+template <int P>
+__device__ __forceinline__ void lag_max_wave(const float2 (&z)[P], int rev, int lane, int m,
+                                             int F, float& va, int& ka, float& vb, int& kb) {
+  constexpr int L = kWave;
+  auto is_lag = [&](int s) {
+    const int tau = lane + L * s;
+    return tau <= m - 1 || tau >= F - m + 1;
+  };
+  auto find = [&](float g, bool isx, int& key) -> bool {
+    key = 0x7fffffff;
+    if (!(g > -INFINITY)) return true;  // no lag taken (uniform)
+    auto row = [&](int s) -> bool {
+      uint64_t b = __ballot((isx ? z[s].x : z[s].y) == g);
+      if (!b) return false;
+      b &= __ballot(is_lag(s));  // only on a row holding g
+      if (!b) return false;
+      const int ln = rev ? 63 - __builtin_clzll(b) : __builtin_ctzll(b);
+      key = lag_key(ln + L * s, m, F, rev);
+      return true;
+    };
+    if (!rev) {
+#pragma unroll
+      for (int s = P / 2; s < P; ++s)
+        if (row(s)) return true;
+#pragma unroll
+      for (int s = 0; s < P / 2; ++s)
+        if (row(s)) return true;
+    } else {
+#pragma unroll
+      for (int s = P / 2 - 1; s >= 0; --s)
+        if (row(s)) return true;
+#pragma unroll
+      for (int s = P - 1; s >= P / 2; --s)
+        if (row(s)) return true;
+    }
+    return false;
+  };
+  float mx = -INFINITY, my = -INFINITY;
+#pragma unroll
+  for (int s = 0; s < P; ++s) {
+    mx = fmaxf(mx, z[s].x);
+    my = fmaxf(my, z[s].y);
+  }
+  va = wave_max(mx);
+  vb = wave_max(my);
+  if (find(va, true, ka) && find(vb, false, kb)) return;
+  // rare: the maximum over the lags alone
+  mx = -INFINITY;
+  my = -INFINITY;
+#pragma unroll
+  for (int s = 0; s < P; ++s) {
+    const bool ok = is_lag(s);
+    mx = ok ? fmaxf(mx, z[s].x) : mx;
+    my = ok ? fmaxf(my, z[s].y) : my;
+  }
+  va = wave_max(mx);
+  vb = wave_max(my);
+  find(va, true, ka);
+  find(vb, false, kb);
+}
+
 }  // namespace tcx
