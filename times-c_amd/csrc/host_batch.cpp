@@ -212,7 +212,11 @@ int tcx_batch_run_grouped(tcx_batch* b, tcx_job* jobs, int njobs) {
       load[w] += job_cost(jobs[i]);
     }
     size_t budget = b->budget;
-    if (!budget) budget = tcx::device_free_bytes(b->ctx[0]) / 10 * 8;
+    if (!budget) {  // most of the free memory and of what the cohorts keep for reuse
+      size_t kept = 0;
+      for (int w = 0; w < W; ++w) kept += tcx::ctx_buffer_device_bytes(b->ctx[w]);
+      budget = (tcx::device_free_bytes(b->ctx[0]) + kept) / 10 * 8;
+    }
     std::vector<int> rcs(W, TCX_OK);
     std::vector<std::string> msgs(W);
     auto work = [&](int w) {

@@ -176,13 +176,18 @@ size_t layout(GJob& j, unsigned char* base) {
 // Host-assembled job tables uploaded by one copy (through a pinned area)
 // into a grow-only device buffer.  Reused only after the stream has
 // consumed the previous upload (every upload follows a synchronization).
+// The buffers are the context's (ctx_buffer ids dev_id, pin_id), kept
+// between calls.
 class Uploader {
  public:
   ~Uploader() {
     if (copied_) (void)hipEventSynchronize(copied_);
     if (copied_) (void)hipEventDestroy(copied_);
-    if (pin_) (void)hipHostFree(pin_);
-    if (dev_) (void)hipFree(dev_);
+  }
+  void bind(tcx_ctx* c, int dev_id, int pin_id) {
+    ctx_ = c;
+    dev_id_ = dev_id;
+    pin_id_ = pin_id;
   }
   void reset() { buf_.clear(); }
   size_t put(const void* p, size_t bytes) {
@@ -199,21 +204,20 @@ class Uploader {
     const size_t n = buf_.size();
     if (n == 0) return TCX_OK;
     if (n > dev_cap_ || n > pin_cap_) {
+      // growing frees the old buffers (hipFree waits for the device)
       GX_HIP(hipStreamSynchronize(st));
+      if (copied_) GX_HIP(hipEventSynchronize(copied_));
       const size_t cap = std::max(n * 2, (size_t)1 << 20);
       if (n > dev_cap_) {
-        if (dev_) GX_HIP(hipFree(dev_));
         dev_ = nullptr;
         dev_cap_ = 0;
-        if (hipMalloc(&dev_, cap) != hipSuccess)
-          return fail(TCX_ERR_NOMEM, "grouped batch: table buffer allocation failed");
+        GX_TRY(ctx_buffer(ctx_, dev_id_, cap, false, &dev_));
         dev_cap_ = cap;
       }
       if (n > pin_cap_) {
-        if (pin_) GX_HIP(hipHostFree(pin_));
         pin_ = nullptr;
         pin_cap_ = 0;
-        GX_HIP(hipHostMalloc(&pin_, cap, hipHostMallocDefault));
+        GX_TRY(ctx_buffer(ctx_, pin_id_, cap, true, &pin_));
         pin_cap_ = cap;
       }
     }
@@ -230,6 +234,8 @@ class Uploader {
 
  private:
   std::vector<unsigned char> buf_;
+  tcx_ctx* ctx_ = nullptr;
+  int dev_id_ = 0, pin_id_ = 0;
   void* dev_ = nullptr;
   void* pin_ = nullptr;
   hipEvent_t copied_ = nullptr;
@@ -282,18 +288,26 @@ struct Keyed {
   }
 };
 
+// Context buffers (ctx_buffer ids) of the executor.
+enum CtxBuf {
+  kCbSlab, kCbGinfo, kCbInfo, kCbEvc, kCbGinfoH, kCbInfoH, kCbEvcH, kCbDoneD, kCbDoneH,
+  kCbOutH, kCbInitH, kCbUpDev, kCbUpPin, kCbAsgDev, kCbAsgPin, kCbCount
+};
+static_assert(kCbCount <= kCtxBuffers, "context buffers");
+
 struct Batch {
   tcx_ctx* ctx = nullptr;
   hipStream_t st = nullptr;
   // second stream: the block-form Gram Lanczos runs beside the one-wave
   // form (default priority; joined back into st by an event)
   hipStream_t st2 = nullptr;
-  bool own_st2 = true;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int cus = 0;
   std::vector<GJob> jobs;
   Uploader up;
-  // readback pools (device + pinned host), sized for every job of the run
+  // readback pools (device + pinned host), sized for every job of the run;
+  // these, the done flags, the slab and the pinned staging are the
+  // context's buffers (kept between calls)
   unsigned char *ginfo_d = nullptr, *info_d = nullptr, *evc_d = nullptr;
   unsigned char *ginfo_h = nullptr, *info_h = nullptr, *evc_h = nullptr;
   size_t ginfo_bytes = 0, info_bytes = 0, evc_bytes = 0;
@@ -314,20 +328,12 @@ struct Batch {
   std::vector<GroupSyrkTile> tiles;
 
   ~Batch() {
+    if (st) (void)hipStreamSynchronize(st);
     if (st2) (void)hipStreamSynchronize(st2);
-    if (st2 && own_st2) (void)hipStreamDestroy(st2);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
     for (GJob& j : jobs)
       if (j.arena && !slab) (void)hipFree(j.arena);
-    if (slab) (void)hipFree(slab);
-    for (unsigned char* p : {ginfo_d, info_d, evc_d})
-      if (p) (void)hipFree(p);
-    for (unsigned char* p : {ginfo_h, info_h, evc_h})
-      if (p) (void)hipHostFree(p);
-    if (done_h) (void)hipHostFree(done_h);
-    if (done_d) (void)hipFree(done_d);
-    if (out_h) (void)hipHostFree(out_h);
     if (asg_exec) (void)hipGraphExecDestroy(asg_exec);
   }
 
@@ -386,7 +392,11 @@ int setup_jobs(Batch& b, const std::vector<GJob*>& fresh) {
   long total = 0;
   for (GJob* j : fresh) total += j->n;
   int* pin = nullptr;
-  GX_HIP(hipHostMalloc((void**)&pin, sizeof(int) * std::max(total, 1L), hipHostMallocDefault));
+  {
+    void* p = nullptr;
+    GX_TRY(ctx_buffer(b.ctx, kCbInitH, sizeof(int) * std::max(total, 1L), true, &p));
+    pin = static_cast<int*>(p);
+  }
   long at = 0;
   int rc = TCX_OK;
   for (GJob* j : fresh) {
@@ -401,7 +411,6 @@ int setup_jobs(Batch& b, const std::vector<GJob*>& fresh) {
   }
   if (hipStreamSynchronize(b.st) != hipSuccess && rc == TCX_OK)
     rc = fail(TCX_ERR_HIP, "grouped batch: initial label upload failed");
-  GX_HIP(hipHostFree(pin));
   return rc;
 }
 
@@ -474,14 +483,17 @@ int power_split(Batch& b, const std::vector<GJob*>& set, int mode) {
     ndone += j->k;
   }
   if ((size_t)ndone > b.done_cap) {
-    if (b.done_h) GX_HIP(hipHostFree(b.done_h));
-    if (b.done_d) GX_HIP(hipFree(b.done_d));
+    // growing frees the old buffers: nothing in flight may use them
+    GX_HIP(hipStreamSynchronize(b.st));
+    GX_HIP(hipStreamSynchronize(b.st2));
     b.done_h = nullptr;
     b.done_d = nullptr;
     b.done_cap = 0;
-    GX_HIP(hipHostMalloc((void**)&b.done_h, sizeof(int) * ndone, hipHostMallocDefault));
-    if (hipMalloc((void**)&b.done_d, sizeof(int) * ndone) != hipSuccess)
-      return fail(TCX_ERR_NOMEM, "grouped batch: done flags allocation failed");
+    void *ph = nullptr, *pd = nullptr;
+    GX_TRY(ctx_buffer(b.ctx, kCbDoneH, sizeof(int) * ndone, true, &ph));
+    GX_TRY(ctx_buffer(b.ctx, kCbDoneD, sizeof(int) * ndone, false, &pd));
+    b.done_h = static_cast<int*>(ph);
+    b.done_d = static_cast<int*>(pd);
     b.done_cap = (size_t)ndone;
   }
   Launch<ClusterMatvecJob> mv_r, mv_v;
@@ -935,10 +947,13 @@ int grouped_fit(tcx_ctx* ctx, tcx_job* jobs, const int* idx, int count, size_t h
   GX_HIP(hipSetDevice(device_of(ctx)));
   if (second_stream) {
     b.st2 = static_cast<hipStream_t>(second_stream);  // the caller's (idle) stream
-    b.own_st2 = false;
   } else {
-    GX_HIP(hipStreamCreateWithFlags(&b.st2, hipStreamNonBlocking));
+    void* st2 = nullptr;
+    GX_TRY(ctx_aux_stream(ctx, &st2));
+    b.st2 = static_cast<hipStream_t>(st2);
   }
+  b.up.bind(ctx, kCbUpDev, kCbUpPin);
+  b.up_asg.bind(ctx, kCbAsgDev, kCbAsgPin);
   GX_HIP(hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming));
   GX_HIP(hipEventCreateWithFlags(&b.ev_join, hipEventDisableTiming));
   b.jobs.resize(count);
@@ -991,15 +1006,17 @@ int grouped_fit(tcx_ctx* ctx, tcx_job* jobs, const int* idx, int count, size_t h
   unsigned char** pd[3] = {&b.ginfo_d, &b.info_d, &b.evc_d};
   unsigned char** ph[3] = {&b.ginfo_h, &b.info_h, &b.evc_h};
   for (int i = 0; i < 3; ++i) {
-    if (hipMalloc((void**)pd[i], pool[i]) != hipSuccess)
-      return fail(TCX_ERR_NOMEM, "grouped batch: readback pool allocation failed");
-    GX_HIP(hipHostMalloc((void**)ph[i], pool[i], hipHostMallocDefault));
+    void *d = nullptr, *h = nullptr;
+    GX_TRY(ctx_buffer(ctx, kCbGinfo + i, pool[i], false, &d));
+    GX_TRY(ctx_buffer(ctx, kCbGinfoH + i, pool[i], true, &h));
+    *pd[i] = static_cast<unsigned char*>(d);
+    *ph[i] = static_cast<unsigned char*>(h);
   }
   size_t budget = hbm_budget;
-  if (!budget) {
+  if (!budget) {  // most of the free memory and of what this context keeps for reuse
     size_t free_b = 0, total_b = 0;
     GX_HIP(hipMemGetInfo(&free_b, &total_b));
-    budget = free_b / 10 * 8;
+    budget = (free_b + ctx_buffer_device_bytes(ctx)) / 10 * 8;
   }
   size_t used = 0;
   {
@@ -1009,10 +1026,11 @@ int grouped_fit(tcx_ctx* ctx, tcx_job* jobs, const int* idx, int count, size_t h
         j.slab_off = total;
         total += align_up(j.arena_bytes);
       }
-    if (total > 0 && total <= budget && hipMalloc((void**)&b.slab, total) != hipSuccess) {
-      (void)hipGetLastError();
-      b.slab = nullptr;  // per-job allocations instead
-    }
+    void* p = nullptr;
+    if (total > 0 && total <= budget && ctx_buffer(ctx, kCbSlab, total, false, &p) == TCX_OK)
+      b.slab = static_cast<unsigned char*>(p);
+    else
+      clear_error();  // per-job allocations instead
   }
   for (;;) {
     // admit waiting jobs while their arenas fit (largest first, so the
@@ -1105,10 +1123,11 @@ int grouped_fit(tcx_ctx* ctx, tcx_job* jobs, const int* idx, int count, size_t h
         need += align_up(sizeof(int) * j->n) + align_up(sizeof(float) * j->k * j->m);
       if (need > b.out_cap) {
         GX_HIP(hipStreamSynchronize(b.st));
-        if (b.out_h) GX_HIP(hipHostFree(b.out_h));
         b.out_h = nullptr;
         b.out_cap = 0;
-        GX_HIP(hipHostMalloc((void**)&b.out_h, need * 2, hipHostMallocDefault));
+        void* p = nullptr;
+        GX_TRY(ctx_buffer(b.ctx, kCbOutH, need * 2, true, &p));
+        b.out_h = static_cast<unsigned char*>(p);
         b.out_cap = need * 2;
       }
       size_t at = 0;

@@ -112,6 +112,11 @@ struct tcx_ctx {
   long long wall_khz = 0;      // wall_clock64() rate
   void* pinned = nullptr;      // page-locked host area for small readbacks
   size_t pinned_bytes = 0;
+  // the grouped executor's buffers kept between calls, and its second stream
+  void* cb_ptr[tcx::kCtxBuffers] = {};
+  size_t cb_bytes[tcx::kCtxBuffers] = {};
+  bool cb_pinned[tcx::kCtxBuffers] = {};
+  hipStream_t aux_stream = nullptr;
 };
 
 namespace {
@@ -720,6 +725,54 @@ int stream_sync(tcx_ctx* c) {
 
 size_t held_bytes(const tcx_ctx* c) { return c->ws_bytes + c->user_bytes; }
 
+namespace {
+hipError_t release_ctx_buffer(tcx_ctx* c, int id) {
+  hipError_t e = hipSuccess;
+  if (c->cb_ptr[id]) {
+    e = c->cb_pinned[id] ? hipHostFree(c->cb_ptr[id]) : hipFree(c->cb_ptr[id]);
+    if (!c->cb_pinned[id]) c->ws_bytes -= c->cb_bytes[id];
+  }
+  c->cb_ptr[id] = nullptr;
+  c->cb_bytes[id] = 0;
+  return e;
+}
+}  // namespace
+
+int ctx_buffer(tcx_ctx* c, int id, size_t bytes, bool pinned, void** out) {
+  if (id < 0 || id >= kCtxBuffers) return fail(TCX_ERR_ARG, "context buffer: bad id");
+  bytes = std::max<size_t>(bytes, 1);
+  if (c->cb_ptr[id] && c->cb_bytes[id] >= bytes && c->cb_pinned[id] == pinned) {
+    *out = c->cb_ptr[id];
+    return TCX_OK;
+  }
+  TCX_HIP(release_ctx_buffer(c, id));
+  void* p = nullptr;
+  const hipError_t e = pinned ? hipHostMalloc(&p, bytes, hipHostMallocDefault) : hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(TCX_ERR_NOMEM, "context buffer allocation failed");
+  }
+  c->cb_ptr[id] = p;
+  c->cb_bytes[id] = bytes;
+  c->cb_pinned[id] = pinned;
+  if (!pinned) c->ws_bytes += bytes;
+  *out = p;
+  return TCX_OK;
+}
+
+size_t ctx_buffer_device_bytes(const tcx_ctx* c) {
+  size_t b = 0;
+  for (int i = 0; i < kCtxBuffers; ++i)
+    if (!c->cb_pinned[i]) b += c->cb_bytes[i];
+  return b;
+}
+
+int ctx_aux_stream(tcx_ctx* c, void** out) {
+  if (!c->aux_stream) TCX_HIP(hipStreamCreateWithFlags(&c->aux_stream, hipStreamNonBlocking));
+  *out = c->aux_stream;
+  return TCX_OK;
+}
+
 size_t device_free_bytes(tcx_ctx* c) {
   if (set_dev(c) != TCX_OK) return 0;
   size_t free_b = 0, total_b = 0;
@@ -754,6 +807,8 @@ int trim_workspace(tcx_ctx* c) {
     c->ws_bytes -= sizeof(float2) * kv.first;
   }
   c->twiddles.clear();
+  if (c->aux_stream) TCX_HIP(hipStreamSynchronize(c->aux_stream));
+  for (int i = 0; i < kCtxBuffers; ++i) TCX_HIP(release_ctx_buffer(c, i));
   // a new exchange buffer must be zeroed before its first use, even at the
   // same address
   c->lz_xch = nullptr;
@@ -866,6 +921,9 @@ int tcx_destroy(tcx_ctx* c) {
     keep(hipEventDestroy(c->coop_ev));
   }
   if (c->pinned) keep(hipHostFree(c->pinned));
+  if (c->aux_stream) keep(hipStreamSynchronize(c->aux_stream));
+  for (int i = 0; i < tcx::kCtxBuffers; ++i) keep(tcx::release_ctx_buffer(c, i));
+  if (c->aux_stream) keep(hipStreamDestroy(c->aux_stream));
   if (c->stream) keep(hipStreamDestroy(c->stream));
   delete c;
   if (e != hipSuccess) return fail(TCX_ERR_HIP, std::string("destroy: ") + hipGetErrorString(e));

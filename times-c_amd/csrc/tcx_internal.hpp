@@ -36,6 +36,20 @@ int stream_sync(tcx_ctx* c);
 size_t held_bytes(const tcx_ctx* c);
 // Free device memory of the context's GPU (0 when it cannot be read).
 size_t device_free_bytes(tcx_ctx* c);
+// Grow-only buffers the context keeps between calls of the grouped executor
+// (device memory, or page-locked host memory when pinned), so that repeated
+// batches allocate and free nothing; buffer `id` < kCtxBuffers holds at
+// least `bytes` afterwards.  Contents are not kept when a buffer grows, and
+// growing frees the old one: only while nothing in flight uses it.  Device
+// bytes count as workspace (held_bytes); trim_workspace and tcx_destroy
+// release them.
+constexpr int kCtxBuffers = 16;
+int ctx_buffer(tcx_ctx* c, int id, size_t bytes, bool pinned, void** out);
+// Device bytes the context's kept buffers hold (reusable by the next call).
+size_t ctx_buffer_device_bytes(const tcx_ctx* c);
+// A second stream of the context, created on first use (default priority,
+// non-blocking) and destroyed with the context.
+int ctx_aux_stream(tcx_ctx* c, void** out);
 // Frees every workspace slot and twiddle table of an idle context (no fit
 // in progress); the next call grows them again.
 int trim_workspace(tcx_ctx* c);
