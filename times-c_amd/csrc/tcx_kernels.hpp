@@ -184,13 +184,15 @@ struct LzState {
 constexpr int kFusedLanczosMaxM = 768;
 // Acceptance of the Lanczos evaluation of M^I v0: y = T_j^I e1 is formed at
 // j = kLzFirstCheck and every kLzCheckEvery steps after, and accepted when
-// it moved by <= kLzPowerTol since the previous check (for a geometric
+// it moved by <= kLzPowerTol since the previous check.  For a geometric
 // convergence rate rho per step the accepted y is then within about
-// rho^4 / (1 - rho^4) * kLzPowerTol of the limit, far below the ~1e-7
-// relative error that the FP32 S itself leaves in the centroid).
+// rho^4 / (1 - rho^4) * kLzPowerTol of the limit.  S itself, in FP32 from
+// split-BF16 products, leaves the centroids about 2e-6 (relative L2) from
+// the FP64 oracle's at n=100000, m=512, k=16; 1e-7 leaves that and the SBD
+// error unchanged (1e-9 needed 28 steps there, 1e-7 needs 24).
 constexpr int kLzFirstCheck = 12;
 constexpr int kLzCheckEvery = 4;
-constexpr double kLzPowerTol = 1e-9;
+constexpr double kLzPowerTol = 1e-7;
 __host__ __device__ constexpr bool lz_power_check(int len) {
   return len >= kLzFirstCheck && len % kLzCheckEvery == 0;
 }
