@@ -570,9 +570,15 @@ int power_split(Batch& b, const std::vector<GJob*>& set, int mode) {
 }
 
 // A cluster takes the Gram form when its padded segment nd is shorter than
-// m: every Lanczos step then moves nd^2 < m^2 values.
+// m: every Lanczos step then moves nd^2 < m^2 values.  Above order
+// kGramBlockMaxND the Gram Lanczos is one block streaming nd^2 values per
+// step, slower than the S form's two-launch steps over the whole GPU (on
+// the 256-job uneven batch: no limit 190 ms, 1024 182 ms, 768 185 ms, 512
+// 191 ms, 256 229 ms).
+constexpr long kGramBlockMaxND = 1024;
 bool gram_form(int m, long nd, int count) {
-  return count > 0 && m <= kGramMaxM && nd > 0 && nd < m && nd <= kGramMaxND;
+  return count > 0 && m <= kGramMaxM && nd > 0 && nd < m && nd <= kGramMaxND &&
+         nd <= kGramBlockMaxND;
 }
 
 // S = Y^T Y items (one K chunk per tile) of the clusters with sel[c].
