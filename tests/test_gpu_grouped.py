@@ -32,9 +32,10 @@ from parity import CENT_RTOL, rel_l2
 pytestmark = pytest.mark.gpu
 
 # (n, m, k), seed: F = 16 .. 4096; clusters shorter than m (in padded
-# members) take the Gram form (one wave per cluster up to order 256, one
-# block above), the others the S form (one wave for m <= 256, the two-launch
-# form above: (1200, 300, 2)); m > 1024 takes the align + pack from HBM;
+# members, up to 1024) take the Gram form (one wave per cluster up to order
+# 256, one block above), the others the S form (one wave for m <= 256, the
+# two-launch form above: (1200, 300, 2)); m > 1024 takes the align + pack
+# from HBM;
 # k > 16 the wider bound product.  On these seeds the oracle's trajectory
 # has no near-tie: in every
 # iteration the best and second-best SBD of every series, and the best and
@@ -122,9 +123,11 @@ def test_grouped_trajectories_match_oracle(ctx):
 # m > 256 (the two-launch form, m <= 768 and above), thin clusters whose
 # Gram order exceeds 256 (one block per cluster), and the one-wave form
 # (Gram and S).  Free of alignment-lag ties, so any seed does.
+# (2400, 1800, 2): clusters of about 1200 members, fewer than m but above
+# the Gram block form's limit of 1024: the S form
 FIRST_ITER = [((1200, 300, 2), 1801), ((1700, 780, 2), 1802), ((2500, 900, 2), 1803),
               ((1600, 1200, 2), 1804), ((700, 900, 2), 1805), ((600, 128, 4), 1806),
-              ((900, 700, 6), 1807)]
+              ((900, 700, 6), 1807), ((2400, 1800, 2), 1808)]
 
 
 def test_grouped_first_iteration_matches_oracle(ctx):
