@@ -14,6 +14,8 @@ over all jobs in progress), through the C ABI:
     oracle's refinement and assignment;
   - the collapse stop, and jobs the executor leaves to the worker pool
     (centre initialisation, eigenvector centroids) in the same call;
+  - a batch object reused for different job sets gives each set's
+    fresh-batch results (the buffers kept between calls carry nothing);
   - cohorts (batches of 32 jobs or more split over up to
     TCX_GROUPED_COHORTS contexts, with a second stream created or borrowed
     from an idle worker): every layout gives the one-cohort results bit for
@@ -199,6 +201,45 @@ def test_grouped_cohorts_equal_one_cohort(ctx, monkeypatch, workers, cohorts):
         assert np.array_equal(a["labels_per_iter"], b["labels_per_iter"]), i
         assert np.array_equal(a["centroids"], b["centroids"]), i
         assert a["stats"]["iterations"] == b["stats"]["iterations"], i
+
+
+def test_grouped_reused_batch_equals_fresh_batches(ctx):
+    """One batch object runs job set A (40 jobs: two cohorts), then a smaller
+    set B of other shapes, then A again: the executor's buffers, kept in
+    the worker contexts between calls, carry nothing from one run into the
+    next; every run equals the same set on a fresh batch, bit for bit."""
+    sa = [(150 + 41 * i, (40, 96, 200, 320)[i % 4], 2 + i % 6) for i in range(40)]
+    sb = [(900 + 13 * i, (700, 64, 1300)[i % 3], 3 + i % 4) for i in range(7)]
+    seeds_a = [5000 + i for i in range(40)]
+    seeds_b = [6000 + i for i in range(7)]
+    da, specs_a, _ = _specs(ctx, sa, seeds_a, labels_per_iter=True)
+    db, specs_b, _ = _specs(ctx, sb, seeds_b, labels_per_iter=True)
+    Xa = [d.numpy() for d in da]
+    Xb = [d.numpy() for d in db]
+
+    def reload():
+        for d, X in zip(da + db, Xa + Xb):
+            ctx.h2d(d, X)
+
+    try:
+        fresh = []
+        for specs in (specs_a, specs_b):
+            reload()
+            with T.Batch(0, workers=4) as b:
+                fresh.append(b.run(specs, grouped=True))
+        with T.Batch(0, workers=4) as b:
+            runs = []
+            for specs in (specs_a, specs_b, specs_a):
+                reload()
+                runs.append(b.run(specs, grouped=True))
+    finally:
+        for d in da + db:
+            d.free()
+    for got, want in zip(runs, (fresh[0], fresh[1], fresh[0])):
+        for i, (a, r) in enumerate(zip(got, want)):
+            assert a["status"] == 0 and r["status"] == 0, (i, a["message"], r["message"])
+            assert np.array_equal(a["labels_per_iter"], r["labels_per_iter"]), i
+            assert np.array_equal(a["centroids"], r["centroids"]), i
 
 
 def test_grouped_256_uneven_jobs(ctx):
