@@ -127,10 +127,12 @@ def test_pruned_assign_matches_oracle(ctx, m, n, k):
 
 
 # F = 256 (16 lanes), 1024 (one wave: m = 260 leaves 505 non-lag positions
-# over rows 4..11, m = 512 one), 4096 (256 lanes); k up to 64 keeps a
-# one-wave group's bounds in registers, k = 80 in LDS
+# over rows 4..11, m = 512 one), 2048, 4096 and 8192 (two, four and eight
+# waves per group); k up to 64 keeps a one-wave group's bounds in
+# registers, k = 80 in LDS
 @pytest.mark.parametrize("m,n,k", [(96, 3001, 12), (2048, 257, 6), (260, 2001, 16),
-                                   (512, 3001, 16), (300, 1501, 64), (400, 801, 80)])
+                                   (512, 3001, 16), (300, 1501, 64), (400, 801, 80),
+                                   (700, 1201, 12), (1030, 601, 9), (3000, 301, 5)])
 def test_pruned_assign_equals_full_search(ctx, m, n, k):
     """Pruned and full assignment of the same inputs agree bit for bit on
     labels and distances (both pair rows 2p, 2p+1 in one transform)."""
@@ -151,7 +153,7 @@ def test_pruned_assign_equals_full_search(ctx, m, n, k):
     assert np.array_equal(sh, fsh)
 
 
-@pytest.mark.parametrize("m", [260, 512])
+@pytest.mark.parametrize("m", [260, 512, 700, 2048])
 def test_pruned_assign_equals_full_search_on_ties(ctx, m):
     """Series and centroids of +-1 values: many exactly equal correlations
     (across lags and across centroids), so the first-key and first-label

@@ -841,6 +841,10 @@ __device__ __forceinline__ void assign_prune_body(const int bid, const int nblk,
           int mo = m;
           asm volatile("" : "+s"(mo));
           lag_max_wave<P>(z, rev, lane, mo, F, va, ka, vb, kb);
+        } else if constexpr (L > kWave) {
+          int mo = m;
+          asm volatile("" : "+s"(mo));
+          lag_max_group<L, P>(z, rev, lane, mo, F, red, va, ka, vb, kb);
         } else {
           lag_argmax2<L, P>(z, s_lo, s_hi, rev, lane, m, F, va, ka, vb, kb);
           group_argmax<L>(va, ka, red);

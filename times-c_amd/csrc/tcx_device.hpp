@@ -607,4 +607,113 @@ __device__ __forceinline__ void lag_max_wave(const float2 (&z)[P], int rev, int 
   find(vb, false, kb);
 }
 
+// lag_max_wave for a group of L = 64 * NW lanes (NW waves, the whole
+// block, P values per lane): each wave takes the maximum of its lanes; the
+// group maximum g goes through LDS (`red`, 2 * NW words); each wave then
+// finds its first lag holding g in key order over its own lanes (rows in
+// key order; within a row the keys run with the group lane, so the first
+// key of the group is the smallest of the waves' first keys), and the
+// smallest key goes through LDS again.  If no lag holds g, the same over
+// the lags alone.  Equal to lag_argmax2 followed by group_argmax.
+template <int L, int P>
+__device__ __forceinline__ void lag_max_group(const float2 (&z)[P], int rev, int glane, int m,
+                                              int F, float* red, float& va, int& ka, float& vb,
+                                              int& kb) {
+  static_assert(L > kWave && L % kWave == 0, "multi-wave group");
+  constexpr int NW = L / kWave;
+  const int w = glane / kWave, lane = glane % kWave;
+  int* ir = reinterpret_cast<int*>(red);
+  auto is_lag = [&](int s) {
+    const int tau = glane + L * s;
+    return tau <= m - 1 || tau >= F - m + 1;
+  };
+  // the group maxima of the two series from per-lane maxima
+  auto group_max = [&](float mx, float my, float& gx, float& gy) {
+    const float wx = wave_max(mx), wy = wave_max(my);
+    __syncthreads();  // earlier users of red are done
+    if (lane == 0) {
+      ir[w] = f32_ord(wx);
+      ir[NW + w] = f32_ord(wy);
+    }
+    __syncthreads();
+    int ax = ir[0], ay = ir[NW];
+#pragma unroll
+    for (int u = 1; u < NW; ++u) {
+      ax = max(ax, ir[u]);
+      ay = max(ay, ir[NW + u]);
+    }
+    gx = ord_f32(ax);
+    gy = ord_f32(ay);
+  };
+  // this wave's first lag key holding g (INT_MAX if none)
+  auto find = [&](float g, bool isx) -> int {
+    if (!(g > -INFINITY)) return 0x7fffffff;
+    auto row = [&](int s, int& key) -> bool {
+      uint64_t b = __ballot((isx ? z[s].x : z[s].y) == g);
+      if (!b) return false;
+      b &= __ballot(is_lag(s));
+      if (!b) return false;
+      const int ln = rev ? 63 - __builtin_clzll(b) : __builtin_ctzll(b);
+      key = lag_key(ln + kWave * w + L * s, m, F, rev);
+      return true;
+    };
+    int key = 0x7fffffff;
+    if (!rev) {
+#pragma unroll
+      for (int s = P / 2; s < P; ++s)
+        if (row(s, key)) return key;
+#pragma unroll
+      for (int s = 0; s < P / 2; ++s)
+        if (row(s, key)) return key;
+    } else {
+#pragma unroll
+      for (int s = P / 2 - 1; s >= 0; --s)
+        if (row(s, key)) return key;
+#pragma unroll
+      for (int s = P - 1; s >= P / 2; --s)
+        if (row(s, key)) return key;
+    }
+    return key;
+  };
+  // the smallest of the waves' keys
+  auto group_min_keys = [&](int kx, int ky, int& ox, int& oy) {
+    __syncthreads();
+    if (lane == 0) {
+      ir[w] = kx;
+      ir[NW + w] = ky;
+    }
+    __syncthreads();
+    int ax = ir[0], ay = ir[NW];
+#pragma unroll
+    for (int u = 1; u < NW; ++u) {
+      ax = min(ax, ir[u]);
+      ay = min(ay, ir[NW + u]);
+    }
+    ox = ax;
+    oy = ay;
+  };
+  float mx = -INFINITY, my = -INFINITY;
+#pragma unroll
+  for (int s = 0; s < P; ++s) {
+    mx = fmaxf(mx, z[s].x);
+    my = fmaxf(my, z[s].y);
+  }
+  group_max(mx, my, va, vb);
+  group_min_keys(find(va, true), find(vb, false), ka, kb);
+  const bool miss_a = va > -INFINITY && ka == 0x7fffffff;
+  const bool miss_b = vb > -INFINITY && kb == 0x7fffffff;
+  if (miss_a || miss_b) {  // uniform: the maximum over the lags alone
+    mx = -INFINITY;
+    my = -INFINITY;
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+      const bool ok = is_lag(s);
+      mx = ok ? fmaxf(mx, z[s].x) : mx;
+      my = ok ? fmaxf(my, z[s].y) : my;
+    }
+    group_max(mx, my, va, vb);
+    group_min_keys(find(va, true), find(vb, false), ka, kb);
+  }
+}
+
 }  // namespace tcx
