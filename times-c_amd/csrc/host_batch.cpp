@@ -14,6 +14,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -229,9 +230,18 @@ int tcx_batch_run_grouped(tcx_batch* b, tcx_job* jobs, int njobs) {
                                 budget / W, t0, second);
       if (rcs[w] != TCX_OK) msgs[w] = tcx_last_error();  // thread-local
     };
+    // a cohort whose thread cannot be started runs on this thread afterwards
     std::vector<std::thread> th;
-    for (int w = 1; w < W; ++w) th.emplace_back(work, w);
+    std::vector<int> inline_w;
+    for (int w = 1; w < W; ++w) {
+      try {
+        th.emplace_back(work, w);
+      } catch (const std::system_error&) {
+        inline_w.push_back(w);
+      }
+    }
     work(0);
+    for (int w : inline_w) work(w);
     for (std::thread& t : th) t.join();
     for (int w = 0; w < W; ++w) {
       if (rcs[w] == TCX_OK) continue;
