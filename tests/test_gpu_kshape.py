@@ -168,3 +168,45 @@ def test_fit_stops_on_collapse_like_the_gpu_loop(ctx, init_mode):
     assert np.array_equal(res["labels"], hist[-1])
     assert res["stats"]["sum_class"] == k // 2
     assert np.array_equal(idx, hist[-1] + 1) and sc.value == k // 2 and it.value == 10
+
+
+def test_lanczos_wait_limit_falls_back_to_power_iteration(ctx, monkeypatch):
+    """The multi-block Lanczos (m = 512, 8 clusters x 16 member blocks)
+    with a negative wait limit set from the host: every member gives up at
+    its first wait and exits, the clusters report "not accepted" and run
+    the stepwise power iteration.  One refinement from the seed's labels:
+    both fits' centroids are within the parity tolerance of the FP64
+    oracle's refinement, and each fit's labels are the FP64 assignment of
+    its own centroids up to SBD ties."""
+    from parity import assert_labels_match
+
+    n, m, k = 6000, 512, 8
+    X, _ = T.generate(31, n, m, k, 0.3)
+
+    def fit():
+        d = ctx.to_device(X)
+        try:
+            ctx.znorm(d, ddof=0)
+            r = ctx.fit(d, k, max_iter=1, seed=31)
+            Z = d.numpy().astype(np.float64)
+        finally:
+            d.free()
+        return r, Z
+
+    ref, Z = fit()
+    monkeypatch.setenv("TCX_LZ_COOP_TIMEOUT_US", "-1")
+    got, _ = fit()
+    print(f"power steps: Lanczos {ref['stats']['power_steps_max']}, forced "
+          f"{got['stats']['power_steps_max']}", flush=True)
+    # the stepwise iteration ran: up to min(m, 1000) steps, stopping at its
+    # own tolerance, far more than the Lanczos steps
+    assert ref["stats"]["power_steps_max"] < 64
+    assert 64 < got["stats"]["power_steps_max"] <= m
+    cent, _, _ = O.refine(Z, T.init_labels(31, n, k), np.zeros((k, m)), seed=31, it=0)
+    for r, what in ((got, "forced fallback"), (ref, "Lanczos fit")):
+        errs = [rel_l2(a, b) for a, b in zip(r["centroids"], cent) if np.any(b)]
+        print(f"  {what}: centroids vs oracle max rel L2 {max(errs):.2e}", flush=True)
+        assert max(errs) <= CENT_RTOL, (what, max(errs))
+        olab, _, _, odm = O.assign(Z, r["centroids"].astype(np.float64), dmat=True)
+        assert_labels_match(r["labels"], olab, odm, what=what)
+

@@ -2133,6 +2133,7 @@ __device__ __forceinline__ bool poll_tagged(const uint64_t* p, long stride, int 
                                             double* v, const uint32_t* fail, uint32_t gen,
                                             long long timeout) {
   if (n <= 0) return true;
+  if (timeout < 0) return false;  // a negative limit: give up at once
   const long long t0 = wall_clock64();
   for (int round = 1;; ++round) {
     uint64_t h[N], l[N];
@@ -2522,7 +2523,8 @@ __global__ __launch_bounds__(kLzCoopThreads) void k_lz_coop(
   if (tid == 0) st_dev_u64(Done + b, (uint64_t)tag << 32);
   if (b == 0) {
     bool good = true;
-    if (tid < kWave) {
+    if (tid < kWave && timeout < 0) good = false;  // a negative limit: give up at once
+    if (tid < kWave && timeout >= 0) {
       const long long t0 = wall_clock64();
       for (;;) {
         bool mine = true;

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -444,7 +445,15 @@ int power_lanczos(tcx_ctx* c, const float* S, int m, int k, const int* counts, u
       auto prev = g_coop_last.find(c->device);
       if (prev != g_coop_last.end() && prev->second != c->coop_ev)
         TCX_HIP(hipStreamWaitEvent(st, prev->second, 0));
-      const long long timeout = c->wall_khz * 1000LL * 2;  // 2 s
+      // every wait of a member gives up after 2 s (TCX_LZ_COOP_TIMEOUT_US
+      // sets another limit, a negative one giving up at the first wait, for
+      // tests of that path: the cluster then takes the stepwise power
+      // iteration)
+      long long timeout = c->wall_khz * 1000LL * 2;
+      if (const char* e = std::getenv("TCX_LZ_COOP_TIMEOUT_US")) {
+        const long long us = std::atoll(e);
+        timeout = us < 0 ? -1 : c->wall_khz * us / 1000;
+      }
       TCX_HIP(tcx::launch_lz_coop(cp, S, m, k, counts, seed, iter, steps, tcx::kLzPowerTol, xch,
                                   fails, gen, timeout, v, r, st_out, cv_out, st));
       TCX_HIP(hipEventRecord(c->coop_ev, st));
