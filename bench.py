@@ -13,7 +13,7 @@ evaluated) and iterations 2..10 are reported apart; the steady state of a
 longer fit is an extra key.  Then the uneven batch (256 fits) against the
 same jobs launched one after another, and the Poisson serving stream
 against sequential execution; last the FP64 oracle on the host cores: the
-CPU baseline and the checker of the fit's SBD distances and centroids.
+CPU baseline and the reference for the fit's SBD distances and centroids.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -439,7 +439,7 @@ def capture_fit(ctx, dx, n, m, k, seed):
 
 
 def accuracy(Z, k, steps, seed):
-    """Checker leg (FP64 oracle): centroid relative L2 error of iterations 1,
+    """Accuracy leg (FP64 oracle): centroid relative L2 error of iterations 1,
     2 and the last, each refined by the oracle from the GPU's own previous
     labels, centroids and alignment lags (the lags differ from the oracle's
     own only at NCC ties) over all series, and the SBD distance error of the

@@ -3,8 +3,8 @@
  * (Code/test/test_timesC.py).
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
- * calls this code.  It is used by tests/ (as the parity checker), by
- * __graft_entry__.smoke() (as the checker of one small run) and by bench.py's
+ * calls this code.  It is used by tests/ (the parity reference), by
+ * __graft_entry__.smoke() (to check one small run) and by bench.py's
  * cpu_baseline leg (timed on the host, never as the measured GPU value).
  *
  * Every function cites the reference line range it restates.  Pinning (what
@@ -99,7 +99,7 @@ void orc_refine(const double* X, long n, int m, int k, const int* labels,
                 int nthreads, int skip_rule, int cal_select);
 
 /* orc_refine with the alignment shift of every row given (shifts, n values;
- * NULL: computed as _preSBD_extract_shape does).  Lets a checker refine
+ * NULL: computed as _preSBD_extract_shape does).  Lets a test refine
  * from the exact shifts an FP32 engine chose where two lags tie. */
 void orc_refine_shifts(const double* X, long n, int m, int k, const int* labels,
                        double* C, int convention, uint64_t seed, int iter,

@@ -1,7 +1,7 @@
 """ctypes binding of the FP64 CPU oracle (oracle/liboracle.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
-the checker of one small run, and by bench.py's cpu_baseline leg.  The
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to
+check one small run, and by bench.py's cpu_baseline leg.  The
 product library never uses it.
 """
 from __future__ import annotations
