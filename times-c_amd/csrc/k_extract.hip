@@ -584,7 +584,9 @@ __device__ __forceinline__ void align_pack_lds_body(const int bid, const int nbl
       s_isd[q] = 1.0 / sqrt(ss / (double)(m - 1));
     }
   }
-  // the lowest-index member of a cluster writes its aligned row to a0
+  __syncthreads();
+  // the lowest-index member of a cluster writes its aligned row to a0 (after
+  // the barrier: its row may have been staged by another wave)
   if (wid == 0) {
     unsigned long long fm = __ballot(first >= 0);
     while (fm) {
@@ -600,7 +602,6 @@ __device__ __forceinline__ void align_pack_lds_body(const int bid, const int nbl
       }
     }
   }
-  __syncthreads();
   // panel chunks (R/16 column chunks x tm row chunks), 16 bytes per lane:
   // lane ln packs columns qb..qb+7 (their parameters held in registers) at
   // t = 32 * r32 + ln % 32 for the row chunks r32 = w, w+4, ...
