@@ -666,31 +666,22 @@ __global__ __launch_bounds__(256) void k_align_pack_lds(
 }
 
 // --------------------------------------------------------------- S = Y^T Y
-// One 256x256 tile of S_j over packed columns [p0, p1) per block: four waves
-// in a 2x2 arrangement, each a 128x128 quarter as 4x4 MFMA 32x32x16 BF16
-// blocks with FP32 accumulation.  split != 0 adds hi*lo + lo*hi
-// (split-BF16).  The panel is in fragment order (panel_blocks), so every
-// operand is one coalesced 1 KB load per wave, 16 bytes per lane; the next
-// 16 columns' operands load while the current ones multiply.  On diagonal
-// tiles the lower-left quarter (the mirror of the upper-right one) is not
-// computed.
+// One wave's 128x128 block of S_j over the packed column chunks [c0, c1)
+// (16 columns each): panel rows of the 32-row blocks ra0..ra0+3 against
+// rb0..rb0+3, as 4x4 MFMA 32x32x16 BF16 blocks with FP32 accumulation;
+// kSplit adds hi*lo + lo*hi (split-BF16).  The panel is in fragment order
+// (panel_blocks), so every operand is one coalesced 1 KB load per wave, 16
+// bytes per lane; the next 16 columns' operands load while the current ones
+// multiply.  The block is stored at o with row stride ostride.
 template <bool kSplit>
-__device__ __forceinline__ void syrk_body(const int bid, const int nblk,
-    const SyrkItem* __restrict__ items,
-                                              const uint16_t* __restrict__ yhi,
-                                              const uint16_t* __restrict__ ylo, int m,
-                                              float* __restrict__ part) {
-  const SyrkItem it = items[bid];
-  if (it.cluster < 0) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wi = wave >> 1, wj = wave & 1;
-  if (it.ti == it.tj && wi > wj) return;
-  if (it.ti * kSyrkTile + wi * 128 >= m || it.tj * kSyrkTile + wj * 128 >= m) return;
-  const int tm = panel_blocks(m);
-  const int ra0 = it.ti * 8 + wi * 4, rb0 = it.tj * 8 + wj * 4;  // first 32-row blocks
+__device__ __forceinline__ void syrk_wave(const int ra0, const int rb0, const long c0,
+                                          const long c1, const int tm,
+                                          const uint16_t* __restrict__ yhi,
+                                          const uint16_t* __restrict__ ylo,
+                                          float* __restrict__ o, const int ostride) {
+  const int lane = threadIdx.x & 63;
   const bf16x8* H = reinterpret_cast<const bf16x8*>(yhi) + lane;
   const bf16x8* Lo = reinterpret_cast<const bf16x8*>(ylo) + lane;
-  const long c0 = it.p0 >> 4, c1 = it.p1 >> 4;
   f32x16 acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -750,55 +741,107 @@ __device__ __forceinline__ void syrk_body(const int bid, const int nblk,
     __builtin_amdgcn_sched_barrier(0);
   }
   if (c < c1) mma(X);
-  float* o = part + (long)it.part * kSyrkTile * kSyrkTile;
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const int i = wi * 128 + a * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-        const int j = wj * 128 + b * 32 + (lane & 31);
-        o[i * kSyrkTile + j] = acc[a][b][r];
+        const int i = a * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+        const int j = b * 32 + (lane & 31);
+        o[i * ostride + j] = acc[a][b][r];
       }
 }
 
+// One 256x256 tile of S_j over packed columns [p0, p1) per block (grouped
+// executor): four waves in a 2x2 arrangement, each a 128x128 quarter.  On
+// diagonal tiles the lower-left quarter (the mirror of the upper-right one)
+// is not computed.
 template <bool kSplit>
-__global__ __launch_bounds__(256) void k_syrk(const SyrkItem* __restrict__ items,
+__device__ __forceinline__ void syrk_body(const int bid, const int nblk,
+                                          const SyrkItem* __restrict__ items,
+                                          const uint16_t* __restrict__ yhi,
+                                          const uint16_t* __restrict__ ylo, int m,
+                                          float* __restrict__ part) {
+  const SyrkItem it = items[bid];
+  if (it.cluster < 0) return;
+  const int wave = threadIdx.x >> 6;
+  const int wi = wave >> 1, wj = wave & 1;
+  if (it.ti == it.tj && wi > wj) return;
+  if (it.ti * kSyrkTile + wi * 128 >= m || it.tj * kSyrkTile + wj * 128 >= m) return;
+  syrk_wave<kSplit>(it.ti * 8 + wi * 4, it.tj * 8 + wj * 4, it.p0 >> 4, it.p1 >> 4,
+                    panel_blocks(m), yhi, ylo,
+                    part + (long)it.part * kSyrkTile * kSyrkTile + (long)wi * 128 * kSyrkTile +
+                        wj * 128,
+                    kSyrkTile);
+}
+
+// Block u (row-major order of the upper triangle) of a T x T block grid:
+// its row and column.
+__device__ __forceinline__ void syrk_block_of(int u, const int T, int& ti, int& tj) {
+  ti = 0;
+  while (u >= T - ti) {
+    u -= T - ti;
+    ++ti;
+  }
+  tj = ti + u;
+}
+
+// Single fit: every wave is its own work item, derived from its slot, so
+// diagonal blocks leave no SIMD idle.  Blocks are dealt to the XCDs round
+// robin (block b on XCD b % 8); group g takes the U consecutive wave slots
+// 4 * (b / 8) + wave of XCD g % 8 from (g / 8) * U on, so the waves that
+// read one group's panel rows share an L2.  Wave u of a group computes the
+// kSyrkSub block u of S_j over the group's columns into that block's
+// partial.  Slots past the last group are holes.
+template <bool kSplit>
+__global__ __launch_bounds__(256) void k_syrk(const SyrkGroup* __restrict__ groups, int ngroups,
                                               const uint16_t* __restrict__ yhi,
                                               const uint16_t* __restrict__ ylo, int m,
                                               float* __restrict__ part) {
-  syrk_body<kSplit>(blockIdx.x, gridDim.x, items, yhi, ylo, m, part);
+  static_assert(kSyrkSub == 128, "one wave's block");
+  const int T = (m + kSyrkSub - 1) / kSyrkSub, U = T * (T + 1) / 2;
+  // the slot is uniform per wave: held in scalar registers, like all that
+  // is derived from it
+  const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x >> 3) * 4 + (int)(threadIdx.x >> 6));
+  const int g = (blockIdx.x & 7) + 8 * (w / U);
+  if (g >= ngroups) return;
+  const int u = w % U;
+  int ti, tj;
+  syrk_block_of(u, T, ti, tj);
+  const SyrkGroup gr = groups[g];
+  syrk_wave<kSplit>(ti * 4, tj * 4, gr.p0 >> 4, gr.p1 >> 4, panel_blocks(m), yhi, ylo,
+                    part + (long)(gr.part + u * gr.stride) * kSyrkSub * kSyrkSub, kSyrkSub);
 }
 
-// S_j from the K-chunk partials of each tile, summed in a fixed order.  One
-// block per 64x64 sub-block (16 per tile, the lower ones of diagonal tiles
-// skipped): 16-byte partial loads, four rows per thread in flight, the sums
-// staged in LDS; then the entries with i <= j and their mirrors are stored
-// as coalesced row segments, so S_j is exactly symmetric.
-__device__ __forceinline__ void syrk_reduce_body(const int bid, const int nblk,
-    const SyrkTile* __restrict__ tiles,
-                                                     const float* __restrict__ part, int m,
-                                                     float* __restrict__ S) {
-  constexpr int kSub = kSyrkTile / 64;
-  const SyrkTile tl = tiles[bid / (kSub * kSub)];
-  const int sb = bid % (kSub * kSub), si = sb / kSub, sj = sb % kSub;
+// S_j from the K-chunk partials of each tile (edge TS: kSyrkTile or
+// kSyrkSub), summed in a fixed order.  One block per 64x64 sub-block (the
+// lower ones of diagonal tiles skipped): 16-byte partial loads, four rows
+// per thread in flight, the sums staged in LDS; then the entries with
+// i <= j and their mirrors are stored as coalesced row segments, so S_j is
+// exactly symmetric.
+template <int TS>
+__device__ __forceinline__ void syrk_reduce_body(const int sb, const SyrkTile tl,
+                                                 const float* __restrict__ part, int m,
+                                                 float* __restrict__ S) {
+  constexpr int kSub = TS / 64;
+  const int si = sb / kSub, sj = sb % kSub;
   if (tl.ti == tl.tj && si > sj) return;
-  const int gi0 = tl.ti * kSyrkTile + si * 64, gj0 = tl.tj * kSyrkTile + sj * 64;
+  const int gi0 = tl.ti * TS + si * 64, gj0 = tl.tj * TS + sj * 64;
   if (gi0 >= m || gj0 >= m) return;
   __shared__ float tr[64][65];
   const int c4 = (threadIdx.x & 15) * 4, rr = threadIdx.x >> 4;
-  const float* pp = part + (long)tl.part0 * kSyrkTile * kSyrkTile + (long)(si * 64 + rr) * kSyrkTile +
+  const float* pp = part + (long)tl.part0 * TS * TS + (long)(si * 64 + rr) * TS +
                     sj * 64 + c4;
   float4 acc[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) acc[u] = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll 2
   for (int c = 0; c < tl.nparts; ++c) {
-    const float* pc = pp + (long)c * kSyrkTile * kSyrkTile;
+    const float* pc = pp + (long)c * TS * TS;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const float4 v = *reinterpret_cast<const float4*>(pc + (long)16 * u * kSyrkTile);
+      const float4 v = *reinterpret_cast<const float4*>(pc + (long)16 * u * TS);
       acc[u].x += v.x;
       acc[u].y += v.y;
       acc[u].z += v.z;
@@ -825,10 +868,20 @@ __device__ __forceinline__ void syrk_reduce_body(const int bid, const int nblk,
   }
 }
 
-__global__ __launch_bounds__(256) void k_syrk_reduce(const SyrkTile* __restrict__ tiles,
+// Single fit: (kSyrkSub / 64)^2 blocks per kSyrkSub block of each cluster
+// with partials, clusters[c] covering the blocks from c * T(T+1)/2 on.
+__global__ __launch_bounds__(256) void k_syrk_reduce(const SyrkCluster* __restrict__ clusters,
                                                      const float* __restrict__ part, int m,
                                                      float* __restrict__ S) {
-  syrk_reduce_body(blockIdx.x, gridDim.x, tiles, part, m, S);
+  constexpr int kSub = kSyrkSub / 64;
+  const int T = (m + kSyrkSub - 1) / kSyrkSub, U = T * (T + 1) / 2;
+  const int t = blockIdx.x / (kSub * kSub), u = t % U;
+  const SyrkCluster c = clusters[t / U];
+  int ti, tj;
+  syrk_block_of(u, T, ti, tj);
+  syrk_reduce_body<kSyrkSub>(blockIdx.x % (kSub * kSub),
+                             SyrkTile{c.cluster, ti, tj, c.part0 + u * c.nparts, c.nparts}, part,
+                             m, S);
 }
 
 // ---------------------------------------------------------- power iteration
@@ -2629,7 +2682,7 @@ __global__ __launch_bounds__(256) void k_syrk_reduce_group(const SyrkJob* __rest
   constexpr int kSub = kSyrkTile / 64;
   const GroupSyrkTile* gt = tiles + blockIdx.x / (kSub * kSub);
   const SyrkJob jb = jobs[gt->job];
-  syrk_reduce_body(blockIdx.x % (kSub * kSub), kSub * kSub, &gt->t, jb.part, jb.m, jb.S);
+  syrk_reduce_body<kSyrkTile>(blockIdx.x % (kSub * kSub), gt->t, jb.part, jb.m, jb.S);
 }
 
 __global__ __launch_bounds__(kPowerThreads) void k_power_lanczos_group(
@@ -2769,22 +2822,29 @@ hipError_t launch_align_pack(const float* x, int m, const int* perm, long n_pad,
   return hipGetLastError();
 }
 
-hipError_t launch_syrk(const SyrkItem* items, int nitems, const uint16_t* yhi,
+long syrk_blocks(int m, int ngroups) {
+  const long T = (m + kSyrkSub - 1) / kSyrkSub, U = T * (T + 1) / 2;
+  return 8 * (((ngroups + 7) / 8 * U + 3) / 4);
+}
+
+hipError_t launch_syrk(const SyrkGroup* groups, int ngroups, const uint16_t* yhi,
                        const uint16_t* ylo, int m, int split, float* part, hipStream_t st) {
-  if (nitems <= 0) return hipSuccess;
+  if (ngroups <= 0) return hipSuccess;
+  const dim3 grid((unsigned)syrk_blocks(m, ngroups));
   if (split)
-    hipLaunchKernelGGL(k_syrk<true>, dim3(nitems), dim3(256), 0, st, items, yhi, ylo, m, part);
+    hipLaunchKernelGGL(k_syrk<true>, grid, dim3(256), 0, st, groups, ngroups, yhi, ylo, m, part);
   else
-    hipLaunchKernelGGL(k_syrk<false>, dim3(nitems), dim3(256), 0, st, items, yhi, ylo, m, part);
+    hipLaunchKernelGGL(k_syrk<false>, grid, dim3(256), 0, st, groups, ngroups, yhi, ylo, m, part);
   return hipGetLastError();
 }
 
-hipError_t launch_syrk_reduce(const SyrkTile* tiles, int ntiles, const float* part, int m,
+hipError_t launch_syrk_reduce(const SyrkCluster* clusters, int nclusters, const float* part, int m,
                               float* S, hipStream_t st) {
-  if (ntiles <= 0) return hipSuccess;
-  constexpr int kSub = kSyrkTile / 64;
-  hipLaunchKernelGGL(k_syrk_reduce, dim3((unsigned)ntiles * kSub * kSub), dim3(256), 0, st, tiles,
-                     part, m, S);
+  if (nclusters <= 0) return hipSuccess;
+  constexpr int kSub = kSyrkSub / 64;
+  const long T = (m + kSyrkSub - 1) / kSyrkSub, U = T * (T + 1) / 2;
+  hipLaunchKernelGGL(k_syrk_reduce, dim3((unsigned)(nclusters * U * kSub * kSub)), dim3(256), 0,
+                     st, clusters, part, m, S);
   return hipGetLastError();
 }
 

@@ -105,10 +105,9 @@ struct tcx_ctx {
   void* lz_xch = nullptr;        // exchange buffer (and its size) the generation below refers to
   size_t lz_xch_bytes = 0;
   uint32_t lz_gen = 0;           // generation of the last multi-block Lanczos launch
-  // host copies kept alive until the stream has consumed them
-  std::vector<tcx::SyrkItem> items;
-  std::vector<tcx::SyrkTile> tiles;
-  std::vector<unsigned char> plan;  // items and tiles, uploaded as one copy
+  // the last S plan (host copies; the upload is staged in `pinned`)
+  std::vector<tcx::SyrkGroup> sgroups;
+  std::vector<tcx::SyrkCluster> sclusters;
   int cus = 0;                 // compute units of the device
   long long wall_khz = 0;      // wall_clock64() rate
   void* pinned = nullptr;      // page-locked host area for small readbacks
@@ -321,64 +320,56 @@ int noalloc_scratch(int dev, unsigned char** out) {
   return TCX_OK;
 }
 
-// Work list of a grouped S = Y^T Y: for each cluster with seg_len[j] > 0
-// packed columns starting at off[j], the upper triangle of
-// ceil(m/256)^2 tiles, each split along K into chunks of one length.  The
-// tiles of one (cluster, chunk) read the same panel rows, so they are
-// placed on one XCD (consecutive blocks run on consecutive XCDs, so blocks
-// with equal b % 8 share one) at adjacent dispatch slots; free slots are
-// holes.  The chunk is the shortest for which every
-// XCD holds at most one block per CU (one 4-wave block fills a CU's
-// register file), so all blocks run in one round.
-void plan_syrk(int m, int k, const long* off, const long* seg_len, int cus,
-               std::vector<tcx::SyrkItem>& items, std::vector<tcx::SyrkTile>& tiles) {
-  constexpr int kXcds = 8;
-  const int T = (m + tcx::kSyrkTile - 1) / tcx::kSyrkTile;
-  const int tiles_per = T * (T + 1) / 2;
-  items.clear();
-  tiles.clear();
+// Work list of S = Y^T Y for k_syrk: for each cluster with seg_len[j] > 0
+// packed columns starting at off[j], the upper triangle of ceil(m/128)^2
+// blocks of 128 x 128 (one wave each) over every K chunk of the cluster.
+// Cluster j is split into nc_j chunks of nearly equal length (whole
+// 16-column steps, at least 16 of them): the chunk bound L is the shortest
+// for which every (cluster, chunk) group fits one round of the device (one
+// wave per SIMD: the kernel's registers fill a SIMD) with each group's
+// waves on one XCD, and nc_j = ceil(steps_j / L).
+// The partials of block u of cluster j are consecutive over its chunks
+// (summed in chunk order); returns their total count.
+long plan_syrk(int m, int k, const long* off, const long* seg_len, int cus,
+               std::vector<tcx::SyrkGroup>& groups, std::vector<tcx::SyrkCluster>& clusters) {
+  constexpr long kXcds = 8, kMinSteps = 16;
+  const long T = (m + tcx::kSyrkSub - 1) / tcx::kSyrkSub, U = T * (T + 1) / 2;  // waves per group
+  groups.clear();
+  clusters.clear();
+  std::vector<long> steps(k);
   long max_steps = 0;
-  for (int j = 0; j < k; ++j) max_steps = std::max(max_steps, (seg_len[j] + 15) / 16);
-  if (max_steps == 0) return;
-  const long slots = std::max(1, cus / kXcds);  // blocks per XCD in one round
-  auto fits = [&](long cs) {
-    long g = 0;
-    for (int j = 0; j < k; ++j) g += (seg_len[j] + 16 * cs - 1) / (16 * cs);
-    return (g + kXcds - 1) / kXcds * tiles_per <= slots;
-  };
-  long lo = 1, hi = max_steps;  // smallest chunk (in 16-column steps) that fits
-  while (lo < hi) {
-    const long mid = (lo + hi) / 2;
-    if (fits(mid)) hi = mid; else lo = mid + 1;
-  }
-  const long cs = std::min(max_steps, std::max(lo, 16L)) * 16;  // columns per chunk
-  struct Group { int j; long p0, p1; int c; int tile0; };
-  std::vector<Group> groups;
   for (int j = 0; j < k; ++j) {
-    const long K = seg_len[j];
-    if (K == 0) continue;
-    const int nc = (int)((K + cs - 1) / cs);
-    const int tile0 = (int)tiles.size();
-    int part = tiles.empty() ? 0 : tiles.back().part0 + tiles.back().nparts;
-    for (int ti = 0; ti < T; ++ti)
-      for (int tj = ti; tj < T; ++tj) {
-        tiles.push_back(tcx::SyrkTile{j, ti, tj, part, nc});
-        part += nc;
-      }
-    for (int c = 0; c < nc; ++c)
-      groups.push_back(Group{j, off[j] + c * cs, off[j] + std::min(K, (c + 1) * cs), c, tile0});
+    steps[j] = (seg_len[j] + 15) / 16;
+    max_steps = std::max(max_steps, steps[j]);
   }
-  const long per_xcd = ((long)groups.size() + kXcds - 1) / kXcds;
-  items.assign((size_t)(kXcds * tiles_per * per_xcd), tcx::SyrkItem{-1, 0, 0, 0, 0, 0});
-  for (long g = 0; g < (long)groups.size(); ++g) {
-    const Group& gr = groups[g];
-    const long x = g % kXcds, gx = g / kXcds;
-    for (int r = 0; r < tiles_per; ++r) {
-      const tcx::SyrkTile& tl = tiles[gr.tile0 + r];
-      items[kXcds * (tiles_per * gx + r) + x] =
-          tcx::SyrkItem{gr.j, tl.ti, tl.tj, tl.part0 + gr.c, gr.p0, gr.p1};
+  if (max_steps == 0) return 0;
+  const long slots = 4L * std::max(cus, 1);  // one wave per SIMD
+  const long per_xcd = std::max(slots / kXcds, 1L);
+  const long gmax = U <= per_xcd ? kXcds * (per_xcd / U) : std::max(slots / U, 1L);
+  auto groups_for = [&](long L) {
+    long g = 0;
+    for (int j = 0; j < k; ++j) g += (steps[j] + L - 1) / L;
+    return g;
+  };
+  long lo = std::min(kMinSteps, max_steps), hi = max_steps;
+  while (lo < hi) {  // shortest chunk bound whose groups fit (else one chunk per cluster)
+    const long mid = (lo + hi) / 2;
+    if (groups_for(mid) <= gmax) hi = mid; else lo = mid + 1;
+  }
+  const long L = lo;
+  long part = 0;
+  for (int j = 0; j < k; ++j) {
+    if (steps[j] == 0) continue;
+    const long nc = (steps[j] + L - 1) / L;
+    clusters.push_back(tcx::SyrkCluster{j, (int)part, (int)nc});
+    for (long c = 0; c < nc; ++c) {
+      const long s0 = steps[j] * c / nc, s1 = steps[j] * (c + 1) / nc;
+      groups.push_back(tcx::SyrkGroup{(int)(part + c), (int)nc, off[j] + 16 * s0,
+                                      off[j] + std::min(seg_len[j], 16 * s1)});
     }
+    part += U * nc;
   }
+  return part;
 }
 
 struct RefineResult {
@@ -585,29 +576,36 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
   for (int j = 0; j < k; ++j) nonempty += h_counts[j] > 0;
   std::vector<long> seg_len(k);
   for (int j = 0; j < k; ++j) seg_len[j] = h_counts[j] > 0 ? h_off[j + 1] - h_off[j] : 0;
-  plan_syrk(m, k, h_off.data(), seg_len.data(), c->cus, c->items, c->tiles);
-  // items and tiles in one buffer, uploaded by one copy
-  const size_t items_bytes = (sizeof(tcx::SyrkItem) * c->items.size() + 255) & ~(size_t)255;
-  const size_t plan_bytes = items_bytes + sizeof(tcx::SyrkTile) * c->tiles.size();
+  const long nparts =
+      plan_syrk(m, k, h_off.data(), seg_len.data(), c->cus, c->sgroups, c->sclusters);
+  // groups and clusters in one buffer, uploaded by one copy
+  const size_t groups_bytes = (sizeof(tcx::SyrkGroup) * c->sgroups.size() + 255) & ~(size_t)255;
+  const size_t plan_bytes = groups_bytes + sizeof(tcx::SyrkCluster) * c->sclusters.size();
   unsigned char* d_plan;
   TCX_TRY(ws(c, kSlotPlan, plan_bytes, &d_plan));
-  tcx::SyrkItem* d_items = reinterpret_cast<tcx::SyrkItem*>(d_plan);
-  tcx::SyrkTile* d_tiles = reinterpret_cast<tcx::SyrkTile*>(d_plan + items_bytes);
+  tcx::SyrkGroup* d_groups = reinterpret_cast<tcx::SyrkGroup*>(d_plan);
+  tcx::SyrkCluster* d_clusters = reinterpret_cast<tcx::SyrkCluster*>(d_plan + groups_bytes);
   float *part, *S;
   tcx::PowerInfo* info;
-  const size_t nparts = c->tiles.empty() ? 1 : c->tiles.back().part0 + c->tiles.back().nparts;
-  TCX_TRY(ws(c, kSlotPart, nparts * tcx::kSyrkTile * tcx::kSyrkTile, &part));
+  TCX_TRY(ws(c, kSlotPart, (size_t)std::max(nparts, 1L) * tcx::kSyrkSub * tcx::kSyrkSub, &part));
   TCX_TRY(ws(c, kSlotS, (size_t)k * m * m, &S));
   TCX_TRY(ws(c, kSlotInfo, (size_t)k, &info));
-  if (!c->items.empty()) {
-    c->plan.resize(plan_bytes);
-    std::memcpy(c->plan.data(), c->items.data(), sizeof(tcx::SyrkItem) * c->items.size());
-    std::memcpy(c->plan.data() + items_bytes, c->tiles.data(),
-                sizeof(tcx::SyrkTile) * c->tiles.size());
-    TCX_HIP(hipMemcpyAsync(d_plan, c->plan.data(), plan_bytes, hipMemcpyHostToDevice, st));
+  const int ngroups = (int)c->sgroups.size(), nclusters = (int)c->sclusters.size();
+  if (ngroups > 0) {
+    // staged in the pinned area (idle here: its readback above was waited
+    // for and consumed; its next use, the power stage's readback, is ordered
+    // after this copy on the same stream and is sized in now, so the area is
+    // not regrown while the copy reads it), so the copy does not block the
+    // host
+    void* stage;
+    TCX_TRY(pinned_area(c, std::max(plan_bytes, sizeof(tcx::PowerInfo) * k), &stage));
+    unsigned char* h_plan = static_cast<unsigned char*>(stage);
+    std::memcpy(h_plan, c->sgroups.data(), sizeof(tcx::SyrkGroup) * ngroups);
+    std::memcpy(h_plan + groups_bytes, c->sclusters.data(), sizeof(tcx::SyrkCluster) * nclusters);
+    TCX_HIP(hipMemcpyAsync(d_plan, h_plan, plan_bytes, hipMemcpyHostToDevice, st));
   }
-  TCX_HIP(tcx::launch_syrk(d_items, (int)c->items.size(), yhi, ylo, m, split, part, st));
-  TCX_HIP(tcx::launch_syrk_reduce(d_tiles, (int)c->tiles.size(), part, m, S, st));
+  TCX_HIP(tcx::launch_syrk(d_groups, ngroups, yhi, ylo, m, split, part, st));
+  TCX_HIP(tcx::launch_syrk_reduce(d_clusters, nclusters, part, m, S, st));
   const int max_steps = m > 1000 ? 1000 : m;  // test_timesC.py:152-156
   const double* vin = nullptr;
   const double* svin = nullptr;
@@ -654,8 +652,8 @@ int refine(tcx_ctx* c, const float* d_x, long n, int m, int k, const int* d_labe
     // the budget check of TCX_S_AUTO: the same S from one BF16 product (the
     // hi panel only) through the same power stage; the largest relative L2
     // difference between its centroids and the split ones above
-    TCX_HIP(tcx::launch_syrk(d_items, (int)c->items.size(), yhi, ylo, m, 0, part, st));
-    TCX_HIP(tcx::launch_syrk_reduce(d_tiles, (int)c->tiles.size(), part, m, S, st));
+    TCX_HIP(tcx::launch_syrk(d_groups, ngroups, yhi, ylo, m, 0, part, st));
+    TCX_HIP(tcx::launch_syrk_reduce(d_clusters, nclusters, part, m, S, st));
     TCX_TRY(power_lanczos(c, S, m, k, counts, seed, iter, max_steps, 0, nullptr, &vin, &svin,
                           &rsum, &steps_in, &conv_in));
     tcx::PowerInfo* cinfo;
@@ -847,14 +845,16 @@ size_t fit_workspace_bound(long n, int m, int k, int cus) {
   // row order, Y panels (hi, lo), sign-fix rows
   const size_t npad = std::max<size_t>((N + Kc * (kPackPad - 1)) / kPackPad * kPackPad, kPackPad);
   b += r(4 * npad) + 2 * r(2 * (size_t)panel_blocks(m) * 32 * npad) + r(4 * Kc * Mm);
-  // S work list and partials: plan_syrk keeps (groups x tiles) within one
-  // round of the device (or one group per cluster) and a chunk of at least
-  // 256 columns
-  const size_t T = (Mm + kSyrkTile - 1) / kSyrkTile, tp = T * (T + 1) / 2;
-  const size_t groups = std::min(std::max((cus + 8 * tp) / tp, Kc), npad / 256 + Kc);
-  const size_t nparts = groups * tp;
-  b += r(sizeof(SyrkItem) * (nparts + 8 * tp) + 256 + sizeof(SyrkTile) * Kc * tp);
-  b += r(4 * nparts * kSyrkTile * kSyrkTile) + r(4 * Kc * Mm * Mm) + r(sizeof(PowerInfo) * Kc);
+  // S plan and partials: plan_syrk keeps its groups (U waves and partials
+  // each) within one round of the device or makes one group per cluster,
+  // and a cluster's chunks are at least 256 columns but for one (so at most
+  // npad / 256 + Kc groups)
+  const size_t T = (Mm + kSyrkSub - 1) / kSyrkSub, U = T * (T + 1) / 2;
+  const size_t groups =
+      std::min(std::max((size_t)4 * std::max(cus, 1) / U + 8, Kc), npad / 256 + Kc);
+  const size_t nparts = groups * U;
+  b += r(sizeof(SyrkGroup) * groups + 256 + sizeof(SyrkCluster) * Kc);
+  b += r(4 * nparts * kSyrkSub * kSyrkSub) + r(4 * Kc * Mm * Mm) + r(sizeof(PowerInfo) * Kc);
   // power stage: basis, v, S v, r, w, counters, states, exchange words, mask
   const size_t xch = Kc * (2 * (2 * (size_t)kLzCoopMaxG * (kLanczosMax + 2) + Mm) + kLzCoopMaxG);
   b += r(8 * Kc * (kLanczosMax + 1) * Mm) + 4 * r(8 * Kc * Mm) + 2 * r(4 * Kc) +

@@ -100,7 +100,8 @@ hipError_t configure_sbd_kernels();
 // ------------------------------------------------------------ extraction
 constexpr int kGroupChunk = 512;  // rows per grouping block
 constexpr int kPackPad = 64;       // cluster segments padded to this many columns
-constexpr int kSyrkTile = 256;     // S tile edge (one block)
+constexpr int kSyrkTile = 256;     // S tile edge of the grouped executor (one block)
+constexpr int kSyrkSub = 128;      // S block edge of the single fit (one wave)
 
 // The packed panel Y (time t x packed column p, BF16 hi and lo parts) is
 // stored in MFMA operand order: chunk (c = p / 16, r = t / 32) holds 512
@@ -111,17 +112,34 @@ __host__ __device__ constexpr int panel_blocks(int m) {
   return (m + kSyrkTile - 1) / kSyrkTile * (kSyrkTile / 32);
 }
 
+// A work item of S = Y^T Y: one kSyrkSub block (single fit, one wave) or
+// one kSyrkTile tile (grouped executor, one block) of S_j over a K chunk.
 struct SyrkItem {
   int cluster;        // < 0: no work (keeps the grid's XCD grouping)
-  int ti, tj;         // tile row / column (ti <= tj)
-  int part;           // index of this item's partial tile
+  int ti, tj;         // block / tile row and column (ti <= tj)
+  int part;           // index of this item's partial
   long p0, p1;        // packed column range [p0, p1), multiples of 16
 };
 
+// A block / tile of S_j and its K-chunk partials.
 struct SyrkTile {
   int cluster;
   int ti, tj;
   int part0, nparts;  // partials part0 .. part0+nparts-1, summed in order
+};
+
+// A (cluster, K chunk) group of the single fit's S = Y^T Y: the upper
+// triangle of kSyrkSub blocks of S_j (row-major order) over the packed
+// columns [p0, p1); block u's partial is part + u * stride.
+struct SyrkGroup {
+  int part, stride;
+  long p0, p1;        // multiples of 16
+};
+
+// A cluster of the single fit with partials: block u of S_j sums the
+// nparts partials from part0 + u * nparts on, in order.
+struct SyrkCluster {
+  int cluster, part0, nparts;
 };
 
 struct PowerInfo {
@@ -156,9 +174,13 @@ hipError_t launch_align_pack(const float* x, int m, const int* perm, long n_pad,
                              const long* offsets, const int* minrow, int k, const int* shift,
                              const int* czero, const double2* stats, uint16_t* yhi,
                              uint16_t* ylo, float* a0, hipStream_t st);
-hipError_t launch_syrk(const SyrkItem* items, int nitems, const uint16_t* yhi,
+// Single fit: blocks of k_syrk for ngroups groups (each group's waves on
+// one XCD), the launch over the groups, and the reduction of the clusters'
+// kSyrkSub x kSyrkSub partials into S.
+long syrk_blocks(int m, int ngroups);
+hipError_t launch_syrk(const SyrkGroup* groups, int ngroups, const uint16_t* yhi,
                        const uint16_t* ylo, int m, int split, float* part, hipStream_t st);
-hipError_t launch_syrk_reduce(const SyrkTile* tiles, int ntiles, const float* part, int m,
+hipError_t launch_syrk_reduce(const SyrkCluster* clusters, int nclusters, const float* part, int m,
                               float* S, hipStream_t st);
 hipError_t launch_power(const float* S, int m, int k, const int* counts, const float* a0,
                         uint64_t seed, int iter, int max_steps, double tol, const double* vin,
