@@ -65,6 +65,39 @@ def test_extract_shape_edge_clusters(ctx, m):
     assert abs(got[0].mean()) <= 1e-5 and abs(got[0].std(ddof=1) - 1) <= 1e-5
 
 
+@pytest.mark.parametrize("n,m,frac", [
+    (20000, 256, [0.4, 0.3, 0.15, 0.1, 0.05]),  # many chunks of the 16-step minimum
+    (30000, 1024, [0.5, 0.33, 0.17]),           # chunk bound searched, uneven chunk counts
+    (6000, 2100, [0.6, 0.4]),                   # one (cluster, chunk) group wider than an XCD
+])
+def test_extract_shape_chunked_s_matches_oracle(ctx, n, m, frac):
+    """Clusters large enough that S = Y^T Y is split into several K chunks
+    per cluster, each summed from its own partials: the centroids match the
+    oracle's; the last of k+1 centroids has no members and stays zero.  Both
+    sides align every member by the oracle's FP64 shift, so that a member
+    whose two best shifts tie within FP32 rounding (one in cluster 1 at
+    m=256: 9.6e-8 apart) cannot move a centroid by a different alignment."""
+    k = len(frac)
+    d, Z, y = _znormed(ctx, 40 + k, n, m, k, frac=frac)
+    labels = y.astype(np.int32)
+    C0 = np.zeros((k + 1, m), np.float32)
+    C0[:k] = O.zscore(Z[T.init_rows(11, n, k)].astype(np.float64), ddof=1)
+    Z64, C64 = Z.astype(np.float64), C0.astype(np.float64)
+    shifts = np.array([O.align_shift(Z64[i], C64[labels[i]]) for i in range(n)], np.int32)
+    dl, dc, ds = ctx.to_device(labels), ctx.to_device(C0), ctx.to_device(shifts)
+    try:
+        sc = ctx.extract_shape(dl, d, ds, dc, it=3)
+        got = dc.numpy()
+    finally:
+        for a in (dl, dc, ds, d):
+            a.free()
+    ref, counts, _ = O.refine(Z64, labels, C64, seed=0, it=3, shifts=shifts)
+    assert sc == k and counts[k] == 0
+    assert np.all(got[k] == 0)
+    errs = centroid_errors(got, ref)
+    assert max(errs) <= CENT_RTOL, errs
+
+
 def test_stepwise_parity_cfg0(ctx):
     """Each refine and assign step from identical inputs (the GPU's previous
     state) matches the oracle: centroids to 1e-4 rel-L2, labels up to ties."""
