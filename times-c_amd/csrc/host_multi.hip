@@ -157,8 +157,8 @@ size_t layout(GJob& j, unsigned char* base) {
   take(j.yhi, (size_t)panel_blocks(j.m) * 32 * (size_t)j.n_pad_max);
   take(j.ylo, (size_t)panel_blocks(j.m) * 32 * (size_t)j.n_pad_max);
   take(j.a0, k * m);
-  const size_t T = (m + kSyrkTile - 1) / kSyrkTile, tp = T * (T + 1) / 2;
-  take(j.part, k * tp * kSyrkTile * kSyrkTile);
+  const size_t T = (m + kSyrkSub - 1) / kSyrkSub, U = T * (T + 1) / 2;
+  take(j.part, k * U * kSyrkSub * kSyrkSub);
   take(j.S, k * m * m);
   take(j.Q, k * (kLanczosMax + 1) * m);
   take(j.v, k * m);
@@ -581,9 +581,10 @@ bool gram_form(int m, long nd, int count) {
          nd <= kGramBlockMaxND;
 }
 
-// S = Y^T Y items (one K chunk per tile) of the clusters with sel[c].
+// S = Y^T Y items of the clusters with sel[c]: one per kSyrkSub block of
+// the upper triangle (one wave each), over the cluster's whole column range.
 void add_syrk_items(Batch& b, int q, const GJob& j, const long* off, const std::vector<char>& sel) {
-  const int T = (j.m + kSyrkTile - 1) / kSyrkTile;
+  const int T = (j.m + kSyrkSub - 1) / kSyrkSub;
   int part = 0;
   for (int c = 0; c < j.k; ++c) {
     if (!sel[c]) continue;

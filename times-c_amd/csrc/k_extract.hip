@@ -753,29 +753,6 @@ __device__ __forceinline__ void syrk_wave(const int ra0, const int rb0, const lo
       }
 }
 
-// One 256x256 tile of S_j over packed columns [p0, p1) per block (grouped
-// executor): four waves in a 2x2 arrangement, each a 128x128 quarter.  On
-// diagonal tiles the lower-left quarter (the mirror of the upper-right one)
-// is not computed.
-template <bool kSplit>
-__device__ __forceinline__ void syrk_body(const int bid, const int nblk,
-                                          const SyrkItem* __restrict__ items,
-                                          const uint16_t* __restrict__ yhi,
-                                          const uint16_t* __restrict__ ylo, int m,
-                                          float* __restrict__ part) {
-  const SyrkItem it = items[bid];
-  if (it.cluster < 0) return;
-  const int wave = threadIdx.x >> 6;
-  const int wi = wave >> 1, wj = wave & 1;
-  if (it.ti == it.tj && wi > wj) return;
-  if (it.ti * kSyrkTile + wi * 128 >= m || it.tj * kSyrkTile + wj * 128 >= m) return;
-  syrk_wave<kSplit>(it.ti * 8 + wi * 4, it.tj * 8 + wj * 4, it.p0 >> 4, it.p1 >> 4,
-                    panel_blocks(m), yhi, ylo,
-                    part + (long)it.part * kSyrkTile * kSyrkTile + (long)wi * 128 * kSyrkTile +
-                        wj * 128,
-                    kSyrkTile);
-}
-
 // Block u (row-major order of the upper triangle) of a T x T block grid:
 // its row and column.
 __device__ __forceinline__ void syrk_block_of(int u, const int T, int& ti, int& tj) {
@@ -814,8 +791,8 @@ __global__ __launch_bounds__(256) void k_syrk(const SyrkGroup* __restrict__ grou
                     part + (long)(gr.part + u * gr.stride) * kSyrkSub * kSyrkSub, kSyrkSub);
 }
 
-// S_j from the K-chunk partials of each tile (edge TS: kSyrkTile or
-// kSyrkSub), summed in a fixed order.  One block per 64x64 sub-block (the
+// S_j from the K-chunk partials of each tile (edge TS), summed in a fixed
+// order.  One block per 64x64 sub-block (the
 // lower ones of diagonal tiles skipped): 16-byte partial loads, four rows
 // per thread in flight, the sums staged in LDS; then the entries with
 // i <= j and their mirrors are stored as coalesced row segments, so S_j is
@@ -2668,21 +2645,26 @@ __global__ __launch_bounds__(256) void k_align_pack_lds_group(
                          a.czero, a.yhi, a.ylo, a.a0);
 }
 
-// One item (tile of one job's cluster) per block.
+// One item (a kSyrkSub block of one job's cluster over the cluster's
+// columns) per wave: items[4 * block + wave], the waves past nitems idle.
 template <bool kSplit>
 __global__ __launch_bounds__(256) void k_syrk_group(const SyrkJob* __restrict__ jobs,
-                                                    const GroupSyrkItem* __restrict__ items) {
-  const GroupSyrkItem* gi = items + blockIdx.x;
-  const SyrkJob jb = jobs[gi->job];
-  syrk_body<kSplit>(0, 1, &gi->it, jb.yhi, jb.ylo, jb.m, jb.part);
+                                                    const GroupSyrkItem* __restrict__ items,
+                                                    int nitems) {
+  const int i = __builtin_amdgcn_readfirstlane((int)blockIdx.x * 4 + (int)(threadIdx.x >> 6));
+  if (i >= nitems) return;
+  const GroupSyrkItem gi = items[i];
+  const SyrkJob jb = jobs[gi.job];
+  syrk_wave<kSplit>(gi.it.ti * 4, gi.it.tj * 4, gi.it.p0 >> 4, gi.it.p1 >> 4, panel_blocks(jb.m),
+                    jb.yhi, jb.ylo, jb.part + (long)gi.it.part * kSyrkSub * kSyrkSub, kSyrkSub);
 }
 
 __global__ __launch_bounds__(256) void k_syrk_reduce_group(const SyrkJob* __restrict__ jobs,
                                                            const GroupSyrkTile* __restrict__ tiles) {
-  constexpr int kSub = kSyrkTile / 64;
+  constexpr int kSub = kSyrkSub / 64;
   const GroupSyrkTile* gt = tiles + blockIdx.x / (kSub * kSub);
   const SyrkJob jb = jobs[gt->job];
-  syrk_reduce_body<kSyrkTile>(blockIdx.x % (kSub * kSub), gt->t, jb.part, jb.m, jb.S);
+  syrk_reduce_body<kSyrkSub>(blockIdx.x % (kSub * kSub), gt->t, jb.part, jb.m, jb.S);
 }
 
 __global__ __launch_bounds__(kPowerThreads) void k_power_lanczos_group(
@@ -3625,17 +3607,18 @@ hipError_t launch_align_pack_group(int form, const GroupTable& g, hipStream_t st
 hipError_t launch_syrk_group(const SyrkJob* jobs, const GroupSyrkItem* items, int nitems,
                              int split, hipStream_t st) {
   if (nitems <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((nitems + 3) / 4));
   if (split)
-    hipLaunchKernelGGL(k_syrk_group<true>, dim3(nitems), dim3(256), 0, st, jobs, items);
+    hipLaunchKernelGGL(k_syrk_group<true>, grid, dim3(256), 0, st, jobs, items, nitems);
   else
-    hipLaunchKernelGGL(k_syrk_group<false>, dim3(nitems), dim3(256), 0, st, jobs, items);
+    hipLaunchKernelGGL(k_syrk_group<false>, grid, dim3(256), 0, st, jobs, items, nitems);
   return hipGetLastError();
 }
 
 hipError_t launch_syrk_reduce_group(const SyrkJob* jobs, const GroupSyrkTile* tiles, int ntiles,
                                     hipStream_t st) {
   if (ntiles <= 0) return hipSuccess;
-  constexpr int kSub = kSyrkTile / 64;
+  constexpr int kSub = kSyrkSub / 64;
   hipLaunchKernelGGL(k_syrk_reduce_group, dim3((unsigned)ntiles * kSub * kSub), dim3(256), 0, st,
                      jobs, tiles);
   return hipGetLastError();

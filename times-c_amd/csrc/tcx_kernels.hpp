@@ -100,28 +100,28 @@ hipError_t configure_sbd_kernels();
 // ------------------------------------------------------------ extraction
 constexpr int kGroupChunk = 512;  // rows per grouping block
 constexpr int kPackPad = 64;       // cluster segments padded to this many columns
-constexpr int kSyrkTile = 256;     // S tile edge of the grouped executor (one block)
-constexpr int kSyrkSub = 128;      // S block edge of the single fit (one wave)
+constexpr int kSyrkTile = 256;     // panel rows are padded to a multiple of this
+constexpr int kSyrkSub = 128;      // S block edge (one wave)
 
 // The packed panel Y (time t x packed column p, BF16 hi and lo parts) is
 // stored in MFMA operand order: chunk (c = p / 16, r = t / 32) holds 512
 // values at element offset (c * panel_blocks(m) + r) * 512, value (t, p) at
 // ((t % 32) + 32 * ((p / 8) % 2)) * 8 + p % 8 inside it.  Rows run to a
-// whole number of S tiles, rows t >= m are zero.
+// multiple of kSyrkTile, rows t >= m are zero.
 __host__ __device__ constexpr int panel_blocks(int m) {
   return (m + kSyrkTile - 1) / kSyrkTile * (kSyrkTile / 32);
 }
 
-// A work item of S = Y^T Y: one kSyrkSub block (single fit, one wave) or
-// one kSyrkTile tile (grouped executor, one block) of S_j over a K chunk.
+// A work item of S = Y^T Y (grouped executor): one kSyrkSub block of S_j
+// (one wave) over a packed column range.
 struct SyrkItem {
-  int cluster;        // < 0: no work (keeps the grid's XCD grouping)
-  int ti, tj;         // block / tile row and column (ti <= tj)
+  int cluster;
+  int ti, tj;         // block row and column (ti <= tj)
   int part;           // index of this item's partial
   long p0, p1;        // packed column range [p0, p1), multiples of 16
 };
 
-// A block / tile of S_j and its K-chunk partials.
+// A kSyrkSub block of S_j and its K-chunk partials.
 struct SyrkTile {
   int cluster;
   int ti, tj;
