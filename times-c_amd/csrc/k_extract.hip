@@ -791,17 +791,16 @@ __global__ __launch_bounds__(256) void k_syrk(const SyrkGroup* __restrict__ grou
                     part + (long)(gr.part + u * gr.stride) * kSyrkSub * kSyrkSub, kSyrkSub);
 }
 
-// S_j from the K-chunk partials of each tile (edge TS), summed in a fixed
-// order.  One block per 64x64 sub-block (the
-// lower ones of diagonal tiles skipped): 16-byte partial loads, four rows
-// per thread in flight, the sums staged in LDS; then the entries with
-// i <= j and their mirrors are stored as coalesced row segments, so S_j is
-// exactly symmetric.
-template <int TS>
+// S_j from the K-chunk partials of each kSyrkSub block, summed in a fixed
+// order.  One thread block per 64x64 sub-block sb (the lower ones of
+// diagonal blocks skipped): 16-byte partial loads, four rows per thread in
+// flight, the sums staged in LDS; then the entries with i <= j and their
+// mirrors are stored as coalesced row segments, so S_j is exactly
+// symmetric.
 __device__ __forceinline__ void syrk_reduce_body(const int sb, const SyrkTile tl,
                                                  const float* __restrict__ part, int m,
                                                  float* __restrict__ S) {
-  constexpr int kSub = TS / 64;
+  constexpr int TS = kSyrkSub, kSub = TS / 64;
   const int si = sb / kSub, sj = sb % kSub;
   if (tl.ti == tl.tj && si > sj) return;
   const int gi0 = tl.ti * TS + si * 64, gj0 = tl.tj * TS + sj * 64;
@@ -856,7 +855,7 @@ __global__ __launch_bounds__(256) void k_syrk_reduce(const SyrkCluster* __restri
   const SyrkCluster c = clusters[t / U];
   int ti, tj;
   syrk_block_of(u, T, ti, tj);
-  syrk_reduce_body<kSyrkSub>(blockIdx.x % (kSub * kSub),
+  syrk_reduce_body(blockIdx.x % (kSub * kSub),
                              SyrkTile{c.cluster, ti, tj, c.part0 + u * c.nparts, c.nparts}, part,
                              m, S);
 }
@@ -2664,7 +2663,7 @@ __global__ __launch_bounds__(256) void k_syrk_reduce_group(const SyrkJob* __rest
   constexpr int kSub = kSyrkSub / 64;
   const GroupSyrkTile* gt = tiles + blockIdx.x / (kSub * kSub);
   const SyrkJob jb = jobs[gt->job];
-  syrk_reduce_body<kSyrkSub>(blockIdx.x % (kSub * kSub), gt->t, jb.part, jb.m, jb.S);
+  syrk_reduce_body(blockIdx.x % (kSub * kSub), gt->t, jb.part, jb.m, jb.S);
 }
 
 __global__ __launch_bounds__(kPowerThreads) void k_power_lanczos_group(
